@@ -1,0 +1,188 @@
+"""Benchmark: full-scene training step of the Gaussian rasteriser on one MI355X.
+
+Workload (one step = one iteration of the CF-3DGS training step on a synthetic
+scene): 500,000 Gaussians, SH degree 3, means in 32 isotropic clusters
+(sigma 0.3) with centres U([-3,3]^3), a 960x540 camera 10 units from the
+centroid; forward colour + depth through the drop-in
+``diff_gaussian_rasterization``, loss 0.8 L1 + 0.2 (1 - SSIM 11x11) against a
+seeded target, full backward, Adam update of all Gaussian parameters.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu-baseline]
+
+Prints one JSON line.  ``roofline`` describes the stage with the largest
+share of the timed region, timed with HIP events recorded on the launch
+stream by libgs_hip.so; ``cpu_baseline`` times the CPU oracle (the
+rasteriser restated in C++/OpenMP) on the same scene.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "cf-3dgs_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "fwd+bwd iters/s @500k Gaussians 960x540; pose-solves/s + p99 at fixed pose err"
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, passes: int = 6) -> float:
+    """Algorithmic HBM bytes of one launch of a stage (the bytes the algorithm
+    must move, independent of how the kernel moves them)."""
+    hw = W * H
+    if stage == "preprocess":   # in: xyz 12, scale 12, rot 16, opac 4, SH 12M; out: radii 4, splat 48, cov 24, tiles 4, scan 4
+        return P * (44 + 12 * M + 84)
+    if stage == "duplicate":    # in: splat 48 + tiles 4 + scan 4 (+ block prefix); out: 12 B per duplicated key
+        return P * 56 + 12 * n_dup
+    if stage == "sort":         # each 8-bit pass reads and writes (key 8, value 4)
+        return passes * 24 * n_dup
+    if stage == "ranges":
+        return 8 * n_dup + 8 * ((W + 15) // 16) * ((H + 15) // 16)
+    if stage == "blend":        # per list entry: id 4 + splat 48; per pixel out: rgb 12, depth 4, alpha 4, T 4, n 4
+        return 52 * n_dup + 28 * hw
+    if stage == "render_bwd":   # per list entry: id 4 + splat 48 + 40 B of gradient; per pixel in: 5 cotangents + T + n
+        return 92 * n_dup + 28 * hw
+    if stage == "preprocess_bwd":  # in: render grads 48, splat 16, cov 24, radii 4, xyz 12, scale 12, rot 16, SH 12M
+        return P * (132 + 12 * M + 92 + 12 * M)
+    return 0.0
+
+
+def build_state(device, P, W, H):
+    import torch
+    from gs_hip import scenes
+    from gs_hip.model import GaussianParams
+    sc = scenes.training_scene(seed=0, P=P, W=W, H=H)
+    g = GaussianParams(sc["xyz"], sc["features_dc"], sc["features_rest"], sc["scaling"], sc["rotation"],
+                       sc["opacity"], sc["sh_degree"], device=device)
+    g.training_setup()
+    return g, sc["cam"].to(device), sc["bg"].to(device), sc["target"].to(device), sc
+
+
+def cpu_baseline(sc, budget_s: float = 12.0):
+    """Oracle fwd + bwd iterations/s on the same scene (rasteriser only)."""
+    import numpy as np
+    import torch
+    from gs_hip.camera import oracle_camera
+    from oracle import oracle as O
+    O.build()
+    scene = dict(means=sc["xyz"].numpy(), scales=torch.exp(sc["scaling"]).numpy(),
+                 rotations=torch.nn.functional.normalize(sc["rotation"]).numpy(),
+                 opacities=torch.sigmoid(sc["opacity"]).numpy(),
+                 shs=torch.cat([sc["features_dc"], sc["features_rest"]], 1).numpy())
+    cam = oracle_camera(sc["cam"], sc["bg"], 3)
+    tgt = sc["target"].numpy()
+
+    def one():
+        f = O.forward(scene, cam)
+        dc = (np.sign(f["color"] - tgt) * (0.8 / f["color"].size)).astype(np.float32)
+        O.backward(scene, cam, f, dc, np.zeros_like(f["depth"]))
+
+    one()  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while n < 2 or (time.perf_counter() - t0 < budget_s and n < 10):
+        one()
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "iters/s", "cores": O.get_threads(), "kind": "port",
+            "sample": f"{n} rasteriser fwd+bwd iterations (preprocess, stable sort, blend, analytic backward; "
+                      f"no SSIM/Adam) of the same 500k-Gaussian 960x540 scene, C++/OpenMP oracle, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--P", type=int, default=500_000)
+    ap.add_argument("--width", type=int, default=960)
+    ap.add_argument("--height", type=int, default=540)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+
+    import torch
+    from gs_hip import _lib
+    from gs_hip.loss import photometric_loss
+    from gs_hip.model import render
+
+    device = torch.device("cuda:0")
+    g, cam, bg, target, sc = build_state(device, args.P, args.width, args.height)
+    opt = g.optimizer
+
+    def step():
+        out = render(g, cam, bg)
+        loss = photometric_loss(out["image"], target)
+        loss.backward()
+        opt.step()
+        opt.zero_grad(set_to_none=True)
+        return out
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.timing_enable(True)
+    _lib.timing_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.timing_enable(False)
+    ms = (t1 - t0) * 1000.0 / args.steps
+
+    # per-stage event timings over the timed region
+    stages = {}
+    for s in _lib.STAGES:
+        tot, n = _lib.timing_read(s)
+        if n:
+            stages[s] = {"ms_per_launch": tot / n, "launches": n}
+    _lib.timing_reset()
+    # N_dup of the last step (constant up to parameter drift)
+    with torch.no_grad():
+        from gs_hip import raster
+        from gs_hip.camera import raster_settings
+        rs = raster_settings(cam, bg, g.active_sh_degree)
+        _, _, _, _, st = raster.forward(rs, g.get_xyz, g.get_features, None, g.get_opacity, g.get_scaling,
+                                        g.get_rotation, None)
+        n_dup = st.num_rendered
+    L = _lib.binning_layout(n_dup, args.width, args.height)
+    for s, d in stages.items():
+        b = stage_bytes(s, args.P, n_dup, args.width, args.height, passes=L.sort_passes)
+        d["alg_bytes"] = b
+        d["GB_s"] = b / (d["ms_per_launch"] * 1e-3) / 1e9 if d["ms_per_launch"] > 0 else 0.0
+    dom = max(stages, key=lambda s: stages[s]["ms_per_launch"] * stages[s]["launches"]) if stages else None
+    roofline = None
+    if dom:
+        d = stages[dom]
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(d["GB_s"], 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(d["GB_s"] / HBM_PEAK_GBS, 4), "traffic": None,
+                    "ms_per_launch": round(d["ms_per_launch"], 4)}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
+    res = {
+        "metric": METRIC, "value": round(1000.0 / ms, 3), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene and target, random-init Gaussians)",
+        "config": {"workload": "full-scene training step: 500k Gaussians SH3, 32 clusters, 960x540, "
+                               "fwd colour+depth, 0.8 L1 + 0.2 (1-SSIM), full bwd, Adam",
+                   "P": args.P, "width": args.width, "height": args.height, "n_dup": n_dup,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                   for k, v in stages.items()},
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,511 @@
+// gs_backward.hip -- backward rasteriser for gfx950.
+//
+//   k_zero_f32       clears the per-Gaussian render-gradient accumulators and
+//                    the pose accumulator (a kernel, not a memset, so the
+//                    sequence can be captured in a graph)
+//   k_render_bwd     one 16x16 tile per workgroup, back to front over the
+//                    tile's list, starting at the workgroup's largest
+//                    contributor count (entries past it never contributed).
+//                    T is recovered by division as in the forward's algebra.
+//                    The 10 per-(pixel, Gaussian) gradient terms are summed
+//                    across the wave with DPP before one lane issues the
+//                    atomics, so a Gaussian receives at most 4 atomics per
+//                    term per tile instead of 256.
+//   k_preprocess_bwd per Gaussian: conic -> 2-D cov -> 3-D cov / mean,
+//                    projection, depth, SH direction; scale/rotation; the
+//                    rigid pose gradient is reduced wave -> atomic.
+//   k_pose_finalize  d/dR, d/dt -> d/d(q, t) through the quaternion
+//                    normalisation.
+#include "gs_common.hpp"
+#include "gs_internal.hpp"
+
+namespace gs {
+
+constexpr int kRG = 12;  // render-gradient record: m2x m2y ca cb cc opac r g b depth - -
+
+__global__ __launch_bounds__(256) void k_zero_f32(float* p, long long n) {
+  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256) p[k] = 0.0f;
+}
+
+struct RBArgs {
+  int W, H, gx;
+  const uint32_t* ranges;
+  const uint32_t* list;
+  const float4* splat;
+  const float* bg;
+  const float* final_T;
+  const uint32_t* n_contrib;
+  const float* dL_dcolor;
+  const float* dL_ddepth;
+  const float* dL_dalpha;
+  float* g_render;
+};
+
+__global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
+  __shared__ uint32_t s_id[256];
+  __shared__ float4 s_a[256];
+  __shared__ float4 s_b[256];
+  __shared__ float4 s_c[256];
+  __shared__ uint32_t s_max;
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = tile % a.gx, ty = tile / a.gx;
+  const int px = tx * kBX + (threadIdx.x & 15), py = ty * kBY + (threadIdx.x >> 4);
+  const bool inside = px < a.W && py < a.H;
+  const int pix = py * a.W + px;
+  const int HW = a.W * a.H;
+  const float pfx = (float)px, pfy = (float)py;
+  const uint32_t r0 = a.ranges[2 * tile];
+  const float T_fin = inside ? a.final_T[pix] : 0.0f;
+  const uint32_t last = inside ? a.n_contrib[pix] : 0u;
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
+  if (last) atomicMax(&s_max, last);
+  __syncthreads();
+  const uint32_t todo = s_max;
+  float dpix0 = 0.f, dpix1 = 0.f, dpix2 = 0.f, dD = 0.f, dA = 0.f;
+  if (inside) {
+    dpix0 = a.dL_dcolor[pix];
+    dpix1 = a.dL_dcolor[HW + pix];
+    dpix2 = a.dL_dcolor[2 * HW + pix];
+    if (a.dL_ddepth) dD = a.dL_ddepth[pix];
+    if (a.dL_dalpha) dA = a.dL_dalpha[pix];
+  }
+  const float bg_dot = (a.bg[0] * dpix0 + a.bg[1] * dpix1) + a.bg[2] * dpix2;
+  const float ddelx = 0.5f * (float)a.W, ddely = 0.5f * (float)a.H;
+  const float a_min = 1.0f / 255.0f;
+  float T = T_fin;
+  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc_d = 0.f, acc_a = 0.f;
+  float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f, last_d = 0.f;
+  const int lane = threadIdx.x & 63;
+
+  for (int bend = (int)todo; bend > 0; bend -= 256) {
+    const int bstart = bend > 256 ? bend - 256 : 0;
+    __syncthreads();
+    {
+      const int q = bstart + (int)threadIdx.x;
+      if (q < bend) {
+        const uint32_t g = a.list[r0 + q];
+        s_id[threadIdx.x] = g;
+        s_a[threadIdx.x] = a.splat[3 * g];
+        s_b[threadIdx.x] = a.splat[3 * g + 1];
+        s_c[threadIdx.x] = a.splat[3 * g + 2];
+      }
+    }
+    __syncthreads();
+    for (int j = bend - bstart - 1; j >= 0; --j) {
+      const int q = bstart + j;  // 0-based position; contributor index q+1
+      bool active = inside && (uint32_t)q < last;
+      float v[10];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) v[k] = 0.f;
+      const float4 ga = s_a[j];
+      const float4 gb = s_b[j];
+      if (active) {
+        const float dx = ga.x - pfx, dy = ga.y - pfy;
+        const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
+        float G = 0.f, alpha = 0.f;
+        if (power > 0.0f) {
+          active = false;
+        } else {
+          G = spec_expf(power);
+          alpha = fminf(0.99f, gb.y * G);
+          if (alpha < a_min) active = false;
+        }
+        if (active) {
+          const float4 gc = s_c[j];
+          T = T / (1.0f - alpha);
+          const float w = alpha * T;
+          float dL_da = 0.f;
+          acc0 = last_alpha * lc0 + (1.0f - last_alpha) * acc0;
+          acc1 = last_alpha * lc1 + (1.0f - last_alpha) * acc1;
+          acc2 = last_alpha * lc2 + (1.0f - last_alpha) * acc2;
+          lc0 = gc.x;
+          lc1 = gc.y;
+          lc2 = gc.z;
+          dL_da += (gc.x - acc0) * dpix0;
+          dL_da += (gc.y - acc1) * dpix1;
+          dL_da += (gc.z - acc2) * dpix2;
+          v[6] = w * dpix0;
+          v[7] = w * dpix1;
+          v[8] = w * dpix2;
+          acc_d = last_alpha * last_d + (1.0f - last_alpha) * acc_d;
+          last_d = gb.z;
+          dL_da += (gb.z - acc_d) * dD;
+          v[9] = w * dD;
+          acc_a = last_alpha + (1.0f - last_alpha) * acc_a;
+          dL_da += (1.0f - acc_a) * dA;
+          dL_da *= T;
+          last_alpha = alpha;
+          dL_da += (-T_fin / (1.0f - alpha)) * bg_dot;
+          const float dL_dG = gb.y * dL_da;
+          const float gdx = G * dx, gdy = G * dy;
+          const float dG_dx = -gdx * ga.z - gdy * ga.w;
+          const float dG_dy = -gdy * gb.x - gdx * ga.w;
+          v[0] = dL_dG * dG_dx * ddelx;
+          v[1] = dL_dG * dG_dy * ddely;
+          v[2] = -0.5f * gdx * dx * dL_dG;
+          v[3] = -gdx * dy * dL_dG;
+          v[4] = -0.5f * gdy * dy * dL_dG;
+          v[5] = G * dL_da;
+        }
+      }
+      if (__ballot(active) != 0ull) {
+        float* dst = a.g_render + (size_t)s_id[j] * kRG;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+          const float s = wave_sum(v[k]);
+          if (lane == 0 && s != 0.0f) unsafeAtomicAdd(dst + k, s);
+        }
+      }
+    }
+  }
+}
+
+struct PBArgs {
+  int P, M, D, W, H;
+  float tanfovx, tanfovy, fx, fy, scale_mod;
+  const float* view;
+  const float* proj;
+  const float* campos;
+  const float* pose;
+  const float* means;
+  const float* shs;
+  const float* scales;
+  const float* rots;
+  const float* cov_pre;
+  const int* radii;
+  const float4* splat;
+  const float* cov3D;
+  const float* g_render;
+  float* dL_dmeans2D;
+  float* dL_dcolors;
+  float* dL_dopacity;
+  float* dL_dmeans3D;
+  float* dL_dcov3D;
+  float* dL_dsh;
+  float* dL_dscales;
+  float* dL_drots;
+  float* pose_acc;  // [12]: dL/dt (3), dL/dR row-major (9)
+};
+
+__global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  float pacc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) pacc[k] = 0.f;
+  Pose ps;
+  if (a.pose) make_pose(a.pose, ps);
+  if (i < a.P) {
+    const bool vis = a.radii[i] > 0;
+    const float* rg = a.g_render + (size_t)i * kRG;
+    const float4 rA = reinterpret_cast<const float4*>(rg)[0];  // m2x m2y ca cb
+    const float4 rB = reinterpret_cast<const float4*>(rg)[1];  // cc opac r g
+    const float4 rC = reinterpret_cast<const float4*>(rg)[2];  // b depth - -
+    a.dL_dmeans2D[3 * i] = vis ? rA.x : 0.f;
+    a.dL_dmeans2D[3 * i + 1] = vis ? rA.y : 0.f;
+    a.dL_dmeans2D[3 * i + 2] = 0.f;
+    a.dL_dopacity[i] = vis ? rB.y : 0.f;
+    a.dL_dcolors[3 * i] = vis ? rB.z : 0.f;
+    a.dL_dcolors[3 * i + 1] = vis ? rB.w : 0.f;
+    a.dL_dcolors[3 * i + 2] = vis ? rC.x : 0.f;
+    float gp[3] = {0.f, 0.f, 0.f};
+    float gS[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const int nc = (a.D + 1) * (a.D + 1);
+    if (vis) {
+      f3 p0 = {a.means[3 * i], a.means[3 * i + 1], a.means[3 * i + 2]};
+      const f3 p = a.pose ? pose_apply(ps, p0) : p0;
+      const float* c3 = a.cov3D + 6 * i;
+      // ---- conic -> 2-D covariance -> 3-D covariance and mean (through J)
+      const f3 t = xf43(a.view, p);
+      const float limx = 1.3f * a.tanfovx, limy = 1.3f * a.tanfovy;
+      const float txtz = t.x / t.z, tytz = t.y / t.z;
+      const float tx = fminf(limx, fmaxf(-limx, txtz)) * t.z;
+      const float ty = fminf(limy, fmaxf(-limy, tytz)) * t.z;
+      const float xm = (txtz < -limx || txtz > limx) ? 0.f : 1.f;
+      const float ym = (tytz < -limy || tytz > limy) ? 0.f : 1.f;
+      const float tz = t.z;
+      const float J00 = a.fx / tz, J02 = -(a.fx * tx) / (tz * tz);
+      const float J11 = a.fy / tz, J12 = -(a.fy * ty) / (tz * tz);
+      float T0[3], T1[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        T0[c] = J00 * a.view[c * 4 + 0] + J02 * a.view[c * 4 + 2];
+        T1[c] = J11 * a.view[c * 4 + 1] + J12 * a.view[c * 4 + 2];
+      }
+      const float S[3][3] = {{c3[0], c3[1], c3[2]}, {c3[1], c3[3], c3[4]}, {c3[2], c3[4], c3[5]}};
+      float ST0[3], ST1[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        ST0[k] = S[k][0] * T0[0] + S[k][1] * T0[1] + S[k][2] * T0[2];
+        ST1[k] = S[k][0] * T1[0] + S[k][1] * T1[1] + S[k][2] * T1[2];
+      }
+      const float ca = T0[0] * ST0[0] + T0[1] * ST0[1] + T0[2] * ST0[2] + 0.3f;
+      const float cb = T0[0] * ST1[0] + T0[1] * ST1[1] + T0[2] * ST1[2];
+      const float cc = T1[0] * ST1[0] + T1[1] * ST1[1] + T1[2] * ST1[2] + 0.3f;
+      const float den = ca * cc - cb * cb;
+      const float gA = rA.z, gB = rA.w, gC = rB.x;
+      const float d2i = 1.0f / (den * den + 0.0000001f);
+      float dLa = 0.f, dLb = 0.f, dLc = 0.f;
+      if (d2i != 0.0f) {
+        dLa = d2i * (-cc * cc * gA + cb * cc * gB + (den - ca * cc) * gC);
+        dLc = d2i * (-ca * ca * gC + ca * cb * gB + (den - ca * cc) * gA);
+        dLb = d2i * (2.0f * cb * cc * gA - (den + 2.0f * cb * cb) * gB + 2.0f * ca * cb * gC);
+        gS[0] = T0[0] * T0[0] * dLa + T0[0] * T1[0] * dLb + T1[0] * T1[0] * dLc;
+        gS[3] = T0[1] * T0[1] * dLa + T0[1] * T1[1] * dLb + T1[1] * T1[1] * dLc;
+        gS[5] = T0[2] * T0[2] * dLa + T0[2] * T1[2] * dLb + T1[2] * T1[2] * dLc;
+        gS[1] = 2.0f * T0[0] * T0[1] * dLa + (T0[0] * T1[1] + T0[1] * T1[0]) * dLb + 2.0f * T1[0] * T1[1] * dLc;
+        gS[2] = 2.0f * T0[0] * T0[2] * dLa + (T0[0] * T1[2] + T0[2] * T1[0]) * dLb + 2.0f * T1[0] * T1[2] * dLc;
+        gS[4] = 2.0f * T0[1] * T0[2] * dLa + (T0[1] * T1[2] + T0[2] * T1[1]) * dLb + 2.0f * T1[1] * T1[2] * dLc;
+      }
+      float gJ00 = 0.f, gJ02 = 0.f, gJ11 = 0.f, gJ12 = 0.f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float gT0 = 2.0f * dLa * ST0[c] + dLb * ST1[c];
+        const float gT1 = 2.0f * dLc * ST1[c] + dLb * ST0[c];
+        gJ00 += gT0 * a.view[c * 4 + 0];
+        gJ02 += gT0 * a.view[c * 4 + 2];
+        gJ11 += gT1 * a.view[c * 4 + 1];
+        gJ12 += gT1 * a.view[c * 4 + 2];
+      }
+      const float iz = 1.0f / tz, iz2 = iz * iz, iz3 = iz2 * iz;
+      const float gtx = xm * -a.fx * iz2 * gJ02;
+      const float gty = ym * -a.fy * iz2 * gJ12;
+      const float gtz = -a.fx * iz2 * gJ00 - a.fy * iz2 * gJ11 + (2.0f * a.fx * tx) * iz3 * gJ02 +
+                        (2.0f * a.fy * ty) * iz3 * gJ12;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gp[k] = a.view[k * 4 + 0] * gtx + a.view[k * 4 + 1] * gty + a.view[k * 4 + 2] * gtz;
+      // ---- projection of the mean (NDC units)
+      const f3 ph = xf43(a.proj, p);
+      const float mw = 1.0f / (xf44w(a.proj, p) + 0.0000001f);
+      const float mul1 = ph.x * mw * mw, mul2 = ph.y * mw * mw;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        gp[k] += (a.proj[4 * k + 0] * mw - a.proj[4 * k + 3] * mul1) * rA.x +
+                 (a.proj[4 * k + 1] * mw - a.proj[4 * k + 3] * mul2) * rA.y;
+      // ---- depth = view-space z
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gp[k] += a.view[4 * k + 2] * rC.y;
+      // ---- SH colour
+      if (a.shs) {
+        const float cbits = a.splat[3 * i + 1].w;
+        const uint32_t cm = __float_as_uint(cbits);
+        const float gc[3] = {(cm & 1u) ? 0.f : rB.z, (cm & 2u) ? 0.f : rB.w, (cm & 4u) ? 0.f : rC.x};
+        const f3 dv = {p.x - a.campos[0], p.y - a.campos[1], p.z - a.campos[2]};
+        const float len = sqrtf(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
+        const float dx = dv.x / len, dy = dv.y / len, dz = dv.z / len;
+        float b[16], bg3[16][3];
+        sh_basis(a.D, dx, dy, dz, b);
+        sh_basis_grad(a.D, dx, dy, dz, bg3);
+        const float* sh = a.shs + (size_t)i * a.M * 3;
+        float* gsh = a.dL_dsh + (size_t)i * a.M * 3;
+        float gdir0 = 0.f, gdir1 = 0.f, gdir2 = 0.f;
+        for (int k = 0; k < nc; ++k) {
+          const float s0 = sh[3 * k], s1 = sh[3 * k + 1], s2 = sh[3 * k + 2];
+          gsh[3 * k] = b[k] * gc[0];
+          gsh[3 * k + 1] = b[k] * gc[1];
+          gsh[3 * k + 2] = b[k] * gc[2];
+          const float sd = s0 * gc[0] + s1 * gc[1] + s2 * gc[2];
+          gdir0 += sd * bg3[k][0];
+          gdir1 += sd * bg3[k][1];
+          gdir2 += sd * bg3[k][2];
+        }
+        for (int k = 3 * nc; k < 3 * a.M; ++k) gsh[k] = 0.f;
+        const float ndot = dx * gdir0 + dy * gdir1 + dz * gdir2;
+        gp[0] += (gdir0 - dx * ndot) / len;
+        gp[1] += (gdir1 - dy * ndot) / len;
+        gp[2] += (gdir2 - dz * ndot) / len;
+      }
+      // ---- scale / rotation
+      if (!a.cov_pre) {
+        const float4 q = reinterpret_cast<const float4*>(a.rots)[i];
+        float R[9];
+        quat_to_R(q.x, q.y, q.z, q.w, R);
+        const float s[3] = {a.scale_mod * a.scales[3 * i], a.scale_mod * a.scales[3 * i + 1],
+                            a.scale_mod * a.scales[3 * i + 2]};
+        const float G[9] = {gS[0], 0.5f * gS[1], 0.5f * gS[2], 0.5f * gS[1], gS[3],
+                            0.5f * gS[4], 0.5f * gS[2], 0.5f * gS[4], gS[5]};
+        float L[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) L[3 * r + k] = R[3 * r + k] * s[k];
+        float gR[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          float gsk = 0.f;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const float gL = 2.0f * (G[3 * r] * L[k] + G[3 * r + 1] * L[3 + k] + G[3 * r + 2] * L[6 + k]);
+            gsk += gL * R[3 * r + k];
+            gR[3 * r + k] = gL * s[k];
+          }
+          a.dL_dscales[3 * i + k] = gsk;
+        }
+        float dq[4];
+        quat_R_grad(q.x, q.y, q.z, q.w, gR, dq);
+        reinterpret_cast<float4*>(a.dL_drots)[i] = make_float4(dq[0], dq[1], dq[2], dq[3]);
+      }
+      if (a.pose) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          pacc[r] = gp[r];
+          pacc[3 + 3 * r + 0] = gp[r] * p0.x;
+          pacc[3 + 3 * r + 1] = gp[r] * p0.y;
+          pacc[3 + 3 * r + 2] = gp[r] * p0.z;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          a.dL_dmeans3D[3 * i + k] = ps.R[k] * gp[0] + ps.R[3 + k] * gp[1] + ps.R[6 + k] * gp[2];
+      } else {
+        a.dL_dmeans3D[3 * i] = gp[0];
+        a.dL_dmeans3D[3 * i + 1] = gp[1];
+        a.dL_dmeans3D[3 * i + 2] = gp[2];
+      }
+    } else {
+      a.dL_dmeans3D[3 * i] = a.dL_dmeans3D[3 * i + 1] = a.dL_dmeans3D[3 * i + 2] = 0.f;
+      if (a.shs)
+        for (int k = 0; k < 3 * a.M; ++k) a.dL_dsh[(size_t)i * a.M * 3 + k] = 0.f;
+      if (!a.cov_pre) {
+        a.dL_dscales[3 * i] = a.dL_dscales[3 * i + 1] = a.dL_dscales[3 * i + 2] = 0.f;
+        reinterpret_cast<float4*>(a.dL_drots)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * i + k] = gS[k];
+  }
+  if (a.pose) {
+    // all lanes reach here (no early return above)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const float s = wave_sum(pacc[k]);
+      if ((threadIdx.x & 63) == 0 && s != 0.0f) unsafeAtomicAdd(a.pose_acc + k, s);
+    }
+  }
+}
+
+__global__ void k_pose_finalize(const float* pose7, const float* acc, float* dL_dpose) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Pose ps;
+  make_pose(pose7, ps);
+  float GR[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) GR[k] = acc[3 + k];
+  float dqh[4];
+  quat_R_grad(ps.q[0], ps.q[1], ps.q[2], ps.q[3], GR, dqh);
+  const float dot = ps.q[0] * dqh[0] + ps.q[1] * dqh[1] + ps.q[2] * dqh[2] + ps.q[3] * dqh[3];
+  float dq[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) dq[k] = (dqh[k] - ps.q[k] * dot) * ps.inv_norm;
+  dL_dpose[0] = acc[0];
+  dL_dpose[1] = acc[1];
+  dL_dpose[2] = acc[2];
+  dL_dpose[3] = dq[1];
+  dL_dpose[4] = dq[2];
+  dL_dpose[5] = dq[3];
+  dL_dpose[6] = dq[0];
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" size_t gs_backward_scratch_bytes(int P) {
+  return align_up((size_t)(P > 0 ? P : 0) * kRG * sizeof(float), 256) + 256;
+}
+
+extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
+                           const void* binning, long long capacity, const void* image, const uint32_t* counters,
+                           const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
+                           const gs_grads_t* out, void* scratch, size_t scratch_bytes, void* stream) {
+  if (!f || !g || !out || !counters || !image || !dL_dcolor) return GS_E_ARG;
+  if (f->width <= 0 || f->height <= 0 || !f->viewmatrix || !f->projmatrix || !f->bg) return GS_E_ARG;
+  const int P = g->P;
+  if (P < 0) return GS_E_ARG;
+  if (P > 0 && (g->shs == nullptr) == (g->colors_precomp == nullptr)) return GS_E_ARG;
+  const bool has_sr = g->scales && g->rotations;
+  if (P > 0 && has_sr == (g->cov3D_precomp != nullptr)) return GS_E_ARG;
+  if (P > 0 && (!radii || !geom || !out->dL_dmeans2D || !out->dL_dcolors || !out->dL_dopacity ||
+                !out->dL_dmeans3D || !out->dL_dcov3D))
+    return GS_E_ARG;
+  if (g->shs && !out->dL_dsh) return GS_E_ARG;
+  if (has_sr && (!out->dL_dscales || !out->dL_drotations)) return GS_E_ARG;
+  if (out->dL_dpose && !f->pose) return GS_E_ARG;
+  if (scratch_bytes < gs_backward_scratch_bytes(P) || !scratch) return GS_E_WORKSPACE;
+  gs_geom_layout_t LG;
+  gs_binning_layout_t LB;
+  gs_image_layout_t LI;
+  gs_geom_layout(P, &LG);
+  if (gs_binning_layout(capacity, f->width, f->height, &LB) != GS_OK) return GS_E_ARG;
+  gs_image_layout(f->width, f->height, &LI);
+  hipStream_t s = (hipStream_t)stream;
+  const int gx = (f->width + kBX - 1) / kBX, gy = (f->height + kBY - 1) / kBY;
+  const int nt = gx * gy;
+  const char* gb = (const char*)geom;
+  const char* bb = (const char*)binning;
+  const char* ib = (const char*)image;
+  float* g_render = (float*)scratch;
+  float* pose_acc = (float*)((char*)scratch + align_up((size_t)P * kRG * sizeof(float), 256));
+  const long long nz = (long long)P * kRG;
+  const long long nzb = (nz + 255) / 256;
+  hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)(nzb < 4096 ? (nzb > 0 ? nzb : 1) : 4096)), dim3(256), 0, s, g_render,
+                     nz);
+  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, pose_acc, 16LL);
+  if (capacity > 0 && P > 0) {
+    RBArgs r;
+    r.W = f->width;
+    r.H = f->height;
+    r.gx = gx;
+    r.ranges = (const uint32_t*)(ib + LI.ranges);
+    r.list = (const uint32_t*)(bb + LB.vals_sorted);
+    r.splat = (const float4*)(gb + LG.splat);
+    r.bg = f->bg;
+    r.final_T = (const float*)(ib + LI.final_T);
+    r.n_contrib = (const uint32_t*)(ib + LI.n_contrib);
+    r.dL_dcolor = dL_dcolor;
+    r.dL_ddepth = dL_ddepth;
+    r.dL_dalpha = dL_dalpha;
+    r.g_render = g_render;
+    timing::Scope ts(timing::kRenderBwd, s);
+    hipLaunchKernelGGL(k_render_bwd, dim3(nt), dim3(256), 0, s, r);
+  }
+  if (P > 0) {
+    PBArgs b;
+    b.P = P;
+    b.M = g->M;
+    b.D = f->sh_degree;
+    b.W = f->width;
+    b.H = f->height;
+    b.tanfovx = f->tanfovx;
+    b.tanfovy = f->tanfovy;
+    b.fx = (float)f->width / (2.0f * f->tanfovx);
+    b.fy = (float)f->height / (2.0f * f->tanfovy);
+    b.scale_mod = f->scale_modifier;
+    b.view = f->viewmatrix;
+    b.proj = f->projmatrix;
+    b.campos = f->campos;
+    b.pose = f->pose;
+    b.means = g->means3D;
+    b.shs = g->shs;
+    b.scales = g->scales;
+    b.rots = g->rotations;
+    b.cov_pre = g->cov3D_precomp;
+    b.radii = radii;
+    b.splat = (const float4*)(gb + LG.splat);
+    b.cov3D = (const float*)(gb + LG.cov3D);
+    b.g_render = g_render;
+    b.dL_dmeans2D = out->dL_dmeans2D;
+    b.dL_dcolors = out->dL_dcolors;
+    b.dL_dopacity = out->dL_dopacity;
+    b.dL_dmeans3D = out->dL_dmeans3D;
+    b.dL_dcov3D = out->dL_dcov3D;
+    b.dL_dsh = out->dL_dsh;
+    b.dL_dscales = out->dL_dscales;
+    b.dL_drots = out->dL_drotations;
+    b.pose_acc = pose_acc;
+    timing::Scope ts(timing::kPreprocessBwd, s);
+    hipLaunchKernelGGL(k_preprocess_bwd, dim3((P + 255) / 256), dim3(256), 0, s, b);
+  }
+  if (out->dL_dpose) hipLaunchKernelGGL(k_pose_finalize, dim3(1), dim3(64), 0, s, f->pose, pose_acc, out->dL_dpose);
+  (void)counters;
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}

@@ -1,0 +1,241 @@
+// gs_common.hpp -- device-side building blocks shared by the rasteriser kernels.
+//
+// Rounding contract: every kernel is compiled with -ffp-contract=off, keeps
+// HIP's correctly rounded fp32 division and square root, and spells out each
+// fused multiply-add it wants.  Together with the exponential below this
+// makes radii, tile rects, keys, ranges, contributor counts and the forward
+// image bit-identical to the fp32 CPU restatement kept in oracle/.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gs_raster.h"
+
+namespace gs {
+
+constexpr int kBX = 16;
+constexpr int kBY = 16;
+constexpr int kBlock = kBX * kBY;  // one 16x16 tile per 256-thread workgroup
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Exponential of the blend: Cody & Waite reduction (Cody, Waite, "Software
+// Manual for the Elementary Functions", 1980) + degree-7 Taylor polynomial,
+// all steps explicit fma / rint / ldexp, which gfx950 and the host round
+// identically.  Domain used by the rasteriser: x <= 0.
+__device__ __forceinline__ float spec_expf(float x) {
+  x = fmaxf(x, -80.0f);
+  const float k = rintf(x * 1.44269502f);
+  float r = fmaf(k, -0.693115234375f, x);
+  r = fmaf(k, -3.19461833e-05f, r);
+  float p = 1.98412701e-04f;
+  p = fmaf(p, r, 1.38888892e-03f);
+  p = fmaf(p, r, 8.33333377e-03f);
+  p = fmaf(p, r, 4.16666679e-02f);
+  p = fmaf(p, r, 1.66666672e-01f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)k);
+}
+
+// -0.5 (a dx^2 + c dy^2) - b dx dy with the oracle's fma placement.
+__device__ __forceinline__ float blend_power(float a, float b, float c, float dx, float dy) {
+  const float q = fmaf(a * dx, dx, (c * dy) * dy);
+  return fmaf(-0.5f, q, -((b * dx) * dy));
+}
+
+// Saturating float -> int (NaN -> 0).
+__device__ __forceinline__ int sat_i(float f) {
+  if (f != f) return 0;
+  if (f >= 2147483647.0f) return 2147483647;
+  if (f <= -2147483648.0f) return (-2147483647 - 1);
+  return (int)f;
+}
+
+// ((v + 1) * S - 1) * 0.5 evaluated in double, rounded once to float.
+__device__ __forceinline__ float ndc2pix(float v, int S) {
+  return (float)((((double)v + 1.0) * (double)S - 1.0) * 0.5);
+}
+
+struct f3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ f3 xf43(const float* m, f3 p) {
+  return {m[0] * p.x + m[4] * p.y + m[8] * p.z + m[12], m[1] * p.x + m[5] * p.y + m[9] * p.z + m[13],
+          m[2] * p.x + m[6] * p.y + m[10] * p.z + m[14]};
+}
+__device__ __forceinline__ float xf44w(const float* m, f3 p) {
+  return m[3] * p.x + m[7] * p.y + m[11] * p.z + m[15];
+}
+
+// Row-major rotation of quaternion (r, x, y, z); utils/general_utils.py:88-96.
+__device__ __forceinline__ void quat_to_R(float r, float x, float y, float z, float R[9]) {
+  R[0] = 1.0f - 2.0f * (y * y + z * z);
+  R[1] = 2.0f * (x * y - r * z);
+  R[2] = 2.0f * (x * z + r * y);
+  R[3] = 2.0f * (x * y + r * z);
+  R[4] = 1.0f - 2.0f * (x * x + z * z);
+  R[5] = 2.0f * (y * z - r * x);
+  R[6] = 2.0f * (x * z - r * y);
+  R[7] = 2.0f * (y * z + r * x);
+  R[8] = 1.0f - 2.0f * (x * x + y * y);
+}
+
+// d(loss)/d(r,x,y,z) given d(loss)/dR (row-major) for the polynomial above.
+__device__ __forceinline__ void quat_R_grad(float r, float x, float y, float z, const float G[9], float dq[4]) {
+  dq[0] = 2.0f * (-z * G[1] + y * G[2] + z * G[3] - x * G[5] - y * G[6] + x * G[7]);
+  dq[1] = 2.0f * (y * G[1] + z * G[2] + y * G[3] - 2.0f * x * G[4] - r * G[5] + z * G[6] + r * G[7] -
+                  2.0f * x * G[8]);
+  dq[2] = 2.0f * (-2.0f * y * G[0] + x * G[1] + r * G[2] + x * G[3] + z * G[5] - r * G[6] + z * G[7] -
+                  2.0f * y * G[8]);
+  dq[3] = 2.0f * (-2.0f * z * G[0] - r * G[1] + x * G[2] + r * G[3] - 2.0f * z * G[4] + y * G[5] +
+                  x * G[6] + y * G[7]);
+}
+
+// Pose [tx, ty, tz, qx, qy, qz, qw] -> normalised quaternion + rotation.
+struct Pose {
+  float R[9];
+  float t[3];
+  float q[4];  // (w, x, y, z), normalised
+  float inv_norm;
+};
+__device__ __forceinline__ void make_pose(const float* p7, Pose& P) {
+  const float qx = p7[3], qy = p7[4], qz = p7[5], qw = p7[6];
+  const float n2 = ((qx * qx + qy * qy) + qz * qz) + qw * qw;
+  P.inv_norm = 1.0f / sqrtf(n2);
+  P.q[0] = qw * P.inv_norm;
+  P.q[1] = qx * P.inv_norm;
+  P.q[2] = qy * P.inv_norm;
+  P.q[3] = qz * P.inv_norm;
+  quat_to_R(P.q[0], P.q[1], P.q[2], P.q[3], P.R);
+  P.t[0] = p7[0];
+  P.t[1] = p7[1];
+  P.t[2] = p7[2];
+}
+__device__ __forceinline__ f3 pose_apply(const Pose& P, f3 p) {
+  return {((P.R[0] * p.x + P.R[1] * p.y) + P.R[2] * p.z) + P.t[0],
+          ((P.R[3] * p.x + P.R[4] * p.y) + P.R[5] * p.z) + P.t[1],
+          ((P.R[6] * p.x + P.R[7] * p.y) + P.R[8] * p.z) + P.t[2]};
+}
+
+// Real SH basis, constants of utils/sh_utils.py:26-45, polynomial of :74-100.
+#define GS_SH_C0 0.28209479177387814f
+#define GS_SH_C1 0.4886025119029199f
+#define GS_SH_C2_0 1.0925484305920792f
+#define GS_SH_C2_1 -1.0925484305920792f
+#define GS_SH_C2_2 0.31539156525252005f
+#define GS_SH_C2_3 -1.0925484305920792f
+#define GS_SH_C2_4 0.5462742152960396f
+#define GS_SH_C3_0 -0.5900435899266435f
+#define GS_SH_C3_1 2.890611442640554f
+#define GS_SH_C3_2 -0.4570457994644658f
+#define GS_SH_C3_3 0.3731763325901154f
+#define GS_SH_C3_4 -0.4570457994644658f
+#define GS_SH_C3_5 1.445305721320277f
+#define GS_SH_C3_6 -0.5900435899266435f
+
+__device__ __forceinline__ void sh_basis(int deg, float x, float y, float z, float b[16]) {
+  b[0] = GS_SH_C0;
+  if (deg < 1) return;
+  b[1] = -GS_SH_C1 * y;
+  b[2] = GS_SH_C1 * z;
+  b[3] = -GS_SH_C1 * x;
+  if (deg < 2) return;
+  const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+  b[4] = GS_SH_C2_0 * xy;
+  b[5] = GS_SH_C2_1 * yz;
+  b[6] = GS_SH_C2_2 * ((2.0f * zz - xx) - yy);
+  b[7] = GS_SH_C2_3 * xz;
+  b[8] = GS_SH_C2_4 * (xx - yy);
+  if (deg < 3) return;
+  b[9] = (GS_SH_C3_0 * y) * (3.0f * xx - yy);
+  b[10] = (GS_SH_C3_1 * xy) * z;
+  b[11] = (GS_SH_C3_2 * y) * ((4.0f * zz - xx) - yy);
+  b[12] = (GS_SH_C3_3 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+  b[13] = (GS_SH_C3_4 * x) * ((4.0f * zz - xx) - yy);
+  b[14] = (GS_SH_C3_5 * z) * (xx - yy);
+  b[15] = (GS_SH_C3_6 * x) * (xx - 3.0f * yy);
+}
+
+// d(basis_k)/d(x,y,z).
+__device__ __forceinline__ void sh_basis_grad(int deg, float x, float y, float z, float g[16][3]) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) g[k][0] = g[k][1] = g[k][2] = 0.0f;
+  if (deg < 1) return;
+  g[1][1] = -GS_SH_C1;
+  g[2][2] = GS_SH_C1;
+  g[3][0] = -GS_SH_C1;
+  if (deg < 2) return;
+  const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+  g[4][0] = GS_SH_C2_0 * y;
+  g[4][1] = GS_SH_C2_0 * x;
+  g[5][1] = GS_SH_C2_1 * z;
+  g[5][2] = GS_SH_C2_1 * y;
+  g[6][0] = -2.0f * GS_SH_C2_2 * x;
+  g[6][1] = -2.0f * GS_SH_C2_2 * y;
+  g[6][2] = 4.0f * GS_SH_C2_2 * z;
+  g[7][0] = GS_SH_C2_3 * z;
+  g[7][2] = GS_SH_C2_3 * x;
+  g[8][0] = 2.0f * GS_SH_C2_4 * x;
+  g[8][1] = -2.0f * GS_SH_C2_4 * y;
+  if (deg < 3) return;
+  g[9][0] = GS_SH_C3_0 * 6.0f * xy;
+  g[9][1] = GS_SH_C3_0 * (3.0f * xx - 3.0f * yy);
+  g[10][0] = GS_SH_C3_1 * yz;
+  g[10][1] = GS_SH_C3_1 * xz;
+  g[10][2] = GS_SH_C3_1 * xy;
+  g[11][0] = GS_SH_C3_2 * (-2.0f * xy);
+  g[11][1] = GS_SH_C3_2 * (4.0f * zz - xx - 3.0f * yy);
+  g[11][2] = GS_SH_C3_2 * (8.0f * yz);
+  g[12][0] = GS_SH_C3_3 * (-6.0f * xz);
+  g[12][1] = GS_SH_C3_3 * (-6.0f * yz);
+  g[12][2] = GS_SH_C3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy);
+  g[13][0] = GS_SH_C3_4 * (4.0f * zz - 3.0f * xx - yy);
+  g[13][1] = GS_SH_C3_4 * (-2.0f * xy);
+  g[13][2] = GS_SH_C3_4 * (8.0f * xz);
+  g[14][0] = GS_SH_C3_5 * (2.0f * xz);
+  g[14][1] = GS_SH_C3_5 * (-2.0f * yz);
+  g[14][2] = GS_SH_C3_5 * (xx - yy);
+  g[15][0] = GS_SH_C3_6 * (3.0f * xx - 3.0f * yy);
+  g[15][1] = GS_SH_C3_6 * (-6.0f * xy);
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 helpers.
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over all 64 lanes; every lane must be active.  Result is wave-uniform.
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp<0x141>(v);  // row_half_mirror
+  v += dpp<0x140>(v);  // row_mirror
+  const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0));
+  const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16));
+  const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+  const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48));
+  return (r0 + r1) + (r2 + r3);
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Block id -> logical id so that each of the 8 XCDs (dispatch is round-robin
+// over XCDs) receives a contiguous run of logical work items: neighbouring
+// tiles share Gaussians, so this keeps their gathers in one L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+  const uint32_t full = nb & ~7u;
+  if (b >= full) return b;
+  const uint32_t per = full >> 3;
+  return (b & 7u) * per + (b >> 3);
+}
+
+}  // namespace gs

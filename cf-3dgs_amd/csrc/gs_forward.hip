@@ -1,0 +1,653 @@
+// gs_forward.hip -- forward rasteriser for gfx950.
+//
+// Pipeline (one frame):
+//   k_preprocess   per Gaussian: optional rigid pose, near cull, 3-D and EWA
+//                  2-D covariance, conic, radius, tile rect, SH -> RGB; writes a
+//                  48-byte splat record and a workgroup-local prefix sum of the
+//                  tiles touched (SH coefficients staged through LDS so the
+//                  [P,M,3] array is read with coalesced 16-byte loads)
+//   k_scan_top     one workgroup: exclusive prefix of the per-workgroup sums,
+//                  total -> counters[0], status -> counters[1]
+//   k_duplicate    per Gaussian: (tile << 32 | depth bits, id) pairs
+//   radix sort     gs_sort.hip, 8-bit LSD passes over bits [0, 32 + bits(T))
+//   k_ranges       [start, end) per tile in the sorted list
+//   k_blend        one 16x16 tile per 256-thread workgroup; splats staged in
+//                  LDS 256 at a time; block-wide vote ends the tile early once
+//                  every pixel is saturated; XCD-aware tile order.
+#include "gs_common.hpp"
+#include "gs_internal.hpp"
+
+namespace gs {
+
+struct PreArgs {
+  int P, M, D, W, H, gx, gy, prefiltered;
+  float tanfovx, tanfovy, fx, fy, scale_mod;
+  const float* view;
+  const float* proj;
+  const float* campos;
+  const float* pose;
+  const float* means;
+  const float* shs;
+  const float* colors;
+  const float* opac;
+  const float* scales;
+  const float* rots;
+  const float* cov_pre;
+  int* radii;
+  float4* splat;
+  float* cov3D;
+  uint32_t* tiles;
+  uint32_t* local_incl;  // workgroup-local inclusive prefix of tiles
+  uint32_t* block_sum;   // [nblocks] sums, then [nblocks] status flags
+};
+
+__device__ __forceinline__ void tile_rect(float px, float py, int r, int gx, int gy, int& x0, int& y0, int& x1,
+                                          int& y1) {
+  const float rr = (float)r;
+  x0 = min(gx, max(0, sat_i((px - rr) / (float)kBX)));
+  y0 = min(gy, max(0, sat_i((py - rr) / (float)kBY)));
+  x1 = min(gx, max(0, sat_i((((px + rr) + (float)kBX) - 1.0f) / (float)kBX)));
+  y1 = min(gy, max(0, sat_i((((py + rr) + (float)kBY) - 1.0f) / (float)kBY)));
+}
+
+// Inclusive prefix sum over the 256 threads of a workgroup (uint32).
+__device__ __forceinline__ uint32_t block_incl_scan_256(uint32_t v, uint32_t* s_tmp /*[256]*/, uint32_t& total) {
+  const int t = threadIdx.x;
+  s_tmp[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t add = t >= off ? s_tmp[t - off] : 0u;
+    __syncthreads();
+    s_tmp[t] += add;
+    __syncthreads();
+  }
+  total = s_tmp[255];
+  const uint32_t r = s_tmp[t];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
+  extern __shared__ float s_sh[];  // [256][3*nc] staged SH coefficients
+  __shared__ uint32_t s_scan[256];
+  __shared__ uint32_t s_flag;
+  const int base = blockIdx.x * 256;
+  const int i = base + threadIdx.x;
+  const int nc = (a.D + 1) * (a.D + 1);
+  const int stride = 3 * nc;
+  if (threadIdx.x == 0) s_flag = 0;
+  if (a.shs) {
+    const int nloc = min(256, a.P - base);
+    if (nc == a.M) {
+      // contiguous [nloc * 3M] floats; 3M*4 bytes per Gaussian is a multiple of 16 when M % 4 == 0
+      const float* src = a.shs + (size_t)base * 3 * a.M;
+      const int total = nloc * stride;
+      if ((a.M & 3) == 0) {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        float4* d4 = reinterpret_cast<float4*>(s_sh);
+        for (int k = threadIdx.x; k < (total >> 2); k += 256) d4[k] = s4[k];
+      } else {
+        for (int k = threadIdx.x; k < total; k += 256) s_sh[k] = src[k];
+      }
+    } else {
+      for (int k = threadIdx.x; k < nloc * stride; k += 256) {
+        const int g = k / stride, e = k - g * stride;
+        s_sh[k] = a.shs[(size_t)(base + g) * 3 * a.M + e];
+      }
+    }
+  }
+  __syncthreads();
+
+  uint32_t my_tiles = 0;
+  if (i < a.P) {
+    a.radii[i] = 0;
+    f3 p = {a.means[3 * i], a.means[3 * i + 1], a.means[3 * i + 2]};
+    if (a.pose) {
+      Pose ps;
+      make_pose(a.pose, ps);
+      p = pose_apply(ps, p);
+    }
+    const f3 pv = xf43(a.view, p);
+    bool ok = true;
+    if (pv.z <= 0.2f) {
+      if (a.prefiltered) s_flag = GS_STATUS_PREFILTERED;
+      ok = false;
+    }
+    float c3[6];
+    float px = 0.f, py = 0.f, con_a = 0.f, con_b = 0.f, con_c = 0.f;
+    int ir = 0;
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    if (ok) {
+      const f3 ph = xf43(a.proj, p);
+      const float pw = 1.0f / (xf44w(a.proj, p) + 0.0000001f);
+      const float ndc_x = ph.x * pw, ndc_y = ph.y * pw;
+      if (a.cov_pre) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) c3[k] = a.cov_pre[6 * i + k];
+      } else {
+        float R[9];
+        const float4 q = reinterpret_cast<const float4*>(a.rots)[i];
+        quat_to_R(q.x, q.y, q.z, q.w, R);
+        const float s0 = a.scale_mod * a.scales[3 * i], s1 = a.scale_mod * a.scales[3 * i + 1],
+                    s2 = a.scale_mod * a.scales[3 * i + 2];
+        float L[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          L[3 * r] = R[3 * r] * s0;
+          L[3 * r + 1] = R[3 * r + 1] * s1;
+          L[3 * r + 2] = R[3 * r + 2] * s2;
+        }
+#define GS_SIG(u, v) ((L[3 * (u)] * L[3 * (v)] + L[3 * (u) + 1] * L[3 * (v) + 1]) + L[3 * (u) + 2] * L[3 * (v) + 2])
+        c3[0] = GS_SIG(0, 0);
+        c3[1] = GS_SIG(0, 1);
+        c3[2] = GS_SIG(0, 2);
+        c3[3] = GS_SIG(1, 1);
+        c3[4] = GS_SIG(1, 2);
+        c3[5] = GS_SIG(2, 2);
+#undef GS_SIG
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) a.cov3D[6 * i + k] = c3[k];
+      const float limx = 1.3f * a.tanfovx, limy = 1.3f * a.tanfovy;
+      const float txtz = pv.x / pv.z, tytz = pv.y / pv.z;
+      const float tx = fminf(limx, fmaxf(-limx, txtz)) * pv.z;
+      const float ty = fminf(limy, fmaxf(-limy, tytz)) * pv.z;
+      const float tz = pv.z;
+      const float J00 = a.fx / tz, J02 = -(a.fx * tx) / (tz * tz);
+      const float J11 = a.fy / tz, J12 = -(a.fy * ty) / (tz * tz);
+      float Tm0[3], Tm1[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Tm0[c] = J00 * a.view[c * 4 + 0] + J02 * a.view[c * 4 + 2];
+        Tm1[c] = J11 * a.view[c * 4 + 1] + J12 * a.view[c * 4 + 2];
+      }
+      const float S[3][3] = {{c3[0], c3[1], c3[2]}, {c3[1], c3[3], c3[4]}, {c3[2], c3[4], c3[5]}};
+      float MS0[3], MS1[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        MS0[c] = (Tm0[0] * S[0][c] + Tm0[1] * S[1][c]) + Tm0[2] * S[2][c];
+        MS1[c] = (Tm1[0] * S[0][c] + Tm1[1] * S[1][c]) + Tm1[2] * S[2][c];
+      }
+      const float ca = ((MS0[0] * Tm0[0] + MS0[1] * Tm0[1]) + MS0[2] * Tm0[2]) + 0.3f;
+      const float cb = (MS0[0] * Tm1[0] + MS0[1] * Tm1[1]) + MS0[2] * Tm1[2];
+      const float cc = ((MS1[0] * Tm1[0] + MS1[1] * Tm1[1]) + MS1[2] * Tm1[2]) + 0.3f;
+      const float det = ca * cc - cb * cb;
+      if (det == 0.0f) ok = false;
+      if (ok) {
+        const float det_inv = 1.0f / det;
+        con_a = cc * det_inv;
+        con_b = -cb * det_inv;
+        con_c = ca * det_inv;
+        const float mid = 0.5f * (ca + cc);
+        const float disc = sqrtf(fmaxf(0.1f, mid * mid - det));
+        const float l1 = mid + disc, l2 = mid - disc;
+        const float radius = ceilf(3.0f * sqrtf(fmaxf(l1, l2)));
+        px = ndc2pix(ndc_x, a.W);
+        py = ndc2pix(ndc_y, a.H);
+        if (!isfinite(px) || !isfinite(py) || !isfinite(radius) || !isfinite(con_a) || !isfinite(con_b) ||
+            !isfinite(con_c) || !isfinite(pv.z))
+          ok = false;
+        if (ok) {
+          ir = sat_i(radius);
+          tile_rect(px, py, ir, a.gx, a.gy, x0, y0, x1, y1);
+          my_tiles = (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
+          if (my_tiles == 0) ok = false;
+        }
+      }
+    }
+    if (ok) {
+      float rgb[3];
+      uint32_t cbits = 0;
+      if (a.shs) {
+        const f3 dv = {p.x - a.campos[0], p.y - a.campos[1], p.z - a.campos[2]};
+        const float len = sqrtf((dv.x * dv.x + dv.y * dv.y) + dv.z * dv.z);
+        const float dx = dv.x / len, dy = dv.y / len, dz = dv.z / len;
+        float b[16];
+        sh_basis(a.D, dx, dy, dz, b);
+        const float* sh = s_sh + threadIdx.x * stride;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          float acc = b[0] * sh[ch];
+          for (int k = 1; k < nc; ++k) acc = acc + b[k] * sh[3 * k + ch];
+          acc = acc + 0.5f;
+          if (acc < 0.0f) cbits |= 1u << ch;
+          rgb[ch] = fmaxf(acc, 0.0f);
+        }
+      } else {
+        rgb[0] = a.colors[3 * i];
+        rgb[1] = a.colors[3 * i + 1];
+        rgb[2] = a.colors[3 * i + 2];
+      }
+      a.radii[i] = ir;
+      a.splat[3 * i + 0] = make_float4(px, py, con_a, con_b);
+      a.splat[3 * i + 1] = make_float4(con_c, a.opac[i], pv.z, __uint_as_float(cbits));
+      a.splat[3 * i + 2] = make_float4(rgb[0], rgb[1], rgb[2], __int_as_float(ir));
+    } else {
+      my_tiles = 0;
+    }
+    a.tiles[i] = my_tiles;
+  }
+  uint32_t total;
+  const uint32_t incl = block_incl_scan_256(my_tiles, s_scan, total);
+  if (i < a.P) a.local_incl[i] = incl;
+  if (threadIdx.x == 0) {
+    a.block_sum[blockIdx.x] = total;
+    a.block_sum[gridDim.x + blockIdx.x] = s_flag;
+  }
+}
+
+// Single workgroup: exclusive prefix of the per-workgroup tile sums (in place),
+// num_rendered and the OR of the status flags.
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* block_sum, int nblocks, uint32_t* counters) {
+  __shared__ uint32_t s[1024];
+  __shared__ uint32_t s_carry, s_flags;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    s_carry = 0;
+    s_flags = 0;
+  }
+  __syncthreads();
+  uint32_t myflags = 0;
+  for (int b0 = 0; b0 < nblocks; b0 += 1024) {
+    const int b = b0 + t;
+    const uint32_t v = b < nblocks ? block_sum[b] : 0u;
+    if (b < nblocks) myflags |= block_sum[nblocks + b];
+    s[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const uint32_t add = t >= off ? s[t - off] : 0u;
+      __syncthreads();
+      s[t] += add;
+      __syncthreads();
+    }
+    const uint32_t carry = s_carry;
+    if (b < nblocks) block_sum[b] = carry + s[t] - v;
+    __syncthreads();
+    if (t == 1023) s_carry = carry + s[1023];
+    __syncthreads();
+  }
+  if (myflags) atomicOr(&s_flags, myflags);
+  __syncthreads();
+  if (t == 0) {
+    counters[0] = s_carry;
+    counters[1] = s_flags;
+  }
+}
+
+struct DupArgs {
+  int P, gx, gy;
+  long long capacity;
+  const float4* splat;
+  const uint32_t* tiles;
+  const uint32_t* local_incl;
+  const uint32_t* block_excl;
+  uint32_t* counters;
+  uint64_t* keys;
+  uint32_t* vals;
+};
+
+__global__ __launch_bounds__(256) void k_duplicate(DupArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t n = a.counters[0];
+  if ((long long)n > a.capacity) {
+    if (i == 0) atomicOr(&a.counters[1], GS_STATUS_CAPACITY);
+    return;
+  }
+  if (i >= a.P) return;
+  const uint32_t nt = a.tiles[i];
+  if (nt == 0) return;
+  uint32_t off = a.block_excl[i >> 8] + a.local_incl[i] - nt;
+  const float4 s0 = a.splat[3 * i];
+  const float4 s1 = a.splat[3 * i + 1];
+  const float4 s2 = a.splat[3 * i + 2];
+  int x0, y0, x1, y1;
+  tile_rect(s0.x, s0.y, __float_as_int(s2.w), a.gx, a.gy, x0, y0, x1, y1);
+  const uint64_t dbits = (uint64_t)__float_as_uint(s1.z);
+  for (int y = y0; y < y1; ++y)
+    for (int x = x0; x < x1; ++x) {
+      a.keys[off] = ((uint64_t)(uint32_t)(y * a.gx + x) << 32) | dbits;
+      a.vals[off] = (uint32_t)i;
+      ++off;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
+  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256) p[k] = v;
+}
+
+__global__ __launch_bounds__(256) void k_ranges(const uint64_t* keys, const uint32_t* counters, long long capacity,
+                                                uint32_t* ranges) {
+  const uint32_t n0 = counters[0];
+  const uint32_t n = ((long long)n0 > capacity) ? 0u : n0;
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t t = (uint32_t)(keys[k] >> 32);
+  if (k == 0) {
+    ranges[2 * t] = 0;
+  } else {
+    const uint32_t tp = (uint32_t)(keys[k - 1] >> 32);
+    if (t != tp) {
+      ranges[2 * tp + 1] = k;
+      ranges[2 * t] = k;
+    }
+  }
+  if (k == n - 1) ranges[2 * t + 1] = n;
+}
+
+struct BlendArgs {
+  int W, H, gx;
+  const uint32_t* ranges;
+  const uint32_t* list;
+  const float4* splat;
+  const float* bg;
+  float* out_color;
+  float* out_depth;
+  float* out_alpha;
+  float* final_T;
+  uint32_t* n_contrib;
+};
+
+__global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
+  __shared__ float4 s_a[256];  // x, y, conic a, conic b
+  __shared__ float4 s_b[256];  // conic c, opacity, depth, -
+  __shared__ float4 s_c[256];  // r, g, b, -
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = tile % a.gx, ty = tile / a.gx;
+  const int px = tx * kBX + (threadIdx.x & 15), py = ty * kBY + (threadIdx.x >> 4);
+  const bool inside = px < a.W && py < a.H;
+  const float pfx = (float)px, pfy = (float)py;
+  const uint32_t r0 = a.ranges[2 * tile], r1 = a.ranges[2 * tile + 1];
+  bool done = !inside;
+  float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
+  uint32_t contributor = 0, last = 0;
+  const float a_min = 1.0f / 255.0f;
+  for (uint32_t base = r0; base < r1; base += 256) {
+    if (__syncthreads_count(done) == 256) break;
+    const uint32_t k = base + threadIdx.x;
+    if (k < r1) {
+      const uint32_t g = a.list[k];
+      s_a[threadIdx.x] = a.splat[3 * g];
+      s_b[threadIdx.x] = a.splat[3 * g + 1];
+      s_c[threadIdx.x] = a.splat[3 * g + 2];
+    }
+    __syncthreads();
+    const int n = (int)min(256u, r1 - base);
+    for (int j = 0; !done && j < n; ++j) {
+      ++contributor;
+      const float4 ga = s_a[j];
+      const float4 gb = s_b[j];
+      const float dx = ga.x - pfx, dy = ga.y - pfy;
+      const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
+      if (power > 0.0f) continue;
+      const float alpha = fminf(0.99f, gb.y * spec_expf(power));
+      if (alpha < a_min) continue;
+      const float test_T = T * (1.0f - alpha);
+      if (test_T < 0.0001f) {
+        done = true;
+        continue;
+      }
+      const float w = alpha * T;
+      const float4 gc = s_c[j];
+      C0 = fmaf(gc.x, w, C0);
+      C1 = fmaf(gc.y, w, C1);
+      C2 = fmaf(gc.z, w, C2);
+      D = fmaf(gb.z, w, D);
+      T = test_T;
+      last = contributor;
+    }
+  }
+  if (inside) {
+    const int pix = py * a.W + px;
+    const int HW = a.H * a.W;
+    a.final_T[pix] = T;
+    a.n_contrib[pix] = last;
+    a.out_color[pix] = fmaf(T, a.bg[0], C0);
+    a.out_color[HW + pix] = fmaf(T, a.bg[1], C1);
+    a.out_color[2 * HW + pix] = fmaf(T, a.bg[2], C2);
+    a.out_depth[pix] = D;
+    if (a.out_alpha) a.out_alpha[pix] = 1.0f - T;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mark_visible(const float* means, int P, const float* view,
+                                                      uint8_t* visible) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P) return;
+  const f3 p = {means[3 * i], means[3 * i + 1], means[3 * i + 2]};
+  visible[i] = xf43(view, p).z > 0.2f ? 1 : 0;
+}
+
+}  // namespace gs
+
+// ---------------------------------------------------------------------------
+// C ABI
+using namespace gs;
+
+namespace {
+inline int grid_tiles(int W, int H, int& gx, int& gy) {
+  gx = (W + kBX - 1) / kBX;
+  gy = (H + kBY - 1) / kBY;
+  return gx * gy;
+}
+inline int nblocks256(long long n) { return (int)((n + 255) / 256); }
+}  // namespace
+
+extern "C" int gs_geom_layout(int P, gs_geom_layout_t* out) {
+  if (P < 0 || !out) return GS_E_ARG;
+  const size_t p = (size_t)P;
+  size_t o = 0;
+  out->splat = o;
+  o = align_up(o + 48 * p, 256);
+  out->cov3D = o;
+  o = align_up(o + 24 * p, 256);
+  out->tiles = o;
+  o = align_up(o + 4 * p, 256);
+  out->offsets = o;
+  o = align_up(o + 4 * p, 256);
+  out->scan_tmp = o;
+  o = align_up(o + 8 * (size_t)nblocks256(P) + 8, 256);
+  out->total = o;
+  return GS_OK;
+}
+
+extern "C" int gs_binning_layout(long long capacity, int width, int height, gs_binning_layout_t* out) {
+  if (capacity < 0 || capacity > 0x7fffffffLL || width <= 0 || height <= 0 || !out) return GS_E_ARG;
+  int gx, gy;
+  const int nt = grid_tiles(width, height, gx, gy);
+  const size_t c = (size_t)capacity;
+  size_t o = 0;
+  out->keys_dup = o;
+  o = align_up(o + 8 * c, 256);
+  out->keys_alt = o;
+  o = align_up(o + 8 * c, 256);
+  out->vals_dup = o;
+  o = align_up(o + 4 * c, 256);
+  out->vals_alt = o;
+  o = align_up(o + 4 * c, 256);
+  out->hist = o;
+  o = align_up(o + sort_tmp_bytes(capacity), 256);
+  out->total = o;
+  const int end_bit = 32 + bits_for((uint32_t)nt);
+  out->sort_passes = sort_passes(end_bit);
+  const bool in_b = (out->sort_passes & 1) != 0;
+  out->keys_sorted = in_b ? out->keys_alt : out->keys_dup;
+  out->vals_sorted = in_b ? out->vals_alt : out->vals_dup;
+  return GS_OK;
+}
+
+extern "C" int gs_image_layout(int width, int height, gs_image_layout_t* out) {
+  if (width <= 0 || height <= 0 || !out) return GS_E_ARG;
+  int gx, gy;
+  const int nt = grid_tiles(width, height, gx, gy);
+  const size_t hw = (size_t)width * height;
+  size_t o = 0;
+  out->ranges = o;
+  o = align_up(o + 8 * (size_t)nt, 256);
+  out->final_T = o;
+  o = align_up(o + 4 * hw, 256);
+  out->n_contrib = o;
+  o = align_up(o + 4 * hw, 256);
+  out->total = o;
+  return GS_OK;
+}
+
+static int check_frame(const gs_frame_t* f) {
+  if (!f || f->width <= 0 || f->height <= 0 || !f->viewmatrix || !f->projmatrix || !f->bg) return GS_E_ARG;
+  if (f->sh_degree < 0 || f->sh_degree > 3) return GS_E_ARG;
+  return GS_OK;
+}
+
+extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* radii, void* geom,
+                             size_t geom_bytes, uint32_t* counters, void* stream) {
+  if (check_frame(f) != GS_OK || !g || g->P < 0 || !counters) return GS_E_ARG;
+  const int P = g->P;
+  if (P > 0 && (!g->means3D || !g->opacities || !radii || !geom)) return GS_E_ARG;
+  // (empty inputs carry no pointers, so the exactly-one-of rules apply for P > 0)
+  if (P > 0 && (g->shs == nullptr) == (g->colors_precomp == nullptr)) return GS_E_ARG;
+  const bool has_sr = g->scales && g->rotations;
+  if (P > 0 && has_sr == (g->cov3D_precomp != nullptr)) return GS_E_ARG;
+  if (g->shs && (f->sh_degree + 1) * (f->sh_degree + 1) > g->M) return GS_E_ARG;
+  if (g->shs && !f->campos) return GS_E_ARG;
+  gs_geom_layout_t L;
+  gs_geom_layout(P, &L);
+  if (geom_bytes < L.total) return GS_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)geom;
+  PreArgs a;
+  a.P = P;
+  a.M = g->M;
+  a.D = f->sh_degree;
+  a.W = f->width;
+  a.H = f->height;
+  grid_tiles(a.W, a.H, a.gx, a.gy);
+  a.prefiltered = f->prefiltered;
+  a.tanfovx = f->tanfovx;
+  a.tanfovy = f->tanfovy;
+  a.fx = (float)a.W / (2.0f * f->tanfovx);
+  a.fy = (float)a.H / (2.0f * f->tanfovy);
+  a.scale_mod = f->scale_modifier;
+  a.view = f->viewmatrix;
+  a.proj = f->projmatrix;
+  a.campos = f->campos;
+  a.pose = f->pose;
+  a.means = g->means3D;
+  a.shs = g->shs;
+  a.colors = g->colors_precomp;
+  a.opac = g->opacities;
+  a.scales = g->scales;
+  a.rots = g->rotations;
+  a.cov_pre = g->cov3D_precomp;
+  a.radii = radii;
+  a.splat = (float4*)(base + L.splat);
+  a.cov3D = (float*)(base + L.cov3D);
+  a.tiles = (uint32_t*)(base + L.tiles);
+  a.local_incl = (uint32_t*)(base + L.offsets);
+  a.block_sum = (uint32_t*)(base + L.scan_tmp);
+  const int nb = nblocks256(P);
+  if (nb > 0) {
+    const int nc = (a.D + 1) * (a.D + 1);
+    const size_t shm = g->shs ? (size_t)256 * 3 * nc * sizeof(float) : 0;
+    timing::Scope ts(timing::kPreprocess, s);
+    hipLaunchKernelGGL(k_preprocess, dim3(nb), dim3(256), shm, s, a);
+  }
+  {
+    timing::Scope ts(timing::kScan, s);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, a.block_sum, nb, counters);
+  }
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
+                         void* binning, size_t binning_bytes, void* image, size_t image_bytes, float* out_color,
+                         float* out_depth, float* out_alpha, void* stream) {
+  if (check_frame(f) != GS_OK || P < 0 || !counters || !image || !out_color || !out_depth) return GS_E_ARG;
+  if (P > 0 && !geom) return GS_E_ARG;
+  if (capacity < 0 || capacity > 0x7fffffffLL) return GS_E_ARG;
+  gs_geom_layout_t LG;
+  gs_binning_layout_t LB;
+  gs_image_layout_t LI;
+  gs_geom_layout(P, &LG);
+  gs_binning_layout(capacity, f->width, f->height, &LB);
+  gs_image_layout(f->width, f->height, &LI);
+  if (binning_bytes < LB.total || image_bytes < LI.total) return GS_E_WORKSPACE;
+  if (capacity > 0 && !binning) return GS_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  int gx, gy;
+  const int nt = grid_tiles(f->width, f->height, gx, gy);
+  const char* gb = (const char*)geom;
+  char* bb = (char*)binning;
+  char* ib = (char*)image;
+  uint32_t* ranges = (uint32_t*)(ib + LI.ranges);
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nblocks256(2LL * nt), 1024)), dim3(256), 0, s, ranges, 2LL * nt, 0u);
+  if (capacity > 0 && P > 0) {
+    DupArgs d;
+    d.P = P;
+    d.gx = gx;
+    d.gy = gy;
+    d.capacity = capacity;
+    d.splat = (const float4*)(gb + LG.splat);
+    d.tiles = (const uint32_t*)(gb + LG.tiles);
+    d.local_incl = (const uint32_t*)(gb + LG.offsets);
+    d.block_excl = (const uint32_t*)(gb + LG.scan_tmp);
+    d.counters = counters;
+    d.keys = (uint64_t*)(bb + LB.keys_dup);
+    d.vals = (uint32_t*)(bb + LB.vals_dup);
+    {
+      timing::Scope ts(timing::kDuplicate, s);
+      hipLaunchKernelGGL(k_duplicate, dim3(nblocks256(P)), dim3(256), 0, s, d);
+    }
+    const int end_bit = 32 + bits_for((uint32_t)nt);
+    int in_b = 0;
+    hipError_t e;
+    {
+      timing::Scope ts_sort(timing::kSort, s);
+      e = sort_pairs_u64((uint64_t*)(bb + LB.keys_dup), (uint32_t*)(bb + LB.vals_dup), (uint64_t*)(bb + LB.keys_alt),
+                         (uint32_t*)(bb + LB.vals_alt), counters, capacity, end_bit, bb + LB.hist, s, &in_b);
+    }
+    if (e != hipSuccess) return GS_E_LAUNCH;
+    timing::Scope ts(timing::kRanges, s);
+    hipLaunchKernelGGL(k_ranges, dim3(nblocks256(capacity)), dim3(256), 0, s,
+                       (const uint64_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, ranges);
+  }
+  BlendArgs b;
+  b.W = f->width;
+  b.H = f->height;
+  b.gx = gx;
+  b.ranges = ranges;
+  b.list = capacity > 0 ? (const uint32_t*)(bb + LB.vals_sorted) : nullptr;
+  b.splat = (const float4*)(gb + LG.splat);
+  b.bg = f->bg;
+  b.out_color = out_color;
+  b.out_depth = out_depth;
+  b.out_alpha = out_alpha;
+  b.final_T = (float*)(ib + LI.final_T);
+  b.n_contrib = (uint32_t*)(ib + LI.n_contrib);
+  {
+    timing::Scope ts(timing::kBlend, s);
+    hipLaunchKernelGGL(k_blend, dim3(nt), dim3(256), 0, s, b);
+  }
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_mark_visible(const float* means3D, int P, const float* viewmatrix, const float* projmatrix,
+                               uint8_t* visible, void* stream) {
+  (void)projmatrix;
+  if (P < 0 || (P > 0 && (!means3D || !viewmatrix || !visible))) return GS_E_ARG;
+  if (P == 0) return GS_OK;
+  hipLaunchKernelGGL(k_mark_visible, dim3(nblocks256(P)), dim3(256), 0, (hipStream_t)stream, means3D, P,
+                     viewmatrix, visible);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" const char* gs_error_string(int code) {
+  switch (code) {
+    case GS_OK: return "ok";
+    case GS_E_ARG: return "invalid argument";
+    case GS_E_WORKSPACE: return "workspace too small";
+    case GS_E_LAUNCH: return "kernel launch failed";
+    default: return "unknown error";
+  }
+}
+
+extern "C" int gs_abi_version(void) { return 1; }

@@ -1,0 +1,50 @@
+// gs_internal.hpp -- host-side declarations shared between the translation
+// units of libgs_hip.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gs {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort workgroup
+constexpr int kRadix = 256;
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+inline int bits_for(uint32_t n) {  // number of bits needed to hold values < n
+  int b = 0;
+  while (b < 32 && (1ull << b) < (uint64_t)n) ++b;
+  return b;
+}
+inline int sort_passes(int end_bit) { return (end_bit + 7) / 8; }
+
+namespace timing {
+enum Stage { kPreprocess = 0, kScan, kDuplicate, kSort, kRanges, kBlend, kRenderBwd, kPreprocessBwd, kKnn, kZero };
+// RAII pair of events around the launches issued inside the scope (no-op
+// unless gs_timing_enable(1) was called).
+class Scope {
+ public:
+  Scope(int stage, hipStream_t s);
+  ~Scope();
+  Scope(const Scope&) = delete;
+  Scope& operator=(const Scope&) = delete;
+
+ private:
+  int stage_;
+  hipStream_t s_;
+  void* a_;
+};
+bool enabled();
+}  // namespace timing
+
+size_t sort_tmp_bytes(long long capacity);
+// Returns hipSuccess or the launch error.  *result_in_b tells where the
+// sorted data ends up.
+hipError_t sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b,
+                          const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
+                          int* result_in_b);
+
+}  // namespace gs

@@ -1,0 +1,218 @@
+// gs_sort.hip -- stable LSD radix sort of (uint64 key, uint32 value) pairs.
+//
+// The element count is read on the device (so the binning pipeline never
+// needs the host to know num_rendered); grids are sized by capacity and
+// workgroups past the count exit at once.
+//
+// Per 8-bit digit pass:
+//   k_sort_upsweep  4096 keys per workgroup (4 waves x 16 rounds x 64 lanes);
+//                   wave64 ballot "match" groups equal digits so each group
+//                   adds its count once -> per-workgroup digit counts
+//                   (digit-major, so one row scan per digit orders blocks)
+//   k_sort_scan     one workgroup per digit: exclusive scan of its row, total
+//   k_sort_scatter  recomputes the matches; rank = prior count of the digit in
+//                   this wave + lanes below with the same digit; waves are
+//                   ordered by an exclusive prefix over the 4 wave counts, so
+//                   the order of equal digits is the input order (stable).
+#include "gs_common.hpp"
+#include "gs_internal.hpp"
+
+namespace gs {
+
+// Lanes whose digit equals this lane's digit (only lanes with valid set).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid, int bits) {
+  uint64_t m = __ballot(valid);
+  for (int b = 0; b < bits; ++b) {
+    const bool on = (d >> b) & 1u;
+    const uint64_t v = __ballot(on);
+    m &= on ? v : ~v;
+  }
+  return m;
+}
+
+__device__ __forceinline__ uint32_t n_eff(const uint32_t* n_dev, long long cap) {
+  const uint32_t n = *n_dev;
+  return ((long long)n > cap) ? 0u : n;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const uint64_t* __restrict__ keys,
+                                                               const uint32_t* n_dev, long long cap, int shift,
+                                                               int bits, uint32_t* hist, int nb_max) {
+  __shared__ uint32_t s_h[4][kRadix];
+  const uint32_t n = n_eff(n_dev, cap);
+  const uint32_t start = blockIdx.x * (uint32_t)kSortTile;
+  if (start >= n) return;
+  for (int k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) (&s_h[0][0])[k] = 0;
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t mask = (1u << bits) - 1u;
+  const uint32_t wbase = start + (uint32_t)w * (kSortItems * 64);
+#pragma unroll 4
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint32_t idx = wbase + r * 64 + lane;
+    const bool valid = idx < n;
+    const uint32_t d = valid ? (uint32_t)(keys[idx] >> shift) & mask : 0u;
+    const uint64_t peers = match_digit(d, valid, bits);
+    const bool leader = valid && ((peers & lanemask_lt()) == 0);
+    if (leader) s_h[w][d] += (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < kRadix; d += kSortThreads)
+    hist[(size_t)d * nb_max + blockIdx.x] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
+}
+
+// One workgroup per digit: exclusive scan over the active blocks' counts.
+__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* totals, const uint32_t* n_dev,
+                                                   long long cap, int nb_max) {
+  __shared__ uint32_t s[256];
+  __shared__ uint32_t s_carry;
+  const uint32_t n = n_eff(n_dev, cap);
+  const int nb = (int)((n + kSortTile - 1) / kSortTile);
+  const int d = blockIdx.x;
+  uint32_t* row = hist + (size_t)d * nb_max;
+  const int t = threadIdx.x;
+  if (t == 0) s_carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nb; b0 += 256) {
+    const int b = b0 + t;
+    const uint32_t v = b < nb ? row[b] : 0u;
+    s[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const uint32_t add = t >= off ? s[t - off] : 0u;
+      __syncthreads();
+      s[t] += add;
+      __syncthreads();
+    }
+    const uint32_t carry = s_carry;
+    if (b < nb) row[b] = carry + s[t] - v;
+    __syncthreads();
+    if (t == 255) s_carry = carry + s[255];
+    __syncthreads();
+  }
+  if (t == 0) totals[d] = s_carry;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint64_t* __restrict__ keys_in,
+                                                               const uint32_t* __restrict__ vals_in,
+                                                               uint64_t* __restrict__ keys_out,
+                                                               uint32_t* __restrict__ vals_out,
+                                                               const uint32_t* n_dev, long long cap, int shift,
+                                                               int bits, const uint32_t* hist,
+                                                               const uint32_t* totals, int nb_max) {
+  __shared__ uint32_t s_cnt[4][kRadix];
+  __shared__ uint32_t s_scan[kRadix];
+  const uint32_t n = n_eff(n_dev, cap);
+  const uint32_t start = blockIdx.x * (uint32_t)kSortTile;
+  if (start >= n) return;
+  const int t = threadIdx.x;
+  // global base of each digit for this block = exclusive scan of totals + row offset
+  {
+    const uint32_t v = totals[t];
+    s_scan[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kRadix; off <<= 1) {
+      const uint32_t add = t >= off ? s_scan[t - off] : 0u;
+      __syncthreads();
+      s_scan[t] += add;
+      __syncthreads();
+    }
+    const uint32_t excl = s_scan[t] - v + hist[(size_t)t * nb_max + blockIdx.x];
+    __syncthreads();
+    s_scan[t] = excl;
+  }
+  for (int k = t; k < 4 * kRadix; k += kSortThreads) (&s_cnt[0][0])[k] = 0;
+  __syncthreads();
+  const int w = t >> 6, lane = t & 63;
+  const uint32_t mask = (1u << bits) - 1u;
+  const uint32_t wbase = start + (uint32_t)w * (kSortItems * 64);
+  uint64_t key[kSortItems];
+  uint32_t val[kSortItems];
+  uint32_t pos[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint32_t idx = wbase + r * 64 + lane;
+    const bool valid = idx < n;
+    key[r] = valid ? keys_in[idx] : 0ull;
+    val[r] = valid ? vals_in[idx] : 0u;
+    const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
+    const uint64_t peers = match_digit(d, valid, bits);
+    const uint64_t below = peers & lanemask_lt();
+    const uint32_t prior = s_cnt[w][d];
+    pos[r] = prior + (uint32_t)__popcll(below);
+    if (valid && below == 0) s_cnt[w][d] = prior + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  {
+    // per digit t: prefix over the 4 waves, plus the block's global base
+    const uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t];
+    const uint32_t b = s_scan[t];
+    s_cnt[0][t] = b;
+    s_cnt[1][t] = b + c0;
+    s_cnt[2][t] = b + c0 + c1;
+    s_cnt[3][t] = b + c0 + c1 + c2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint32_t idx = wbase + r * 64 + lane;
+    if (idx < n) {
+      const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
+      const uint32_t dst = s_cnt[w][d] + pos[r];
+      keys_out[dst] = key[r];
+      vals_out[dst] = val[r];
+    }
+  }
+}
+
+size_t sort_tmp_bytes(long long capacity) {
+  const long long nb = (capacity + kSortTile - 1) / kSortTile;
+  return align_up((size_t)kRadix * 4 * (size_t)(nb > 0 ? nb : 1), 256) + kRadix * 4;
+}
+
+hipError_t sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b,
+                          const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
+                          int* result_in_b) {
+  const int passes = sort_passes(end_bit);
+  *result_in_b = passes & 1;
+  if (capacity <= 0) return hipSuccess;
+  const int nb_max = (int)((capacity + kSortTile - 1) / kSortTile);
+  uint32_t* hist = (uint32_t*)tmp;
+  uint32_t* totals = hist + align_up((size_t)kRadix * (size_t)nb_max, 64);
+  uint64_t* kin = keys_a;
+  uint32_t* vin = vals_a;
+  uint64_t* kout = keys_b;
+  uint32_t* vout = vals_b;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
+    hipLaunchKernelGGL(k_sort_upsweep, dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity, shift, bits,
+                       hist, nb_max);
+    hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_dev,
+                       capacity, shift, bits, hist, totals, nb_max);
+    uint64_t* tk = kin;
+    kin = kout;
+    kout = tk;
+    uint32_t* tv = vin;
+    vin = vout;
+    vout = tv;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gs
+
+extern "C" size_t gs_sort_tmp_bytes(long long capacity) { return gs::sort_tmp_bytes(capacity); }
+
+extern "C" int gs_sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b,
+                                 const uint32_t* n_dev, long long capacity, int end_bit, void* tmp,
+                                 size_t tmp_bytes, int* result_in_b, void* stream) {
+  if (capacity < 0 || capacity > 0x7fffffffLL || end_bit < 1 || end_bit > 64 || !n_dev || !result_in_b)
+    return GS_E_ARG;
+  if (capacity > 0 && (!keys_a || !vals_a || !keys_b || !vals_b || !tmp)) return GS_E_ARG;
+  if (tmp_bytes < gs::sort_tmp_bytes(capacity)) return GS_E_WORKSPACE;
+  const hipError_t e = gs::sort_pairs_u64(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, end_bit, tmp,
+                                          (hipStream_t)stream, result_in_b);
+  return e == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}

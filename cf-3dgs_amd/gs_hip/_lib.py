@@ -1,0 +1,155 @@
+"""ctypes binding of libgs_hip.so (C ABI declared in include/gs_raster.h).
+
+The library is loaded from this directory.  There is no fallback: if the
+library is missing, importing the rasteriser fails with instructions to
+build it, and every entry point requires tensors on a HIP device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgs_hip.so")
+
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_size_t = ctypes.c_size_t
+c_void_p = ctypes.c_void_p
+c_longlong = ctypes.c_longlong
+
+GS_OK = 0
+GS_STATUS_CAPACITY = 1
+GS_STATUS_PREFILTERED = 2
+TILE = 16
+
+
+class GsFrame(ctypes.Structure):
+    _fields_ = [
+        ("width", c_int), ("height", c_int), ("tanfovx", c_float), ("tanfovy", c_float),
+        ("scale_modifier", c_float), ("sh_degree", c_int), ("prefiltered", c_int),
+        ("viewmatrix", c_void_p), ("projmatrix", c_void_p), ("campos", c_void_p), ("bg", c_void_p),
+        ("pose", c_void_p),
+    ]
+
+
+class GsGaussians(ctypes.Structure):
+    _fields_ = [
+        ("P", c_int), ("M", c_int), ("means3D", c_void_p), ("shs", c_void_p), ("colors_precomp", c_void_p),
+        ("opacities", c_void_p), ("scales", c_void_p), ("rotations", c_void_p), ("cov3D_precomp", c_void_p),
+    ]
+
+
+class GsGrads(ctypes.Structure):
+    _fields_ = [
+        ("dL_dmeans2D", c_void_p), ("dL_dcolors", c_void_p), ("dL_dopacity", c_void_p),
+        ("dL_dmeans3D", c_void_p), ("dL_dcov3D", c_void_p), ("dL_dsh", c_void_p), ("dL_dscales", c_void_p),
+        ("dL_drotations", c_void_p), ("dL_dpose", c_void_p),
+    ]
+
+
+class GsGeomLayout(ctypes.Structure):
+    _fields_ = [("splat", c_size_t), ("cov3D", c_size_t), ("tiles", c_size_t), ("offsets", c_size_t),
+                ("scan_tmp", c_size_t), ("total", c_size_t)]
+
+
+class GsBinningLayout(ctypes.Structure):
+    _fields_ = [("keys_dup", c_size_t), ("vals_dup", c_size_t), ("keys_alt", c_size_t), ("vals_alt", c_size_t),
+                ("keys_sorted", c_size_t), ("vals_sorted", c_size_t), ("hist", c_size_t), ("sort_passes", c_int),
+                ("total", c_size_t)]
+
+
+class GsImageLayout(ctypes.Structure):
+    _fields_ = [("ranges", c_size_t), ("final_T", c_size_t), ("n_contrib", c_size_t), ("total", c_size_t)]
+
+
+EXPORTS = {
+    "gs_geom_layout": (c_int, [c_int, ctypes.POINTER(GsGeomLayout)]),
+    "gs_binning_layout": (c_int, [c_longlong, c_int, c_int, ctypes.POINTER(GsBinningLayout)]),
+    "gs_image_layout": (c_int, [c_int, c_int, ctypes.POINTER(GsImageLayout)]),
+    "gs_backward_scratch_bytes": (c_size_t, [c_int]),
+    "gs_preprocess": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p, c_size_t,
+                              c_void_p, c_void_p]),
+    "gs_render": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong, c_void_p, c_size_t,
+                          c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gs_backward": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p, c_void_p,
+                            c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(GsGrads),
+                            c_void_p, c_size_t, c_void_p]),
+    "gs_mark_visible": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gs_sort_tmp_bytes": (c_size_t, [c_longlong]),
+    "gs_sort_pairs_u64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_longlong, c_int, c_void_p,
+                                  c_size_t, ctypes.POINTER(c_int), c_void_p]),
+    "gs_knn_workspace_bytes": (c_size_t, [c_int]),
+    "gs_knn3_meandist": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "gs_timing_enable": (c_int, [c_int]),
+    "gs_timing_reset": (c_int, []),
+    "gs_timing_read": (c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int)]),
+    "gs_error_string": (ctypes.c_char_p, [c_int]),
+    "gs_abi_version": (c_int, []),
+}
+
+_lib = None
+
+
+class GsError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libgs_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python cf-3dgs_amd/build.py` "
+                "(hipcc, gfx950). There is no CPU fallback.")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(code: int, what: str) -> None:
+    if code != GS_OK:
+        msg = load().gs_error_string(code).decode()
+        raise GsError(f"{what} failed: {msg} (code {code})")
+
+
+def geom_layout(P: int) -> GsGeomLayout:
+    L = GsGeomLayout()
+    check(load().gs_geom_layout(P, ctypes.byref(L)), "gs_geom_layout")
+    return L
+
+
+def binning_layout(capacity: int, W: int, H: int) -> GsBinningLayout:
+    L = GsBinningLayout()
+    check(load().gs_binning_layout(capacity, W, H, ctypes.byref(L)), "gs_binning_layout")
+    return L
+
+
+def image_layout(W: int, H: int) -> GsImageLayout:
+    L = GsImageLayout()
+    check(load().gs_image_layout(W, H, ctypes.byref(L)), "gs_image_layout")
+    return L
+
+
+STAGES = ("preprocess", "scan", "duplicate", "sort", "ranges", "blend", "render_bwd", "preprocess_bwd", "knn")
+
+
+def timing_enable(on: bool) -> None:
+    check(load().gs_timing_enable(int(bool(on))), "gs_timing_enable")
+
+
+def timing_reset() -> None:
+    check(load().gs_timing_reset(), "gs_timing_reset")
+
+
+def timing_read(stage: str):
+    """(total milliseconds, launches) recorded for `stage` since the last reset."""
+    t = ctypes.c_double(0.0)
+    n = c_int(0)
+    check(load().gs_timing_read(stage.encode(), ctypes.byref(t), ctypes.byref(n)), "gs_timing_read")
+    return t.value, n.value

@@ -1,0 +1,56 @@
+"""Photometric loss of the CF-3DGS training step, in torch.
+
+``Loss.forward`` (trainer/losses.py:98-136) with the default config:
+  (1 - lambda_dssim) * L1 + lambda_dssim * (1 - SSIM), lambda_dssim = 0.2
+(arguments/__init__.py:113), the depth term off (lambda_depth = 0, :114).
+SSIM is ``SSIM_V2`` (trainer/losses.py:212-251): 11x11 Gaussian window,
+sigma 1.5 (:143-159), zero padding, C1 = 0.01^2, C2 = 0.03^2 (:164-209).
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+import torch.nn.functional as F
+
+_WINDOWS = {}
+
+
+def gaussian_window(window_size: int = 11, sigma: float = 1.5, channel: int = 3, device=None,
+                    dtype=torch.float32) -> torch.Tensor:
+    key = (window_size, sigma, channel, str(device), dtype)
+    w = _WINDOWS.get(key)
+    if w is None:
+        g = torch.tensor([math.exp(-(x - window_size // 2) ** 2 / float(2 * sigma ** 2)) for x in range(window_size)])
+        g = g / g.sum()
+        w2 = (g[:, None] @ g[None, :]).float()
+        w = w2.expand(channel, 1, window_size, window_size).contiguous().to(device=device, dtype=dtype)
+        _WINDOWS[key] = w
+    return w
+
+
+def ssim(img1: torch.Tensor, img2: torch.Tensor, window_size: int = 11) -> torch.Tensor:
+    if img1.dim() == 3:
+        img1 = img1.unsqueeze(0)
+    if img2.dim() == 3:
+        img2 = img2.unsqueeze(0)
+    channel = img1.shape[1]
+    w = gaussian_window(window_size, 1.5, channel, img1.device, img1.dtype)
+    pad = window_size // 2
+    mu1 = F.conv2d(img1, w, padding=pad, groups=channel)
+    mu2 = F.conv2d(img2, w, padding=pad, groups=channel)
+    mu1_sq, mu2_sq, mu1_mu2 = mu1.pow(2), mu2.pow(2), mu1 * mu2
+    s1 = F.conv2d(img1 * img1, w, padding=pad, groups=channel) - mu1_sq
+    s2 = F.conv2d(img2 * img2, w, padding=pad, groups=channel) - mu2_sq
+    s12 = F.conv2d(img1 * img2, w, padding=pad, groups=channel) - mu1_mu2
+    C1, C2 = 0.01 ** 2, 0.03 ** 2
+    m = ((2 * mu1_mu2 + C1) * (2 * s12 + C2)) / ((mu1_sq + mu2_sq + C1) * (s1 + s2 + C2))
+    return m.mean()
+
+
+def l1_loss(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    return torch.abs(a - b).mean()
+
+
+def photometric_loss(pred: torch.Tensor, gt: torch.Tensor, lambda_dssim: float = 0.2) -> torch.Tensor:
+    return (1.0 - lambda_dssim) * l1_loss(pred, gt) + lambda_dssim * (1.0 - ssim(pred, gt))

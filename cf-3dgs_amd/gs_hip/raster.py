@@ -1,0 +1,311 @@
+"""Torch-facing forward/backward of the gfx950 rasteriser.
+
+Workspaces are torch uint8 tensors from the caching allocator; the library
+itself never allocates.  ``num_rendered`` is read back once per forward after
+preprocess, which is the reference's own synchronisation point (the binning
+buffers are sized from it); everything else is asynchronous on the current
+stream.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from . import _lib
+
+
+def _dev_f32(t: Optional[torch.Tensor], device: torch.device, name: str) -> Optional[torch.Tensor]:
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(t, dtype=torch.float32)
+    if t.numel() == 0:
+        return None
+    if t.device.type == "cpu" and device.type == "cuda":
+        t = t.to(device)
+    if t.device != device:
+        raise ValueError(f"{name} is on {t.device}, expected {device}")
+    if t.dtype != torch.float32:
+        t = t.float()
+    t = t.contiguous()
+    if t.data_ptr() % 16:
+        t = t.clone()  # kernels read [P,4] / [P,M,3] rows with 16-byte loads
+    return t
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(device: torch.device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_device(t: torch.Tensor, name: str = "means3D") -> torch.device:
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise RuntimeError(
+            f"{name} must be a tensor on a HIP device: the rasteriser runs only on the GPU "
+            "(libgs_hip.so, gfx950); there is no CPU path")
+    return t.device
+
+
+@dataclass
+class FrameArgs:
+    """Settings tensors kept alive for the duration of the launches."""
+    frame: _lib.GsFrame
+    view: torch.Tensor
+    proj: torch.Tensor
+    campos: Optional[torch.Tensor]
+    bg: torch.Tensor
+    pose: Optional[torch.Tensor]
+
+
+def make_frame(settings, device: torch.device, pose: Optional[torch.Tensor] = None) -> FrameArgs:
+    view = _dev_f32(settings.viewmatrix, device, "viewmatrix")
+    proj = _dev_f32(settings.projmatrix, device, "projmatrix")
+    campos = _dev_f32(settings.campos, device, "campos")
+    bg = _dev_f32(settings.bg, device, "bg")
+    pose_t = _dev_f32(pose, device, "pose")
+    if view is None or view.numel() != 16 or proj is None or proj.numel() != 16:
+        raise ValueError("viewmatrix and projmatrix must hold 16 floats")
+    if bg is None or bg.numel() != 3:
+        raise ValueError("bg must hold 3 floats")
+    if pose_t is not None and pose_t.numel() != 7:
+        raise ValueError("pose must hold 7 floats [tx, ty, tz, qx, qy, qz, qw]")
+    f = _lib.GsFrame()
+    f.width = int(settings.image_width)
+    f.height = int(settings.image_height)
+    f.tanfovx = float(settings.tanfovx)
+    f.tanfovy = float(settings.tanfovy)
+    f.scale_modifier = float(settings.scale_modifier)
+    f.sh_degree = int(settings.sh_degree)
+    f.prefiltered = int(bool(settings.prefiltered))
+    f.viewmatrix = view.data_ptr()
+    f.projmatrix = proj.data_ptr()
+    f.campos = campos.data_ptr() if campos is not None else None
+    f.bg = bg.data_ptr()
+    f.pose = pose_t.data_ptr() if pose_t is not None else None
+    return FrameArgs(f, view, proj, campos, bg, pose_t)
+
+
+@dataclass
+class GaussianArgs:
+    g: _lib.GsGaussians
+    tensors: tuple
+
+
+def make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp,
+                   device: torch.device) -> GaussianArgs:
+    P = means3D.shape[0]
+    means = _dev_f32(means3D, device, "means3D")
+    if means is None:
+        means = torch.zeros((0, 3), device=device)
+    if means.dim() != 2 or means.shape[1] != 3:
+        raise ValueError("means3D must be [P, 3]")
+    sh = _dev_f32(shs, device, "shs")
+    col = _dev_f32(colors_precomp, device, "colors_precomp")
+    op = _dev_f32(opacities, device, "opacities")
+    sc = _dev_f32(scales, device, "scales")
+    ro = _dev_f32(rotations, device, "rotations")
+    cv = _dev_f32(cov3D_precomp, device, "cov3D_precomp")
+    M = 0
+    if sh is not None:
+        if sh.dim() != 3 or sh.shape[0] != P or sh.shape[2] != 3:
+            raise ValueError("shs must be [P, M, 3]")
+        M = sh.shape[1]
+    if col is not None and col.numel() != 3 * P:
+        raise ValueError("colors_precomp must be [P, 3]")
+    if (op is None and P > 0) or (op is not None and op.numel() != P):
+        raise ValueError("opacities must hold P values")
+    if sc is not None and sc.numel() != 3 * P:
+        raise ValueError("scales must be [P, 3]")
+    if ro is not None and ro.numel() != 4 * P:
+        raise ValueError("rotations must be [P, 4]")
+    if cv is not None and cv.numel() != 6 * P:
+        raise ValueError("cov3D_precomp must be [P, 6]")
+    g = _lib.GsGaussians()
+    g.P = P
+    g.M = M
+    g.means3D = means.data_ptr()
+    g.shs = sh.data_ptr() if sh is not None else None
+    g.colors_precomp = col.data_ptr() if col is not None else None
+    g.opacities = op.data_ptr() if op is not None else None
+    g.scales = sc.data_ptr() if sc is not None else None
+    g.rotations = ro.data_ptr() if ro is not None else None
+    g.cov3D_precomp = cv.data_ptr() if cv is not None else None
+    return GaussianArgs(g, (means, sh, col, op, sc, ro, cv))
+
+
+@dataclass
+class ForwardState:
+    """Buffers the backward needs (the reference keeps geomBuffer,
+    binningBuffer and imgBuffer alive in the autograd context)."""
+    geom: torch.Tensor
+    binning: torch.Tensor
+    image: torch.Tensor
+    counters: torch.Tensor
+    radii: torch.Tensor
+    capacity: int
+    num_rendered: int
+
+
+def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp,
+            pose=None, want_alpha: bool = True):
+    """Returns (color [3,H,W], radii [P] int32, depth [1,H,W], alpha [1,H,W] or None, state)."""
+    lib = _lib.load()
+    device = require_device(means3D)
+    fa = make_frame(settings, device, pose)
+    ga = make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp, device)
+    P = ga.g.P
+    W, H = fa.frame.width, fa.frame.height
+    stream = _stream(device)
+    LG = _lib.geom_layout(P)
+    geom = torch.empty(max(LG.total, 256), dtype=torch.uint8, device=device)
+    radii = torch.empty(P, dtype=torch.int32, device=device)
+    counters = torch.empty(4, dtype=torch.int32, device=device)
+    _lib.check(lib.gs_preprocess(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(radii), _ptr(geom),
+                                 geom.numel(), _ptr(counters), stream), "gs_preprocess")
+    host = counters[:2].cpu()
+    num_rendered = int(host[0].item()) & 0xFFFFFFFF
+    status = int(host[1].item())
+    if status & _lib.GS_STATUS_PREFILTERED:
+        raise RuntimeError("a point was culled by the near plane although prefiltered=True")
+    capacity = num_rendered
+    LB = _lib.binning_layout(capacity, W, H)
+    LI = _lib.image_layout(W, H)
+    binning = torch.empty(max(LB.total, 256), dtype=torch.uint8, device=device)
+    image = torch.empty(LI.total, dtype=torch.uint8, device=device)
+    color = torch.empty((3, H, W), dtype=torch.float32, device=device)
+    depth = torch.empty((1, H, W), dtype=torch.float32, device=device)
+    alpha = torch.empty((1, H, W), dtype=torch.float32, device=device) if want_alpha else None
+    _lib.check(lib.gs_render(ctypes.byref(fa.frame), P, _ptr(geom), _ptr(counters), capacity, _ptr(binning),
+                             binning.numel(), _ptr(image), image.numel(), _ptr(color), _ptr(depth), _ptr(alpha),
+                             stream), "gs_render")
+    st = ForwardState(geom, binning, image, counters, radii, capacity, num_rendered)
+    return color, radii, depth, alpha, st
+
+
+def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities, scales, rotations,
+             cov3D_precomp, pose, grad_color, grad_depth=None, grad_alpha=None):
+    """Returns a dict of gradients (None where the input was absent)."""
+    lib = _lib.load()
+    device = require_device(means3D)
+    fa = make_frame(settings, device, pose)
+    ga = make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp, device)
+    P = ga.g.P
+    M = ga.g.M
+    stream = _stream(device)
+    dc = _dev_f32(grad_color, device, "grad_color")
+    dd = _dev_f32(grad_depth, device, "grad_depth")
+    da = _dev_f32(grad_alpha, device, "grad_alpha")
+    if dc is None:
+        dc = torch.zeros((3, fa.frame.height, fa.frame.width), device=device)
+    f32 = dict(dtype=torch.float32, device=device)
+    out = dict(
+        means2D=torch.empty((P, 3), **f32),
+        colors=torch.empty((P, 3), **f32),
+        opacities=torch.empty((P,), **f32),
+        means3D=torch.empty((P, 3), **f32),
+        cov3D=torch.empty((P, 6), **f32),
+        shs=torch.empty((P, M, 3), **f32) if ga.g.shs else None,
+        scales=torch.empty((P, 3), **f32) if ga.g.scales else None,
+        rotations=torch.empty((P, 4), **f32) if ga.g.rotations else None,
+        pose=torch.empty((7,), **f32) if fa.pose is not None else None,
+    )
+    gr = _lib.GsGrads()
+    gr.dL_dmeans2D = out["means2D"].data_ptr()
+    gr.dL_dcolors = out["colors"].data_ptr()
+    gr.dL_dopacity = out["opacities"].data_ptr()
+    gr.dL_dmeans3D = out["means3D"].data_ptr()
+    gr.dL_dcov3D = out["cov3D"].data_ptr()
+    gr.dL_dsh = out["shs"].data_ptr() if out["shs"] is not None else None
+    gr.dL_dscales = out["scales"].data_ptr() if out["scales"] is not None else None
+    gr.dL_drotations = out["rotations"].data_ptr() if out["rotations"] is not None else None
+    gr.dL_dpose = out["pose"].data_ptr() if out["pose"] is not None else None
+    scratch = torch.empty(lib.gs_backward_scratch_bytes(P), dtype=torch.uint8, device=device)
+    _lib.check(lib.gs_backward(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(st.radii), _ptr(st.geom),
+                               _ptr(st.binning), st.capacity, _ptr(st.image), _ptr(st.counters), _ptr(dc), _ptr(dd),
+                               _ptr(da), ctypes.byref(gr), _ptr(scratch), scratch.numel(), stream), "gs_backward")
+    return out
+
+
+def mark_visible(positions: torch.Tensor, viewmatrix, projmatrix) -> torch.Tensor:
+    lib = _lib.load()
+    device = require_device(positions, "positions")
+    pos = _dev_f32(positions, device, "positions")
+    view = _dev_f32(viewmatrix, device, "viewmatrix")
+    proj = _dev_f32(projmatrix, device, "projmatrix")
+    P = positions.shape[0]
+    vis = torch.empty(P, dtype=torch.uint8, device=device)
+    if P:
+        _lib.check(lib.gs_mark_visible(_ptr(pos), P, _ptr(view), _ptr(proj), _ptr(vis), _stream(device)),
+                   "gs_mark_visible")
+    return vis.bool()
+
+
+def knn3_meandist(points: torch.Tensor) -> torch.Tensor:
+    lib = _lib.load()
+    device = require_device(points, "points")
+    pts = _dev_f32(points, device, "points")
+    if pts is None:
+        return torch.empty(0, dtype=torch.float32, device=device)
+    if pts.dim() != 2 or pts.shape[1] != 3:
+        raise ValueError("points must be [P, 3]")
+    P = pts.shape[0]
+    out = torch.empty(P, dtype=torch.float32, device=device)
+    ws = torch.empty(max(int(lib.gs_knn_workspace_bytes(P)), 256), dtype=torch.uint8, device=device)
+    _lib.check(lib.gs_knn3_meandist(_ptr(pts), P, _ptr(out), _ptr(ws), ws.numel(), _stream(device)),
+               "gs_knn3_meandist")
+    return out
+
+
+def sort_pairs_u64(keys: torch.Tensor, vals: torch.Tensor, end_bit: int):
+    """Stable radix sort of int64 (bit pattern of uint64) keys with int32 values (test entry)."""
+    lib = _lib.load()
+    device = require_device(keys, "keys")
+    n = keys.numel()
+    ka = keys.contiguous().clone()
+    va = vals.to(torch.int32).contiguous().clone()
+    kb = torch.empty_like(ka)
+    vb = torch.empty_like(va)
+    n_dev = torch.tensor([n], dtype=torch.int32, device=device)
+    tmp = torch.empty(max(int(lib.gs_sort_tmp_bytes(n)), 256), dtype=torch.uint8, device=device)
+    in_b = ctypes.c_int(0)
+    _lib.check(lib.gs_sort_pairs_u64(_ptr(ka), _ptr(va), _ptr(kb), _ptr(vb), _ptr(n_dev), n, end_bit, _ptr(tmp),
+                                     tmp.numel(), ctypes.byref(in_b), _stream(device)), "gs_sort_pairs_u64")
+    return (kb, vb) if in_b.value else (ka, va)
+
+
+def views(st: ForwardState, P: int, W: int, H: int) -> dict:
+    """Typed views of the internal buffers (keys, sorted ids, ranges, ...) for parity checks."""
+    LG = _lib.geom_layout(P)
+    LB = _lib.binning_layout(st.capacity, W, H)
+    LI = _lib.image_layout(W, H)
+    gx, gy = (W + 15) // 16, (H + 15) // 16
+    n = st.num_rendered if st.num_rendered <= st.capacity else 0
+
+    def sl(buf, off, nbytes, dtype):
+        return buf[off:off + nbytes].view(dtype)
+
+    splat = sl(st.geom, LG.splat, 48 * P, torch.float32).view(P, 3, 4)
+    return dict(
+        xy=splat[:, 0, 0:2],
+        conic=splat[:, 0, 2:4],
+        conic_c=splat[:, 1, 0],
+        opacity=splat[:, 1, 1],
+        depth=splat[:, 1, 2],
+        clamp_bits=splat[:, 1, 3].view(torch.int32) if P else torch.zeros(0, dtype=torch.int32),
+        rgb=splat[:, 2, 0:3],
+        cov3D=sl(st.geom, LG.cov3D, 24 * P, torch.float32).view(P, 6),
+        tiles=sl(st.geom, LG.tiles, 4 * P, torch.int32),
+        keys_dup=sl(st.binning, LB.keys_dup, 8 * n, torch.int64),
+        vals_dup=sl(st.binning, LB.vals_dup, 4 * n, torch.int32),
+        keys=sl(st.binning, LB.keys_sorted, 8 * n, torch.int64),
+        vals=sl(st.binning, LB.vals_sorted, 4 * n, torch.int32),
+        ranges=sl(st.image, LI.ranges, 8 * gx * gy, torch.int32).view(gx * gy, 2),
+        final_T=sl(st.image, LI.final_T, 4 * W * H, torch.float32).view(H, W),
+        n_contrib=sl(st.image, LI.n_contrib, 4 * W * H, torch.int32).view(H, W),
+    )

@@ -1,0 +1,176 @@
+/* gs_raster.h -- C ABI of libgs_hip.so, the MI355X (gfx950) Gaussian rasteriser.
+ *
+ * Drop-in native layer beneath the Python packages `diff_gaussian_rasterization`
+ * and `simple_knn`, the two un-vendored submodules CF-3DGS imports
+ * (reference .gitmodules:1-6; imported at scene/gaussian_model_cf.py:20,28-31).
+ * Each entry point names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every pointer is a DEVICE pointer owned by the caller (e.g. torch
+ *    tensor data_ptr()); the library allocates nothing on these paths.
+ *  - Work is enqueued on `stream` (a hipStream_t passed as void*); calls return
+ *    after enqueueing.  No host synchronisation happens inside the library.
+ *  - Return value 0 = GS_OK, otherwise a GS_E_* code (argument checks are
+ *    done on the host before anything is enqueued).  Conditions only the
+ *    device can see (binning capacity exceeded, a culled point under
+ *    `prefiltered`) are reported in the device status word counters[1].
+ *  - Matrices are the 16 floats of the reference's torch tensors in memory
+ *    order: viewmatrix = world_view_transform, projmatrix =
+ *    full_proj_transform (scene/cameras.py:58-73), i.e. column-major.
+ */
+#ifndef GS_RASTER_H
+#define GS_RASTER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_OK 0
+#define GS_E_ARG 1       /* bad argument (shape, null pointer, size) */
+#define GS_E_WORKSPACE 2 /* workspace smaller than the layout requires */
+#define GS_E_LAUNCH 3    /* the runtime refused a launch */
+
+/* counters[1] bits */
+#define GS_STATUS_CAPACITY 1u    /* num_rendered exceeded the binning capacity */
+#define GS_STATUS_PREFILTERED 2u /* a point was near-culled while prefiltered=1 */
+
+#define GS_TILE 16
+
+/* One camera / frame.  Mirrors GaussianRasterizationSettings
+ * (constructed at scene/gaussian_model_cf.py:811-824), plus an optional rigid
+ * pose applied to the means, which fuses the reference's
+ * `P[i].retr().act(xyz)` (scene/gaussian_model_cf.py:123-133) into preprocess. */
+typedef struct gs_frame {
+  int width, height;
+  float tanfovx, tanfovy;
+  float scale_modifier;
+  int sh_degree;          /* active SH degree D, (D+1)^2 <= M */
+  int prefiltered;
+  const float* viewmatrix; /* [16] */
+  const float* projmatrix; /* [16] */
+  const float* campos;     /* [3] */
+  const float* bg;         /* [3] */
+  const float* pose;       /* [7] = tx,ty,tz,qx,qy,qz,qw (lietorch SE3 layout) or NULL */
+} gs_frame_t;
+
+/* The Gaussians: exactly one of shs / colors_precomp and exactly one of
+ * (scales, rotations) / cov3D_precomp, as GaussianRasterizer.forward demands. */
+typedef struct gs_gaussians {
+  int P;
+  int M;                     /* SH coefficients per Gaussian in shs */
+  const float* means3D;      /* [P,3] */
+  const float* shs;          /* [P,M,3] or NULL */
+  const float* colors_precomp; /* [P,3] or NULL */
+  const float* opacities;    /* [P] */
+  const float* scales;       /* [P,3] or NULL */
+  const float* rotations;    /* [P,4] (w,x,y,z) or NULL */
+  const float* cov3D_precomp; /* [P,6] or NULL */
+} gs_gaussians_t;
+
+/* Gradients written by gs_backward (every entry is written, no accumulation). */
+typedef struct gs_grads {
+  float* dL_dmeans2D;   /* [P,3] NDC-space screen mean gradient (z = 0) */
+  float* dL_dcolors;    /* [P,3] gradient w.r.t. the blended per-Gaussian colour */
+  float* dL_dopacity;   /* [P] */
+  float* dL_dmeans3D;   /* [P,3] (w.r.t. the means before the pose, if any) */
+  float* dL_dcov3D;     /* [P,6] */
+  float* dL_dsh;        /* [P,M,3] or NULL */
+  float* dL_dscales;    /* [P,3] or NULL */
+  float* dL_drotations; /* [P,4] or NULL */
+  float* dL_dpose;      /* [7] or NULL (requires frame.pose) */
+} gs_grads_t;
+
+/* Byte offsets of the sub-buffers of the three workspaces. */
+typedef struct gs_geom_layout {
+  size_t splat;    /* float4[3P]: (x,y,conic_a,conic_b) (conic_c,opacity,depth,clamp bits) (r,g,b,0) */
+  size_t cov3D;    /* float[6P] */
+  size_t tiles;    /* uint32[P] tiles touched */
+  size_t offsets;  /* uint32[P] inclusive prefix sum of tiles */
+  size_t scan_tmp; /* scan partials */
+  size_t total;
+} gs_geom_layout_t;
+
+typedef struct gs_binning_layout {
+  size_t keys_dup, vals_dup;   /* uint64/uint32 [capacity]: duplicated (tile|depth) keys, Gaussian ids */
+  size_t keys_alt, vals_alt;   /* ping-pong buffers of the radix sort */
+  size_t keys_sorted, vals_sorted; /* whichever of the above holds the sorted result */
+  size_t hist;                 /* sort digit histograms */
+  int sort_passes;             /* 8-bit LSD passes over bits [0, 32 + bits(n_tiles)) */
+  size_t total;
+} gs_binning_layout_t;
+
+typedef struct gs_image_layout {
+  size_t ranges;    /* uint32[2*n_tiles] [start, end) into the sorted list */
+  size_t final_T;   /* float[H*W] */
+  size_t n_contrib; /* uint32[H*W] index (1-based) of the last contributor */
+  size_t total;
+} gs_image_layout_t;
+
+int gs_geom_layout(int P, gs_geom_layout_t* out);
+int gs_binning_layout(long long capacity, int width, int height, gs_binning_layout_t* out);
+int gs_image_layout(int width, int height, gs_image_layout_t* out);
+size_t gs_backward_scratch_bytes(int P);
+
+/* Forward, phase 1 (replaces the preprocess + InclusiveSum part of
+ * diff_gaussian_rasterization._C.rasterize_gaussians, called from
+ * _RasterizeGaussians.forward; reached at scene/gaussian_model_cf.py:873-882).
+ * Writes radii[P] and the geometry workspace; counters[0] = num_rendered
+ * (sum of tiles touched), counters[1] = status (cleared, then flagged). */
+int gs_preprocess(const gs_frame_t* frame, const gs_gaussians_t* g, int* radii, void* geom,
+                  size_t geom_bytes, uint32_t* counters, void* stream);
+
+/* Forward, phase 2 (replaces duplicateWithKeys + SortPairs + identifyTileRanges
+ * + render of the same call).  Reads num_rendered from counters[0]; if it
+ * exceeds `capacity` nothing is binned and counters[1] |= GS_STATUS_CAPACITY.
+ * out_color [3,H,W], out_depth [H,W], out_alpha [H,W] (out_alpha may be NULL). */
+int gs_render(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters, long long capacity,
+              void* binning, size_t binning_bytes, void* image, size_t image_bytes, float* out_color,
+              float* out_depth, float* out_alpha, void* stream);
+
+/* Backward (replaces diff_gaussian_rasterization._C.rasterize_gaussians_backward,
+ * triggered by loss.backward() at trainer/cf3dgs_trainer.py:133).  dL_ddepth and
+ * dL_dalpha may be NULL (treated as zero).  scratch: gs_backward_scratch_bytes(P). */
+int gs_backward(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
+                const void* binning, long long capacity, const void* image, const uint32_t* counters,
+                const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha, const gs_grads_t* out,
+                void* scratch, size_t scratch_bytes, void* stream);
+
+/* Near-plane visibility (replaces diff_gaussian_rasterization._C.mark_visible,
+ * GaussianRasterizer.markVisible). visible[P] = 1 if the view-space z > 0.2. */
+int gs_mark_visible(const float* means3D, int P, const float* viewmatrix, const float* projmatrix,
+                    uint8_t* visible, void* stream);
+
+/* Stable LSD radix sort of (uint64 key, uint32 value) pairs on bits
+ * [0, end_bit), n read from *n_dev (<= capacity).  Input in (keys_a, vals_a);
+ * the result lands in a or b: *result_in_b is set on the host (depends only on
+ * end_bit).  tmp: gs_sort_tmp_bytes(capacity). */
+size_t gs_sort_tmp_bytes(long long capacity);
+int gs_sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b,
+                      const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, size_t tmp_bytes,
+                      int* result_in_b, void* stream);
+
+/* Mean squared distance to the 3 nearest other points (replaces
+ * simple_knn._C.distCUDA2, called at scene/gaussian_model_cf.py:194-196).
+ * Exact (same value as a brute-force search).  ws: gs_knn_workspace_bytes(P). */
+size_t gs_knn_workspace_bytes(int P);
+int gs_knn3_meandist(const float* points, int P, float* out, void* ws, size_t ws_bytes, void* stream);
+
+/* Opt-in per-stage timing with HIP events recorded on the launch stream (no
+ * reference counterpart; the reference's debug mode only synchronises).
+ * Stages: preprocess, scan, duplicate, sort, ranges, blend, render_bwd,
+ * preprocess_bwd, knn.  gs_timing_read waits for the recorded events. */
+int gs_timing_enable(int on);
+int gs_timing_reset(void);
+int gs_timing_read(const char* stage, double* total_ms, int* launches);
+
+const char* gs_error_string(int code);
+int gs_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GS_RASTER_H */

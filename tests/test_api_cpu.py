@@ -1,0 +1,137 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every entry point include/gs_raster.h declares, workspace layouts, the
+drop-in packages' argument errors (raised before any device work), the
+loud failure on CPU tensors, and the camera conventions."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "gs_raster.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from gs_hip import _lib
+    lib = _lib.load()
+    names = header_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.gs_abi_version() == 1
+
+
+def test_layouts():
+    from gs_hip import _lib
+    L = _lib.binning_layout(1000, 960, 540)
+    assert L.sort_passes == 6  # 2040 tiles -> 11 bits -> 43-bit keys
+    assert _lib.binning_layout(10, 256, 256).sort_passes == 5  # 256 tiles -> 8 bits
+    G = _lib.geom_layout(1000)
+    assert G.total >= 1000 * (48 + 24 + 8) and G.splat % 256 == 0 and G.cov3D % 256 == 0
+    I = _lib.image_layout(250, 131)
+    assert I.total >= 8 * 16 * 9 + 8 * 250 * 131
+    with pytest.raises(_lib.GsError):
+        _lib.binning_layout(-1, 10, 10)
+
+
+def test_c_abi_rejects_bad_arguments_without_device():
+    import ctypes
+    from gs_hip import _lib
+    lib = _lib.load()
+    f = _lib.GsFrame()
+    f.width, f.height = 0, 10  # invalid
+    g = _lib.GsGaussians()
+    assert lib.gs_preprocess(ctypes.byref(f), ctypes.byref(g), None, None, 0, None, None) == 1
+    assert lib.gs_knn3_meandist(None, -1, None, None, 0, None) == 1
+    assert lib.gs_error_string(1) == b"invalid argument"
+
+
+def _settings():
+    from diff_gaussian_rasterization import GaussianRasterizationSettings
+    return GaussianRasterizationSettings(
+        image_height=32, image_width=32, tanfovx=0.5, tanfovy=0.5, bg=torch.zeros(3), scale_modifier=1.0,
+        viewmatrix=torch.eye(4), projmatrix=torch.eye(4), sh_degree=0, campos=torch.zeros(3),
+        prefiltered=False, debug=False)
+
+
+def test_rasterizer_argument_errors():
+    from diff_gaussian_rasterization import GaussianRasterizer
+    r = GaussianRasterizer(_settings())
+    m = torch.zeros(4, 3)
+    with pytest.raises(Exception, match="exactly one of either SHs|excatly one of either SHs"):
+        r(means3D=m, means2D=m, opacities=torch.ones(4, 1), shs=None, colors_precomp=None,
+          scales=torch.ones(4, 3), rotations=torch.ones(4, 4))
+    with pytest.raises(Exception, match="SHs or precomputed colors"):
+        r(means3D=m, means2D=m, opacities=torch.ones(4, 1), shs=torch.zeros(4, 1, 3), colors_precomp=torch.zeros(4, 3),
+          scales=torch.ones(4, 3), rotations=torch.ones(4, 4))
+    with pytest.raises(Exception, match="scale/rotation pair or precomputed 3D covariance"):
+        r(means3D=m, means2D=m, opacities=torch.ones(4, 1), colors_precomp=torch.zeros(4, 3),
+          scales=torch.ones(4, 3), rotations=torch.ones(4, 4), cov3D_precomp=torch.zeros(4, 6))
+    with pytest.raises(Exception, match="scale/rotation pair or precomputed 3D covariance"):
+        r(means3D=m, means2D=m, opacities=torch.ones(4, 1), colors_precomp=torch.zeros(4, 3))
+
+
+def test_cpu_tensors_fail_loudly():
+    from diff_gaussian_rasterization import GaussianRasterizer
+    r = GaussianRasterizer(_settings())
+    m = torch.zeros(4, 3)
+    with pytest.raises(RuntimeError, match="HIP device"):
+        r(means3D=m, means2D=m, opacities=torch.ones(4, 1), colors_precomp=torch.zeros(4, 3),
+          scales=torch.ones(4, 3), rotations=torch.ones(4, 4))
+    from simple_knn._C import distCUDA2
+    with pytest.raises(RuntimeError, match="HIP device"):
+        distCUDA2(torch.zeros(10, 3))
+
+
+def test_camera_matrices_follow_reference_conventions():
+    from gs_hip.camera import fov2focal, look_at, make_camera
+    W, H = 64, 48
+    fovx = math.radians(60)
+    fovy = 2 * math.atan(math.tan(fovx / 2) * H / W)
+    R, t = look_at([1.0, 2.0, -5.0], [0.0, 0.0, 0.0])
+    cam = make_camera(R, t, fovx, fovy, W, H)
+    # camera centre recovered from the inverse view matrix
+    np.testing.assert_allclose(cam.camera_center.numpy(), [1.0, 2.0, -5.0], atol=1e-5)
+    # a point on the optical axis projects to NDC (0, 0) with w = its depth
+    p = np.array([0.0, 0.0, 0.0, 1.0], np.float32)
+    h = p @ cam.full_proj_transform.numpy()
+    np.testing.assert_allclose(h[:2] / h[3], [0.0, 0.0], atol=1e-6)
+    assert h[3] == pytest.approx(math.sqrt(1 + 4 + 25), rel=1e-5)
+    # x right, y down, z forward: a point to the camera's right has positive NDC x
+    right = R[:, 0]
+    h2 = np.concatenate([right * 0.5, [1.0]]).astype(np.float32) @ cam.full_proj_transform.numpy()
+    assert h2[0] / h2[3] > 0
+    assert fov2focal(fovx, W) == pytest.approx(W / (2 * math.tan(fovx / 2)))
+
+
+def test_loss_matches_reference_formula():
+    """The torch SSIM equals a direct per-pixel evaluation of the windowed
+    statistics (zero padding) at a few pixels."""
+    from gs_hip.loss import gaussian_window, ssim
+    g = torch.Generator().manual_seed(0)
+    a = torch.rand((3, 20, 24), generator=g)
+    b = torch.rand((3, 20, 24), generator=g)
+    s = ssim(a, b)
+    w = gaussian_window(11, 1.5, 1)[0, 0].double()
+    ap = torch.nn.functional.pad(a.double(), (5, 5, 5, 5))
+    bp = torch.nn.functional.pad(b.double(), (5, 5, 5, 5))
+    tot = 0.0
+    for c in range(3):
+        for y in range(20):
+            for x in range(24):
+                pa, pb = ap[c, y:y + 11, x:x + 11], bp[c, y:y + 11, x:x + 11]
+                m1, m2 = (w * pa).sum(), (w * pb).sum()
+                v1 = (w * pa * pa).sum() - m1 * m1
+                v2 = (w * pb * pb).sum() - m2 * m2
+                c12 = (w * pa * pb).sum() - m1 * m2
+                C1, C2 = 1e-4, 9e-4
+                tot += ((2 * m1 * m2 + C1) * (2 * c12 + C2) / ((m1 * m1 + m2 * m2 + C1) * (v1 + v2 + C2))).item()
+    assert float(s) == pytest.approx(tot / (3 * 20 * 24), rel=1e-5)
