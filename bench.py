@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--width", type=int, default=960)
     ap.add_argument("--height", type=int, default=540)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--torch-loss", action="store_true", help="loss with torch conv2d windows instead of the fused kernel")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy 5 times (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -110,8 +113,17 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_launch": a.numel() * 4,
+                          "bytes_written_per_launch": b.numel() * 4, "launches": 5}))
+        return
     from gs_hip import _lib
-    from gs_hip.loss import photometric_loss
+    from gs_hip.loss import fused_photometric_loss, photometric_loss
     from gs_hip.model import render
 
     device = torch.device("cuda:0")
@@ -120,7 +132,10 @@ def main():
 
     def step():
         out = render(g, cam, bg)
-        loss = photometric_loss(out["image"], target)
+        if args.torch_loss:
+            loss = photometric_loss(out["image"], target)
+        else:
+            loss, _ = fused_photometric_loss(out["image"], target)
         loss.backward()
         opt.step()
         opt.zero_grad(set_to_none=True)

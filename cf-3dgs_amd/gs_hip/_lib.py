@@ -81,6 +81,11 @@ EXPORTS = {
                                   c_size_t, ctypes.POINTER(c_int), c_void_p]),
     "gs_knn_workspace_bytes": (c_size_t, [c_int]),
     "gs_knn3_meandist": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "gs_loss_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "gs_ssim_l1_forward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_float), c_float,
+                                   c_void_p, c_void_p, c_size_t, c_void_p]),
+    "gs_ssim_l1_backward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_float), c_float,
+                                    c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
     "gs_timing_enable": (c_int, [c_int]),
     "gs_timing_reset": (c_int, []),
     "gs_timing_read": (c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int)]),

@@ -53,4 +53,67 @@ def l1_loss(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
 
 
 def photometric_loss(pred: torch.Tensor, gt: torch.Tensor, lambda_dssim: float = 0.2) -> torch.Tensor:
+    """The reference formula in torch ops (conv2d windows)."""
     return (1.0 - lambda_dssim) * l1_loss(pred, gt) + lambda_dssim * (1.0 - ssim(pred, gt))
+
+
+def _window_taps(window_size: int = 11, sigma: float = 1.5):
+    """The 1-D taps exactly as trainer/losses.py:143-146 forms them (float32)."""
+    g = torch.tensor([math.exp(-(x - window_size // 2) ** 2 / float(2 * sigma ** 2)) for x in range(window_size)])
+    g = g / g.sum()
+    import ctypes
+    return (ctypes.c_float * window_size)(*[float(v) for v in g])
+
+
+_TAPS = None
+
+
+class _FusedPhotometric(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pred, gt, lambda_dssim):
+        import ctypes
+        from . import _lib
+        global _TAPS
+        if _TAPS is None:
+            _TAPS = _window_taps()
+        lib = _lib.load()
+        if pred.device.type != "cuda":
+            raise RuntimeError("the fused loss runs on the GPU only")
+        x = pred.contiguous().float()
+        y = gt.to(x.device).contiguous().float()
+        if x.dim() == 4:
+            x, y = x[0], y[0]
+        C, H, W = x.shape
+        ws = torch.empty(lib.gs_loss_workspace_bytes(C, H, W), dtype=torch.uint8, device=x.device)
+        out = torch.empty(3, dtype=torch.float32, device=x.device)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
+        _lib.check(lib.gs_ssim_l1_forward(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), C, H, W,
+                                          _TAPS, float(lambda_dssim), ctypes.c_void_p(out.data_ptr()),
+                                          ctypes.c_void_p(ws.data_ptr()), ws.numel(), stream), "gs_ssim_l1_forward")
+        ctx.save_for_backward(x, y, ws)
+        ctx.shape = pred.shape
+        ctx.lam = float(lambda_dssim)
+        ctx.mark_non_differentiable(out)
+        return out[0], out
+
+    @staticmethod
+    def backward(ctx, g_loss, g_out):
+        import ctypes
+        from . import _lib
+        lib = _lib.load()
+        x, y, ws = ctx.saved_tensors
+        C, H, W = x.shape
+        go = g_loss.reshape(1).contiguous().float()
+        dx = torch.empty_like(x)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
+        _lib.check(lib.gs_ssim_l1_backward(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), C, H, W,
+                                           _TAPS, ctx.lam, ctypes.c_void_p(go.data_ptr()),
+                                           ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.c_void_p(dx.data_ptr()),
+                                           stream), "gs_ssim_l1_backward")
+        return dx.view(ctx.shape), None, None
+
+
+def fused_photometric_loss(pred: torch.Tensor, gt: torch.Tensor, lambda_dssim: float = 0.2):
+    """(1 - l) L1 + l (1 - SSIM) in two HIP kernels forward and one backward.
+    Returns (loss, stats) with stats = [loss, mean SSIM, mean L1] on the device."""
+    return _FusedPhotometric.apply(pred, gt, lambda_dssim)

@@ -158,6 +158,21 @@ int gs_sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint
 size_t gs_knn_workspace_bytes(int P);
 int gs_knn3_meandist(const float* points, int P, float* out, void* ws, size_t ws_bytes, void* stream);
 
+/* Fused photometric loss of the training step (replaces Loss.forward's
+ * (1-l)*l1_loss + l*(1 - SSIM_V2) in torch, trainer/losses.py:98-136 and
+ * :164-251; l = lambda_dssim = 0.2 at arguments/__init__.py:113).  pred and
+ * target are [C,H,W]; window11 is a HOST array of the 11 separable window
+ * taps (the normalised 1-D Gaussian, sigma 1.5, of trainer/losses.py:143-159).
+ * out3 (device) = {loss, mean SSIM, mean |pred - target|}.  The workspace keeps
+ * the per-pixel SSIM partials for the backward, which writes
+ * dpred = grad_out[0] * d loss / d pred. */
+size_t gs_loss_workspace_bytes(int C, int H, int W);
+int gs_ssim_l1_forward(const float* pred, const float* target, int C, int H, int W, const float* window11,
+                       float lambda_dssim, float* out3, void* ws, size_t ws_bytes, void* stream);
+int gs_ssim_l1_backward(const float* pred, const float* target, int C, int H, int W, const float* window11,
+                        float lambda_dssim, const float* grad_out, const void* ws, size_t ws_bytes, float* dpred,
+                        void* stream);
+
 /* Opt-in per-stage timing with HIP events recorded on the launch stream (no
  * reference counterpart; the reference's debug mode only synchronises).
  * Stages: preprocess, scan, duplicate, sort, ranges, blend, render_bwd,
