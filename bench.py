@@ -31,16 +31,21 @@ METRIC = "fwd+bwd iters/s @500k Gaussians 960x540; pose-solves/s + p99 at fixed 
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, passes: int = 6) -> float:
-    """Algorithmic HBM bytes of one launch of a stage (the bytes the algorithm
-    must move, independent of how the kernel moves them)."""
+def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, passes: int = 6,
+                depth_first: bool = True) -> float:
+    """Algorithmic HBM bytes of one step's worth of a stage (the bytes the
+    algorithm must move, independent of how the kernels move them)."""
     hw = W * H
     if stage == "preprocess":   # in: xyz 12, scale 12, rot 16, opac 4, SH 12M; out: radii 4, splat 48, cov 24, tiles 4, scan 4
         return P * (44 + 12 * M + 84)
-    if stage == "duplicate":    # in: splat 48 + tiles 4 + scan 4 (+ block prefix); out: 12 B per duplicated key
-        return P * 56 + 12 * n_dup
-    if stage == "sort":         # each 8-bit pass reads and writes (key 8, value 4)
-        return passes * 24 * n_dup
+    if stage == "duplicate":
+        if depth_first:         # scan of tiles in depth order (id 4, tiles 4, prefix 4) + dup (id, tiles, prefix, 2 splat records)
+            return P * 12 + P * (12 + 32) + 8 * n_dup
+        return P * 56 + 12 * n_dup  # splat 48 + tiles 4 + scan 4; out: key 8 + id 4 per duplicated entry
+    if stage == "sort":
+        if depth_first:         # 4 passes over P (key 4, id 4, in + out) + tile passes over the list (4 + 4, in + out)
+            return 4 * 16 * P + passes * 16 * n_dup
+        return passes * 24 * n_dup  # each 8-bit pass reads and writes (key 8, value 4)
     if stage == "ranges":
         return 8 * n_dup + 8 * ((W + 15) // 16) * ((H + 15) // 16)
     if stage == "blend":        # per list entry: id 4 + splat 48; per pixel out: rgb 12, depth 4, alpha 4, T 4, n 4
@@ -160,7 +165,7 @@ def main():
     for s in _lib.STAGES:
         tot, n = _lib.timing_read(s)
         if n:
-            stages[s] = {"ms_per_launch": tot / n, "launches": n}
+            stages[s] = {"ms_per_step": tot / args.steps, "event_pairs_per_step": n // args.steps}
     _lib.timing_reset()
     # N_dup of the last step (constant up to parameter drift)
     with torch.no_grad():
@@ -170,18 +175,19 @@ def main():
         _, _, _, _, st = raster.forward(rs, g.get_xyz, g.get_features, None, g.get_opacity, g.get_scaling,
                                         g.get_rotation, None)
         n_dup = st.num_rendered
-    L = _lib.binning_layout(n_dup, args.width, args.height)
+    L = _lib.binning_layout(n_dup, args.P, args.width, args.height, raster.BINNING)
     for s, d in stages.items():
-        b = stage_bytes(s, args.P, n_dup, args.width, args.height, passes=L.sort_passes)
+        b = stage_bytes(s, args.P, n_dup, args.width, args.height, passes=L.sort_passes,
+                        depth_first=raster.BINNING == _lib.BINNING_DEPTH_FIRST)
         d["alg_bytes"] = b
-        d["GB_s"] = b / (d["ms_per_launch"] * 1e-3) / 1e9 if d["ms_per_launch"] > 0 else 0.0
-    dom = max(stages, key=lambda s: stages[s]["ms_per_launch"] * stages[s]["launches"]) if stages else None
+        d["GB_s"] = b / (d["ms_per_step"] * 1e-3) / 1e9 if d["ms_per_step"] > 0 else 0.0
+    dom = max(stages, key=lambda s: stages[s]["ms_per_step"]) if stages else None
     roofline = None
     if dom:
         d = stages[dom]
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(d["GB_s"], 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(d["GB_s"] / HBM_PEAK_GBS, 4), "traffic": None,
-                    "ms_per_launch": round(d["ms_per_launch"], 4)}
+                    "ms_per_launch": round(d["ms_per_step"], 4)}
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
     res = {
         "metric": METRIC, "value": round(1000.0 / ms, 3), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,

@@ -3,14 +3,20 @@
 //   k_zero_f32       clears the per-Gaussian render-gradient accumulators and
 //                    the pose accumulator (a kernel, not a memset, so the
 //                    sequence can be captured in a graph)
-//   k_render_bwd     one 16x16 tile per workgroup, back to front over the
-//                    tile's list, starting at the workgroup's largest
-//                    contributor count (entries past it never contributed).
-//                    T is recovered by division as in the forward's algebra.
-//                    The 10 per-(pixel, Gaussian) gradient terms are summed
-//                    across the wave with DPP before one lane issues the
-//                    atomics, so a Gaussian receives at most 4 atomics per
-//                    term per tile instead of 256.
+//   k_bwd_items      one workgroup: work items = 256-entry chunks of each
+//                    tile's list up to the tile's largest contributor count
+//                    (entries past it never contributed)
+//   k_render_bwd     one chunk per workgroup (16x16 pixels), back to front.
+//                    A chunk that is not the tile's last starts from the
+//                    forward's checkpoint at its end: T_b, and the suffix sums
+//                    (C_final - C_b) / T_b that the back-to-front recurrence
+//                    would have accumulated, so heavy tiles spread over many
+//                    workgroups instead of serialising in one.  T is then
+//                    recovered by division as in the forward's algebra.  The
+//                    10 per-(pixel, Gaussian) terms are summed over each
+//                    16-lane DPP row, the 4 row sums per wave go to LDS
+//                    accumulators, and each Gaussian of the chunk receives
+//                    one global atomic per term.
 //   k_preprocess_bwd per Gaussian: conic -> 2-D cov -> 3-D cov / mean,
 //                    projection, depth, SH direction; scale/rotation; the
 //                    rigid pose gradient is reduced wave -> atomic.
@@ -28,26 +34,86 @@ __global__ __launch_bounds__(256) void k_zero_f32(float* p, long long n) {
 }
 
 struct RBArgs {
-  int W, H, gx;
+  int W, H, gx, nt;
   const uint32_t* ranges;
   const uint32_t* list;
   const float4* splat;
   const float* bg;
   const float* final_T;
   const uint32_t* n_contrib;
+  const float* accum;         // [4][H*W] colour sums before the background, depth
+  const float* ckpt;          // [..][5][256] forward state at batch boundaries
+  const uint32_t* ckpt_base;  // [nt + 1]
+  const uint32_t* tile_maxc;  // [nt]
+  const uint32_t* item_base;  // [nt + 1]
   const float* dL_dcolor;
   const float* dL_ddepth;
   const float* dL_dalpha;
   float* g_render;
 };
 
+// One workgroup: item_base = exclusive prefix over tiles of ceil(maxc / 256),
+// the number of 256-entry chunks the backward processes for each tile.
+__global__ __launch_bounds__(1024) void k_bwd_items(const uint32_t* tile_maxc, int nt, uint32_t* item_base) {
+  __shared__ uint32_t s[1024];
+  __shared__ uint32_t s_carry;
+  const int t = threadIdx.x;
+  if (t == 0) s_carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nt; b0 += 1024) {
+    const int b = b0 + t;
+    const uint32_t v = b < nt ? (tile_maxc[b] + 255) / 256 : 0u;
+    s[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const uint32_t add = t >= off ? s[t - off] : 0u;
+      __syncthreads();
+      s[t] += add;
+      __syncthreads();
+    }
+    const uint32_t carry = s_carry;
+    if (b < nt) item_base[b] = carry + s[t] - v;
+    __syncthreads();
+    if (t == 1023) s_carry = carry + s[1023];
+    __syncthreads();
+  }
+  if (t == 0) item_base[nt] = s_carry;
+}
+
+// Sum over the 16 lanes of each DPP row; every lane of the row gets the sum.
+__device__ __forceinline__ float row_sum16(float v) {
+  v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp<0x141>(v);  // row_half_mirror
+  v += dpp<0x140>(v);  // row_mirror
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
   __shared__ uint32_t s_id[256];
   __shared__ float4 s_a[256];
   __shared__ float4 s_b[256];
   __shared__ float4 s_c[256];
-  __shared__ uint32_t s_max;
-  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  __shared__ float s_acc[10][256];
+  const uint32_t nitems = a.item_base[a.nt];
+  // The grid is an upper bound on the item count, so items are taken in
+  // dispatch order: live items then go round-robin over all 8 XCDs (a
+  // contiguous-per-XCD remap would put them all on the first few).
+  const uint32_t item = blockIdx.x;
+  if (item >= nitems) return;
+  // tile = last t with item_base[t] <= item (tiles without items share the next base)
+  int lo = 0, hi = a.nt - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.item_base[mid] <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  const int tile = lo;
+  const uint32_t chunk = item - a.item_base[tile];
+  const uint32_t maxc = a.tile_maxc[tile];
+  const uint32_t nchunk = (maxc + 255) / 256;
+  const uint32_t cstart = chunk * 256;
+  const uint32_t cend = min(cstart + 256, maxc);
   const int tx = tile % a.gx, ty = tile / a.gx;
   const int px = tx * kBX + (threadIdx.x & 15), py = ty * kBY + (threadIdx.x >> 4);
   const bool inside = px < a.W && py < a.H;
@@ -55,108 +121,121 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
   const int HW = a.W * a.H;
   const float pfx = (float)px, pfy = (float)py;
   const uint32_t r0 = a.ranges[2 * tile];
+  {
+    const uint32_t q = cstart + threadIdx.x;
+    if (q < cend) {
+      const uint32_t g = a.list[r0 + q];
+      s_id[threadIdx.x] = g;
+      s_a[threadIdx.x] = a.splat[3 * g];
+      s_b[threadIdx.x] = a.splat[3 * g + 1];
+      s_c[threadIdx.x] = a.splat[3 * g + 2];
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) s_acc[k][threadIdx.x] = 0.0f;
+  }
   const float T_fin = inside ? a.final_T[pix] : 0.0f;
   const uint32_t last = inside ? a.n_contrib[pix] : 0u;
-  if (threadIdx.x == 0) s_max = 0;
-  __syncthreads();
-  if (last) atomicMax(&s_max, last);
-  __syncthreads();
-  const uint32_t todo = s_max;
   float dpix0 = 0.f, dpix1 = 0.f, dpix2 = 0.f, dD = 0.f, dA = 0.f;
-  if (inside) {
+  // per-pixel state at position cend (the end of this chunk)
+  float T = T_fin;
+  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc_d = 0.f, acc_a = 0.f;
+  if (inside && last > cstart) {
     dpix0 = a.dL_dcolor[pix];
     dpix1 = a.dL_dcolor[HW + pix];
     dpix2 = a.dL_dcolor[2 * HW + pix];
     if (a.dL_ddepth) dD = a.dL_ddepth[pix];
     if (a.dL_dalpha) dA = a.dL_dalpha[pix];
+    if (chunk + 1 < nchunk) {
+      // suffix sums from the forward's state at the boundary: sum_{k >= b} c_k a_k T_k = C_final - C_b
+      const float* ck = a.ckpt + (size_t)(a.ckpt_base[tile] + chunk) * 5 * 256 + threadIdx.x;
+      const float Tb = ck[0];
+      const float inv = 1.0f / Tb;
+      T = Tb;
+      acc0 = (a.accum[pix] - ck[256]) * inv;
+      acc1 = (a.accum[HW + pix] - ck[512]) * inv;
+      acc2 = (a.accum[2 * HW + pix] - ck[768]) * inv;
+      acc_d = (a.accum[3 * HW + pix] - ck[1024]) * inv;
+      acc_a = 1.0f - T_fin * inv;
+    }
   }
   const float bg_dot = (a.bg[0] * dpix0 + a.bg[1] * dpix1) + a.bg[2] * dpix2;
   const float ddelx = 0.5f * (float)a.W, ddely = 0.5f * (float)a.H;
   const float a_min = 1.0f / 255.0f;
-  float T = T_fin;
-  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc_d = 0.f, acc_a = 0.f;
   float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f, last_d = 0.f;
   const int lane = threadIdx.x & 63;
-
-  for (int bend = (int)todo; bend > 0; bend -= 256) {
-    const int bstart = bend > 256 ? bend - 256 : 0;
-    __syncthreads();
-    {
-      const int q = bstart + (int)threadIdx.x;
-      if (q < bend) {
-        const uint32_t g = a.list[r0 + q];
-        s_id[threadIdx.x] = g;
-        s_a[threadIdx.x] = a.splat[3 * g];
-        s_b[threadIdx.x] = a.splat[3 * g + 1];
-        s_c[threadIdx.x] = a.splat[3 * g + 2];
-      }
-    }
-    __syncthreads();
-    for (int j = bend - bstart - 1; j >= 0; --j) {
-      const int q = bstart + j;  // 0-based position; contributor index q+1
-      bool active = inside && (uint32_t)q < last;
-      float v[10];
+  __syncthreads();
+  for (int j = (int)(cend - cstart) - 1; j >= 0; --j) {
+    const uint32_t q = cstart + j;  // 0-based list position; contributor index q + 1
+    bool active = inside && q < last;
+    float v[10];
 #pragma unroll
-      for (int k = 0; k < 10; ++k) v[k] = 0.f;
+    for (int k = 0; k < 10; ++k) v[k] = 0.f;
+    if (active) {
       const float4 ga = s_a[j];
       const float4 gb = s_b[j];
+      const float dx = ga.x - pfx, dy = ga.y - pfy;
+      const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
+      float G = 0.f, alpha = 0.f;
+      if (power > 0.0f) {
+        active = false;
+      } else {
+        G = spec_expf(power);
+        alpha = fminf(0.99f, gb.y * G);
+        if (alpha < a_min) active = false;
+      }
       if (active) {
-        const float dx = ga.x - pfx, dy = ga.y - pfy;
-        const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
-        float G = 0.f, alpha = 0.f;
-        if (power > 0.0f) {
-          active = false;
-        } else {
-          G = spec_expf(power);
-          alpha = fminf(0.99f, gb.y * G);
-          if (alpha < a_min) active = false;
-        }
-        if (active) {
-          const float4 gc = s_c[j];
-          T = T / (1.0f - alpha);
-          const float w = alpha * T;
-          float dL_da = 0.f;
-          acc0 = last_alpha * lc0 + (1.0f - last_alpha) * acc0;
-          acc1 = last_alpha * lc1 + (1.0f - last_alpha) * acc1;
-          acc2 = last_alpha * lc2 + (1.0f - last_alpha) * acc2;
-          lc0 = gc.x;
-          lc1 = gc.y;
-          lc2 = gc.z;
-          dL_da += (gc.x - acc0) * dpix0;
-          dL_da += (gc.y - acc1) * dpix1;
-          dL_da += (gc.z - acc2) * dpix2;
-          v[6] = w * dpix0;
-          v[7] = w * dpix1;
-          v[8] = w * dpix2;
-          acc_d = last_alpha * last_d + (1.0f - last_alpha) * acc_d;
-          last_d = gb.z;
-          dL_da += (gb.z - acc_d) * dD;
-          v[9] = w * dD;
-          acc_a = last_alpha + (1.0f - last_alpha) * acc_a;
-          dL_da += (1.0f - acc_a) * dA;
-          dL_da *= T;
-          last_alpha = alpha;
-          dL_da += (-T_fin / (1.0f - alpha)) * bg_dot;
-          const float dL_dG = gb.y * dL_da;
-          const float gdx = G * dx, gdy = G * dy;
-          const float dG_dx = -gdx * ga.z - gdy * ga.w;
-          const float dG_dy = -gdy * gb.x - gdx * ga.w;
-          v[0] = dL_dG * dG_dx * ddelx;
-          v[1] = dL_dG * dG_dy * ddely;
-          v[2] = -0.5f * gdx * dx * dL_dG;
-          v[3] = -gdx * dy * dL_dG;
-          v[4] = -0.5f * gdy * dy * dL_dG;
-          v[5] = G * dL_da;
-        }
+        const float4 gc = s_c[j];
+        T = T / (1.0f - alpha);
+        const float w = alpha * T;
+        float dL_da = 0.f;
+        acc0 = last_alpha * lc0 + (1.0f - last_alpha) * acc0;
+        acc1 = last_alpha * lc1 + (1.0f - last_alpha) * acc1;
+        acc2 = last_alpha * lc2 + (1.0f - last_alpha) * acc2;
+        lc0 = gc.x;
+        lc1 = gc.y;
+        lc2 = gc.z;
+        dL_da += (gc.x - acc0) * dpix0;
+        dL_da += (gc.y - acc1) * dpix1;
+        dL_da += (gc.z - acc2) * dpix2;
+        v[6] = w * dpix0;
+        v[7] = w * dpix1;
+        v[8] = w * dpix2;
+        acc_d = last_alpha * last_d + (1.0f - last_alpha) * acc_d;
+        last_d = gb.z;
+        dL_da += (gb.z - acc_d) * dD;
+        v[9] = w * dD;
+        acc_a = last_alpha + (1.0f - last_alpha) * acc_a;
+        dL_da += (1.0f - acc_a) * dA;
+        dL_da *= T;
+        last_alpha = alpha;
+        dL_da += (-T_fin / (1.0f - alpha)) * bg_dot;
+        const float dL_dG = gb.y * dL_da;
+        const float gdx = G * dx, gdy = G * dy;
+        const float dG_dx = -gdx * ga.z - gdy * ga.w;
+        const float dG_dy = -gdy * gb.x - gdx * ga.w;
+        v[0] = dL_dG * dG_dx * ddelx;
+        v[1] = dL_dG * dG_dy * ddely;
+        v[2] = -0.5f * gdx * dx * dL_dG;
+        v[3] = -gdx * dy * dL_dG;
+        v[4] = -0.5f * gdy * dy * dL_dG;
+        v[5] = G * dL_da;
       }
-      if (__ballot(active) != 0ull) {
-        float* dst = a.g_render + (size_t)s_id[j] * kRG;
+    }
+    if (__ballot(active) != 0ull) {
 #pragma unroll
-        for (int k = 0; k < 10; ++k) {
-          const float s = wave_sum(v[k]);
-          if (lane == 0 && s != 0.0f) unsafeAtomicAdd(dst + k, s);
-        }
+      for (int k = 0; k < 10; ++k) {
+        const float s = row_sum16(v[k]);
+        if ((lane & 15) == 0 && s != 0.0f) atomicAdd(&s_acc[k][j], s);
       }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < cend - cstart) {
+    float* dst = a.g_render + (size_t)s_id[threadIdx.x] * kRG;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      const float s = s_acc[k][threadIdx.x];
+      if (s != 0.0f) unsafeAtomicAdd(dst + k, s);
     }
   }
 }
@@ -189,7 +268,26 @@ struct PBArgs {
 };
 
 __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
+  // [256][3M] SH rows of this workgroup: coefficients are read from and their
+  // gradients written back to the thread's own row, then the whole block is
+  // stored with coalesced 16-byte writes (M <= 16; larger M goes direct)
+  extern __shared__ float s_sh[];
   const int i = blockIdx.x * 256 + threadIdx.x;
+  const int base = blockIdx.x * 256;
+  const int nloc = min(256, a.P - base);
+  const bool staged = a.shs && a.M <= 16;
+  const int row = 3 * a.M;
+  if (staged) {
+    const float* src = a.shs + (size_t)base * row;
+    const int total = nloc * row;
+    if ((a.M & 3) == 0) {
+      for (int k = threadIdx.x; k < (total >> 2); k += 256)
+        reinterpret_cast<float4*>(s_sh)[k] = reinterpret_cast<const float4*>(src)[k];
+    } else {
+      for (int k = threadIdx.x; k < total; k += 256) s_sh[k] = src[k];
+    }
+  }
+  __syncthreads();
   float pacc[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) pacc[k] = 0.f;
@@ -296,8 +394,8 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
         float b[16], bg3[16][3];
         sh_basis(a.D, dx, dy, dz, b);
         sh_basis_grad(a.D, dx, dy, dz, bg3);
-        const float* sh = a.shs + (size_t)i * a.M * 3;
-        float* gsh = a.dL_dsh + (size_t)i * a.M * 3;
+        const float* sh = staged ? s_sh + threadIdx.x * row : a.shs + (size_t)i * row;
+        float* gsh = staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row;
         float gdir0 = 0.f, gdir1 = 0.f, gdir2 = 0.f;
         for (int k = 0; k < nc; ++k) {
           const float s0 = sh[3 * k], s1 = sh[3 * k + 1], s2 = sh[3 * k + 2];
@@ -363,8 +461,10 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
       }
     } else {
       a.dL_dmeans3D[3 * i] = a.dL_dmeans3D[3 * i + 1] = a.dL_dmeans3D[3 * i + 2] = 0.f;
-      if (a.shs)
-        for (int k = 0; k < 3 * a.M; ++k) a.dL_dsh[(size_t)i * a.M * 3 + k] = 0.f;
+      if (a.shs) {
+        float* gsh = staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row;
+        for (int k = 0; k < row; ++k) gsh[k] = 0.f;
+      }
       if (!a.cov_pre) {
         a.dL_dscales[3 * i] = a.dL_dscales[3 * i + 1] = a.dL_dscales[3 * i + 2] = 0.f;
         reinterpret_cast<float4*>(a.dL_drots)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -372,6 +472,17 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * i + k] = gS[k];
+  }
+  if (staged) {
+    __syncthreads();
+    float* dst = a.dL_dsh + (size_t)base * row;
+    const int total = nloc * row;
+    if ((a.M & 3) == 0) {
+      for (int k = threadIdx.x; k < (total >> 2); k += 256)
+        reinterpret_cast<float4*>(dst)[k] = reinterpret_cast<const float4*>(s_sh)[k];
+    } else {
+      for (int k = threadIdx.x; k < total; k += 256) dst[k] = s_sh[k];
+    }
   }
   if (a.pose) {
     // all lanes reach here (no early return above)
@@ -414,7 +525,7 @@ extern "C" size_t gs_backward_scratch_bytes(int P) {
 }
 
 extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
-                           const void* binning, long long capacity, const void* image, const uint32_t* counters,
+                           const void* binning, long long capacity, void* image, const uint32_t* counters,
                            const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
                            const gs_grads_t* out, void* scratch, size_t scratch_bytes, void* stream) {
   if (!f || !g || !out || !counters || !image || !dL_dcolor) return GS_E_ARG;
@@ -435,14 +546,14 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
   gs_binning_layout_t LB;
   gs_image_layout_t LI;
   gs_geom_layout(P, &LG);
-  if (gs_binning_layout(capacity, f->width, f->height, &LB) != GS_OK) return GS_E_ARG;
+  if (gs_binning_layout(capacity, P, f->width, f->height, f->binning, &LB) != GS_OK) return GS_E_ARG;
   gs_image_layout(f->width, f->height, &LI);
   hipStream_t s = (hipStream_t)stream;
   const int gx = (f->width + kBX - 1) / kBX, gy = (f->height + kBY - 1) / kBY;
   const int nt = gx * gy;
   const char* gb = (const char*)geom;
   const char* bb = (const char*)binning;
-  const char* ib = (const char*)image;
+  char* ib = (char*)image;
   float* g_render = (float*)scratch;
   float* pose_acc = (float*)((char*)scratch + align_up((size_t)P * kRG * sizeof(float), 256));
   const long long nz = (long long)P * kRG;
@@ -461,12 +572,23 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
     r.bg = f->bg;
     r.final_T = (const float*)(ib + LI.final_T);
     r.n_contrib = (const uint32_t*)(ib + LI.n_contrib);
+    r.nt = nt;
+    r.accum = (const float*)(ib + LI.accum);
+    r.ckpt = (const float*)(bb + LB.ckpt);
+    r.ckpt_base = (const uint32_t*)(ib + LI.ckpt_base);
+    r.tile_maxc = (const uint32_t*)(ib + LI.tile_maxc);
+    r.item_base = (const uint32_t*)(ib + LI.item_base);
     r.dL_dcolor = dL_dcolor;
     r.dL_ddepth = dL_ddepth;
     r.dL_dalpha = dL_dalpha;
     r.g_render = g_render;
+    // (the item table lives in the image workspace, which the forward owns;
+    // the backward rewrites it from tile_maxc before use)
+    hipLaunchKernelGGL(k_bwd_items, dim3(1), dim3(1024), 0, s, r.tile_maxc, nt, (uint32_t*)(ib + LI.item_base));
+    // upper bound on items: sum over tiles of ceil(list length / 256)
+    const long long max_items = capacity / 256 + nt + 1;
     timing::Scope ts(timing::kRenderBwd, s);
-    hipLaunchKernelGGL(k_render_bwd, dim3(nt), dim3(256), 0, s, r);
+    hipLaunchKernelGGL(k_render_bwd, dim3((unsigned)max_items), dim3(256), 0, s, r);
   }
   if (P > 0) {
     PBArgs b;
@@ -503,7 +625,8 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
     b.dL_drots = out->dL_drotations;
     b.pose_acc = pose_acc;
     timing::Scope ts(timing::kPreprocessBwd, s);
-    hipLaunchKernelGGL(k_preprocess_bwd, dim3((P + 255) / 256), dim3(256), 0, s, b);
+    const size_t shm = (g->shs && g->M <= 16) ? (size_t)256 * 3 * g->M * sizeof(float) : 0;
+    hipLaunchKernelGGL(k_preprocess_bwd, dim3((P + 255) / 256), dim3(256), shm, s, b);
   }
   if (out->dL_dpose) hipLaunchKernelGGL(k_pose_finalize, dim3(1), dim3(64), 0, s, f->pose, pose_acc, out->dL_dpose);
   (void)counters;

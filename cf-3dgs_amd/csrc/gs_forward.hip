@@ -335,6 +335,114 @@ __global__ __launch_bounds__(256) void k_ranges(const uint64_t* keys, const uint
   if (k == n - 1) ranges[2 * t + 1] = n;
 }
 
+// ---- depth-first binning -------------------------------------------------
+// Keys for the depth sort: depth bits of visible Gaussians (positive floats
+// order as their bit patterns), 0xffffffff for the rest (sorted last, and they
+// touch no tile).  counters[2] = P is the sort's element count.
+__global__ __launch_bounds__(256) void k_df_keys(int P, const uint32_t* tiles, const float4* splat, uint32_t* keys,
+                                                 uint32_t* vals, uint32_t* counters) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) counters[2] = (uint32_t)P;
+  if (i >= P) return;
+  keys[i] = tiles[i] ? __float_as_uint(splat[3 * i + 1].z) : 0xffffffffu;
+  vals[i] = (uint32_t)i;
+}
+
+// Workgroup-local inclusive prefix of the tile counts in depth order.
+__global__ __launch_bounds__(256) void k_df_scan(int P, const uint32_t* dvals, const uint32_t* tiles, uint32_t* dscan,
+                                                 uint32_t* dblocks) {
+  __shared__ uint32_t s_scan[256];
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t v = k < P ? tiles[dvals[k]] : 0u;
+  uint32_t total;
+  const uint32_t incl = block_incl_scan_256(v, s_scan, total);
+  if (k < P) dscan[k] = incl;
+  if (threadIdx.x == 0) dblocks[blockIdx.x] = total;
+}
+
+// One workgroup: in-place exclusive prefix of n values.
+__global__ __launch_bounds__(1024) void k_excl_scan_1wg(uint32_t* data, int n) {
+  __shared__ uint32_t s[1024];
+  __shared__ uint32_t s_carry;
+  const int t = threadIdx.x;
+  if (t == 0) s_carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < n; b0 += 1024) {
+    const int b = b0 + t;
+    const uint32_t v = b < n ? data[b] : 0u;
+    s[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const uint32_t add = t >= off ? s[t - off] : 0u;
+      __syncthreads();
+      s[t] += add;
+      __syncthreads();
+    }
+    const uint32_t carry = s_carry;
+    if (b < n) data[b] = carry + s[t] - v;
+    __syncthreads();
+    if (t == 1023) s_carry = carry + s[1023];
+    __syncthreads();
+  }
+}
+
+struct DfDupArgs {
+  int P, gx, gy;
+  long long capacity;
+  const float4* splat;
+  const uint32_t* tiles;
+  const uint32_t* dvals;
+  const uint32_t* dscan;
+  const uint32_t* dblocks;
+  uint32_t* counters;
+  uint32_t* keys;  // tile ids
+  uint32_t* vals;
+};
+
+// Duplicate tile ids in depth order (sorted position k -> Gaussian dvals[k]).
+__global__ __launch_bounds__(256) void k_df_duplicate(DfDupArgs a) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t n = a.counters[0];
+  if ((long long)n > a.capacity) {
+    if (k == 0) atomicOr(&a.counters[1], GS_STATUS_CAPACITY);
+    return;
+  }
+  if (k >= a.P) return;
+  const uint32_t g = a.dvals[k];
+  const uint32_t nt = a.tiles[g];
+  if (nt == 0) return;
+  uint32_t off = a.dblocks[k >> 8] + a.dscan[k] - nt;
+  const float4 s0 = a.splat[3 * g];
+  const float4 s2 = a.splat[3 * g + 2];
+  int x0, y0, x1, y1;
+  tile_rect(s0.x, s0.y, __float_as_int(s2.w), a.gx, a.gy, x0, y0, x1, y1);
+  for (int y = y0; y < y1; ++y)
+    for (int x = x0; x < x1; ++x) {
+      a.keys[off] = (uint32_t)(y * a.gx + x);
+      a.vals[off] = g;
+      ++off;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ranges32(const uint32_t* keys, const uint32_t* counters, long long capacity,
+                                                  uint32_t* ranges) {
+  const uint32_t n0 = counters[0];
+  const uint32_t n = ((long long)n0 > capacity) ? 0u : n0;
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t t = keys[k];
+  if (k == 0) {
+    ranges[2 * t] = 0;
+  } else {
+    const uint32_t tp = keys[k - 1];
+    if (t != tp) {
+      ranges[2 * tp + 1] = k;
+      ranges[2 * t] = k;
+    }
+  }
+  if (k == n - 1) ranges[2 * t + 1] = n;
+}
+
 struct BlendArgs {
   int W, H, gx;
   const uint32_t* ranges;
@@ -346,12 +454,20 @@ struct BlendArgs {
   float* out_alpha;
   float* final_T;
   uint32_t* n_contrib;
+  // backward support: per-pixel state at every 256-entry batch boundary of
+  // the tile's list ([ckpt][T, C0, C1, C2, D][256]), the tile's largest
+  // contributor count and the pre-background sums ([C0, C1, C2, D][H*W])
+  const uint32_t* ckpt_base;
+  float* ckpt;
+  uint32_t* tile_maxc;
+  float* accum;
 };
 
 __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
   __shared__ float4 s_a[256];  // x, y, conic a, conic b
   __shared__ float4 s_b[256];  // conic c, opacity, depth, -
   __shared__ float4 s_c[256];  // r, g, b, -
+  __shared__ uint32_t s_maxc;
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.gx, ty = tile / a.gx;
   const int px = tx * kBX + (threadIdx.x & 15), py = ty * kBY + (threadIdx.x >> 4);
@@ -362,7 +478,17 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
   uint32_t contributor = 0, last = 0;
   const float a_min = 1.0f / 255.0f;
-  for (uint32_t base = r0; base < r1; base += 256) {
+  if (threadIdx.x == 0) s_maxc = 0;
+  uint32_t batch = 0;
+  for (uint32_t base = r0; base < r1; base += 256, ++batch) {
+    if (batch > 0 && inside) {
+      float* ck = a.ckpt + (size_t)(a.ckpt_base[tile] + batch - 1) * 5 * 256 + threadIdx.x;
+      ck[0] = T;
+      ck[256] = C0;
+      ck[512] = C1;
+      ck[768] = C2;
+      ck[1024] = D;
+    }
     if (__syncthreads_count(done) == 256) break;
     const uint32_t k = base + threadIdx.x;
     if (k < r1) {
@@ -407,7 +533,47 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
     a.out_color[2 * HW + pix] = fmaf(T, a.bg[2], C2);
     a.out_depth[pix] = D;
     if (a.out_alpha) a.out_alpha[pix] = 1.0f - T;
+    a.accum[pix] = C0;
+    a.accum[HW + pix] = C1;
+    a.accum[2 * HW + pix] = C2;
+    a.accum[3 * HW + pix] = D;
+    if (last) atomicMax(&s_maxc, last);
   }
+  __syncthreads();
+  if (threadIdx.x == 0) a.tile_maxc[tile] = s_maxc;
+}
+
+// One workgroup: checkpoint base of every tile = exclusive prefix over tiles
+// of (batches in its list - 1); ckpt_base[n_tiles] = total checkpoints.
+__global__ __launch_bounds__(1024) void k_tile_ckpt_scan(const uint32_t* ranges, int nt, uint32_t* ckpt_base) {
+  __shared__ uint32_t s[1024];
+  __shared__ uint32_t s_carry;
+  const int t = threadIdx.x;
+  if (t == 0) s_carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nt; b0 += 1024) {
+    const int b = b0 + t;
+    uint32_t v = 0;
+    if (b < nt) {
+      const uint32_t len = ranges[2 * b + 1] - ranges[2 * b];
+      const uint32_t nb = (len + 255) / 256;
+      v = nb > 0 ? nb - 1 : 0;
+    }
+    s[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const uint32_t add = t >= off ? s[t - off] : 0u;
+      __syncthreads();
+      s[t] += add;
+      __syncthreads();
+    }
+    const uint32_t carry = s_carry;
+    if (b < nt) ckpt_base[b] = carry + s[t] - v;
+    __syncthreads();
+    if (t == 1023) s_carry = carry + s[1023];
+    __syncthreads();
+  }
+  if (t == 0) ckpt_base[nt] = s_carry;
 }
 
 __global__ __launch_bounds__(256) void k_mark_visible(const float* means, int P, const float* view,
@@ -451,25 +617,46 @@ extern "C" int gs_geom_layout(int P, gs_geom_layout_t* out) {
   return GS_OK;
 }
 
-extern "C" int gs_binning_layout(long long capacity, int width, int height, gs_binning_layout_t* out) {
-  if (capacity < 0 || capacity > 0x7fffffffLL || width <= 0 || height <= 0 || !out) return GS_E_ARG;
+extern "C" int gs_binning_layout(long long capacity, int P, int width, int height, int binning,
+                                 gs_binning_layout_t* out) {
+  if (capacity < 0 || capacity > 0x7fffffffLL || P < 0 || width <= 0 || height <= 0 || !out) return GS_E_ARG;
+  if (binning != GS_BINNING_DEPTH_FIRST && binning != GS_BINNING_SORT64) return GS_E_ARG;
   int gx, gy;
   const int nt = grid_tiles(width, height, gx, gy);
   const size_t c = (size_t)capacity;
+  const bool df = binning == GS_BINNING_DEPTH_FIRST;
+  const size_t kb = df ? 4 : 8;
   size_t o = 0;
   out->keys_dup = o;
-  o = align_up(o + 8 * c, 256);
+  o = align_up(o + kb * c, 256);
   out->keys_alt = o;
-  o = align_up(o + 8 * c, 256);
+  o = align_up(o + kb * c, 256);
   out->vals_dup = o;
   o = align_up(o + 4 * c, 256);
   out->vals_alt = o;
   o = align_up(o + 4 * c, 256);
+  const size_t p = df ? (size_t)P : 0;
+  out->dkeys_a = o;
+  o = align_up(o + 4 * p, 256);
+  out->dvals_a = o;
+  o = align_up(o + 4 * p, 256);
+  out->dkeys_b = o;
+  o = align_up(o + 4 * p, 256);
+  out->dvals_b = o;
+  o = align_up(o + 4 * p, 256);
+  out->dscan = o;
+  o = align_up(o + 4 * p, 256);
+  out->dblocks = o;
+  o = align_up(o + 4 * (p / 256 + 2), 256);
   out->hist = o;
-  o = align_up(o + sort_tmp_bytes(capacity), 256);
+  o = align_up(o + sort_tmp_bytes(capacity > (long long)p ? capacity : (long long)p), 256);
+  // blend checkpoints: at most capacity / 256 batch boundaries, 5 x 256 floats each
+  out->ckpt = o;
+  o = align_up(o + (size_t)(c / 256 + 1) * 5 * 256 * sizeof(float), 256);
   out->total = o;
-  const int end_bit = 32 + bits_for((uint32_t)nt);
-  out->sort_passes = sort_passes(end_bit);
+  const int end_bit = (df ? 0 : 32) + bits_for((uint32_t)nt);
+  out->sort_passes = sort_passes(end_bit > 0 ? end_bit : 1);
+  out->key_bytes = (int)kb;
   const bool in_b = (out->sort_passes & 1) != 0;
   out->keys_sorted = in_b ? out->keys_alt : out->keys_dup;
   out->vals_sorted = in_b ? out->vals_alt : out->vals_dup;
@@ -488,6 +675,14 @@ extern "C" int gs_image_layout(int width, int height, gs_image_layout_t* out) {
   o = align_up(o + 4 * hw, 256);
   out->n_contrib = o;
   o = align_up(o + 4 * hw, 256);
+  out->accum = o;
+  o = align_up(o + 16 * hw, 256);
+  out->ckpt_base = o;
+  o = align_up(o + 4 * ((size_t)nt + 1), 256);
+  out->tile_maxc = o;
+  o = align_up(o + 4 * (size_t)nt, 256);
+  out->item_base = o;
+  o = align_up(o + 4 * ((size_t)nt + 1), 256);
   out->total = o;
   return GS_OK;
 }
@@ -568,7 +763,7 @@ extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t*
   gs_binning_layout_t LB;
   gs_image_layout_t LI;
   gs_geom_layout(P, &LG);
-  gs_binning_layout(capacity, f->width, f->height, &LB);
+  if (gs_binning_layout(capacity, P, f->width, f->height, f->binning, &LB) != GS_OK) return GS_E_ARG;
   gs_image_layout(f->width, f->height, &LI);
   if (binning_bytes < LB.total || image_bytes < LI.total) return GS_E_WORKSPACE;
   if (capacity > 0 && !binning) return GS_E_ARG;
@@ -580,7 +775,53 @@ extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t*
   char* ib = (char*)image;
   uint32_t* ranges = (uint32_t*)(ib + LI.ranges);
   hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nblocks256(2LL * nt), 1024)), dim3(256), 0, s, ranges, 2LL * nt, 0u);
-  if (capacity > 0 && P > 0) {
+  if (capacity > 0 && P > 0 && f->binning == GS_BINNING_DEPTH_FIRST) {
+    const float4* splat = (const float4*)(gb + LG.splat);
+    const uint32_t* tiles = (const uint32_t*)(gb + LG.tiles);
+    uint32_t* dka = (uint32_t*)(bb + LB.dkeys_a);
+    uint32_t* dva = (uint32_t*)(bb + LB.dvals_a);
+    const int nbp = nblocks256(P);
+    hipError_t e;
+    int in_b = 0;
+    {
+      timing::Scope ts(timing::kSort, s);
+      hipLaunchKernelGGL(k_df_keys, dim3(nbp), dim3(256), 0, s, P, tiles, splat, dka, dva, counters);
+      e = sort_pairs_u32(dka, dva, (uint32_t*)(bb + LB.dkeys_b), (uint32_t*)(bb + LB.dvals_b), counters + 2, P, 32,
+                         bb + LB.hist, s, &in_b);
+    }
+    if (e != hipSuccess) return GS_E_LAUNCH;
+    const uint32_t* dvals = (const uint32_t*)(bb + (in_b ? LB.dvals_b : LB.dvals_a));
+    {
+      timing::Scope ts(timing::kDuplicate, s);
+      hipLaunchKernelGGL(k_df_scan, dim3(nbp), dim3(256), 0, s, P, dvals, tiles, (uint32_t*)(bb + LB.dscan),
+                         (uint32_t*)(bb + LB.dblocks));
+      hipLaunchKernelGGL(k_excl_scan_1wg, dim3(1), dim3(1024), 0, s, (uint32_t*)(bb + LB.dblocks), nbp);
+      DfDupArgs d;
+      d.P = P;
+      d.gx = gx;
+      d.gy = gy;
+      d.capacity = capacity;
+      d.splat = splat;
+      d.tiles = tiles;
+      d.dvals = dvals;
+      d.dscan = (const uint32_t*)(bb + LB.dscan);
+      d.dblocks = (const uint32_t*)(bb + LB.dblocks);
+      d.counters = counters;
+      d.keys = (uint32_t*)(bb + LB.keys_dup);
+      d.vals = (uint32_t*)(bb + LB.vals_dup);
+      hipLaunchKernelGGL(k_df_duplicate, dim3(nbp), dim3(256), 0, s, d);
+    }
+    {
+      timing::Scope ts(timing::kSort, s);
+      const int tb = bits_for((uint32_t)nt);
+      e = sort_pairs_u32((uint32_t*)(bb + LB.keys_dup), (uint32_t*)(bb + LB.vals_dup), (uint32_t*)(bb + LB.keys_alt),
+                         (uint32_t*)(bb + LB.vals_alt), counters, capacity, tb > 0 ? tb : 1, bb + LB.hist, s, &in_b);
+    }
+    if (e != hipSuccess) return GS_E_LAUNCH;
+    timing::Scope ts(timing::kRanges, s);
+    hipLaunchKernelGGL(k_ranges32, dim3(nblocks256(capacity)), dim3(256), 0, s,
+                       (const uint32_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, ranges);
+  } else if (capacity > 0 && P > 0) {
     DupArgs d;
     d.P = P;
     d.gx = gx;
@@ -623,6 +864,12 @@ extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t*
   b.out_alpha = out_alpha;
   b.final_T = (float*)(ib + LI.final_T);
   b.n_contrib = (uint32_t*)(ib + LI.n_contrib);
+  b.ckpt_base = (const uint32_t*)(ib + LI.ckpt_base);
+  b.ckpt = (float*)(bb + LB.ckpt);
+  b.tile_maxc = (uint32_t*)(ib + LI.tile_maxc);
+  b.accum = (float*)(ib + LI.accum);
+  hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)ranges, nt,
+                     (uint32_t*)(ib + LI.ckpt_base));
   {
     timing::Scope ts(timing::kBlend, s);
     hipLaunchKernelGGL(k_blend, dim3(nt), dim3(256), 0, s, b);

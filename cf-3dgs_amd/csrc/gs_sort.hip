@@ -1,4 +1,6 @@
-// gs_sort.hip -- stable LSD radix sort of (uint64 key, uint32 value) pairs.
+// gs_sort.hip -- stable LSD radix sort of (key, uint32 value) pairs, with
+// 64-bit keys (tile << 32 | depth bits, the reference's key) or 32-bit keys
+// (depth bits, or tile ids in the depth-first binning).
 //
 // The element count is read on the device (so the binning pipeline never
 // needs the host to know num_rendered); grids are sized by capacity and
@@ -35,9 +37,10 @@ __device__ __forceinline__ uint32_t n_eff(const uint32_t* n_dev, long long cap) 
   return ((long long)n > cap) ? 0u : n;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const uint64_t* __restrict__ keys,
-                                                               const uint32_t* n_dev, long long cap, int shift,
-                                                               int bits, uint32_t* hist, int nb_max) {
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restrict__ keys, const uint32_t* n_dev,
+                                                               long long cap, int shift, int bits, uint32_t* hist,
+                                                               int nb_max) {
   __shared__ uint32_t s_h[4][kRadix];
   const uint32_t n = n_eff(n_dev, cap);
   const uint32_t start = blockIdx.x * (uint32_t)kSortTile;
@@ -93,9 +96,10 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* tot
   if (t == 0) totals[d] = s_carry;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint64_t* __restrict__ keys_in,
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restrict__ keys_in,
                                                                const uint32_t* __restrict__ vals_in,
-                                                               uint64_t* __restrict__ keys_out,
+                                                               K* __restrict__ keys_out,
                                                                uint32_t* __restrict__ vals_out,
                                                                const uint32_t* n_dev, long long cap, int shift,
                                                                int bits, const uint32_t* hist,
@@ -126,14 +130,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint64_t* _
   const int w = t >> 6, lane = t & 63;
   const uint32_t mask = (1u << bits) - 1u;
   const uint32_t wbase = start + (uint32_t)w * (kSortItems * 64);
-  uint64_t key[kSortItems];
+  K key[kSortItems];
   uint32_t val[kSortItems];
   uint32_t pos[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const uint32_t idx = wbase + r * 64 + lane;
     const bool valid = idx < n;
-    key[r] = valid ? keys_in[idx] : 0ull;
+    key[r] = valid ? keys_in[idx] : (K)0;
     val[r] = valid ? vals_in[idx] : 0u;
     const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
     const uint64_t peers = match_digit(d, valid, bits);
@@ -170,28 +174,28 @@ size_t sort_tmp_bytes(long long capacity) {
   return align_up((size_t)kRadix * 4 * (size_t)(nb > 0 ? nb : 1), 256) + kRadix * 4;
 }
 
-hipError_t sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b,
-                          const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
-                          int* result_in_b) {
+template <typename K>
+hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, const uint32_t* n_dev,
+                      long long capacity, int end_bit, void* tmp, hipStream_t s, int* result_in_b) {
   const int passes = sort_passes(end_bit);
   *result_in_b = passes & 1;
   if (capacity <= 0) return hipSuccess;
   const int nb_max = (int)((capacity + kSortTile - 1) / kSortTile);
   uint32_t* hist = (uint32_t*)tmp;
   uint32_t* totals = hist + align_up((size_t)kRadix * (size_t)nb_max, 64);
-  uint64_t* kin = keys_a;
+  K* kin = keys_a;
   uint32_t* vin = vals_a;
-  uint64_t* kout = keys_b;
+  K* kout = keys_b;
   uint32_t* vout = vals_b;
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
     const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
-    hipLaunchKernelGGL(k_sort_upsweep, dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity, shift, bits,
+    hipLaunchKernelGGL(k_sort_upsweep<K>, dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity, shift, bits,
                        hist, nb_max);
     hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_dev,
+    hipLaunchKernelGGL(k_sort_scatter<K>, dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_dev,
                        capacity, shift, bits, hist, totals, nb_max);
-    uint64_t* tk = kin;
+    K* tk = kin;
     kin = kout;
     kout = tk;
     uint32_t* tv = vin;
@@ -199,6 +203,18 @@ hipError_t sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, 
     vout = tv;
   }
   return hipGetLastError();
+}
+
+hipError_t sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b,
+                          const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
+                          int* result_in_b) {
+  return sort_pairs<uint64_t>(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, end_bit, tmp, s, result_in_b);
+}
+
+hipError_t sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, uint32_t* vals_b,
+                          const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
+                          int* result_in_b) {
+  return sort_pairs<uint32_t>(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, end_bit, tmp, s, result_in_b);
 }
 
 }  // namespace gs
@@ -213,6 +229,18 @@ extern "C" int gs_sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* k
   if (capacity > 0 && (!keys_a || !vals_a || !keys_b || !vals_b || !tmp)) return GS_E_ARG;
   if (tmp_bytes < gs::sort_tmp_bytes(capacity)) return GS_E_WORKSPACE;
   const hipError_t e = gs::sort_pairs_u64(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, end_bit, tmp,
+                                          (hipStream_t)stream, result_in_b);
+  return e == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, uint32_t* vals_b,
+                                 const uint32_t* n_dev, long long capacity, int end_bit, void* tmp,
+                                 size_t tmp_bytes, int* result_in_b, void* stream) {
+  if (capacity < 0 || capacity > 0x7fffffffLL || end_bit < 1 || end_bit > 32 || !n_dev || !result_in_b)
+    return GS_E_ARG;
+  if (capacity > 0 && (!keys_a || !vals_a || !keys_b || !vals_b || !tmp)) return GS_E_ARG;
+  if (tmp_bytes < gs::sort_tmp_bytes(capacity)) return GS_E_WORKSPACE;
+  const hipError_t e = gs::sort_pairs_u32(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, end_bit, tmp,
                                           (hipStream_t)stream, result_in_b);
   return e == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }

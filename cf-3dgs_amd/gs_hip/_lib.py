@@ -29,8 +29,12 @@ class GsFrame(ctypes.Structure):
         ("width", c_int), ("height", c_int), ("tanfovx", c_float), ("tanfovy", c_float),
         ("scale_modifier", c_float), ("sh_degree", c_int), ("prefiltered", c_int),
         ("viewmatrix", c_void_p), ("projmatrix", c_void_p), ("campos", c_void_p), ("bg", c_void_p),
-        ("pose", c_void_p),
+        ("pose", c_void_p), ("binning", c_int),
     ]
+
+
+BINNING_DEPTH_FIRST = 0
+BINNING_SORT64 = 1
 
 
 class GsGaussians(ctypes.Structure):
@@ -56,16 +60,19 @@ class GsGeomLayout(ctypes.Structure):
 class GsBinningLayout(ctypes.Structure):
     _fields_ = [("keys_dup", c_size_t), ("vals_dup", c_size_t), ("keys_alt", c_size_t), ("vals_alt", c_size_t),
                 ("keys_sorted", c_size_t), ("vals_sorted", c_size_t), ("hist", c_size_t), ("sort_passes", c_int),
+                ("key_bytes", c_int), ("dkeys_a", c_size_t), ("dvals_a", c_size_t), ("dkeys_b", c_size_t),
+                ("dvals_b", c_size_t), ("dscan", c_size_t), ("dblocks", c_size_t), ("ckpt", c_size_t),
                 ("total", c_size_t)]
 
 
 class GsImageLayout(ctypes.Structure):
-    _fields_ = [("ranges", c_size_t), ("final_T", c_size_t), ("n_contrib", c_size_t), ("total", c_size_t)]
+    _fields_ = [("ranges", c_size_t), ("final_T", c_size_t), ("n_contrib", c_size_t), ("accum", c_size_t),
+                ("ckpt_base", c_size_t), ("tile_maxc", c_size_t), ("item_base", c_size_t), ("total", c_size_t)]
 
 
 EXPORTS = {
     "gs_geom_layout": (c_int, [c_int, ctypes.POINTER(GsGeomLayout)]),
-    "gs_binning_layout": (c_int, [c_longlong, c_int, c_int, ctypes.POINTER(GsBinningLayout)]),
+    "gs_binning_layout": (c_int, [c_longlong, c_int, c_int, c_int, c_int, ctypes.POINTER(GsBinningLayout)]),
     "gs_image_layout": (c_int, [c_int, c_int, ctypes.POINTER(GsImageLayout)]),
     "gs_backward_scratch_bytes": (c_size_t, [c_int]),
     "gs_preprocess": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p, c_size_t,
@@ -79,6 +86,8 @@ EXPORTS = {
     "gs_sort_tmp_bytes": (c_size_t, [c_longlong]),
     "gs_sort_pairs_u64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_longlong, c_int, c_void_p,
                                   c_size_t, ctypes.POINTER(c_int), c_void_p]),
+    "gs_sort_pairs_u32": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_longlong, c_int, c_void_p,
+                                  c_size_t, ctypes.POINTER(c_int), c_void_p]),
     "gs_knn_workspace_bytes": (c_size_t, [c_int]),
     "gs_knn3_meandist": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "gs_loss_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
@@ -86,6 +95,9 @@ EXPORTS = {
                                    c_void_p, c_void_p, c_size_t, c_void_p]),
     "gs_ssim_l1_backward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_float), c_float,
                                     c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
+    "gs_adam_step": (c_int, [c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                             ctypes.POINTER(c_void_p), ctypes.POINTER(c_longlong), ctypes.POINTER(c_float), c_float,
+                             c_float, c_float, c_int, c_void_p]),
     "gs_timing_enable": (c_int, [c_int]),
     "gs_timing_reset": (c_int, []),
     "gs_timing_read": (c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int)]),
@@ -129,9 +141,9 @@ def geom_layout(P: int) -> GsGeomLayout:
     return L
 
 
-def binning_layout(capacity: int, W: int, H: int) -> GsBinningLayout:
+def binning_layout(capacity: int, P: int, W: int, H: int, binning: int = BINNING_DEPTH_FIRST) -> GsBinningLayout:
     L = GsBinningLayout()
-    check(load().gs_binning_layout(capacity, W, H, ctypes.byref(L)), "gs_binning_layout")
+    check(load().gs_binning_layout(capacity, P, W, H, binning, ctypes.byref(L)), "gs_binning_layout")
     return L
 
 

@@ -67,9 +67,10 @@ class GaussianParams:
             {"params": [self._scaling], "lr": scaling_lr, "name": "scaling"},
             {"params": [self._rotation], "lr": rotation_lr, "name": "rotation"},
         ]
-        try:
-            self.optimizer = torch.optim.Adam(groups, lr=0.0, eps=1e-15, fused=fused)
-        except (RuntimeError, TypeError):
+        if fused:
+            from .optim import FusedAdam
+            self.optimizer = FusedAdam(groups, lr=0.0, eps=1e-15)
+        else:
             self.optimizer = torch.optim.Adam(groups, lr=0.0, eps=1e-15)
         return self.optimizer
 

@@ -1,0 +1,56 @@
+"""Adam for the Gaussian parameters in one HIP launch per step.
+
+Same update and defaults as the torch.optim.Adam that CFGaussianModel builds
+(scene/gaussian_model_cf.py:256-268: per-group learning rates, betas
+(0.9, 0.999), eps 1e-15), with the same ``param_groups`` / ``zero_grad``
+surface so learning-rate schedules that edit ``param_groups`` keep working.
+Parameters without a gradient are skipped, as torch does.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+class FusedAdam(torch.optim.Optimizer):
+    def __init__(self, params, lr=0.0, betas=(0.9, 0.999), eps=1e-15):
+        super().__init__(params, dict(lr=lr, betas=betas, eps=eps))
+        self._steps = 0
+
+    @torch.no_grad()
+    def step(self, closure=None):
+        loss = closure() if closure is not None else None
+        lib = _lib.load()
+        by_hyper = {}
+        for group in self.param_groups:
+            b1, b2 = group["betas"]
+            for p in group["params"]:
+                if p.grad is None:
+                    continue
+                if p.device.type != "cuda" or p.dtype != torch.float32 or not p.is_contiguous():
+                    raise RuntimeError("FusedAdam needs contiguous float32 parameters on the GPU")
+                st = self.state[p]
+                if not st:
+                    st["step"] = 0
+                    st["exp_avg"] = torch.zeros_like(p)
+                    st["exp_avg_sq"] = torch.zeros_like(p)
+                st["step"] += 1
+                g = p.grad if p.grad.is_contiguous() else p.grad.contiguous()
+                key = (float(b1), float(b2), float(group["eps"]), int(st["step"]), p.device)
+                by_hyper.setdefault(key, []).append((p, g, st["exp_avg"], st["exp_avg_sq"], float(group["lr"])))
+        for (b1, b2, eps, step, dev), items in by_hyper.items():
+            for i in range(0, len(items), 16):
+                chunk = items[i:i + 16]
+                n = len(chunk)
+                P = (ctypes.c_void_p * n)(*[c[0].data_ptr() for c in chunk])
+                G = (ctypes.c_void_p * n)(*[c[1].data_ptr() for c in chunk])
+                M = (ctypes.c_void_p * n)(*[c[2].data_ptr() for c in chunk])
+                V = (ctypes.c_void_p * n)(*[c[3].data_ptr() for c in chunk])
+                N = (ctypes.c_longlong * n)(*[c[0].numel() for c in chunk])
+                LR = (ctypes.c_float * n)(*[c[4] for c in chunk])
+                stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+                _lib.check(lib.gs_adam_step(n, P, G, M, V, N, LR, b1, b2, eps, step, stream), "gs_adam_step")
+        return loss

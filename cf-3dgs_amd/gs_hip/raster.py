@@ -9,12 +9,18 @@ stream.
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass
 from typing import Optional
 
 import torch
 
 from . import _lib
+
+# Binning schedule for every forward in this process: depth-first (default) or
+# the reference's 64-bit key sort; both give identical sorted lists and ranges.
+BINNING = (_lib.BINNING_SORT64 if os.environ.get("GS_BINNING", "").lower() == "sort64"
+           else _lib.BINNING_DEPTH_FIRST)
 
 
 def _dev_f32(t: Optional[torch.Tensor], device: torch.device, name: str) -> Optional[torch.Tensor]:
@@ -63,7 +69,8 @@ class FrameArgs:
     pose: Optional[torch.Tensor]
 
 
-def make_frame(settings, device: torch.device, pose: Optional[torch.Tensor] = None) -> FrameArgs:
+def make_frame(settings, device: torch.device, pose: Optional[torch.Tensor] = None,
+               binning: Optional[int] = None) -> FrameArgs:
     view = _dev_f32(settings.viewmatrix, device, "viewmatrix")
     proj = _dev_f32(settings.projmatrix, device, "projmatrix")
     campos = _dev_f32(settings.campos, device, "campos")
@@ -88,6 +95,7 @@ def make_frame(settings, device: torch.device, pose: Optional[torch.Tensor] = No
     f.campos = campos.data_ptr() if campos is not None else None
     f.bg = bg.data_ptr()
     f.pose = pose_t.data_ptr() if pose_t is not None else None
+    f.binning = BINNING if binning is None else int(binning)
     return FrameArgs(f, view, proj, campos, bg, pose_t)
 
 
@@ -150,14 +158,15 @@ class ForwardState:
     radii: torch.Tensor
     capacity: int
     num_rendered: int
+    binning_mode: int = 0
 
 
 def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp,
-            pose=None, want_alpha: bool = True):
+            pose=None, want_alpha: bool = True, binning: Optional[int] = None):
     """Returns (color [3,H,W], radii [P] int32, depth [1,H,W], alpha [1,H,W] or None, state)."""
     lib = _lib.load()
     device = require_device(means3D)
-    fa = make_frame(settings, device, pose)
+    fa = make_frame(settings, device, pose, binning)
     ga = make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp, device)
     P = ga.g.P
     W, H = fa.frame.width, fa.frame.height
@@ -174,7 +183,7 @@ def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations
     if status & _lib.GS_STATUS_PREFILTERED:
         raise RuntimeError("a point was culled by the near plane although prefiltered=True")
     capacity = num_rendered
-    LB = _lib.binning_layout(capacity, W, H)
+    LB = _lib.binning_layout(capacity, P, W, H, fa.frame.binning)
     LI = _lib.image_layout(W, H)
     binning = torch.empty(max(LB.total, 256), dtype=torch.uint8, device=device)
     image = torch.empty(LI.total, dtype=torch.uint8, device=device)
@@ -184,7 +193,7 @@ def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations
     _lib.check(lib.gs_render(ctypes.byref(fa.frame), P, _ptr(geom), _ptr(counters), capacity, _ptr(binning),
                              binning.numel(), _ptr(image), image.numel(), _ptr(color), _ptr(depth), _ptr(alpha),
                              stream), "gs_render")
-    st = ForwardState(geom, binning, image, counters, radii, capacity, num_rendered)
+    st = ForwardState(geom, binning, image, counters, radii, capacity, num_rendered, fa.frame.binning)
     return color, radii, depth, alpha, st
 
 
@@ -193,7 +202,7 @@ def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities
     """Returns a dict of gradients (None where the input was absent)."""
     lib = _lib.load()
     device = require_device(means3D)
-    fa = make_frame(settings, device, pose)
+    fa = make_frame(settings, device, pose, st.binning_mode)
     ga = make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp, device)
     P = ga.g.P
     M = ga.g.M
@@ -282,7 +291,7 @@ def sort_pairs_u64(keys: torch.Tensor, vals: torch.Tensor, end_bit: int):
 def views(st: ForwardState, P: int, W: int, H: int) -> dict:
     """Typed views of the internal buffers (keys, sorted ids, ranges, ...) for parity checks."""
     LG = _lib.geom_layout(P)
-    LB = _lib.binning_layout(st.capacity, W, H)
+    LB = _lib.binning_layout(st.capacity, P, W, H, st.binning_mode)
     LI = _lib.image_layout(W, H)
     gx, gy = (W + 15) // 16, (H + 15) // 16
     n = st.num_rendered if st.num_rendered <= st.capacity else 0
@@ -291,6 +300,14 @@ def views(st: ForwardState, P: int, W: int, H: int) -> dict:
         return buf[off:off + nbytes].view(dtype)
 
     splat = sl(st.geom, LG.splat, 48 * P, torch.float32).view(P, 3, 4)
+    vals = sl(st.binning, LB.vals_sorted, 4 * n, torch.int32)
+    if LB.key_bytes == 8:
+        keys = sl(st.binning, LB.keys_sorted, 8 * n, torch.int64)
+    else:
+        # (tile << 32 | depth bits) rebuilt from the sorted tile ids and the ids' depths
+        tiles_sorted = sl(st.binning, LB.keys_sorted, 4 * n, torch.int32).to(torch.int64)
+        dbits = splat[:, 1, 2].contiguous().view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+        keys = (tiles_sorted << 32) | dbits[vals.to(torch.int64)] if n else torch.zeros(0, dtype=torch.int64)
     return dict(
         xy=splat[:, 0, 0:2],
         conic=splat[:, 0, 2:4],
@@ -301,10 +318,8 @@ def views(st: ForwardState, P: int, W: int, H: int) -> dict:
         rgb=splat[:, 2, 0:3],
         cov3D=sl(st.geom, LG.cov3D, 24 * P, torch.float32).view(P, 6),
         tiles=sl(st.geom, LG.tiles, 4 * P, torch.int32),
-        keys_dup=sl(st.binning, LB.keys_dup, 8 * n, torch.int64),
-        vals_dup=sl(st.binning, LB.vals_dup, 4 * n, torch.int32),
-        keys=sl(st.binning, LB.keys_sorted, 8 * n, torch.int64),
-        vals=sl(st.binning, LB.vals_sorted, 4 * n, torch.int32),
+        keys=keys,
+        vals=vals,
         ranges=sl(st.image, LI.ranges, 8 * gx * gy, torch.int32).view(gx * gy, 2),
         final_T=sl(st.image, LI.final_T, 4 * W * H, torch.float32).view(H, W),
         n_contrib=sl(st.image, LI.n_contrib, 4 * W * H, torch.int32).view(H, W),

@@ -54,7 +54,18 @@ typedef struct gs_frame {
   const float* campos;     /* [3] */
   const float* bg;         /* [3] */
   const float* pose;       /* [7] = tx,ty,tz,qx,qy,qz,qw (lietorch SE3 layout) or NULL */
+  int binning;             /* GS_BINNING_DEPTH_FIRST (default) or GS_BINNING_SORT64 */
 } gs_frame_t;
+
+/* Two binning schedules with identical results (same sorted Gaussian ids and
+ * tile ranges):
+ *  SORT64       the reference's structure: duplicate (tile << 32 | depth bits)
+ *               keys in Gaussian order, stable radix sort on 32 + bits(T) bits;
+ *  DEPTH_FIRST  stable sort of the visible Gaussians by depth bits, duplicate
+ *               tile ids in that order, stable sort on bits(T) bits -- equal
+ *               (tile, depth) keys keep Gaussian order in both. */
+#define GS_BINNING_DEPTH_FIRST 0
+#define GS_BINNING_SORT64 1
 
 /* The Gaussians: exactly one of shs / colors_precomp and exactly one of
  * (scales, rotations) / cov3D_precomp, as GaussianRasterizer.forward demands. */
@@ -94,11 +105,15 @@ typedef struct gs_geom_layout {
 } gs_geom_layout_t;
 
 typedef struct gs_binning_layout {
-  size_t keys_dup, vals_dup;   /* uint64/uint32 [capacity]: duplicated (tile|depth) keys, Gaussian ids */
+  size_t keys_dup, vals_dup;   /* [capacity] duplicated keys (uint64 tile|depth, or uint32 tile), Gaussian ids */
   size_t keys_alt, vals_alt;   /* ping-pong buffers of the radix sort */
   size_t keys_sorted, vals_sorted; /* whichever of the above holds the sorted result */
   size_t hist;                 /* sort digit histograms */
-  int sort_passes;             /* 8-bit LSD passes over bits [0, 32 + bits(n_tiles)) */
+  int sort_passes;             /* 8-bit LSD passes of the list sort */
+  int key_bytes;               /* 8 (SORT64) or 4 (DEPTH_FIRST) */
+  size_t dkeys_a, dvals_a, dkeys_b, dvals_b; /* uint32[P] depth sort (DEPTH_FIRST) */
+  size_t dscan, dblocks;       /* uint32[P] tiles prefix in depth order, per-256 block sums */
+  size_t ckpt;                 /* float[capacity/256 + 1][5][256] blend state at list batch boundaries */
   size_t total;
 } gs_binning_layout_t;
 
@@ -106,11 +121,15 @@ typedef struct gs_image_layout {
   size_t ranges;    /* uint32[2*n_tiles] [start, end) into the sorted list */
   size_t final_T;   /* float[H*W] */
   size_t n_contrib; /* uint32[H*W] index (1-based) of the last contributor */
+  size_t accum;     /* float[4][H*W] colour sums before the background term, depth */
+  size_t ckpt_base; /* uint32[n_tiles+1] first checkpoint of each tile */
+  size_t tile_maxc; /* uint32[n_tiles] largest contributor index in the tile */
+  size_t item_base; /* uint32[n_tiles+1] backward work items (256-entry chunks) per tile, prefix */
   size_t total;
 } gs_image_layout_t;
 
 int gs_geom_layout(int P, gs_geom_layout_t* out);
-int gs_binning_layout(long long capacity, int width, int height, gs_binning_layout_t* out);
+int gs_binning_layout(long long capacity, int P, int width, int height, int binning, gs_binning_layout_t* out);
 int gs_image_layout(int width, int height, gs_image_layout_t* out);
 size_t gs_backward_scratch_bytes(int P);
 
@@ -132,9 +151,11 @@ int gs_render(const gs_frame_t* frame, int P, const void* geom, uint32_t* counte
 
 /* Backward (replaces diff_gaussian_rasterization._C.rasterize_gaussians_backward,
  * triggered by loss.backward() at trainer/cf3dgs_trainer.py:133).  dL_ddepth and
- * dL_dalpha may be NULL (treated as zero).  scratch: gs_backward_scratch_bytes(P). */
+ * dL_dalpha may be NULL (treated as zero).  scratch: gs_backward_scratch_bytes(P).
+ * The image workspace is the forward's; the backward rewrites its work-item
+ * table (item_base) and reads everything else. */
 int gs_backward(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
-                const void* binning, long long capacity, const void* image, const uint32_t* counters,
+                const void* binning, long long capacity, void* image, const uint32_t* counters,
                 const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha, const gs_grads_t* out,
                 void* scratch, size_t scratch_bytes, void* stream);
 
@@ -149,6 +170,9 @@ int gs_mark_visible(const float* means3D, int P, const float* viewmatrix, const 
  * end_bit).  tmp: gs_sort_tmp_bytes(capacity). */
 size_t gs_sort_tmp_bytes(long long capacity);
 int gs_sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b,
+                      const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, size_t tmp_bytes,
+                      int* result_in_b, void* stream);
+int gs_sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, uint32_t* vals_b,
                       const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, size_t tmp_bytes,
                       int* result_in_b, void* stream);
 
@@ -172,6 +196,16 @@ int gs_ssim_l1_forward(const float* pred, const float* target, int C, int H, int
 int gs_ssim_l1_backward(const float* pred, const float* target, int C, int H, int W, const float* window11,
                         float lambda_dssim, const float* grad_out, const void* ws, size_t ws_bytes, float* dpred,
                         void* stream);
+
+/* Adam over up to 16 float32 tensors in one launch (replaces
+ * torch.optim.Adam(..., eps=1e-15).step() of training_setup,
+ * scene/gaussian_model_cf.py:256-268; the step loop calls it at
+ * trainer/cf3dgs_trainer.py:163-165).  Arrays are HOST arrays of device
+ * pointers (16-byte aligned), element counts and per-tensor learning rates;
+ * `step` is the 1-based step count used for the bias corrections. */
+int gs_adam_step(int ntensors, float* const* params, const float* const* grads, float* const* exp_avg,
+                 float* const* exp_avg_sq, const long long* numel, const float* lr, float beta1, float beta2,
+                 float eps, int step, void* stream);
 
 /* Opt-in per-stage timing with HIP events recorded on the launch stream (no
  * reference counterpart; the reference's debug mode only synchronises).

@@ -11,7 +11,7 @@ from oracle import oracle as O
 DEV = torch.device("cuda:0")
 
 
-def run_gpu(sc: dict, sh_degree: int, pose=None, colors=None, cov3D=None, with_sh=True):
+def run_gpu(sc: dict, sh_degree: int, pose=None, colors=None, cov3D=None, with_sh=True, binning=None):
     cam = sc["cam"].to(DEV)
     rs = gcam.raster_settings(cam, sc["bg"].to(DEV), sh_degree)
     means = sc["means"].to(DEV)
@@ -22,7 +22,7 @@ def run_gpu(sc: dict, sh_degree: int, pose=None, colors=None, cov3D=None, with_s
     cv = cov3D.to(DEV) if cov3D is not None else None
     po = pose.to(DEV) if pose is not None else None
     color, radii, depth, alpha, st = raster.forward(rs, means, shs, col, sc["opacities"].to(DEV), sc_, ro, cv,
-                                                    pose=po)
+                                                    pose=po, binning=binning)
     torch.cuda.synchronize()
     P = means.shape[0]
     v = raster.views(st, P, cam.image_width, cam.image_height)

@@ -31,15 +31,20 @@ def test_library_exports_every_declared_symbol():
 
 def test_layouts():
     from gs_hip import _lib
-    L = _lib.binning_layout(1000, 960, 540)
-    assert L.sort_passes == 6  # 2040 tiles -> 11 bits -> 43-bit keys
-    assert _lib.binning_layout(10, 256, 256).sort_passes == 5  # 256 tiles -> 8 bits
+    L = _lib.binning_layout(1000, 500, 960, 540, _lib.BINNING_SORT64)
+    assert L.sort_passes == 6 and L.key_bytes == 8  # 2040 tiles -> 11 bits -> 43-bit keys
+    assert _lib.binning_layout(10, 5, 256, 256, _lib.BINNING_SORT64).sort_passes == 5  # 256 tiles -> 8 bits
+    D = _lib.binning_layout(1000, 500, 960, 540, _lib.BINNING_DEPTH_FIRST)
+    assert D.sort_passes == 2 and D.key_bytes == 4  # tile ids only: 11 bits
+    assert D.dscan > D.dvals_b > D.dkeys_b > D.dvals_a > D.dkeys_a >= D.vals_alt
     G = _lib.geom_layout(1000)
     assert G.total >= 1000 * (48 + 24 + 8) and G.splat % 256 == 0 and G.cov3D % 256 == 0
     I = _lib.image_layout(250, 131)
     assert I.total >= 8 * 16 * 9 + 8 * 250 * 131
     with pytest.raises(_lib.GsError):
-        _lib.binning_layout(-1, 10, 10)
+        _lib.binning_layout(-1, 10, 10, 10)
+    with pytest.raises(_lib.GsError):
+        _lib.binning_layout(10, 10, 10, 10, 7)
 
 
 def test_c_abi_rejects_bad_arguments_without_device():

@@ -21,8 +21,8 @@ pytestmark = pytest.mark.gpu
 IMG_TOL = 1e-5
 
 
-def _compare(sc, sh_degree, **kw):
-    g = run_gpu(sc, sh_degree, **kw)
+def _compare(sc, sh_degree, binning=None, **kw):
+    g = run_gpu(sc, sh_degree, binning=binning, **kw)
     _, _, o = run_oracle(sc, sh_degree, **kw)
     v = g["views"]
     radii = np_(g["radii"])
@@ -55,11 +55,25 @@ def _compare(sc, sh_degree, **kw):
     return g, o, n
 
 
-def test_plain_scene_forward_bit_exact():
+@pytest.mark.parametrize("binning", [0, 1], ids=["depth_first", "sort64"])
+def test_plain_scene_forward_bit_exact(binning):
     sc = scenes.plain_scene(seed=0)
-    g, o, n = _compare(sc, 3)
+    g, o, n = _compare(sc, 3, binning=binning)
     assert n > 10000
     assert float(o["alpha"].mean()) > 0.1
+
+
+@pytest.mark.parametrize("binning", [0, 1], ids=["depth_first", "sort64"])
+def test_training_scene_binning_bit_exact(binning):
+    """The 500k-Gaussian 960x540 scene: sorted ids, ranges, contributor counts
+    and the images of both binning schedules equal the oracle's."""
+    import torch.nn.functional as F
+    raw = scenes.training_scene(seed=0)
+    sc = dict(means=raw["xyz"], scales=torch.exp(raw["scaling"]), rotations=F.normalize(raw["rotation"]),
+              opacities=torch.sigmoid(raw["opacity"]), shs=torch.cat([raw["features_dc"], raw["features_rest"]], 1),
+              cam=raw["cam"], bg=raw["bg"])
+    g, o, n = _compare(sc, 3, binning=binning)
+    assert n > 5_000_000
 
 
 @pytest.mark.parametrize("deg", [0, 1, 2])
