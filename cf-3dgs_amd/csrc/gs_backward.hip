@@ -127,7 +127,9 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
       const uint32_t g = a.list[r0 + q];
       s_id[threadIdx.x] = g;
       s_a[threadIdx.x] = a.splat[3 * g];
-      s_b[threadIdx.x] = a.splat[3 * g + 1];
+      float4 b = a.splat[3 * g + 1];
+      b.w = skip_threshold(b.y);
+      s_b[threadIdx.x] = b;
       s_c[threadIdx.x] = a.splat[3 * g + 2];
     }
 #pragma unroll
@@ -176,7 +178,7 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
       const float dx = ga.x - pfx, dy = ga.y - pfy;
       const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
       float G = 0.f, alpha = 0.f;
-      if (power > 0.0f) {
+      if (power > 0.0f || power < gb.w) {  // gb.w: alpha < 1/255 for sure below it
         active = false;
       } else {
         G = spec_expf(power);

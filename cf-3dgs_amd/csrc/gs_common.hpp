@@ -46,6 +46,17 @@ __device__ __forceinline__ float blend_power(float a, float b, float c, float dx
   return fmaf(-0.5f, q, -((b * dx) * dy));
 }
 
+// Exponent below which o * exp(power) < 1/255 for certain: ln(1/(255 o)) less
+// a 1e-3 margin that covers the rounding of logf, of spec_expf (a few ulp)
+// and of the product.  A Gaussian skipped by `power < threshold` is one the
+// exact test (alpha < 1/255) skips too, so results are unchanged; the test
+// only spares the exponential.  o = 0 gives +inf (always skipped, as the
+// exact test does); o < 0 or NaN give NaN (never skipped here, left to the
+// exact test).
+__device__ __forceinline__ float skip_threshold(float o) {
+  return logf(1.0f / (255.0f * o)) - 1e-3f;
+}
+
 // Saturating float -> int (NaN -> 0).
 __device__ __forceinline__ int sat_i(float f) {
   if (f != f) return 0;

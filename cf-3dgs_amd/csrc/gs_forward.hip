@@ -494,7 +494,9 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
     if (k < r1) {
       const uint32_t g = a.list[k];
       s_a[threadIdx.x] = a.splat[3 * g];
-      s_b[threadIdx.x] = a.splat[3 * g + 1];
+      float4 b = a.splat[3 * g + 1];
+      b.w = skip_threshold(b.y);  // (the clamp bits in .w are not needed here)
+      s_b[threadIdx.x] = b;
       s_c[threadIdx.x] = a.splat[3 * g + 2];
     }
     __syncthreads();
@@ -505,7 +507,7 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
       const float4 gb = s_b[j];
       const float dx = ga.x - pfx, dy = ga.y - pfy;
       const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
-      if (power > 0.0f) continue;
+      if (power > 0.0f || power < gb.w) continue;  // gb.w: alpha < 1/255 for sure below it
       const float alpha = fminf(0.99f, gb.y * spec_expf(power));
       if (alpha < a_min) continue;
       const float test_T = T * (1.0f - alpha);
@@ -649,7 +651,11 @@ extern "C" int gs_binning_layout(long long capacity, int P, int width, int heigh
   out->dblocks = o;
   o = align_up(o + 4 * (p / 256 + 2), 256);
   out->hist = o;
-  o = align_up(o + sort_tmp_bytes(capacity > (long long)p ? capacity : (long long)p), 256);
+  {
+    // histogram space for both sorts that share it (their tile sizes may differ)
+    const size_t t_list = sort_tmp_bytes(capacity), t_depth = sort_tmp_bytes((long long)p);
+    o = align_up(o + (t_list > t_depth ? t_list : t_depth), 256);
+  }
   // blend checkpoints: at most capacity / 256 batch boundaries, 5 x 256 floats each
   out->ckpt = o;
   o = align_up(o + (size_t)(c / 256 + 1) * 5 * 256 * sizeof(float), 256);

@@ -7,7 +7,8 @@
 // workgroups past the count exit at once.
 //
 // Per 8-bit digit pass:
-//   k_sort_upsweep  4096 keys per workgroup (4 waves x 16 rounds x 64 lanes);
+//   k_sort_upsweep  1024 or 4096 keys per workgroup (4 waves x 4 or 16 rounds
+//                   x 64 lanes);
 //                   wave64 ballot "match" groups equal digits so each group
 //                   adds its count once -> per-workgroup digit counts
 //                   (digit-major, so one row scan per digit orders blocks)
@@ -37,21 +38,21 @@ __device__ __forceinline__ uint32_t n_eff(const uint32_t* n_dev, long long cap) 
   return ((long long)n > cap) ? 0u : n;
 }
 
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restrict__ keys, const uint32_t* n_dev,
                                                                long long cap, int shift, int bits, uint32_t* hist,
                                                                int nb_max) {
   __shared__ uint32_t s_h[4][kRadix];
   const uint32_t n = n_eff(n_dev, cap);
-  const uint32_t start = blockIdx.x * (uint32_t)kSortTile;
+  const uint32_t start = blockIdx.x * (uint32_t)(kSortThreads * ITEMS);
   if (start >= n) return;
   for (int k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) (&s_h[0][0])[k] = 0;
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t mask = (1u << bits) - 1u;
-  const uint32_t wbase = start + (uint32_t)w * (kSortItems * 64);
+  const uint32_t wbase = start + (uint32_t)w * (ITEMS * 64);
 #pragma unroll 4
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint32_t idx = wbase + r * 64 + lane;
     const bool valid = idx < n;
     const uint32_t d = valid ? (uint32_t)(keys[idx] >> shift) & mask : 0u;
@@ -66,11 +67,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
 
 // One workgroup per digit: exclusive scan over the active blocks' counts.
 __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* totals, const uint32_t* n_dev,
-                                                   long long cap, int nb_max) {
+                                                   long long cap, int nb_max, int tile) {
   __shared__ uint32_t s[256];
   __shared__ uint32_t s_carry;
   const uint32_t n = n_eff(n_dev, cap);
-  const int nb = (int)((n + kSortTile - 1) / kSortTile);
+  const int nb = (int)((n + tile - 1) / tile);
   const int d = blockIdx.x;
   uint32_t* row = hist + (size_t)d * nb_max;
   const int t = threadIdx.x;
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* tot
   if (t == 0) totals[d] = s_carry;
 }
 
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restrict__ keys_in,
                                                                const uint32_t* __restrict__ vals_in,
                                                                K* __restrict__ keys_out,
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   __shared__ uint32_t s_cnt[4][kRadix];
   __shared__ uint32_t s_scan[kRadix];
   const uint32_t n = n_eff(n_dev, cap);
-  const uint32_t start = blockIdx.x * (uint32_t)kSortTile;
+  const uint32_t start = blockIdx.x * (uint32_t)(kSortThreads * ITEMS);
   if (start >= n) return;
   const int t = threadIdx.x;
   // global base of each digit for this block = exclusive scan of totals + row offset
@@ -129,12 +130,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   __syncthreads();
   const int w = t >> 6, lane = t & 63;
   const uint32_t mask = (1u << bits) - 1u;
-  const uint32_t wbase = start + (uint32_t)w * (kSortItems * 64);
-  K key[kSortItems];
-  uint32_t val[kSortItems];
-  uint32_t pos[kSortItems];
+  const uint32_t wbase = start + (uint32_t)w * (ITEMS * 64);
+  K key[ITEMS];
+  uint32_t val[ITEMS];
+  uint32_t pos[ITEMS];
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint32_t idx = wbase + r * 64 + lane;
     const bool valid = idx < n;
     key[r] = valid ? keys_in[idx] : (K)0;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint32_t idx = wbase + r * 64 + lane;
     if (idx < n) {
       const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
@@ -169,8 +170,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   }
 }
 
+// Keys per workgroup: short sorts use 1024 (4 per thread) so that they still
+// fill the 256 CUs; long ones 4096 (16 per thread) to amortise the scans.
+int sort_tile(long long capacity) { return capacity <= (1LL << 21) ? kSortThreads * 4 : kSortThreads * 16; }
+
 size_t sort_tmp_bytes(long long capacity) {
-  const long long nb = (capacity + kSortTile - 1) / kSortTile;
+  const long long tile = sort_tile(capacity);
+  const long long nb = (capacity + tile - 1) / tile;
   return align_up((size_t)kRadix * 4 * (size_t)(nb > 0 ? nb : 1), 256) + kRadix * 4;
 }
 
@@ -180,7 +186,8 @@ hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
   const int passes = sort_passes(end_bit);
   *result_in_b = passes & 1;
   if (capacity <= 0) return hipSuccess;
-  const int nb_max = (int)((capacity + kSortTile - 1) / kSortTile);
+  const int tile = sort_tile(capacity);
+  const int nb_max = (int)((capacity + tile - 1) / tile);
   uint32_t* hist = (uint32_t*)tmp;
   uint32_t* totals = hist + align_up((size_t)kRadix * (size_t)nb_max, 64);
   K* kin = keys_a;
@@ -190,11 +197,19 @@ hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
     const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
-    hipLaunchKernelGGL(k_sort_upsweep<K>, dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity, shift, bits,
-                       hist, nb_max);
-    hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max);
-    hipLaunchKernelGGL(k_sort_scatter<K>, dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_dev,
-                       capacity, shift, bits, hist, totals, nb_max);
+    if (tile == kSortThreads * 4) {
+      hipLaunchKernelGGL((k_sort_upsweep<K, 4>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity, shift,
+                         bits, hist, nb_max);
+      hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max, tile);
+      hipLaunchKernelGGL((k_sort_scatter<K, 4>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_dev,
+                         capacity, shift, bits, hist, totals, nb_max);
+    } else {
+      hipLaunchKernelGGL((k_sort_upsweep<K, 16>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity,
+                         shift, bits, hist, nb_max);
+      hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max, tile);
+      hipLaunchKernelGGL((k_sort_scatter<K, 16>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout,
+                         n_dev, capacity, shift, bits, hist, totals, nb_max);
+    }
     K* tk = kin;
     kin = kout;
     kout = tk;

@@ -9,6 +9,7 @@ stream.
 from __future__ import annotations
 
 import ctypes
+import functools
 import os
 from dataclasses import dataclass
 from typing import Optional
@@ -147,6 +148,33 @@ def make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, c
     return GaussianArgs(g, (means, sh, col, op, sc, ro, cv))
 
 
+_CAPACITY_QUANTUM = 1 << 16
+_PINNED = {}
+
+
+def _pinned_counters(device: torch.device) -> torch.Tensor:
+    t = _PINNED.get(device)
+    if t is None:
+        t = torch.empty(4, dtype=torch.int32, pin_memory=True)
+        _PINNED[device] = t
+    return t
+
+
+@functools.lru_cache(maxsize=256)
+def _geom_layout(P: int):
+    return _lib.geom_layout(P)
+
+
+@functools.lru_cache(maxsize=64)
+def _image_layout(W: int, H: int):
+    return _lib.image_layout(W, H)
+
+
+@functools.lru_cache(maxsize=256)
+def _binning_layout(capacity: int, P: int, W: int, H: int, mode: int):
+    return _lib.binning_layout(capacity, P, W, H, mode)
+
+
 @dataclass
 class ForwardState:
     """Buffers the backward needs (the reference keeps geomBuffer,
@@ -171,25 +199,30 @@ def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations
     P = ga.g.P
     W, H = fa.frame.width, fa.frame.height
     stream = _stream(device)
-    LG = _lib.geom_layout(P)
+    LG = _geom_layout(P)
+    LI = _image_layout(W, H)
     geom = torch.empty(max(LG.total, 256), dtype=torch.uint8, device=device)
     radii = torch.empty(P, dtype=torch.int32, device=device)
     counters = torch.empty(4, dtype=torch.int32, device=device)
-    _lib.check(lib.gs_preprocess(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(radii), _ptr(geom),
-                                 geom.numel(), _ptr(counters), stream), "gs_preprocess")
-    host = counters[:2].cpu()
-    num_rendered = int(host[0].item()) & 0xFFFFFFFF
-    status = int(host[1].item())
-    if status & _lib.GS_STATUS_PREFILTERED:
-        raise RuntimeError("a point was culled by the near plane although prefiltered=True")
-    capacity = num_rendered
-    LB = _lib.binning_layout(capacity, P, W, H, fa.frame.binning)
-    LI = _lib.image_layout(W, H)
-    binning = torch.empty(max(LB.total, 256), dtype=torch.uint8, device=device)
+    # everything not sized by num_rendered is allocated before the readback
     image = torch.empty(LI.total, dtype=torch.uint8, device=device)
     color = torch.empty((3, H, W), dtype=torch.float32, device=device)
     depth = torch.empty((1, H, W), dtype=torch.float32, device=device)
     alpha = torch.empty((1, H, W), dtype=torch.float32, device=device) if want_alpha else None
+    _lib.check(lib.gs_preprocess(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(radii), _ptr(geom),
+                                 geom.numel(), _ptr(counters), stream), "gs_preprocess")
+    # the one host synchronisation of the forward (the reference's num_rendered readback)
+    host = _pinned_counters(device)
+    host.copy_(counters, non_blocking=True)
+    torch.cuda.current_stream(device).synchronize()
+    num_rendered = int(host[0]) & 0xFFFFFFFF
+    status = int(host[1])
+    if status & _lib.GS_STATUS_PREFILTERED:
+        raise RuntimeError("a point was culled by the near plane although prefiltered=True")
+    # capacity rounded up so the caching allocator can reuse the binning block
+    capacity = min(-(-num_rendered // _CAPACITY_QUANTUM) * _CAPACITY_QUANTUM, 0x7FFFFFFF)
+    LB = _binning_layout(capacity, P, W, H, fa.frame.binning)
+    binning = torch.empty(max(LB.total, 256), dtype=torch.uint8, device=device)
     _lib.check(lib.gs_render(ctypes.byref(fa.frame), P, _ptr(geom), _ptr(counters), capacity, _ptr(binning),
                              binning.numel(), _ptr(image), image.numel(), _ptr(color), _ptr(depth), _ptr(alpha),
                              stream), "gs_render")
