@@ -75,9 +75,174 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// SE(3) tangent parameters (lietorch LieGroupParameter semantics): the pose is
+// Exp(xi) * X0 with xi = [tau, phi].  The Jacobian d pose / d xi is evaluated
+// by forward-mode dual numbers through the closed form, one pass per tangent
+// direction, in double precision; near phi = 0 the trigonometric ratios use
+// their Taylor series in th^2 so the derivative stays finite at the identity.
+struct Dual {
+  double v, d;
+};
+__device__ inline Dual mk(double v) { return {v, 0.0}; }
+__device__ inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ inline Dual operator*(double s, Dual a) { return {s * a.v, s * a.d}; }
+__device__ inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+__device__ inline Dual dsin(Dual a) { return {sin(a.v), cos(a.v) * a.d}; }
+__device__ inline Dual dcos(Dual a) { return {cos(a.v), -sin(a.v) * a.d}; }
+__device__ inline Dual dsqrt(Dual a) {
+  const double s = sqrt(a.v);
+  return {s, a.d / (2.0 * s)};
+}
+
+// out[7] = Exp(xi) * X0
+__device__ void se3_exp_compose(const Dual xi[6], const double X0[7], Dual out[7]) {
+  const Dual tau[3] = {xi[0], xi[1], xi[2]};
+  const Dual phi[3] = {xi[3], xi[4], xi[5]};
+  const Dual th2 = phi[0] * phi[0] + phi[1] * phi[1] + phi[2] * phi[2];
+  Dual qa_s, qa_w, c1, c2;  // sin(th/2)/th, cos(th/2), (1-cos th)/th^2, (th-sin th)/th^3
+  if (th2.v < 1e-6) {
+    const Dual th4 = th2 * th2;
+    qa_s = mk(0.5) - (1.0 / 48.0) * th2 + (1.0 / 3840.0) * th4;
+    qa_w = mk(1.0) - (1.0 / 8.0) * th2 + (1.0 / 384.0) * th4;
+    c1 = mk(0.5) - (1.0 / 24.0) * th2 + (1.0 / 720.0) * th4;
+    c2 = mk(1.0 / 6.0) - (1.0 / 120.0) * th2 + (1.0 / 5040.0) * th4;
+  } else {
+    const Dual th = dsqrt(th2);
+    const Dual half = 0.5 * th;
+    qa_s = dsin(half) / th;
+    qa_w = dcos(half);
+    c1 = (mk(1.0) - dcos(th)) / th2;
+    c2 = (th - dsin(th)) / (th2 * th);
+  }
+  const Dual qa[3] = {qa_s * phi[0], qa_s * phi[1], qa_s * phi[2]};
+  // t_a = tau + c1 (phi x tau) + c2 (phi x (phi x tau))
+  const Dual k1[3] = {phi[1] * tau[2] - phi[2] * tau[1], phi[2] * tau[0] - phi[0] * tau[2],
+                      phi[0] * tau[1] - phi[1] * tau[0]};
+  const Dual k2[3] = {phi[1] * k1[2] - phi[2] * k1[1], phi[2] * k1[0] - phi[0] * k1[2],
+                      phi[0] * k1[1] - phi[1] * k1[0]};
+  Dual ta[3];
+  for (int k = 0; k < 3; ++k) ta[k] = tau[k] + c1 * k1[k] + c2 * k2[k];
+  // R(qa) t0 = t0 + 2 w (u x t0) + 2 u x (u x t0)
+  const Dual t0[3] = {mk(X0[0]), mk(X0[1]), mk(X0[2])};
+  const Dual uxt[3] = {qa[1] * t0[2] - qa[2] * t0[1], qa[2] * t0[0] - qa[0] * t0[2], qa[0] * t0[1] - qa[1] * t0[0]};
+  const Dual uuxt[3] = {qa[1] * uxt[2] - qa[2] * uxt[1], qa[2] * uxt[0] - qa[0] * uxt[2],
+                        qa[0] * uxt[1] - qa[1] * uxt[0]};
+  for (int k = 0; k < 3; ++k) out[k] = ta[k] + t0[k] + 2.0 * (qa_w * uxt[k]) + 2.0 * uuxt[k];
+  // q = qa (x) q0 (Hamilton product, [x, y, z, w])
+  const Dual u0[3] = {mk(X0[3]), mk(X0[4]), mk(X0[5])};
+  const Dual w0 = mk(X0[6]);
+  const Dual cr[3] = {qa[1] * u0[2] - qa[2] * u0[1], qa[2] * u0[0] - qa[0] * u0[2], qa[0] * u0[1] - qa[1] * u0[0]};
+  for (int k = 0; k < 3; ++k) out[3 + k] = qa_w * u0[k] + w0 * qa[k] + cr[k];
+  out[6] = qa_w * w0 - (qa[0] * u0[0] + qa[1] * u0[1] + qa[2] * u0[2]);
+}
+
+__device__ void se3_load(const float* xi, const float* X0, double x[6], double x0[7]) {
+  for (int k = 0; k < 6; ++k) x[k] = xi[k];
+  if (X0) {
+    for (int k = 0; k < 7; ++k) x0[k] = X0[k];
+  } else {
+    for (int k = 0; k < 7; ++k) x0[k] = k == 6 ? 1.0 : 0.0;
+  }
+}
+
+__device__ void se3_pose(const double x[6], const double x0[7], float* pose) {
+  Dual xi[6], out[7];
+  for (int k = 0; k < 6; ++k) xi[k] = mk(x[k]);
+  se3_exp_compose(xi, x0, out);
+  for (int k = 0; k < 7; ++k) pose[k] = (float)out[k].v;
+}
+
+__device__ void se3_tangent_grad(const double x[6], const double x0[7], const float* gpose, double g[6]) {
+  for (int j = 0; j < 6; ++j) {
+    Dual xi[6], out[7];
+    for (int k = 0; k < 6; ++k) xi[k] = {x[k], k == j ? 1.0 : 0.0};
+    se3_exp_compose(xi, x0, out);
+    double s = 0.0;
+    for (int k = 0; k < 7; ++k) s += out[k].d * (double)gpose[k];
+    g[j] = s;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_se3_retract(int n, const float* xi, const float* X0, float* pose) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  double x[6], x0[7];
+  se3_load(xi + 6 * i, X0 ? X0 + 7 * i : nullptr, x, x0);
+  se3_pose(x, x0, pose + 7 * i);
+}
+
+__global__ __launch_bounds__(64) void k_se3_grad(int n, const float* xi, const float* X0, const float* gpose,
+                                                 float* gxi) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  double x[6], x0[7], g[6];
+  se3_load(xi + 6 * i, X0 ? X0 + 7 * i : nullptr, x, x0);
+  se3_tangent_grad(x, x0, gpose + 7 * i, g);
+  for (int k = 0; k < 6; ++k) gxi[6 * i + k] = (float)g[k];
+}
+
+// One workgroup walks all n poses (one per concurrent solve, so n is small):
+// the shared step count is read once before and incremented once after.
+__global__ __launch_bounds__(64) void k_se3_adam(int n, float* xi, const float* X0, const float* gpose, float* M,
+                                                 float* V, int* step, float lr, float b1, float b2, float eps,
+                                                 float* pose) {
+  __shared__ float s_step, s_inv;
+  if (threadIdx.x == 0) {
+    const double t = (double)(*step + 1);
+    s_step = (float)(lr / (1.0 - pow((double)b1, t)));
+    s_inv = (float)(1.0 / sqrt(1.0 - pow((double)b2, t)));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 64) {
+    double x[6], x0[7], g[6];
+    se3_load(xi + 6 * i, X0 ? X0 + 7 * i : nullptr, x, x0);
+    se3_tangent_grad(x, x0, gpose + 7 * i, g);
+    for (int k = 0; k < 6; ++k) {
+      float p = xi[6 * i + k], m = M[6 * i + k], v = V[6 * i + k];
+      adam_one(p, (float)g[k], m, v, b1, b2, eps, s_inv, s_step);
+      xi[6 * i + k] = p;
+      M[6 * i + k] = m;
+      V[6 * i + k] = v;
+      x[k] = p;
+    }
+    se3_pose(x, x0, pose + 7 * i);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *step += 1;
+}
+
 }  // namespace gs
 
 using namespace gs;
+
+extern "C" int gs_se3_retract(int n, const float* xi, const float* X0, float* pose_out, void* stream) {
+  if (n < 0 || (n > 0 && (!xi || !pose_out))) return GS_E_ARG;
+  if (n == 0) return GS_OK;
+  hipLaunchKernelGGL(k_se3_retract, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, n, xi, X0, pose_out);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_se3_grad(int n, const float* xi, const float* X0, const float* grad_pose, float* grad_xi,
+                           void* stream) {
+  if (n < 0 || (n > 0 && (!xi || !grad_pose || !grad_xi))) return GS_E_ARG;
+  if (n == 0) return GS_OK;
+  hipLaunchKernelGGL(k_se3_grad, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, n, xi, X0, grad_pose,
+                     grad_xi);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_se3_adam(int n, float* xi, const float* X0, const float* grad_pose, float* exp_avg,
+                           float* exp_avg_sq, int* step, float lr, float beta1, float beta2, float eps,
+                           float* pose_out, void* stream) {
+  if (n < 0 || (n > 0 && (!xi || !grad_pose || !exp_avg || !exp_avg_sq || !step || !pose_out))) return GS_E_ARG;
+  if (n == 0) return GS_OK;
+  hipLaunchKernelGGL(k_se3_adam, dim3(1), dim3(64), 0, (hipStream_t)stream, n, xi, X0, grad_pose, exp_avg,
+                     exp_avg_sq, step, lr, beta1, beta2, eps, pose_out);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
 
 extern "C" int gs_adam_step(int ntensors, float* const* params, const float* const* grads, float* const* exp_avg,
                             float* const* exp_avg_sq, const long long* numel, const float* lr, float beta1,

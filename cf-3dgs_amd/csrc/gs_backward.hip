@@ -16,11 +16,20 @@
 //                    10 per-(pixel, Gaussian) terms are summed over each
 //                    16-lane DPP row, the 4 row sums per wave go to LDS
 //                    accumulators, and each Gaussian of the chunk receives
-//                    one global atomic per term.
+//                    one global atomic per term.  Deterministic mode: the 4
+//                    rows of a wave are added in a fixed order, each wave
+//                    keeps its own LDS partials, and the chunk writes one
+//                    10-float partial per (Gaussian, tile) pair into a slot
+//                    of its own (the Gaussian's offset in the Gaussian-order
+//                    tile prefix + the tile's rank in its rect) -- no atomics;
+//   k_det_gather     deterministic mode: per Gaussian, its partials summed in
+//                    rect order;
 //   k_preprocess_bwd per Gaussian: conic -> 2-D cov -> 3-D cov / mean,
 //                    projection, depth, SH direction; scale/rotation; the
-//                    rigid pose gradient is reduced wave -> atomic.
-//   k_pose_finalize  d/dR, d/dt -> d/d(q, t) through the quaternion
+//                    rigid pose gradient is reduced per wave (DPP) and each
+//                    wave stores its 12 partial sums;
+//   k_pose_finalize  sums the wave partials in a fixed tree order, then
+//                    d/dR, d/dt -> d/d(q, t) through the quaternion
 //                    normalisation.
 #include "gs_common.hpp"
 #include "gs_internal.hpp"
@@ -34,7 +43,7 @@ __global__ __launch_bounds__(256) void k_zero_f32(float* p, long long n) {
 }
 
 struct RBArgs {
-  int W, H, gx, nt;
+  int W, H, gx, gy, nt;
   const uint32_t* ranges;
   const uint32_t* list;
   const float4* splat;
@@ -50,6 +59,10 @@ struct RBArgs {
   const float* dL_ddepth;
   const float* dL_dalpha;
   float* g_render;
+  // deterministic mode
+  const uint32_t* local_incl;  // [P] tiles prefix within each 256-Gaussian block
+  const uint32_t* block_excl;  // [ceil(P/256)] exclusive block prefix
+  float* part;                 // [capacity][10] partial per (Gaussian, tile) pair
 };
 
 // One workgroup: item_base = exclusive prefix over tiles of ceil(maxc / 256),
@@ -89,12 +102,14 @@ __device__ __forceinline__ float row_sum16(float v) {
   return v;
 }
 
+template <bool DET>
 __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
+  constexpr int kParts = DET ? 4 : 1;  // LDS partial sets: one per wave when deterministic
   __shared__ uint32_t s_id[256];
   __shared__ float4 s_a[256];
   __shared__ float4 s_b[256];
   __shared__ float4 s_c[256];
-  __shared__ float s_acc[10][256];
+  __shared__ float s_acc[10 * kParts][256];
   const uint32_t nitems = a.item_base[a.nt];
   // The grid is an upper bound on the item count, so items are taken in
   // dispatch order: live items then go round-robin over all 8 XCDs (a
@@ -133,7 +148,7 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
       s_c[threadIdx.x] = a.splat[3 * g + 2];
     }
 #pragma unroll
-    for (int k = 0; k < 10; ++k) s_acc[k][threadIdx.x] = 0.0f;
+    for (int k = 0; k < 10 * kParts; ++k) s_acc[k][threadIdx.x] = 0.0f;
   }
   const float T_fin = inside ? a.final_T[pix] : 0.0f;
   const uint32_t last = inside ? a.n_contrib[pix] : 0u;
@@ -227,19 +242,70 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
 #pragma unroll
       for (int k = 0; k < 10; ++k) {
         const float s = row_sum16(v[k]);
-        if ((lane & 15) == 0 && s != 0.0f) atomicAdd(&s_acc[k][j], s);
+        if constexpr (DET) {
+          const int si = __builtin_bit_cast(int, s);
+          const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(si, 0));
+          const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(si, 16));
+          const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(si, 32));
+          const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(si, 48));
+          if (lane == 0) s_acc[(threadIdx.x >> 6) * 10 + k][j] = (r0 + r1) + (r2 + r3);
+        } else {
+          if ((lane & 15) == 0 && s != 0.0f) atomicAdd(&s_acc[k][j], s);
+        }
       }
     }
   }
   __syncthreads();
-  if (threadIdx.x < cend - cstart) {
-    float* dst = a.g_render + (size_t)s_id[threadIdx.x] * kRG;
+  const uint32_t t = threadIdx.x;
+  if (t < cend - cstart) {
+    if constexpr (DET) {
+      // slot of this (Gaussian, tile) pair: the Gaussian's first slot in the
+      // Gaussian-order prefix of tiles touched + the tile's row-major rank in
+      // its rect (every pair of the list has exactly one chunk)
+      const uint32_t g = s_id[t];
+      int x0, y0, x1, y1;
+      tile_rect(s_a[t].x, s_a[t].y, __float_as_int(s_c[t].w), a.gx, a.gy, x0, y0, x1, y1);
+      const uint32_t w = (uint32_t)(x1 - x0);
+      const uint32_t ntg = w * (uint32_t)(y1 - y0);
+      const uint32_t slot = a.block_excl[g >> 8] + a.local_incl[g] - ntg + (uint32_t)(ty - y0) * w + (uint32_t)(tx - x0);
+      float* dst = a.part + (size_t)slot * 10;
 #pragma unroll
-    for (int k = 0; k < 10; ++k) {
-      const float s = s_acc[k][threadIdx.x];
-      if (s != 0.0f) unsafeAtomicAdd(dst + k, s);
+      for (int k = 0; k < 10; ++k) dst[k] = ((s_acc[k][t] + s_acc[10 + k][t]) + s_acc[20 + k][t]) + s_acc[30 + k][t];
+    } else {
+      float* dst = a.g_render + (size_t)s_id[t] * kRG;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        const float s = s_acc[k][t];
+        if (s != 0.0f) unsafeAtomicAdd(dst + k, s);
+      }
     }
   }
+}
+
+// Deterministic mode: per Gaussian, the partials of its (Gaussian, tile)
+// pairs summed in rect order into its render-gradient record.  Nothing is
+// read when the frame overflowed the binning capacity (no pairs were made).
+__global__ __launch_bounds__(256) void k_det_gather(int P, const uint32_t* tiles, const uint32_t* local_incl,
+                                                    const uint32_t* block_excl, const uint32_t* counters,
+                                                    long long capacity, const float* part, float* g_render) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= P) return;
+  float acc[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) acc[k] = 0.f;
+  const uint32_t nt = tiles[g];
+  if ((long long)counters[0] <= capacity && nt > 0) {
+    const uint32_t off = block_excl[g >> 8] + local_incl[g] - nt;
+    for (uint32_t e = off; e < off + nt; ++e) {
+      const float* src = part + (size_t)e * 10;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) acc[k] += src[k];
+    }
+  }
+  float4* dst = reinterpret_cast<float4*>(g_render + (size_t)g * kRG);
+  dst[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  dst[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+  dst[2] = make_float4(acc[8], acc[9], 0.f, 0.f);
 }
 
 struct PBArgs {
@@ -266,7 +332,7 @@ struct PBArgs {
   float* dL_dsh;
   float* dL_dscales;
   float* dL_drots;
-  float* pose_acc;  // [12]: dL/dt (3), dL/dR row-major (9)
+  float* pose_part;  // [4 * gridDim.x][12] per-wave sums: dL/dt (3), dL/dR row-major (9)
 };
 
 __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
@@ -277,7 +343,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int base = blockIdx.x * 256;
   const int nloc = min(256, a.P - base);
-  const bool staged = a.shs && a.M <= 16;
+  const bool staged = a.shs && a.dL_dsh && a.M <= 16;
   const int row = 3 * a.M;
   if (staged) {
     const float* src = a.shs + (size_t)base * row;
@@ -301,13 +367,18 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     const float4 rA = reinterpret_cast<const float4*>(rg)[0];  // m2x m2y ca cb
     const float4 rB = reinterpret_cast<const float4*>(rg)[1];  // cc opac r g
     const float4 rC = reinterpret_cast<const float4*>(rg)[2];  // b depth - -
-    a.dL_dmeans2D[3 * i] = vis ? rA.x : 0.f;
-    a.dL_dmeans2D[3 * i + 1] = vis ? rA.y : 0.f;
-    a.dL_dmeans2D[3 * i + 2] = 0.f;
-    a.dL_dopacity[i] = vis ? rB.y : 0.f;
-    a.dL_dcolors[3 * i] = vis ? rB.z : 0.f;
-    a.dL_dcolors[3 * i + 1] = vis ? rB.w : 0.f;
-    a.dL_dcolors[3 * i + 2] = vis ? rC.x : 0.f;
+    // (every per-Gaussian output is optional: a pose-only solve asks for none)
+    if (a.dL_dmeans2D) {
+      a.dL_dmeans2D[3 * i] = vis ? rA.x : 0.f;
+      a.dL_dmeans2D[3 * i + 1] = vis ? rA.y : 0.f;
+      a.dL_dmeans2D[3 * i + 2] = 0.f;
+    }
+    if (a.dL_dopacity) a.dL_dopacity[i] = vis ? rB.y : 0.f;
+    if (a.dL_dcolors) {
+      a.dL_dcolors[3 * i] = vis ? rB.z : 0.f;
+      a.dL_dcolors[3 * i + 1] = vis ? rB.w : 0.f;
+      a.dL_dcolors[3 * i + 2] = vis ? rC.x : 0.f;
+    }
     float gp[3] = {0.f, 0.f, 0.f};
     float gS[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int nc = (a.D + 1) * (a.D + 1);
@@ -385,8 +456,8 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
       // ---- depth = view-space z
 #pragma unroll
       for (int k = 0; k < 3; ++k) gp[k] += a.view[4 * k + 2] * rC.y;
-      // ---- SH colour
-      if (a.shs) {
+      // ---- SH colour (at degree 0 the colour does not depend on the mean)
+      if (a.shs && (a.dL_dsh || a.D > 0)) {
         const float cbits = a.splat[3 * i + 1].w;
         const uint32_t cm = __float_as_uint(cbits);
         const float gc[3] = {(cm & 1u) ? 0.f : rB.z, (cm & 2u) ? 0.f : rB.w, (cm & 4u) ? 0.f : rC.x};
@@ -397,26 +468,29 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
         sh_basis(a.D, dx, dy, dz, b);
         sh_basis_grad(a.D, dx, dy, dz, bg3);
         const float* sh = staged ? s_sh + threadIdx.x * row : a.shs + (size_t)i * row;
-        float* gsh = staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row;
+        float* gsh = !a.dL_dsh ? nullptr : (staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row);
         float gdir0 = 0.f, gdir1 = 0.f, gdir2 = 0.f;
         for (int k = 0; k < nc; ++k) {
           const float s0 = sh[3 * k], s1 = sh[3 * k + 1], s2 = sh[3 * k + 2];
-          gsh[3 * k] = b[k] * gc[0];
-          gsh[3 * k + 1] = b[k] * gc[1];
-          gsh[3 * k + 2] = b[k] * gc[2];
+          if (gsh) {
+            gsh[3 * k] = b[k] * gc[0];
+            gsh[3 * k + 1] = b[k] * gc[1];
+            gsh[3 * k + 2] = b[k] * gc[2];
+          }
           const float sd = s0 * gc[0] + s1 * gc[1] + s2 * gc[2];
           gdir0 += sd * bg3[k][0];
           gdir1 += sd * bg3[k][1];
           gdir2 += sd * bg3[k][2];
         }
-        for (int k = 3 * nc; k < 3 * a.M; ++k) gsh[k] = 0.f;
+        if (gsh)
+          for (int k = 3 * nc; k < 3 * a.M; ++k) gsh[k] = 0.f;
         const float ndot = dx * gdir0 + dy * gdir1 + dz * gdir2;
         gp[0] += (gdir0 - dx * ndot) / len;
         gp[1] += (gdir1 - dy * ndot) / len;
         gp[2] += (gdir2 - dz * ndot) / len;
       }
       // ---- scale / rotation
-      if (!a.cov_pre) {
+      if (!a.cov_pre && (a.dL_dscales || a.dL_drots)) {
         const float4 q = reinterpret_cast<const float4*>(a.rots)[i];
         float R[9];
         quat_to_R(q.x, q.y, q.z, q.w, R);
@@ -439,11 +513,11 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
             gsk += gL * R[3 * r + k];
             gR[3 * r + k] = gL * s[k];
           }
-          a.dL_dscales[3 * i + k] = gsk;
+          if (a.dL_dscales) a.dL_dscales[3 * i + k] = gsk;
         }
         float dq[4];
         quat_R_grad(q.x, q.y, q.z, q.w, gR, dq);
-        reinterpret_cast<float4*>(a.dL_drots)[i] = make_float4(dq[0], dq[1], dq[2], dq[3]);
+        if (a.dL_drots) reinterpret_cast<float4*>(a.dL_drots)[i] = make_float4(dq[0], dq[1], dq[2], dq[3]);
       }
       if (a.pose) {
 #pragma unroll
@@ -453,27 +527,27 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
           pacc[3 + 3 * r + 1] = gp[r] * p0.y;
           pacc[3 + 3 * r + 2] = gp[r] * p0.z;
         }
+        if (a.dL_dmeans3D)
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          a.dL_dmeans3D[3 * i + k] = ps.R[k] * gp[0] + ps.R[3 + k] * gp[1] + ps.R[6 + k] * gp[2];
-      } else {
+          for (int k = 0; k < 3; ++k)
+            a.dL_dmeans3D[3 * i + k] = ps.R[k] * gp[0] + ps.R[3 + k] * gp[1] + ps.R[6 + k] * gp[2];
+      } else if (a.dL_dmeans3D) {
         a.dL_dmeans3D[3 * i] = gp[0];
         a.dL_dmeans3D[3 * i + 1] = gp[1];
         a.dL_dmeans3D[3 * i + 2] = gp[2];
       }
     } else {
-      a.dL_dmeans3D[3 * i] = a.dL_dmeans3D[3 * i + 1] = a.dL_dmeans3D[3 * i + 2] = 0.f;
-      if (a.shs) {
+      if (a.dL_dmeans3D) a.dL_dmeans3D[3 * i] = a.dL_dmeans3D[3 * i + 1] = a.dL_dmeans3D[3 * i + 2] = 0.f;
+      if (a.shs && a.dL_dsh) {
         float* gsh = staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row;
         for (int k = 0; k < row; ++k) gsh[k] = 0.f;
       }
-      if (!a.cov_pre) {
-        a.dL_dscales[3 * i] = a.dL_dscales[3 * i + 1] = a.dL_dscales[3 * i + 2] = 0.f;
-        reinterpret_cast<float4*>(a.dL_drots)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
+      if (a.dL_dscales) a.dL_dscales[3 * i] = a.dL_dscales[3 * i + 1] = a.dL_dscales[3 * i + 2] = 0.f;
+      if (a.dL_drots) reinterpret_cast<float4*>(a.dL_drots)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    if (a.dL_dcov3D)
 #pragma unroll
-    for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * i + k] = gS[k];
+      for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * i + k] = gS[k];
   }
   if (staged) {
     __syncthreads();
@@ -487,17 +561,40 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     }
   }
   if (a.pose) {
-    // all lanes reach here (no early return above)
+    // all lanes reach here (no early return above); every wave of the grid
+    // stores its 12 sums, so the finalize reads a fully written array
+    float* dst = a.pose_part + ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 12;
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
       const float s = wave_sum(pacc[k]);
-      if ((threadIdx.x & 63) == 0 && s != 0.0f) unsafeAtomicAdd(a.pose_acc + k, s);
+      if ((threadIdx.x & 63) == 0) dst[k] = s;
     }
   }
 }
 
-__global__ void k_pose_finalize(const float* pose7, const float* acc, float* dL_dpose) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_pose_finalize(const float* pose7, const float* part, int nparts,
+                                                        float* dL_dpose) {
+  __shared__ float s[12][256];
+  const int t = threadIdx.x;
+  float sum[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) sum[k] = 0.f;
+  for (int w = t; w < nparts; w += 256)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) sum[k] += part[(size_t)w * 12 + k];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) s[k][t] = sum[k];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off)
+#pragma unroll
+      for (int k = 0; k < 12; ++k) s[k][t] += s[k][t + off];
+    __syncthreads();
+  }
+  if (t != 0) return;
+  float acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = s[k][0];
   Pose ps;
   make_pose(pose7, ps);
   float GR[9];
@@ -522,8 +619,16 @@ __global__ void k_pose_finalize(const float* pose7, const float* acc, float* dL_
 
 using namespace gs;
 
-extern "C" size_t gs_backward_scratch_bytes(int P) {
-  return align_up((size_t)(P > 0 ? P : 0) * kRG * sizeof(float), 256) + 256;
+// scratch: render-gradient records [P][kRG] | pose wave partials [4 * ceil(P/256)][12]
+//          | deterministic mode: pair partials [capacity][10]
+static size_t pose_parts(int P) { return (size_t)4 * (size_t)((P + 255) / 256); }
+
+extern "C" size_t gs_backward_scratch_bytes(int P, long long capacity, int deterministic) {
+  if (P < 0) P = 0;
+  if (capacity < 0) capacity = 0;
+  size_t n = align_up((size_t)P * kRG * sizeof(float), 256) + align_up(pose_parts(P) * 12 * sizeof(float), 256) + 256;
+  if (deterministic) n += align_up((size_t)capacity * 10 * sizeof(float), 256);
+  return n;
 }
 
 extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
@@ -537,13 +642,13 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
   if (P > 0 && (g->shs == nullptr) == (g->colors_precomp == nullptr)) return GS_E_ARG;
   const bool has_sr = g->scales && g->rotations;
   if (P > 0 && has_sr == (g->cov3D_precomp != nullptr)) return GS_E_ARG;
-  if (P > 0 && (!radii || !geom || !out->dL_dmeans2D || !out->dL_dcolors || !out->dL_dopacity ||
-                !out->dL_dmeans3D || !out->dL_dcov3D))
-    return GS_E_ARG;
-  if (g->shs && !out->dL_dsh) return GS_E_ARG;
-  if (has_sr && (!out->dL_dscales || !out->dL_drotations)) return GS_E_ARG;
+  // per-Gaussian outputs are optional (NULL = not wanted)
+  if (P > 0 && (!radii || !geom)) return GS_E_ARG;
+  if (!g->shs && out->dL_dsh) return GS_E_ARG;
+  if (!has_sr && (out->dL_dscales || out->dL_drotations)) return GS_E_ARG;
   if (out->dL_dpose && !f->pose) return GS_E_ARG;
-  if (scratch_bytes < gs_backward_scratch_bytes(P) || !scratch) return GS_E_WORKSPACE;
+  const bool det = f->deterministic != 0;
+  if (scratch_bytes < gs_backward_scratch_bytes(P, capacity, det) || !scratch) return GS_E_WORKSPACE;
   gs_geom_layout_t LG;
   gs_binning_layout_t LB;
   gs_image_layout_t LI;
@@ -557,17 +662,23 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
   const char* bb = (const char*)binning;
   char* ib = (char*)image;
   float* g_render = (float*)scratch;
-  float* pose_acc = (float*)((char*)scratch + align_up((size_t)P * kRG * sizeof(float), 256));
-  const long long nz = (long long)P * kRG;
-  const long long nzb = (nz + 255) / 256;
-  hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)(nzb < 4096 ? (nzb > 0 ? nzb : 1) : 4096)), dim3(256), 0, s, g_render,
-                     nz);
-  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, pose_acc, 16LL);
+  float* pose_part = (float*)((char*)scratch + align_up((size_t)P * kRG * sizeof(float), 256));
+  float* part = (float*)((char*)pose_part + align_up(pose_parts(P) * 12 * sizeof(float), 256));
+  auto zero = [&](float* p, long long n) {
+    const long long nb = (n + 255) / 256;
+    if (n > 0)
+      hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)(nb < 4096 ? nb : 4096)), dim3(256), 0, s, p, n);
+  };
+  // deterministic mode writes every record in k_det_gather; pairs past a
+  // tile's last contributor are never visited, so their partials start at 0
+  if (det) zero(part, capacity * 10);
+  else zero(g_render, (long long)P * kRG);
   if (capacity > 0 && P > 0) {
     RBArgs r;
     r.W = f->width;
     r.H = f->height;
     r.gx = gx;
+    r.gy = gy;
     r.ranges = (const uint32_t*)(ib + LI.ranges);
     r.list = (const uint32_t*)(bb + LB.vals_sorted);
     r.splat = (const float4*)(gb + LG.splat);
@@ -584,14 +695,24 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
     r.dL_ddepth = dL_ddepth;
     r.dL_dalpha = dL_dalpha;
     r.g_render = g_render;
+    r.local_incl = (const uint32_t*)(gb + LG.offsets);
+    r.block_excl = (const uint32_t*)(gb + LG.scan_tmp);
+    r.part = part;
     // (the item table lives in the image workspace, which the forward owns;
     // the backward rewrites it from tile_maxc before use)
     hipLaunchKernelGGL(k_bwd_items, dim3(1), dim3(1024), 0, s, r.tile_maxc, nt, (uint32_t*)(ib + LI.item_base));
     // upper bound on items: sum over tiles of ceil(list length / 256)
     const long long max_items = capacity / 256 + nt + 1;
     timing::Scope ts(timing::kRenderBwd, s);
-    hipLaunchKernelGGL(k_render_bwd, dim3((unsigned)max_items), dim3(256), 0, s, r);
+    if (det)
+      hipLaunchKernelGGL(k_render_bwd<true>, dim3((unsigned)max_items), dim3(256), 0, s, r);
+    else
+      hipLaunchKernelGGL(k_render_bwd<false>, dim3((unsigned)max_items), dim3(256), 0, s, r);
   }
+  if (det && P > 0)
+    hipLaunchKernelGGL(k_det_gather, dim3((P + 255) / 256), dim3(256), 0, s, P,
+                       (const uint32_t*)(gb + LG.tiles), (const uint32_t*)(gb + LG.offsets),
+                       (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, part, g_render);
   if (P > 0) {
     PBArgs b;
     b.P = P;
@@ -625,12 +746,14 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
     b.dL_dsh = out->dL_dsh;
     b.dL_dscales = out->dL_dscales;
     b.dL_drots = out->dL_drotations;
-    b.pose_acc = pose_acc;
+    b.pose_part = pose_part;
     timing::Scope ts(timing::kPreprocessBwd, s);
-    const size_t shm = (g->shs && g->M <= 16) ? (size_t)256 * 3 * g->M * sizeof(float) : 0;
+    const size_t shm = (g->shs && out->dL_dsh && g->M <= 16) ? (size_t)256 * 3 * g->M * sizeof(float) : 0;
     hipLaunchKernelGGL(k_preprocess_bwd, dim3((P + 255) / 256), dim3(256), shm, s, b);
   }
-  if (out->dL_dpose) hipLaunchKernelGGL(k_pose_finalize, dim3(1), dim3(64), 0, s, f->pose, pose_acc, out->dL_dpose);
-  (void)counters;
+  // (P = 0: no wave partials, the pose gradient is zero)
+  if (out->dL_dpose)
+    hipLaunchKernelGGL(k_pose_finalize, dim3(1), dim3(256), 0, s, f->pose, pose_part, (int)pose_parts(P),
+                       out->dL_dpose);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }

@@ -222,6 +222,17 @@ __device__ __forceinline__ float dpp(float v) {
 }
 
 // Sum over all 64 lanes; every lane must be active.  Result is wave-uniform.
+// Tile rectangle [x0, x1) x [y0, y1) of a splat at (px, py) with radius r
+// (getRect of the reference rasteriser: ((p + r) + 16 - 1) / 16 for the end).
+__device__ __forceinline__ void tile_rect(float px, float py, int r, int gx, int gy, int& x0, int& y0, int& x1,
+                                          int& y1) {
+  const float rr = (float)r;
+  x0 = min(gx, max(0, sat_i((px - rr) / (float)kBX)));
+  y0 = min(gy, max(0, sat_i((py - rr) / (float)kBY)));
+  x1 = min(gx, max(0, sat_i((((px + rr) + (float)kBX) - 1.0f) / (float)kBX)));
+  y1 = min(gy, max(0, sat_i((((py + rr) + (float)kBY) - 1.0f) / (float)kBY)));
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
   v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
   v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]

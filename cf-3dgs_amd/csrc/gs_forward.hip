@@ -41,15 +41,6 @@ struct PreArgs {
   uint32_t* block_sum;   // [nblocks] sums, then [nblocks] status flags
 };
 
-__device__ __forceinline__ void tile_rect(float px, float py, int r, int gx, int gy, int& x0, int& y0, int& x1,
-                                          int& y1) {
-  const float rr = (float)r;
-  x0 = min(gx, max(0, sat_i((px - rr) / (float)kBX)));
-  y0 = min(gy, max(0, sat_i((py - rr) / (float)kBY)));
-  x1 = min(gx, max(0, sat_i((((px + rr) + (float)kBX) - 1.0f) / (float)kBX)));
-  y1 = min(gy, max(0, sat_i((((py + rr) + (float)kBY) - 1.0f) / (float)kBY)));
-}
-
 // Inclusive prefix sum over the 256 threads of a workgroup (uint32).
 __device__ __forceinline__ uint32_t block_incl_scan_256(uint32_t v, uint32_t* s_tmp /*[256]*/, uint32_t& total) {
   const int t = threadIdx.x;
@@ -291,7 +282,10 @@ __global__ __launch_bounds__(256) void k_duplicate(DupArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const uint32_t n = a.counters[0];
   if ((long long)n > a.capacity) {
-    if (i == 0) atomicOr(&a.counters[1], GS_STATUS_CAPACITY);
+    if (i == 0) {
+      atomicOr(&a.counters[1], GS_STATUS_CAPACITY);
+      atomicOr(&a.counters[3], GS_STATUS_CAPACITY);  // sticky copy
+    }
     return;
   }
   if (i >= a.P) return;
@@ -404,7 +398,10 @@ __global__ __launch_bounds__(256) void k_df_duplicate(DfDupArgs a) {
   const int k = blockIdx.x * 256 + threadIdx.x;
   const uint32_t n = a.counters[0];
   if ((long long)n > a.capacity) {
-    if (k == 0) atomicOr(&a.counters[1], GS_STATUS_CAPACITY);
+    if (k == 0) {
+      atomicOr(&a.counters[1], GS_STATUS_CAPACITY);
+      atomicOr(&a.counters[3], GS_STATUS_CAPACITY);  // sticky copy
+    }
     return;
   }
   if (k >= a.P) return;

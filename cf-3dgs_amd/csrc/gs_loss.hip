@@ -28,6 +28,7 @@ struct Win {
 struct LossArgs {
   int C, H, W;
   float lambda;
+  int clamp01;  // loss of clamp(pred, 0, 1), as the caller's image.clamp(0, 1)
   Win win;
   const float* x;  // prediction [C,H,W]
   const float* y;  // target [C,H,W]
@@ -51,7 +52,8 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(LossArgs a) {
   const int t = threadIdx.x;
   for (int k = t; k < kLIH * kLIW; k += 256) {
     const int r = k / kLIW, c = k - r * kLIW;
-    sx[r][c] = at_or_zero(X, a.H, a.W, r0 + r, c0 + c);
+    const float xr = at_or_zero(X, a.H, a.W, r0 + r, c0 + c);
+    sx[r][c] = a.clamp01 ? fminf(fmaxf(xr, 0.0f), 1.0f) : xr;
     sy[r][c] = at_or_zero(Y, a.H, a.W, r0 + r, c0 + c);
   }
   __syncthreads();
@@ -151,6 +153,7 @@ __global__ __launch_bounds__(256) void k_loss_final(const float* partial, int nb
 struct LossBwdArgs {
   int C, H, W;
   float lambda, inv_n;
+  int clamp01;
   Win win;
   const float* x;
   const float* y;
@@ -203,11 +206,14 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(LossBwdArgs a) {
       gc2 = fmaf(g, sh[2][r + j][c], gc2);
     }
     const size_t o = (size_t)ch * HW + (size_t)gr * a.W + gc;
-    const float xv = a.x[o], yv = a.y[o];
+    const float xraw = a.x[o], yv = a.y[o];
+    const float xv = a.clamp01 ? fminf(fmaxf(xraw, 0.0f), 1.0f) : xraw;
+    // clamp's gradient passes where 0 <= x <= 1 (torch's clamp backward)
+    const float pass = (!a.clamp01 || (xraw >= 0.0f && xraw <= 1.0f)) ? 1.0f : 0.0f;
     const float dssim = (ga + 2.f * xv * gb + yv * gc2) * a.inv_n;
     const float d = xv - yv;
     const float sgn = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
-    a.dx[o] = go * ((1.0f - a.lambda) * sgn * a.inv_n - a.lambda * dssim);
+    a.dx[o] = pass * (go * ((1.0f - a.lambda) * sgn * a.inv_n - a.lambda * dssim));
   }
 }
 
@@ -227,7 +233,7 @@ extern "C" size_t gs_loss_workspace_bytes(int C, int H, int W) {
 }
 
 extern "C" int gs_ssim_l1_forward(const float* pred, const float* target, int C, int H, int W,
-                                  const float* window11, float lambda_dssim, float* out3, void* ws,
+                                  const float* window11, float lambda_dssim, int clamp01, float* out3, void* ws,
                                   size_t ws_bytes, void* stream) {
   if (!pred || !target || !window11 || !out3 || !ws || C <= 0 || H <= 0 || W <= 0) return GS_E_ARG;
   if (ws_bytes < gs_loss_workspace_bytes(C, H, W)) return GS_E_WORKSPACE;
@@ -236,6 +242,7 @@ extern "C" int gs_ssim_l1_forward(const float* pred, const float* target, int C,
   a.H = H;
   a.W = W;
   a.lambda = lambda_dssim;
+  a.clamp01 = clamp01;
   for (int k = 0; k < 11; ++k) a.win.g[k] = window11[k];
   a.x = pred;
   a.y = target;
@@ -251,7 +258,7 @@ extern "C" int gs_ssim_l1_forward(const float* pred, const float* target, int C,
 }
 
 extern "C" int gs_ssim_l1_backward(const float* pred, const float* target, int C, int H, int W,
-                                   const float* window11, float lambda_dssim, const float* grad_out,
+                                   const float* window11, float lambda_dssim, int clamp01, const float* grad_out,
                                    const void* ws, size_t ws_bytes, float* dpred, void* stream) {
   if (!pred || !target || !window11 || !grad_out || !ws || !dpred || C <= 0 || H <= 0 || W <= 0) return GS_E_ARG;
   if (ws_bytes < gs_loss_workspace_bytes(C, H, W)) return GS_E_WORKSPACE;
@@ -260,6 +267,7 @@ extern "C" int gs_ssim_l1_backward(const float* pred, const float* target, int C
   a.H = H;
   a.W = W;
   a.lambda = lambda_dssim;
+  a.clamp01 = clamp01;
   a.inv_n = 1.0f / ((float)C * H * W);
   for (int k = 0; k < 11; ++k) a.win.g[k] = window11[k];
   a.x = pred;

@@ -40,6 +40,10 @@ bool enabled() { return g_on; }
 
 Scope::Scope(int stage, hipStream_t s) : stage_(stage), s_(s), a_(nullptr) {
   if (!g_on) return;
+  // launches being captured into a graph are not timed (a recorded event
+  // would become a graph node)
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
   std::lock_guard<std::mutex> lk(g_mu);
   hipEvent_t e = take();
   if (e && hipEventRecord(e, s_) == hipSuccess) a_ = (void*)e;

@@ -95,22 +95,32 @@ class _RasterizeGaussians(torch.autograd.Function):
         means3D, sh, colors_precomp, opacities, scales, rotations, cov3Ds_precomp, pose = ctx.saved_tensors
         has_sh, has_col, has_sc, has_ro, has_cv, has_pose = ctx.has
         st = ctx.state
+        # Only the gradients autograd asks for are written (a pose-only solve
+        # skips every per-Gaussian output).
+        need = ctx.needs_input_grad
+        names = ("means3D", "means2D", "shs", "colors", "opacities", "scales", "rotations", "cov3D", "pose")
+        want = {n for n, k in zip(names, need[:9]) if k}
         g = _raster.backward(
             ctx.raster_settings, st, means3D,
             sh if has_sh else None, colors_precomp if has_col else None, opacities,
             scales if has_sc else None, rotations if has_ro else None, cov3Ds_precomp if has_cv else None,
-            pose if has_pose else None, grad_color, grad_depth, grad_alpha)
+            pose if has_pose else None, grad_color, grad_depth, grad_alpha, want=want)
         if ctx.raster_settings.debug:
             torch.cuda.synchronize(means3D.device)
-        grad_means3D = g["means3D"].view_as(means3D)
+
+        def shaped(name, like):
+            t = g[name]
+            return t.view_as(like) if t is not None else None
+
+        grad_means3D = shaped("means3D", means3D)
         grad_means2D = g["means2D"]
-        grad_sh = g["shs"].view_as(sh) if has_sh else None
-        grad_col = g["colors"].view_as(colors_precomp) if has_col else None
-        grad_opac = g["opacities"].view_as(opacities)
-        grad_sc = g["scales"].view_as(scales) if has_sc else None
-        grad_ro = g["rotations"].view_as(rotations) if has_ro else None
-        grad_cv = g["cov3D"].view_as(cov3Ds_precomp) if has_cv else None
-        grad_pose = g["pose"].view_as(pose) if has_pose else None
+        grad_sh = shaped("shs", sh) if has_sh else None
+        grad_col = shaped("colors", colors_precomp) if has_col else None
+        grad_opac = shaped("opacities", opacities)
+        grad_sc = shaped("scales", scales) if has_sc else None
+        grad_ro = shaped("rotations", rotations) if has_ro else None
+        grad_cv = shaped("cov3D", cov3Ds_precomp) if has_cv else None
+        grad_pose = shaped("pose", pose) if has_pose else None
         return (grad_means3D, grad_means2D, grad_sh, grad_col, grad_opac, grad_sc, grad_ro, grad_cv, grad_pose,
                 None, None)
 

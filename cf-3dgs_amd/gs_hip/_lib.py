@@ -29,7 +29,7 @@ class GsFrame(ctypes.Structure):
         ("width", c_int), ("height", c_int), ("tanfovx", c_float), ("tanfovy", c_float),
         ("scale_modifier", c_float), ("sh_degree", c_int), ("prefiltered", c_int),
         ("viewmatrix", c_void_p), ("projmatrix", c_void_p), ("campos", c_void_p), ("bg", c_void_p),
-        ("pose", c_void_p), ("binning", c_int),
+        ("pose", c_void_p), ("binning", c_int), ("deterministic", c_int),
     ]
 
 
@@ -74,7 +74,7 @@ EXPORTS = {
     "gs_geom_layout": (c_int, [c_int, ctypes.POINTER(GsGeomLayout)]),
     "gs_binning_layout": (c_int, [c_longlong, c_int, c_int, c_int, c_int, ctypes.POINTER(GsBinningLayout)]),
     "gs_image_layout": (c_int, [c_int, c_int, ctypes.POINTER(GsImageLayout)]),
-    "gs_backward_scratch_bytes": (c_size_t, [c_int]),
+    "gs_backward_scratch_bytes": (c_size_t, [c_int, c_longlong, c_int]),
     "gs_preprocess": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p, c_size_t,
                               c_void_p, c_void_p]),
     "gs_render": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong, c_void_p, c_size_t,
@@ -92,9 +92,13 @@ EXPORTS = {
     "gs_knn3_meandist": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "gs_loss_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
     "gs_ssim_l1_forward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_float), c_float,
-                                   c_void_p, c_void_p, c_size_t, c_void_p]),
+                                   c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "gs_ssim_l1_backward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_float), c_float,
-                                    c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
+                                    c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
+    "gs_se3_retract": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gs_se3_grad": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gs_se3_adam": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float,
+                            c_float, c_float, c_void_p, c_void_p]),
     "gs_adam_step": (c_int, [c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
                              ctypes.POINTER(c_void_p), ctypes.POINTER(c_longlong), ctypes.POINTER(c_float), c_float,
                              c_float, c_float, c_int, c_void_p]),

@@ -1,0 +1,307 @@
+"""Many pose solves at once: static workspaces, a captured HIP graph, streams.
+
+The reference solves one frame at a time, each iteration a Python step with
+a host synchronisation (the num_rendered readback of the rasteriser and
+``torch.isnan(loss)``, trainer/losses.py:126).  Here a batch of independent
+solves (one per frame instance) runs in lock-step:
+
+  * every instance owns fixed buffers sized once (geometry, binning at a
+    fixed capacity, image state, loss workspace, gradient scratch), so an
+    iteration allocates nothing and never reads anything back;
+  * one iteration of all instances -- preprocess (pose fused), binning +
+    blend, fused L1+SSIM forward and backward (clamp to [0, 1] inside),
+    render/preprocess backward asking for the pose gradient only -- is
+    captured into a HIP graph, instance i on stream i mod S, joined before a
+    single ``gs_se3_adam`` launch that updates every tangent and writes the
+    next poses; ``iters_per_graph`` iterations are captured back to back and
+    the graph is replayed until the iteration count is reached;
+  * binning capacity is the number of tile-Gaussian pairs at the start pose
+    times a headroom; an overflow raises a sticky device flag (the renderer
+    then draws nothing for that iteration), read once after the solve, and
+    the affected instances are re-solved with a larger capacity.
+
+``autotune`` picks the stream count and binning schedule per size bucket by
+timing graph replays.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import torch
+
+from . import _lib
+from . import raster as _raster
+from .loss import _window_taps
+from .pose import EPS, ITERS, LR, PoseInstance, SE3Adam
+
+_TAPS = None
+
+
+def _taps():
+    global _TAPS
+    if _TAPS is None:
+        _TAPS = _window_taps()
+    return _TAPS
+
+
+def _vp(t: Optional[torch.Tensor], offset_bytes: int = 0):
+    return None if t is None else t.data_ptr() + offset_bytes
+
+
+@dataclass
+class _Slot:
+    inst: PoseInstance
+    capacity: int
+    frame: _lib.GsFrame = None
+    gauss: _lib.GsGaussians = None
+    grads: _lib.GsGrads = None
+    keep: list = field(default_factory=list)
+    radii: torch.Tensor = None
+    geom: torch.Tensor = None
+    counters: torch.Tensor = None
+    binning: torch.Tensor = None
+    image: torch.Tensor = None
+    color: torch.Tensor = None
+    depth: torch.Tensor = None
+    loss_ws: torch.Tensor = None
+    dcolor: torch.Tensor = None
+    scratch: torch.Tensor = None
+
+
+def _round_capacity(n: int, headroom: float) -> int:
+    q = 1 << 14
+    c = int(n * headroom) + 4096
+    return min(-(-c // q) * q, 0x7FFFFFFF)
+
+
+class BatchedPoseSolver:
+    """Solves ``len(instances)`` frames together.  ``run()`` returns
+    (poses [n, 7], losses [n]) on the device."""
+
+    def __init__(self, instances: Sequence[PoseInstance], iters: int = ITERS, lr: float = LR,
+                 streams: int = 4, binning: int = _lib.BINNING_DEPTH_FIRST, iters_per_graph: int = 10,
+                 headroom: float = 1.3, use_graph: bool = True, lambda_dssim: float = 0.2,
+                 X0: Optional[torch.Tensor] = None):
+        if not instances:
+            raise ValueError("no instances")
+        self.instances = list(instances)
+        self.device = instances[0].means.device
+        _raster.require_device(instances[0].means)
+        self.n = len(self.instances)
+        self.iters = int(iters)
+        self.lr = float(lr)
+        self.nstreams = max(1, int(streams))
+        self.binning = int(binning)
+        self.iters_per_graph = max(1, int(iters_per_graph))
+        self.headroom = float(headroom)
+        self.use_graph = bool(use_graph)
+        self.lam = float(lambda_dssim)
+        self.adam = SE3Adam(self.n, self.device, lr=lr, eps=EPS, X0=X0)
+        self.one = torch.ones(1, dtype=torch.float32, device=self.device)
+        self.out3 = torch.zeros((self.n, 3), dtype=torch.float32, device=self.device)
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(self.nstreams)]
+        self.graph = None
+        self.regrows = 0
+        caps = self._start_capacities()
+        self.slots = [self._make_slot(inst, cap, i) for i, (inst, cap) in enumerate(zip(self.instances, caps))]
+
+    # -- set-up -------------------------------------------------------------
+    def _start_capacities(self) -> List[int]:
+        """Tile-Gaussian pairs of every instance at the start pose (one eager
+        preprocess each, one readback for all)."""
+        lib = _lib.load()
+        counts = torch.zeros((self.n, 4), dtype=torch.int32, device=self.device)
+        stream = _raster._stream(self.device)
+        keep = []
+        for i, inst in enumerate(self.instances):
+            fa = _raster.make_frame(inst.settings, self.device, self.adam.pose[i], self.binning)
+            ga = _raster.make_gaussians(inst.means, inst.shs, None, inst.opacities, inst.scales, inst.rotations,
+                                        None, self.device)
+            LG = _raster._geom_layout(inst.P)
+            geom = torch.empty(max(LG.total, 256), dtype=torch.uint8, device=self.device)
+            radii = torch.empty(inst.P, dtype=torch.int32, device=self.device)
+            _lib.check(lib.gs_preprocess(ctypes.byref(fa.frame), ctypes.byref(ga.g), _raster._ptr(radii),
+                                         _raster._ptr(geom), geom.numel(), _raster._ptr(counts[i]), stream),
+                       "gs_preprocess")
+            keep.append((fa, ga, geom, radii))
+        host = counts.cpu()
+        return [_round_capacity(int(host[i, 0]) & 0xFFFFFFFF, self.headroom) for i in range(self.n)]
+
+    def _make_slot(self, inst: PoseInstance, capacity: int, i: int) -> _Slot:
+        lib = _lib.load()
+        dev = self.device
+        s = _Slot(inst=inst, capacity=capacity)
+        fa = _raster.make_frame(inst.settings, dev, None, self.binning, deterministic=True)
+        fa.frame.pose = _vp(self.adam.pose, 28 * i)
+        ga = _raster.make_gaussians(inst.means, inst.shs, None, inst.opacities, inst.scales, inst.rotations, None,
+                                    dev)
+        s.frame, s.gauss = fa.frame, ga.g
+        s.keep = [fa, ga]
+        P, W, H = inst.P, inst.W, inst.H
+        LG = _raster._geom_layout(P)
+        LI = _raster._image_layout(W, H)
+        LB = _raster._binning_layout(capacity, P, W, H, self.binning)
+        u8 = dict(dtype=torch.uint8, device=dev)
+        s.geom = torch.empty(max(LG.total, 256), **u8)
+        s.binning = torch.empty(max(LB.total, 256), **u8)
+        s.image = torch.empty(LI.total, **u8)
+        s.radii = torch.empty(P, dtype=torch.int32, device=dev)
+        s.counters = torch.zeros(4, dtype=torch.int32, device=dev)
+        s.color = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+        s.depth = torch.empty((1, H, W), dtype=torch.float32, device=dev)
+        s.dcolor = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+        s.loss_ws = torch.empty(max(int(lib.gs_loss_workspace_bytes(3, H, W)), 256), **u8)
+        s.scratch = torch.empty(max(int(lib.gs_backward_scratch_bytes(P, capacity, 1)), 256), **u8)
+        g = _lib.GsGrads()
+        g.dL_dpose = _vp(self.adam.grad, 28 * i)
+        s.grads = g
+        return s
+
+    # -- one iteration ------------------------------------------------------
+    def _launch_slot(self, s: _Slot, i: int, stream: ctypes.c_void_p):
+        lib = _lib.load()
+        inst = s.inst
+        c = ctypes.c_void_p
+        _lib.check(lib.gs_preprocess(ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()),
+                                     c(s.geom.data_ptr()), s.geom.numel(), c(s.counters.data_ptr()), stream),
+                   "gs_preprocess")
+        _lib.check(lib.gs_render(ctypes.byref(s.frame), inst.P, c(s.geom.data_ptr()), c(s.counters.data_ptr()),
+                                 s.capacity, c(s.binning.data_ptr()), s.binning.numel(), c(s.image.data_ptr()),
+                                 s.image.numel(), c(s.color.data_ptr()), c(s.depth.data_ptr()), None, stream),
+                   "gs_render")
+        out3 = c(self.out3.data_ptr() + 12 * i)
+        _lib.check(lib.gs_ssim_l1_forward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
+                                          _taps(), self.lam, 1, out3, c(s.loss_ws.data_ptr()), s.loss_ws.numel(),
+                                          stream), "gs_ssim_l1_forward")
+        _lib.check(lib.gs_ssim_l1_backward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
+                                           _taps(), self.lam, 1, c(self.one.data_ptr()), c(s.loss_ws.data_ptr()),
+                                           s.loss_ws.numel(), c(s.dcolor.data_ptr()), stream),
+                   "gs_ssim_l1_backward")
+        _lib.check(lib.gs_backward(ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()),
+                                   c(s.geom.data_ptr()), c(s.binning.data_ptr()), s.capacity, c(s.image.data_ptr()),
+                                   c(s.counters.data_ptr()), c(s.dcolor.data_ptr()), None, None,
+                                   ctypes.byref(s.grads), c(s.scratch.data_ptr()), s.scratch.numel(), stream),
+                   "gs_backward")
+
+    def _launch_iteration(self):
+        cur = torch.cuda.current_stream(self.device)
+        for st in self.streams:
+            st.wait_stream(cur)
+        for i, s in enumerate(self.slots):
+            st = self.streams[i % self.nstreams]
+            with torch.cuda.stream(st):
+                self._launch_slot(s, i, ctypes.c_void_p(st.cuda_stream))
+        for st in self.streams:
+            cur.wait_stream(st)
+        self.adam.step()
+
+    def _reset(self):
+        self.adam.reset()
+        for s in self.slots:
+            s.counters.zero_()
+
+    def _capture(self):
+        # one eager iteration first (lazy loads, first-launch work); callers
+        # reset the state before the captured iterations run
+        self._launch_iteration()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(self.iters_per_graph):
+                self._launch_iteration()
+        torch.cuda.synchronize(self.device)
+        self.graph = g
+
+    # -- solve ----------------------------------------------------------------
+    def _run_once(self):
+        self._reset()
+        if self.use_graph:
+            if self.graph is None:
+                self._capture()
+                self._reset()
+            full, rest = divmod(self.iters, self.iters_per_graph)
+            for _ in range(full):
+                self.graph.replay()
+            for _ in range(rest):
+                self._launch_iteration()
+        else:
+            for _ in range(self.iters):
+                self._launch_iteration()
+
+    def overflowed(self) -> List[int]:
+        flags = torch.stack([s.counters[3] for s in self.slots]).cpu()
+        return [i for i in range(self.n) if int(flags[i]) & _lib.GS_STATUS_CAPACITY]
+
+    def run(self, max_regrow: int = 4) -> Tuple[torch.Tensor, torch.Tensor]:
+        for _ in range(max_regrow + 1):
+            self._run_once()
+            bad = self.overflowed()
+            if not bad:
+                return self.adam.pose.clone(), self.out3[:, 0].clone()
+            # grow the instances that overflowed and solve the batch again
+            self.regrows += 1
+            self.graph = None
+            for i in bad:
+                s = self.slots[i]
+                self.slots[i] = self._make_slot(s.inst, _round_capacity(int(s.capacity * 1.5), 1.0), i)
+        raise RuntimeError(f"binning capacity still exceeded after {max_regrow} regrowths")
+
+    def timed_run(self) -> Dict[str, float]:
+        """run() with wall-clock seconds (graph capture excluded)."""
+        if self.use_graph and self.graph is None:
+            self._reset()
+            self._capture()
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        poses, losses = self.run()
+        torch.cuda.synchronize(self.device)
+        return dict(seconds=time.perf_counter() - t0, poses=poses, losses=losses)
+
+
+# ---------------------------------------------------------------------------
+# autotuning of the batched solver per size bucket
+
+_TUNED: Dict[tuple, dict] = {}
+
+
+def size_bucket(instances: Sequence[PoseInstance]) -> tuple:
+    """(instance count, total Gaussians rounded to 2^k, total pixels rounded to 2^k)."""
+    P = sum(i.P for i in instances)
+    px = sum(i.W * i.H for i in instances)
+    return (len(instances), max(P, 1).bit_length(), max(px, 1).bit_length())
+
+
+def autotune(instances: Sequence[PoseInstance], streams=(1, 2, 4),
+             binnings=(_lib.BINNING_DEPTH_FIRST, _lib.BINNING_SORT64), iters_per_graph: int = 5,
+             replays: int = 2) -> dict:
+    """Times ``replays`` graph replays of every (streams, binning) candidate
+    and returns (and caches per size bucket) the fastest configuration."""
+    key = size_bucket(instances)
+    if key in _TUNED:
+        return _TUNED[key]
+    best = None
+    results = []
+    for b in binnings:
+        for s in streams:
+            solver = BatchedPoseSolver(instances, iters=iters_per_graph * replays, streams=s, binning=b,
+                                       iters_per_graph=iters_per_graph)
+            solver._reset()
+            solver._capture()
+            solver._reset()
+            solver.graph.replay()
+            torch.cuda.synchronize(solver.device)
+            t0 = time.perf_counter()
+            for _ in range(replays):
+                solver.graph.replay()
+            torch.cuda.synchronize(solver.device)
+            dt = (time.perf_counter() - t0) / (replays * iters_per_graph)
+            results.append(dict(streams=s, binning=b, ms_per_iter=dt * 1e3))
+            if best is None or dt < best[0]:
+                best = (dt, s, b)
+            del solver
+    cfg = dict(streams=best[1], binning=best[2], ms_per_iter=best[0] * 1e3, candidates=results)
+    _TUNED[key] = cfg
+    return cfg

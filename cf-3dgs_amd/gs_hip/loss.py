@@ -70,7 +70,7 @@ _TAPS = None
 
 class _FusedPhotometric(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, pred, gt, lambda_dssim):
+    def forward(ctx, pred, gt, lambda_dssim, clamp01):
         import ctypes
         from . import _lib
         global _TAPS
@@ -88,11 +88,13 @@ class _FusedPhotometric(torch.autograd.Function):
         out = torch.empty(3, dtype=torch.float32, device=x.device)
         stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
         _lib.check(lib.gs_ssim_l1_forward(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), C, H, W,
-                                          _TAPS, float(lambda_dssim), ctypes.c_void_p(out.data_ptr()),
+                                          _TAPS, float(lambda_dssim), int(bool(clamp01)),
+                                          ctypes.c_void_p(out.data_ptr()),
                                           ctypes.c_void_p(ws.data_ptr()), ws.numel(), stream), "gs_ssim_l1_forward")
         ctx.save_for_backward(x, y, ws)
         ctx.shape = pred.shape
         ctx.lam = float(lambda_dssim)
+        ctx.clamp01 = int(bool(clamp01))
         ctx.mark_non_differentiable(out)
         return out[0], out
 
@@ -107,13 +109,41 @@ class _FusedPhotometric(torch.autograd.Function):
         dx = torch.empty_like(x)
         stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
         _lib.check(lib.gs_ssim_l1_backward(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), C, H, W,
-                                           _TAPS, ctx.lam, ctypes.c_void_p(go.data_ptr()),
+                                           _TAPS, ctx.lam, ctx.clamp01, ctypes.c_void_p(go.data_ptr()),
                                            ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.c_void_p(dx.data_ptr()),
                                            stream), "gs_ssim_l1_backward")
-        return dx.view(ctx.shape), None, None
+        return dx.view(ctx.shape), None, None, None
 
 
-def fused_photometric_loss(pred: torch.Tensor, gt: torch.Tensor, lambda_dssim: float = 0.2):
+def fused_photometric_loss(pred: torch.Tensor, gt: torch.Tensor, lambda_dssim: float = 0.2, clamp01: bool = False):
     """(1 - l) L1 + l (1 - SSIM) in two HIP kernels forward and one backward.
     Returns (loss, stats) with stats = [loss, mean SSIM, mean L1] on the device."""
-    return _FusedPhotometric.apply(pred, gt, lambda_dssim)
+    return _FusedPhotometric.apply(pred, gt, lambda_dssim, clamp01)
+
+
+def fused_photometric_value_and_grad(pred: torch.Tensor, gt: torch.Tensor, lambda_dssim: float = 0.2,
+                                     clamp01: bool = False):
+    """Forward and backward of the fused loss in one call, outside autograd:
+    returns (stats [loss, SSIM, L1] on the device, d loss / d pred)."""
+    import ctypes
+    from . import _lib
+    global _TAPS
+    if _TAPS is None:
+        _TAPS = _window_taps()
+    lib = _lib.load()
+    if pred.device.type != "cuda":
+        raise RuntimeError("the fused loss runs on the GPU only")
+    x = pred.contiguous().float()
+    y = gt.to(x.device).contiguous().float()
+    C, H, W = x.shape[-3:]
+    ws = torch.empty(lib.gs_loss_workspace_bytes(C, H, W), dtype=torch.uint8, device=x.device)
+    out = torch.empty(3, dtype=torch.float32, device=x.device)
+    one = torch.ones(1, dtype=torch.float32, device=x.device)
+    dx = torch.empty_like(x)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    _lib.check(lib.gs_ssim_l1_forward(p(x), p(y), C, H, W, _TAPS, float(lambda_dssim), int(bool(clamp01)), p(out),
+                                      p(ws), ws.numel(), stream), "gs_ssim_l1_forward")
+    _lib.check(lib.gs_ssim_l1_backward(p(x), p(y), C, H, W, _TAPS, float(lambda_dssim), int(bool(clamp01)), p(one),
+                                       p(ws), ws.numel(), p(dx), stream), "gs_ssim_l1_backward")
+    return out, dx.view(pred.shape)

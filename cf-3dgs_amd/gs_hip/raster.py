@@ -23,6 +23,16 @@ from . import _lib
 BINNING = (_lib.BINNING_SORT64 if os.environ.get("GS_BINNING", "").lower() == "sort64"
            else _lib.BINNING_DEPTH_FIRST)
 
+# Backward reductions in a fixed order (bitwise reproducible gradients) for
+# every backward in this process; off by default (float atomics, like the
+# reference).  GS_DETERMINISTIC=1 or set_deterministic(True) turns it on.
+DETERMINISTIC = os.environ.get("GS_DETERMINISTIC", "") not in ("", "0")
+
+
+def set_deterministic(on: bool) -> None:
+    global DETERMINISTIC
+    DETERMINISTIC = bool(on)
+
 
 def _dev_f32(t: Optional[torch.Tensor], device: torch.device, name: str) -> Optional[torch.Tensor]:
     if t is None:
@@ -71,7 +81,7 @@ class FrameArgs:
 
 
 def make_frame(settings, device: torch.device, pose: Optional[torch.Tensor] = None,
-               binning: Optional[int] = None) -> FrameArgs:
+               binning: Optional[int] = None, deterministic: Optional[bool] = None) -> FrameArgs:
     view = _dev_f32(settings.viewmatrix, device, "viewmatrix")
     proj = _dev_f32(settings.projmatrix, device, "projmatrix")
     campos = _dev_f32(settings.campos, device, "campos")
@@ -97,6 +107,7 @@ def make_frame(settings, device: torch.device, pose: Optional[torch.Tensor] = No
     f.bg = bg.data_ptr()
     f.pose = pose_t.data_ptr() if pose_t is not None else None
     f.binning = BINNING if binning is None else int(binning)
+    f.deterministic = int(DETERMINISTIC if deterministic is None else bool(deterministic))
     return FrameArgs(f, view, proj, campos, bg, pose_t)
 
 
@@ -231,11 +242,13 @@ def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations
 
 
 def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities, scales, rotations,
-             cov3D_precomp, pose, grad_color, grad_depth=None, grad_alpha=None):
-    """Returns a dict of gradients (None where the input was absent)."""
+             cov3D_precomp, pose, grad_color, grad_depth=None, grad_alpha=None, want=None,
+             deterministic: Optional[bool] = None):
+    """Returns a dict of gradients (None where the input was absent or, when
+    `want` (a set of names) is given, not wanted)."""
     lib = _lib.load()
     device = require_device(means3D)
-    fa = make_frame(settings, device, pose, st.binning_mode)
+    fa = make_frame(settings, device, pose, st.binning_mode, deterministic)
     ga = make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp, device)
     P = ga.g.P
     M = ga.g.M
@@ -246,28 +259,40 @@ def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities
     if dc is None:
         dc = torch.zeros((3, fa.frame.height, fa.frame.width), device=device)
     f32 = dict(dtype=torch.float32, device=device)
+
+    def alloc(name, shape, present=True):
+        if not present or (want is not None and name not in want):
+            return None
+        return torch.empty(shape, **f32)
+
     out = dict(
-        means2D=torch.empty((P, 3), **f32),
-        colors=torch.empty((P, 3), **f32),
-        opacities=torch.empty((P,), **f32),
-        means3D=torch.empty((P, 3), **f32),
-        cov3D=torch.empty((P, 6), **f32),
-        shs=torch.empty((P, M, 3), **f32) if ga.g.shs else None,
-        scales=torch.empty((P, 3), **f32) if ga.g.scales else None,
-        rotations=torch.empty((P, 4), **f32) if ga.g.rotations else None,
-        pose=torch.empty((7,), **f32) if fa.pose is not None else None,
+        means2D=alloc("means2D", (P, 3)),
+        colors=alloc("colors", (P, 3)),
+        opacities=alloc("opacities", (P,)),
+        means3D=alloc("means3D", (P, 3)),
+        cov3D=alloc("cov3D", (P, 6)),
+        shs=alloc("shs", (P, M, 3), bool(ga.g.shs)),
+        scales=alloc("scales", (P, 3), bool(ga.g.scales)),
+        rotations=alloc("rotations", (P, 4), bool(ga.g.rotations)),
+        pose=alloc("pose", (7,), fa.pose is not None),
     )
+
+    def ptr(name):
+        t = out[name]
+        return t.data_ptr() if t is not None else None
+
     gr = _lib.GsGrads()
-    gr.dL_dmeans2D = out["means2D"].data_ptr()
-    gr.dL_dcolors = out["colors"].data_ptr()
-    gr.dL_dopacity = out["opacities"].data_ptr()
-    gr.dL_dmeans3D = out["means3D"].data_ptr()
-    gr.dL_dcov3D = out["cov3D"].data_ptr()
-    gr.dL_dsh = out["shs"].data_ptr() if out["shs"] is not None else None
-    gr.dL_dscales = out["scales"].data_ptr() if out["scales"] is not None else None
-    gr.dL_drotations = out["rotations"].data_ptr() if out["rotations"] is not None else None
-    gr.dL_dpose = out["pose"].data_ptr() if out["pose"] is not None else None
-    scratch = torch.empty(lib.gs_backward_scratch_bytes(P), dtype=torch.uint8, device=device)
+    gr.dL_dmeans2D = ptr("means2D")
+    gr.dL_dcolors = ptr("colors")
+    gr.dL_dopacity = ptr("opacities")
+    gr.dL_dmeans3D = ptr("means3D")
+    gr.dL_dcov3D = ptr("cov3D")
+    gr.dL_dsh = ptr("shs")
+    gr.dL_dscales = ptr("scales")
+    gr.dL_drotations = ptr("rotations")
+    gr.dL_dpose = ptr("pose")
+    scratch = torch.empty(lib.gs_backward_scratch_bytes(P, st.capacity, fa.frame.deterministic), dtype=torch.uint8,
+                          device=device)
     _lib.check(lib.gs_backward(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(st.radii), _ptr(st.geom),
                                _ptr(st.binning), st.capacity, _ptr(st.image), _ptr(st.counters), _ptr(dc), _ptr(dd),
                                _ptr(da), ctypes.byref(gr), _ptr(scratch), scratch.numel(), stream), "gs_backward")

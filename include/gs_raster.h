@@ -33,7 +33,13 @@ extern "C" {
 #define GS_E_WORKSPACE 2 /* workspace smaller than the layout requires */
 #define GS_E_LAUNCH 3    /* the runtime refused a launch */
 
-/* counters[1] bits */
+/* counters: uint32[4] on the device.
+ *   [0] num_rendered of the last preprocess
+ *   [1] status of the last frame (cleared by every preprocess)
+ *   [2] scratch (element count of the depth sort)
+ *   [3] sticky status: capacity overflows OR-ed in and never cleared by the
+ *       library, so a caller replaying many frames (e.g. a captured graph)
+ *       clears it once and checks it once at the end. */
 #define GS_STATUS_CAPACITY 1u    /* num_rendered exceeded the binning capacity */
 #define GS_STATUS_PREFILTERED 2u /* a point was near-culled while prefiltered=1 */
 
@@ -55,6 +61,10 @@ typedef struct gs_frame {
   const float* bg;         /* [3] */
   const float* pose;       /* [7] = tx,ty,tz,qx,qy,qz,qw (lietorch SE3 layout) or NULL */
   int binning;             /* GS_BINNING_DEPTH_FIRST (default) or GS_BINNING_SORT64 */
+  int deterministic;       /* backward only: 0 = per-Gaussian gradients summed with float
+                              atomics (order varies run to run, as the reference's);
+                              1 = every reduction in a fixed order, so gradients are
+                              bitwise reproducible (needs the larger scratch below) */
 } gs_frame_t;
 
 /* Two binning schedules with identical results (same sorted Gaussian ids and
@@ -81,7 +91,9 @@ typedef struct gs_gaussians {
   const float* cov3D_precomp; /* [P,6] or NULL */
 } gs_gaussians_t;
 
-/* Gradients written by gs_backward (every entry is written, no accumulation). */
+/* Gradients written by gs_backward (every entry is written, no accumulation).
+ * Any per-Gaussian output may be NULL when it is not wanted (a pose-only
+ * solve passes only dL_dpose). */
 typedef struct gs_grads {
   float* dL_dmeans2D;   /* [P,3] NDC-space screen mean gradient (z = 0) */
   float* dL_dcolors;    /* [P,3] gradient w.r.t. the blended per-Gaussian colour */
@@ -99,8 +111,8 @@ typedef struct gs_geom_layout {
   size_t splat;    /* float4[3P]: (x,y,conic_a,conic_b) (conic_c,opacity,depth,clamp bits) (r,g,b,0) */
   size_t cov3D;    /* float[6P] */
   size_t tiles;    /* uint32[P] tiles touched */
-  size_t offsets;  /* uint32[P] inclusive prefix sum of tiles */
-  size_t scan_tmp; /* scan partials */
+  size_t offsets;  /* uint32[P] inclusive prefix sum of tiles within each 256-Gaussian block */
+  size_t scan_tmp; /* uint32[ceil(P/256)] exclusive prefix of the block totals */
   size_t total;
 } gs_geom_layout_t;
 
@@ -131,7 +143,9 @@ typedef struct gs_image_layout {
 int gs_geom_layout(int P, gs_geom_layout_t* out);
 int gs_binning_layout(long long capacity, int P, int width, int height, int binning, gs_binning_layout_t* out);
 int gs_image_layout(int width, int height, gs_image_layout_t* out);
-size_t gs_backward_scratch_bytes(int P);
+/* Backward scratch for P Gaussians; a deterministic backward also keeps one
+ * 10-float partial per (Gaussian, tile) pair, so it needs the binning capacity. */
+size_t gs_backward_scratch_bytes(int P, long long capacity, int deterministic);
 
 /* Forward, phase 1 (replaces the preprocess + InclusiveSum part of
  * diff_gaussian_rasterization._C.rasterize_gaussians, called from
@@ -151,7 +165,8 @@ int gs_render(const gs_frame_t* frame, int P, const void* geom, uint32_t* counte
 
 /* Backward (replaces diff_gaussian_rasterization._C.rasterize_gaussians_backward,
  * triggered by loss.backward() at trainer/cf3dgs_trainer.py:133).  dL_ddepth and
- * dL_dalpha may be NULL (treated as zero).  scratch: gs_backward_scratch_bytes(P).
+ * dL_dalpha may be NULL (treated as zero).  scratch: gs_backward_scratch_bytes(P,
+ * capacity, frame->deterministic).
  * The image workspace is the forward's; the backward rewrites its work-item
  * table (item_base) and reads everything else. */
 int gs_backward(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
@@ -192,10 +207,35 @@ int gs_knn3_meandist(const float* points, int P, float* out, void* ws, size_t ws
  * dpred = grad_out[0] * d loss / d pred. */
 size_t gs_loss_workspace_bytes(int C, int H, int W);
 int gs_ssim_l1_forward(const float* pred, const float* target, int C, int H, int W, const float* window11,
-                       float lambda_dssim, float* out3, void* ws, size_t ws_bytes, void* stream);
+                       float lambda_dssim, int clamp01, float* out3, void* ws, size_t ws_bytes, void* stream);
 int gs_ssim_l1_backward(const float* pred, const float* target, int C, int H, int W, const float* window11,
-                        float lambda_dssim, const float* grad_out, const void* ws, size_t ws_bytes, float* dpred,
-                        void* stream);
+                        float lambda_dssim, int clamp01, const float* grad_out, const void* ws, size_t ws_bytes,
+                        float* dpred, void* stream);
+/* (clamp01 != 0: the loss of clamp(pred, 0, 1), the caller's image.clamp(0, 1)
+ * at scene/gaussian_model_cf.py:885, with clamp's gradient mask.) */
+
+/* SE(3) pose parameters as lietorch's LieGroupParameter holds them
+ * (scene/gaussian_model_cf.py:328-340 init_RT, :123-133 P[0].retr().act(xyz)):
+ * the optimised tensor is a tangent xi = [tau(3), phi(3)] (zero at start) and
+ * the pose is X = Exp(xi) * X0, with X = [t(3), q(4) x,y,z,w] and
+ * Exp(tau, phi) = (V(phi) tau, quat(phi)),
+ * V = I + (1 - cos th)/th^2 [phi]x + (th - sin th)/th^3 [phi]x^2, th = |phi|;
+ * (t1,q1) * (t2,q2) = (t1 + R(q1) t2, q1 q2).  Arrays are [n][6] / [n][7].
+ *
+ * gs_se3_retract: pose_out = Exp(xi) * X0.
+ * gs_se3_grad:    grad_xi = (d pose / d xi)^T grad_pose, the exact derivative
+ *                 of the map above (forward-mode, double precision), i.e. the
+ *                 gradient lietorch's Exp/retr backward delivers to P[i].
+ * gs_se3_adam:    the per-frame optimiser of training_setup_fix_position
+ *                 (torch.optim.Adam(lr=rotation_lr, eps=1e-15) on P[0],
+ *                 scene/gaussian_model_cf.py:303-322): forms grad_xi as above,
+ *                 one Adam step on xi (step count read from *step as t-1 and
+ *                 incremented on the device, so a captured graph can be
+ *                 replayed), then pose_out = Exp(xi) * X0. */
+int gs_se3_retract(int n, const float* xi, const float* X0, float* pose_out, void* stream);
+int gs_se3_grad(int n, const float* xi, const float* X0, const float* grad_pose, float* grad_xi, void* stream);
+int gs_se3_adam(int n, float* xi, const float* X0, const float* grad_pose, float* exp_avg, float* exp_avg_sq,
+                int* step, float lr, float beta1, float beta2, float eps, float* pose_out, void* stream);
 
 /* Adam over up to 16 float32 tensors in one launch (replaces
  * torch.optim.Adam(..., eps=1e-15).step() of training_setup,

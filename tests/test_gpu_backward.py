@@ -26,15 +26,15 @@ def _cotangents(H, W, seed):
     return dc, dd, da
 
 
-def _check(sc, deg, seed=0, **kw):
+def _check(sc, deg, seed=0, deterministic=None, **kw):
     g = run_gpu(sc, deg, **kw)
-    scene, cam, o = run_oracle(sc, deg, **kw)
+    scene, cam, o = run_oracle(sc, deg, **{k: v for k, v in kw.items() if k != "binning"})
     H, W = cam["H"], cam["W"]
     dc, dd, da = _cotangents(H, W, seed)
     og = O.backward(scene, cam, o, dc.numpy(), dd.numpy(), da.numpy())
     means, shs, col, op, sc_, ro, cv, po = g["inputs"]
     gg = raster.backward(g["rs"], g["st"], means, shs, col, op, sc_, ro, cv, po, dc.to(DEV), dd.to(DEV),
-                         da.to(DEV))
+                         da.to(DEV), deterministic=deterministic)
     torch.cuda.synchronize()
     vis = o["radii"] > 0
     assert_close_grad("means2D", np_(gg["means2D"]), og["means2D"])
@@ -49,7 +49,22 @@ def _check(sc, deg, seed=0, **kw):
         assert_close_grad("rotations", np_(gg["rotations"]), og["rotations"])
     if gg["pose"] is not None:
         assert_close_grad("pose", np_(gg["pose"]), og["pose7"], rtol=5e-3, atol_frac=1e-3)
+    if deterministic:
+        # fixed-order reductions: a second backward of the same forward is bit-identical
+        again = raster.backward(g["rs"], g["st"], means, shs, col, op, sc_, ro, cv, po, dc.to(DEV), dd.to(DEV),
+                                da.to(DEV), deterministic=True)
+        for k, v in gg.items():
+            if v is not None:
+                assert torch.equal(v, again[k]), k
     return gg, og
+
+
+@pytest.mark.parametrize("binning", [0, 1])
+def test_deterministic_backward(binning):
+    """Fixed-order mode: oracle parity at the same tolerances, bitwise repeatable."""
+    sc = scenes.plain_scene(seed=40 + binning, P=6000)
+    pose = scenes.random_pose(9, 0.05, 0.05)
+    _check(sc, 3, deterministic=True, pose=pose, binning=binning)
 
 
 def test_plain_scene_backward():
