@@ -114,7 +114,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+                                   "(sharding across GPUs is designed, not implemented)"}))
         sys.exit(2)
 
     import torch
