@@ -99,6 +99,118 @@ def cpu_baseline(sc, budget_s: float = 12.0):
                       f"{dt:.1f} s"}
 
 
+def _log(msg: str) -> None:
+    """Progress on stderr (stdout carries only the result line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, quick: bool = False):
+    """C4: ``n`` pose solves (P ~ U[20k, 200k], resolution ~ U{320x180,
+    480x270, 640x360, 960x540}, seeded) batched in captured HIP graphs, against
+    the same solves run one at a time on the plain one-solve-per-launch path
+    (all ``n``), plus the torch-op plain path on a sample and C3 (one
+    129,600-Gaussian 480x270 frame) on every path."""
+    import random
+    import torch
+    from gs_hip.batch import BatchedPoseSolver, autotune
+    from gs_hip.pose import build_instance, pose_errors, solve_plain, solve_plain_native
+    rng = random.Random(0)
+    res = [(320, 180), (480, 270), (640, 360), (960, 540)]
+    insts = []
+    for i in range(n):
+        W, H = res[rng.randrange(4)]
+        insts.append(build_instance(1000 + i, W=W, H=H, P=rng.randint(20000, 200000), device=device))
+    torch.cuda.synchronize(device)
+    _log(f"pose: {n} instances built; autotuning")
+    t0 = time.perf_counter()
+    cfg = autotune(insts)
+    t_tune = time.perf_counter() - t0
+    solver = BatchedPoseSolver(insts, iters=iters, streams=cfg["streams"], binning=cfg["binning"])
+    r = solver.timed_run()
+    batched_s = r["seconds"]
+    _log(f"pose: batched {batched_s:.2f} s; plain one-at-a-time solves")
+    if quick:
+        return {"workload": f"{n} pose solves x {iters} Adam iterations (batched path only)",
+                "batched_solves_per_s": round(n / batched_s, 3), "batched_seconds": round(batched_s, 3),
+                "autotune": {"streams": cfg["streams"], "binning": cfg["binning"]}}
+    plain_t, diffs = [], []
+    for k, inst in enumerate(insts):
+        o = solve_plain_native(inst, iters=iters)
+        plain_t.append(o["seconds"])
+        diffs.append(pose_errors(o["pose"], r["poses"][k]))
+    seq_lat = []
+    acc = 0.0
+    for t in plain_t:
+        acc += t
+        seq_lat.append(acc)
+    torch_t = []
+    for k in range(min(torch_sample, n)):
+        torch_t.append(solve_plain(insts[k], iters=iters)["seconds"])
+    errs = sorted(pose_errors(r["poses"][k], insts[k].gt_pose) for k in range(n))
+
+    def pct(xs, q):
+        xs = sorted(xs)
+        return xs[min(len(xs) - 1, max(0, -(-q * len(xs) // 100) - 1))]
+
+    _log("pose: single 480x270 frame on every path")
+    c3 = build_instance(0, W=480, H=270, device=device)
+    c3_torch = solve_plain(c3, iters=iters)
+    c3_native = solve_plain_native(c3, iters=iters)
+    c3_b = BatchedPoseSolver([c3], iters=iters, streams=1).timed_run()
+    rot_gt = [e[0] for e in errs]
+    tr_gt = [e[1] for e in errs]
+    return {
+        "workload": f"{n} pose solves x {iters} Adam iterations, P ~ U[20k,200k], res ~ U{{320x180..960x540}}",
+        "batched_solves_per_s": round(n / batched_s, 3),
+        "batched_seconds": round(batched_s, 3),
+        "batched_p50_s": round(batched_s, 3), "batched_p99_s": round(batched_s, 3),
+        "plain_native_solves_per_s": round(n / sum(plain_t), 3),
+        "plain_native_p50_s": round(pct(seq_lat, 50), 3), "plain_native_p99_s": round(pct(seq_lat, 99), 3),
+        "speedup_vs_plain_native": round(sum(plain_t) / batched_s, 3),
+        "plain_torch_s_per_solve_sample": round(sum(torch_t) / max(1, len(torch_t)), 3),
+        "speedup_vs_plain_torch": round((sum(torch_t) / max(1, len(torch_t))) * n / batched_s, 3),
+        "max_pose_diff_vs_plain_native": [max(d[0] for d in diffs), max(d[1] for d in diffs)],
+        "pose_err_vs_truth_median": [rot_gt[len(rot_gt) // 2], sorted(tr_gt)[len(tr_gt) // 2]],
+        "pose_err_vs_truth_max": [max(rot_gt), max(tr_gt)],
+        "autotune": {"streams": cfg["streams"], "binning": "depth_first" if cfg["binning"] == 0 else "sort64",
+                     "seconds": round(t_tune, 2)},
+        "regrows": solver.regrows,
+        "c3_single_frame_s": {"plain_torch": round(c3_torch["seconds"], 3),
+                              "plain_native": round(c3_native["seconds"], 3),
+                              "graph": round(c3_b["seconds"], 3)},
+    }
+
+
+def serve_bench(device, n_scenes: int = 16, n_requests: int = 60, calib_requests: int = 24):
+    """C5: 16 resident scenes (P log-uniform in [100k, 2M], SH degree 3);
+    Poisson arrivals at 0.7 x the closed-loop capacity measured first; 70%
+    renders from a sphere of radius 4 (480x270 / 960x540 / 1920x1080), 30%
+    200-iteration pose solves of new 480x270 frames; 200 GB budget."""
+    import math
+    import random
+    import torch
+    from gs_hip.serve import Server, make_requests, make_scene
+    rng = random.Random(7)
+    scenes = [make_scene(k, int(math.exp(rng.uniform(math.log(1e5), math.log(2e6)))), device)
+              for k in range(n_scenes)]
+    torch.cuda.synchronize(device)
+    _log("serve: closed-loop capacity run")
+    cal = make_requests(calib_requests, rate=0.0, n_scenes=n_scenes, seed=11)
+    cal_stats = Server(scenes).run(cal, realtime=False)
+    capacity = cal_stats["throughput_rps"]
+    lam = 0.7 * capacity
+    _log(f"serve: Poisson run at {lam:.2f} requests/s")
+    reqs = make_requests(n_requests, rate=lam, n_scenes=n_scenes, seed=12)
+    srv = Server(scenes)
+    stats = srv.run(reqs, realtime=True)
+    out = {"workload": f"{n_scenes} scenes P~logU[100k,2M] SH3, {n_requests} Poisson requests "
+                       "(70% renders 480x270/960x540/1920x1080, 30% 200-iter pose solves 480x270), 200 GB budget",
+           "capacity_rps": round(capacity, 3), "arrival_rate_rps": round(lam, 3),
+           "scene_P": [s.P for s in scenes], "peak_reserved_GB": round(srv.peak_used / 2 ** 30, 2)}
+    out.update({k: (round(v, 4) if isinstance(v, float) else v) for k, v in stats.items()})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +220,10 @@ def main():
     ap.add_argument("--width", type=int, default=960)
     ap.add_argument("--height", type=int, default=540)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pose", action="store_true", help="skip the batched pose-solve measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--pose-instances", type=int, default=64)
+    ap.add_argument("--pose-quick", action="store_true", help="time only the batched pose solves (profiling runs)")
     ap.add_argument("--torch-loss", action="store_true", help="loss with torch conv2d windows instead of the fused kernel")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy 5 times (PMC byte-counter calibration)")
@@ -188,7 +304,10 @@ def main():
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(d["GB_s"], 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(d["GB_s"] / HBM_PEAK_GBS, 4), "traffic": None,
                     "ms_per_launch": round(d["ms_per_step"], 4)}
+    _log(f"training step: {ms:.3f} ms")
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
+    pose = None if args.no_pose else pose_bench(device, n=args.pose_instances, quick=args.pose_quick)
+    serving = None if args.no_serve else serve_bench(device)
     res = {
         "metric": METRIC, "value": round(1000.0 / ms, 3), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
@@ -199,6 +318,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "pose": pose,
+        "serving": serving,
         "stages": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in stages.items()},
     }

@@ -13,12 +13,12 @@
 //                    would have accumulated, so heavy tiles spread over many
 //                    workgroups instead of serialising in one.  T is then
 //                    recovered by division as in the forward's algebra.  The
-//                    10 per-(pixel, Gaussian) terms are summed over each
-//                    16-lane DPP row, the 4 row sums per wave go to LDS
-//                    accumulators, and each Gaussian of the chunk receives
-//                    one global atomic per term.  Deterministic mode: the 4
-//                    rows of a wave are added in a fixed order, each wave
-//                    keeps its own LDS partials, and the chunk writes one
+//                    10 per-(pixel, Gaussian) terms are summed over the
+//                    wave with permlane swaps + DPP (wave_sum12), the wave
+//                    sums go to LDS accumulators, and each Gaussian of the
+//                    chunk receives one global atomic per term.
+//                    Deterministic mode: each wave keeps its own LDS
+//                    partials (summed in wave order), and the chunk writes one
 //                    10-float partial per (Gaussian, tile) pair into a slot
 //                    of its own (the Gaussian's offset in the Gaussian-order
 //                    tile prefix + the tile's rank in its rect) -- no atomics;
@@ -100,6 +100,35 @@ __device__ __forceinline__ float row_sum16(float v) {
   v += dpp<0x141>(v);  // row_half_mirror
   v += dpp<0x140>(v);  // row_mirror
   return v;
+}
+
+// Wave-wide sums of 12 values at once (reduce-scatter).  Stage 1 swaps the
+// upper half-wave of v[2i] with the lower half of v[2i+1] (v_permlane32_swap)
+// and adds: value 2i now sits, half-summed, in lanes 0-31 and 2i+1 in lanes
+// 32-63.  Stage 2 does the same with 16-lane rows (v_permlane16_swap) on
+// pairs of those, leaving four values per register, one per row, each lane
+// holding the sum of its 4 lanes at stride 16.  Stage 3 sums each 16-lane row
+// (DPP).  Afterwards every lane of row q of w[m] holds the wave sum of value
+// 4m + kRowValue[q], kRowValue = {0, 2, 1, 3}.  The order of additions is
+// fixed, so the sums are bitwise reproducible.  30 VALU operations for 12
+// values, against 4 DPP steps per value for separate row sums.
+__device__ __forceinline__ void swap_add32(float a, float b, float& out) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  out = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap_add16(float a, float b, float& out) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  out = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void wave_sum12(const float v[12], float w[3]) {
+  float u[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) swap_add32(v[2 * i], v[2 * i + 1], u[i]);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    swap_add16(u[2 * m], u[2 * m + 1], w[m]);
+    w[m] = row_sum16(w[m]);
+  }
 }
 
 template <bool DET>
@@ -184,9 +213,9 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
   for (int j = (int)(cend - cstart) - 1; j >= 0; --j) {
     const uint32_t q = cstart + j;  // 0-based list position; contributor index q + 1
     bool active = inside && q < last;
-    float v[10];
+    float v[12];
 #pragma unroll
-    for (int k = 0; k < 10; ++k) v[k] = 0.f;
+    for (int k = 0; k < 12; ++k) v[k] = 0.f;
     if (active) {
       const float4 ga = s_a[j];
       const float4 gb = s_b[j];
@@ -239,18 +268,21 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
       }
     }
     if (__ballot(active) != 0ull) {
+      float w[3];
+      wave_sum12(v, w);
+      if ((lane & 15) == 0) {
+        const int q = lane >> 4;
+        const int o = q == 1 ? 2 : (q == 2 ? 1 : q);  // value of row q: 4m + {0, 2, 1, 3}[q]
 #pragma unroll
-      for (int k = 0; k < 10; ++k) {
-        const float s = row_sum16(v[k]);
-        if constexpr (DET) {
-          const int si = __builtin_bit_cast(int, s);
-          const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(si, 0));
-          const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(si, 16));
-          const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(si, 32));
-          const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(si, 48));
-          if (lane == 0) s_acc[(threadIdx.x >> 6) * 10 + k][j] = (r0 + r1) + (r2 + r3);
-        } else {
-          if ((lane & 15) == 0 && s != 0.0f) atomicAdd(&s_acc[k][j], s);
+        for (int m = 0; m < 3; ++m) {
+          const int k = 4 * m + o;
+          if (k < 10) {
+            if constexpr (DET) {
+              s_acc[(threadIdx.x >> 6) * 10 + k][j] = w[m];
+            } else {
+              if (w[m] != 0.0f) atomicAdd(&s_acc[k][j], w[m]);
+            }
+          }
         }
       }
     }
@@ -564,10 +596,14 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     // all lanes reach here (no early return above); every wave of the grid
     // stores its 12 sums, so the finalize reads a fully written array
     float* dst = a.pose_part + ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 12;
+    float w[3];
+    wave_sum12(pacc, w);
+    const int lane = threadIdx.x & 63;
+    if ((lane & 15) == 0) {
+      const int q = lane >> 4;
+      const int o = q == 1 ? 2 : (q == 2 ? 1 : q);  // value of row q: 4m + {0, 2, 1, 3}[q]
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      const float s = wave_sum(pacc[k]);
-      if ((threadIdx.x & 63) == 0) dst[k] = s;
+      for (int m = 0; m < 3; ++m) dst[4 * m + o] = w[m];
     }
   }
 }

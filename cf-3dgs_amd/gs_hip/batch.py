@@ -77,6 +77,101 @@ def _round_capacity(n: int, headroom: float) -> int:
     return min(-(-c // q) * q, 0x7FFFFFFF)
 
 
+def start_capacities(instances: Sequence[PoseInstance], poses: torch.Tensor, binning: int,
+                     headroom: float) -> List[int]:
+    """Binning capacity per instance: tile-Gaussian pairs at the given start
+    poses ([n, 7] on the device) times ``headroom`` (one eager preprocess
+    each, one readback for all)."""
+    lib = _lib.load()
+    n = len(instances)
+    if n == 0:
+        return []
+    device = poses.device
+    counts = torch.zeros((n, 4), dtype=torch.int32, device=device)
+    stream = _raster._stream(device)
+    keep = []
+    for i, inst in enumerate(instances):
+        fa = _raster.make_frame(inst.settings, device, poses[i], binning)
+        ga = _raster.make_gaussians(inst.means, inst.shs, None, inst.opacities, inst.scales, inst.rotations,
+                                    None, device)
+        LG = _raster._geom_layout(inst.P)
+        geom = torch.empty(max(LG.total, 256), dtype=torch.uint8, device=device)
+        radii = torch.empty(inst.P, dtype=torch.int32, device=device)
+        _lib.check(lib.gs_preprocess(ctypes.byref(fa.frame), ctypes.byref(ga.g), _raster._ptr(radii),
+                                     _raster._ptr(geom), geom.numel(), _raster._ptr(counts[i]), stream),
+                   "gs_preprocess")
+        keep.append((fa, ga, geom, radii))
+    host = counts.cpu()
+    return [_round_capacity(int(host[i, 0]) & 0xFFFFFFFF, headroom) for i in range(n)]
+
+
+def slot_bytes(P: int, W: int, H: int, capacity: int, binning: int) -> int:
+    """Device bytes of one solve slot's buffers (the instance's own tensors excluded)."""
+    lib = _lib.load()
+    return int(_raster._geom_layout(P).total + _raster._binning_layout(capacity, P, W, H, binning).total
+               + _raster._image_layout(W, H).total + 4 * P + 16 + 4 * 7 * H * W
+               + lib.gs_loss_workspace_bytes(3, H, W) + lib.gs_backward_scratch_bytes(P, capacity, 1))
+
+
+def make_slot(inst: PoseInstance, capacity: int, binning: int, pose_ptr: int, grad_ptr: int) -> _Slot:
+    """Static buffers of one solve; the renderer reads the pose at
+    ``pose_ptr`` (7 floats) and the backward writes dL/dpose to ``grad_ptr``."""
+    lib = _lib.load()
+    dev = inst.means.device
+    s = _Slot(inst=inst, capacity=capacity)
+    fa = _raster.make_frame(inst.settings, dev, None, binning, deterministic=True)
+    fa.frame.pose = pose_ptr
+    ga = _raster.make_gaussians(inst.means, inst.shs, None, inst.opacities, inst.scales, inst.rotations, None, dev)
+    s.frame, s.gauss = fa.frame, ga.g
+    s.keep = [fa, ga]
+    P, W, H = inst.P, inst.W, inst.H
+    LG = _raster._geom_layout(P)
+    LI = _raster._image_layout(W, H)
+    LB = _raster._binning_layout(capacity, P, W, H, binning)
+    u8 = dict(dtype=torch.uint8, device=dev)
+    s.geom = torch.empty(max(LG.total, 256), **u8)
+    s.binning = torch.empty(max(LB.total, 256), **u8)
+    s.image = torch.empty(LI.total, **u8)
+    s.radii = torch.empty(P, dtype=torch.int32, device=dev)
+    s.counters = torch.zeros(4, dtype=torch.int32, device=dev)
+    s.color = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+    s.depth = torch.empty((1, H, W), dtype=torch.float32, device=dev)
+    s.dcolor = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+    s.loss_ws = torch.empty(max(int(lib.gs_loss_workspace_bytes(3, H, W)), 256), **u8)
+    s.scratch = torch.empty(max(int(lib.gs_backward_scratch_bytes(P, capacity, 1)), 256), **u8)
+    g = _lib.GsGrads()
+    g.dL_dpose = grad_ptr
+    s.grads = g
+    return s
+
+
+def launch_slot(s: _Slot, out3_ptr: int, one_ptr: int, lam: float, stream: ctypes.c_void_p) -> None:
+    """One solve iteration minus the optimiser step, enqueued on ``stream``:
+    preprocess (pose fused), binning + blend, fused loss forward (stats to
+    ``out3_ptr``) and backward (d loss scaled by the float at ``one_ptr``),
+    pose-only backward with fixed-order reductions."""
+    lib = _lib.load()
+    inst = s.inst
+    c = ctypes.c_void_p
+    _lib.check(lib.gs_preprocess(ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()),
+                                 c(s.geom.data_ptr()), s.geom.numel(), c(s.counters.data_ptr()), stream),
+               "gs_preprocess")
+    _lib.check(lib.gs_render(ctypes.byref(s.frame), inst.P, c(s.geom.data_ptr()), c(s.counters.data_ptr()),
+                             s.capacity, c(s.binning.data_ptr()), s.binning.numel(), c(s.image.data_ptr()),
+                             s.image.numel(), c(s.color.data_ptr()), c(s.depth.data_ptr()), None, stream),
+               "gs_render")
+    _lib.check(lib.gs_ssim_l1_forward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
+                                      _taps(), lam, 1, c(out3_ptr), c(s.loss_ws.data_ptr()), s.loss_ws.numel(),
+                                      stream), "gs_ssim_l1_forward")
+    _lib.check(lib.gs_ssim_l1_backward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
+                                       _taps(), lam, 1, c(one_ptr), c(s.loss_ws.data_ptr()), s.loss_ws.numel(),
+                                       c(s.dcolor.data_ptr()), stream), "gs_ssim_l1_backward")
+    _lib.check(lib.gs_backward(ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()),
+                               c(s.geom.data_ptr()), c(s.binning.data_ptr()), s.capacity, c(s.image.data_ptr()),
+                               c(s.counters.data_ptr()), c(s.dcolor.data_ptr()), None, None, ctypes.byref(s.grads),
+                               c(s.scratch.data_ptr()), s.scratch.numel(), stream), "gs_backward")
+
+
 class BatchedPoseSolver:
     """Solves ``len(instances)`` frames together.  ``run()`` returns
     (poses [n, 7], losses [n]) on the device."""
@@ -105,86 +200,15 @@ class BatchedPoseSolver:
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(self.nstreams)]
         self.graph = None
         self.regrows = 0
-        caps = self._start_capacities()
+        caps = start_capacities(self.instances, self.adam.pose, self.binning, self.headroom)
         self.slots = [self._make_slot(inst, cap, i) for i, (inst, cap) in enumerate(zip(self.instances, caps))]
 
-    # -- set-up -------------------------------------------------------------
-    def _start_capacities(self) -> List[int]:
-        """Tile-Gaussian pairs of every instance at the start pose (one eager
-        preprocess each, one readback for all)."""
-        lib = _lib.load()
-        counts = torch.zeros((self.n, 4), dtype=torch.int32, device=self.device)
-        stream = _raster._stream(self.device)
-        keep = []
-        for i, inst in enumerate(self.instances):
-            fa = _raster.make_frame(inst.settings, self.device, self.adam.pose[i], self.binning)
-            ga = _raster.make_gaussians(inst.means, inst.shs, None, inst.opacities, inst.scales, inst.rotations,
-                                        None, self.device)
-            LG = _raster._geom_layout(inst.P)
-            geom = torch.empty(max(LG.total, 256), dtype=torch.uint8, device=self.device)
-            radii = torch.empty(inst.P, dtype=torch.int32, device=self.device)
-            _lib.check(lib.gs_preprocess(ctypes.byref(fa.frame), ctypes.byref(ga.g), _raster._ptr(radii),
-                                         _raster._ptr(geom), geom.numel(), _raster._ptr(counts[i]), stream),
-                       "gs_preprocess")
-            keep.append((fa, ga, geom, radii))
-        host = counts.cpu()
-        return [_round_capacity(int(host[i, 0]) & 0xFFFFFFFF, self.headroom) for i in range(self.n)]
-
     def _make_slot(self, inst: PoseInstance, capacity: int, i: int) -> _Slot:
-        lib = _lib.load()
-        dev = self.device
-        s = _Slot(inst=inst, capacity=capacity)
-        fa = _raster.make_frame(inst.settings, dev, None, self.binning, deterministic=True)
-        fa.frame.pose = _vp(self.adam.pose, 28 * i)
-        ga = _raster.make_gaussians(inst.means, inst.shs, None, inst.opacities, inst.scales, inst.rotations, None,
-                                    dev)
-        s.frame, s.gauss = fa.frame, ga.g
-        s.keep = [fa, ga]
-        P, W, H = inst.P, inst.W, inst.H
-        LG = _raster._geom_layout(P)
-        LI = _raster._image_layout(W, H)
-        LB = _raster._binning_layout(capacity, P, W, H, self.binning)
-        u8 = dict(dtype=torch.uint8, device=dev)
-        s.geom = torch.empty(max(LG.total, 256), **u8)
-        s.binning = torch.empty(max(LB.total, 256), **u8)
-        s.image = torch.empty(LI.total, **u8)
-        s.radii = torch.empty(P, dtype=torch.int32, device=dev)
-        s.counters = torch.zeros(4, dtype=torch.int32, device=dev)
-        s.color = torch.empty((3, H, W), dtype=torch.float32, device=dev)
-        s.depth = torch.empty((1, H, W), dtype=torch.float32, device=dev)
-        s.dcolor = torch.empty((3, H, W), dtype=torch.float32, device=dev)
-        s.loss_ws = torch.empty(max(int(lib.gs_loss_workspace_bytes(3, H, W)), 256), **u8)
-        s.scratch = torch.empty(max(int(lib.gs_backward_scratch_bytes(P, capacity, 1)), 256), **u8)
-        g = _lib.GsGrads()
-        g.dL_dpose = _vp(self.adam.grad, 28 * i)
-        s.grads = g
-        return s
+        return make_slot(inst, capacity, self.binning, _vp(self.adam.pose, 28 * i), _vp(self.adam.grad, 28 * i))
 
     # -- one iteration ------------------------------------------------------
     def _launch_slot(self, s: _Slot, i: int, stream: ctypes.c_void_p):
-        lib = _lib.load()
-        inst = s.inst
-        c = ctypes.c_void_p
-        _lib.check(lib.gs_preprocess(ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()),
-                                     c(s.geom.data_ptr()), s.geom.numel(), c(s.counters.data_ptr()), stream),
-                   "gs_preprocess")
-        _lib.check(lib.gs_render(ctypes.byref(s.frame), inst.P, c(s.geom.data_ptr()), c(s.counters.data_ptr()),
-                                 s.capacity, c(s.binning.data_ptr()), s.binning.numel(), c(s.image.data_ptr()),
-                                 s.image.numel(), c(s.color.data_ptr()), c(s.depth.data_ptr()), None, stream),
-                   "gs_render")
-        out3 = c(self.out3.data_ptr() + 12 * i)
-        _lib.check(lib.gs_ssim_l1_forward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
-                                          _taps(), self.lam, 1, out3, c(s.loss_ws.data_ptr()), s.loss_ws.numel(),
-                                          stream), "gs_ssim_l1_forward")
-        _lib.check(lib.gs_ssim_l1_backward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
-                                           _taps(), self.lam, 1, c(self.one.data_ptr()), c(s.loss_ws.data_ptr()),
-                                           s.loss_ws.numel(), c(s.dcolor.data_ptr()), stream),
-                   "gs_ssim_l1_backward")
-        _lib.check(lib.gs_backward(ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()),
-                                   c(s.geom.data_ptr()), c(s.binning.data_ptr()), s.capacity, c(s.image.data_ptr()),
-                                   c(s.counters.data_ptr()), c(s.dcolor.data_ptr()), None, None,
-                                   ctypes.byref(s.grads), c(s.scratch.data_ptr()), s.scratch.numel(), stream),
-                   "gs_backward")
+        launch_slot(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, stream)
 
     def _launch_iteration(self):
         cur = torch.cuda.current_stream(self.device)

@@ -94,3 +94,23 @@ def test_lift_points_pinhole_and_supergrid():
     # same seed, same points
     xyz4, _, _, _ = lift_points(1, 64, 36, 1000)
     assert torch.equal(xyz3, xyz4)
+
+
+def test_serving_request_stream_and_percentiles():
+    from gs_hip.serve import Request, make_requests, summarize
+    a = make_requests(200, rate=10.0, n_scenes=16, seed=5)
+    b = make_requests(200, rate=10.0, n_scenes=16, seed=5)
+    assert [(r.kind, r.arrival, r.scene, r.W) for r in a] == [(r.kind, r.arrival, r.scene, r.W) for r in b]
+    arr = [r.arrival for r in a]
+    assert arr == sorted(arr)
+    frac = sum(r.kind == "render" for r in a) / len(a)
+    assert 0.6 < frac < 0.8
+    # mean inter-arrival ~ 1 / rate
+    assert abs(arr[-1] / len(a) - 0.1) < 0.03
+    done = []
+    for i in range(100):
+        r = Request(rid=i, kind="render", arrival=0.0)
+        r.done = (i + 1) / 100.0
+        done.append(r)
+    s = summarize(done, 2.0)
+    assert s["p50_s"] == 0.5 and s["p99_s"] == 0.99 and s["throughput_rps"] == 50.0
