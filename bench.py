@@ -43,7 +43,7 @@ def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, pas
             return P * 12 + P * (12 + 32) + 8 * n_dup
         return P * 56 + 12 * n_dup  # splat 48 + tiles 4 + scan 4; out: key 8 + id 4 per duplicated entry
     if stage == "sort":
-        if depth_first:         # 4 passes over P (key 4, id 4, in + out) + tile passes over the list (4 + 4, in + out)
+        if depth_first:         # 4 passes over P (32-bit key 4, id 4, in + out) + tile passes over the list
             return 4 * 16 * P + passes * 16 * n_dup
         return passes * 24 * n_dup  # each 8-bit pass reads and writes (key 8, value 4)
     if stage == "ranges":
@@ -133,11 +133,12 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
         return {"workload": f"{n} pose solves x {iters} Adam iterations (batched path only)",
                 "batched_solves_per_s": round(n / batched_s, 3), "batched_seconds": round(batched_s, 3),
                 "autotune": {"streams": cfg["streams"], "binning": cfg["binning"]}}
-    plain_t, diffs = [], []
+    plain_t, diffs, same = [], [], 0
     for k, inst in enumerate(insts):
         o = solve_plain_native(inst, iters=iters)
         plain_t.append(o["seconds"])
         diffs.append(pose_errors(o["pose"], r["poses"][k]))
+        same += int(torch.equal(o["pose"], r["poses"][k]))
     seq_lat = []
     acc = 0.0
     for t in plain_t:
@@ -170,6 +171,7 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
         "plain_torch_s_per_solve_sample": round(sum(torch_t) / max(1, len(torch_t)), 3),
         "speedup_vs_plain_torch": round((sum(torch_t) / max(1, len(torch_t))) * n / batched_s, 3),
         "max_pose_diff_vs_plain_native": [max(d[0] for d in diffs), max(d[1] for d in diffs)],
+        "bitwise_equal_to_plain_native": f"{same}/{n}",
         "pose_err_vs_truth_median": [rot_gt[len(rot_gt) // 2], sorted(tr_gt)[len(tr_gt) // 2]],
         "pose_err_vs_truth_max": [max(rot_gt), max(tr_gt)],
         "autotune": {"streams": cfg["streams"], "binning": "depth_first" if cfg["binning"] == 0 else "sort64",

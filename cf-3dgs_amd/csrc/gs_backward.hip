@@ -138,6 +138,8 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
   __shared__ float4 s_a[256];
   __shared__ float4 s_b[256];
   __shared__ float4 s_c[256];
+  __shared__ uint32_t s_mask[256];    // wave rows each staged splat can reach (splat_box)
+  __shared__ uint8_t s_list[4][256];  // per wave: the staged splats that reach its pixels
   __shared__ float s_acc[10 * kParts][256];
   const uint32_t nitems = a.item_base[a.nt];
   // The grid is an upper bound on the item count, so items are taken in
@@ -167,15 +169,19 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
   const uint32_t r0 = a.ranges[2 * tile];
   {
     const uint32_t q = cstart + threadIdx.x;
+    uint32_t m = 0u;
     if (q < cend) {
       const uint32_t g = a.list[r0 + q];
       s_id[threadIdx.x] = g;
-      s_a[threadIdx.x] = a.splat[3 * g];
+      const float4 sa = a.splat[3 * g];
+      s_a[threadIdx.x] = sa;
       float4 b = a.splat[3 * g + 1];
       b.w = skip_threshold(b.y);
       s_b[threadIdx.x] = b;
       s_c[threadIdx.x] = a.splat[3 * g + 2];
+      m = wave_rows_hit(splat_box(sa.x, sa.y, sa.z, sa.w, b.x, b.w), (float)(tx * kBX), (float)(ty * kBY));
     }
+    s_mask[threadIdx.x] = m;
 #pragma unroll
     for (int k = 0; k < 10 * kParts; ++k) s_acc[k][threadIdx.x] = 0.0f;
   }
@@ -210,7 +216,10 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
   float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f, last_d = 0.f;
   const int lane = threadIdx.x & 63;
   __syncthreads();
-  for (int j = (int)(cend - cstart) - 1; j >= 0; --j) {
+  // the wave's own list: splats that miss its 16 x 4 pixels are ones no lane would use
+  const int nw = compact_wave_list(s_mask, s_list[threadIdx.x >> 6], threadIdx.x >> 6, lane);
+  for (int i = nw - 1; i >= 0; --i) {
+    const int j = s_list[threadIdx.x >> 6][i];
     const uint32_t q = cstart + j;  // 0-based list position; contributor index q + 1
     bool active = inside && q < last;
     float v[12];

@@ -250,6 +250,58 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Pixel-independent form of the same test: an axis-aligned box around the
+// splat centre outside which power < thr - 0.01 everywhere.  The set
+// {power >= thr - 0.01} is the ellipse {Q <= L}, Q = a dx^2 + 2 b dx dy + c dy^2,
+// L = -2 (thr - 0.01), whose bounding box has half-extents sqrt(L c / det) and
+// sqrt(L a / det), det = a c - b^2.  The 0.01 in the exponent, the relative
+// 1e-4 and the absolute 1e-3 pixel on the extents cover the rounding of the
+// per-pixel power and of this computation by orders of magnitude, so a pixel
+// outside the box is one the per-pixel test skips as well.  Returned as
+// (x0, x1, y0, y1); thr NaN or a non-positive-definite conic give an infinite
+// box (never skipped), thr >= 0.01 an empty one (power <= 0 < thr anyway).
+__device__ __forceinline__ float4 splat_box(float x, float y, float a, float b, float c, float thr) {
+  const float inf = __builtin_inff();
+  if (thr != thr) return make_float4(-inf, inf, -inf, inf);
+  const float L = -2.0f * (thr - 0.01f);
+  if (!(L > 0.0f)) return make_float4(inf, -inf, inf, -inf);
+  const float det = a * c - b * b;
+  if (!(det > 0.0f)) return make_float4(-inf, inf, -inf, inf);
+  const float ex = sqrtf(L * c / det) * 1.0001f + 1e-3f;
+  const float ey = sqrtf(L * a / det) * 1.0001f + 1e-3f;
+  if (!(ex == ex) || !(ey == ey)) return make_float4(-inf, inf, -inf, inf);
+  return make_float4(x - ex, x + ex, y - ey, y + ey);
+}
+
+// Bit r set when the box meets pixel rows [y0 + 4r, y0 + 4r + 3] of the tile
+// columns [x0, x0 + 15]: the 16 x 4 pixels of wave r of a 16 x 16 tile.
+__device__ __forceinline__ uint32_t wave_rows_hit(float4 bx, float x0, float y0) {
+  if (bx.y < x0 || bx.x > x0 + 15.0f) return 0u;
+  uint32_t m = 0u;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const float ya = y0 + 4.0f * (float)r;
+    if (!(bx.w < ya || bx.z > ya + 3.0f)) m |= 1u << r;
+  }
+  return m;
+}
+
+// The wave's own list of the staged splats that meet its pixels: positions j
+// (ascending) with bit `wv` set in s_mask[j]; returns the count (wave-uniform).
+// Needs all 64 lanes active.
+__device__ __forceinline__ int compact_wave_list(const uint32_t* s_mask, uint8_t* s_list, int wv, int lane) {
+  int n = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int idx = c * 64 + lane;
+    const bool hit = (s_mask[idx] >> wv) & 1u;
+    const uint64_t bal = __ballot(hit);
+    if (hit) s_list[n + __popcll(bal & lanemask_lt())] = (uint8_t)idx;
+    n += __popcll(bal);
+  }
+  return n;
+}
+
 // Block id -> logical id so that each of the 8 XCDs (dispatch is round-robin
 // over XCDs) receives a contiguous run of logical work items: neighbouring
 // tiles share Gaussians, so this keeps their gathers in one L2.

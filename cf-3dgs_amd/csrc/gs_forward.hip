@@ -12,8 +12,10 @@
 //   radix sort     gs_sort.hip, 8-bit LSD passes over bits [0, 32 + bits(T))
 //   k_ranges       [start, end) per tile in the sorted list
 //   k_blend        one 16x16 tile per 256-thread workgroup; splats staged in
-//                  LDS 256 at a time; block-wide vote ends the tile early once
-//                  every pixel is saturated; XCD-aware tile order.
+//                  LDS 256 at a time, each wave keeping its own list of the
+//                  staged splats whose alpha can reach 1/255 on its 16 x 4
+//                  pixels (splat_box); block-wide vote ends the tile early
+//                  once every pixel is saturated; XCD-aware tile order.
 #include "gs_common.hpp"
 #include "gs_internal.hpp"
 
@@ -464,16 +466,19 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
   __shared__ float4 s_a[256];  // x, y, conic a, conic b
   __shared__ float4 s_b[256];  // conic c, opacity, depth, -
   __shared__ float4 s_c[256];  // r, g, b, -
+  __shared__ uint32_t s_mask[256];    // wave rows each staged splat can reach (splat_box)
+  __shared__ uint8_t s_list[4][256];  // per wave: the staged splats that reach its 16 x 4 pixels
   __shared__ uint32_t s_maxc;
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.gx, ty = tile / a.gx;
   const int px = tx * kBX + (threadIdx.x & 15), py = ty * kBY + (threadIdx.x >> 4);
   const bool inside = px < a.W && py < a.H;
   const float pfx = (float)px, pfy = (float)py;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t r0 = a.ranges[2 * tile], r1 = a.ranges[2 * tile + 1];
   bool done = !inside;
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
-  uint32_t contributor = 0, last = 0;
+  uint32_t last = 0;
   const float a_min = 1.0f / 255.0f;
   if (threadIdx.x == 0) s_maxc = 0;
   uint32_t batch = 0;
@@ -488,38 +493,58 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
     }
     if (__syncthreads_count(done) == 256) break;
     const uint32_t k = base + threadIdx.x;
+    uint32_t m = 0u;
     if (k < r1) {
       const uint32_t g = a.list[k];
-      s_a[threadIdx.x] = a.splat[3 * g];
+      const float4 sa = a.splat[3 * g];
+      s_a[threadIdx.x] = sa;
       float4 b = a.splat[3 * g + 1];
       b.w = skip_threshold(b.y);  // (the clamp bits in .w are not needed here)
       s_b[threadIdx.x] = b;
       s_c[threadIdx.x] = a.splat[3 * g + 2];
+      m = wave_rows_hit(splat_box(sa.x, sa.y, sa.z, sa.w, b.x, b.w), (float)(tx * kBX), (float)(ty * kBY));
     }
+    s_mask[threadIdx.x] = m;
     __syncthreads();
-    const int n = (int)min(256u, r1 - base);
-    for (int j = 0; !done && j < n; ++j) {
-      ++contributor;
+    // a splat that misses the wave's pixels is one every lane would skip
+    const int nw = compact_wave_list(s_mask, s_list[wv], wv, lane);
+    const uint32_t first = base - r0 + 1;  // contributor index of staged position 0
+    // alpha of staged splat j at this pixel, 0 when it is skipped
+    auto alpha_at = [&](int j) -> float {
       const float4 ga = s_a[j];
       const float4 gb = s_b[j];
       const float dx = ga.x - pfx, dy = ga.y - pfy;
       const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
-      if (power > 0.0f || power < gb.w) continue;  // gb.w: alpha < 1/255 for sure below it
+      if (power > 0.0f || power < gb.w) return 0.0f;  // gb.w: alpha < 1/255 for sure below it
       const float alpha = fminf(0.99f, gb.y * spec_expf(power));
-      if (alpha < a_min) continue;
+      return alpha < a_min ? 0.0f : alpha;
+    };
+    auto apply = [&](int j, float alpha) {
+      if (alpha == 0.0f) return;
       const float test_T = T * (1.0f - alpha);
       if (test_T < 0.0001f) {
         done = true;
-        continue;
+        return;
       }
       const float w = alpha * T;
       const float4 gc = s_c[j];
       C0 = fmaf(gc.x, w, C0);
       C1 = fmaf(gc.y, w, C1);
       C2 = fmaf(gc.z, w, C2);
-      D = fmaf(gb.z, w, D);
+      D = fmaf(s_b[j].z, w, D);
       T = test_T;
-      last = contributor;
+      last = first + (uint32_t)j;
+    };
+    // two list entries per step: their alphas do not depend on each other, so
+    // both exponentials are in flight together; they are applied in order
+    for (int i = 0; !done && i < nw; i += 2) {
+      const int j0 = s_list[wv][i];
+      const bool two = i + 1 < nw;
+      const int j1 = two ? (int)s_list[wv][i + 1] : j0;
+      const float a0 = alpha_at(j0);
+      const float a1 = two ? alpha_at(j1) : 0.0f;
+      apply(j0, a0);
+      if (!done) apply(j1, a1);
     }
   }
   if (inside) {

@@ -11,7 +11,8 @@ namespace gs {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort workgroup
-constexpr int kRadix = 256;
+constexpr int kRadixBits = 8;  // digit width of the LSD passes: 256 buckets
+constexpr int kRadix = 1 << kRadixBits;
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 inline int bits_for(uint32_t n) {  // number of bits needed to hold values < n
@@ -19,7 +20,8 @@ inline int bits_for(uint32_t n) {  // number of bits needed to hold values < n
   while (b < 32 && (1ull << b) < (uint64_t)n) ++b;
   return b;
 }
-inline int sort_passes(int end_bit) { return (end_bit + 7) / 8; }
+// LSD passes over bits [0, end_bit), kRadixBits at a time.
+inline int sort_passes(int end_bit) { return (end_bit + kRadixBits - 1) / kRadixBits; }
 
 namespace timing {
 enum Stage { kPreprocess = 0, kScan, kDuplicate, kSort, kRanges, kBlend, kRenderBwd, kPreprocessBwd, kKnn, kZero };

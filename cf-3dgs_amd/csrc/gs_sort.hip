@@ -17,6 +17,10 @@
 //                   this wave + lanes below with the same digit; waves are
 //                   ordered by an exclusive prefix over the 4 wave counts, so
 //                   the order of equal digits is the input order (stable).
+//                   The workgroup's keys are first placed in LDS in digit
+//                   order, then written out by consecutive threads: each
+//                   digit's run leaves as contiguous stores instead of one
+//                   scattered 4-byte store per lane.
 #include "gs_common.hpp"
 #include "gs_internal.hpp"
 
@@ -38,15 +42,15 @@ __device__ __forceinline__ uint32_t n_eff(const uint32_t* n_dev, long long cap) 
   return ((long long)n > cap) ? 0u : n;
 }
 
-template <typename K, int ITEMS>
+template <typename K, int ITEMS, int R>
 __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restrict__ keys, const uint32_t* n_dev,
                                                                long long cap, int shift, int bits, uint32_t* hist,
                                                                int nb_max) {
-  __shared__ uint32_t s_h[4][kRadix];
+  __shared__ uint32_t s_h[4][R];
   const uint32_t n = n_eff(n_dev, cap);
   const uint32_t start = blockIdx.x * (uint32_t)(kSortThreads * ITEMS);
   if (start >= n) return;
-  for (int k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) (&s_h[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < 4 * R; k += kSortThreads) (&s_h[0][0])[k] = 0;
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t mask = (1u << bits) - 1u;
@@ -61,7 +65,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
     if (leader) s_h[w][d] += (uint32_t)__popcll(peers);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kRadix; d += kSortThreads)
+  for (int d = threadIdx.x; d < R; d += kSortThreads)
     hist[(size_t)d * nb_max + blockIdx.x] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
 }
 
@@ -97,7 +101,7 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* tot
   if (t == 0) totals[d] = s_carry;
 }
 
-template <typename K, int ITEMS>
+template <typename K, int ITEMS, int R>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restrict__ keys_in,
                                                                const uint32_t* __restrict__ vals_in,
                                                                K* __restrict__ keys_out,
@@ -105,10 +109,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
                                                                const uint32_t* n_dev, long long cap, int shift,
                                                                int bits, const uint32_t* hist,
                                                                const uint32_t* totals, int nb_max) {
-  __shared__ uint32_t s_cnt[4][kRadix];
-  __shared__ uint32_t s_scan[kRadix];
+  static_assert(R == kSortThreads, "one digit per thread");
+  constexpr int TILE = kSortThreads * ITEMS;
+  __shared__ uint32_t s_cnt[4][R];
+  __shared__ uint32_t s_scan[R];
+  __shared__ K s_key[TILE];
+  __shared__ uint32_t s_val[TILE];
   const uint32_t n = n_eff(n_dev, cap);
-  const uint32_t start = blockIdx.x * (uint32_t)(kSortThreads * ITEMS);
+  const uint32_t start = blockIdx.x * (uint32_t)TILE;
   if (start >= n) return;
   const int t = threadIdx.x;
   // global base of each digit for this block = exclusive scan of totals + row offset
@@ -116,7 +124,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     const uint32_t v = totals[t];
     s_scan[t] = v;
     __syncthreads();
-    for (int off = 1; off < kRadix; off <<= 1) {
+    for (int off = 1; off < R; off <<= 1) {
       const uint32_t add = t >= off ? s_scan[t - off] : 0u;
       __syncthreads();
       s_scan[t] += add;
@@ -126,7 +134,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     __syncthreads();
     s_scan[t] = excl;
   }
-  for (int k = t; k < 4 * kRadix; k += kSortThreads) (&s_cnt[0][0])[k] = 0;
+  for (int k = t; k < 4 * R; k += kSortThreads) (&s_cnt[0][0])[k] = 0;
   __syncthreads();
   const int w = t >> 6, lane = t & 63;
   const uint32_t mask = (1u << bits) - 1u;
@@ -148,25 +156,44 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     if (valid && below == 0) s_cnt[w][d] = prior + (uint32_t)__popcll(peers);
   }
   __syncthreads();
-  {
-    // per digit t: prefix over the 4 waves, plus the block's global base
-    const uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t];
-    const uint32_t b = s_scan[t];
-    s_cnt[0][t] = b;
-    s_cnt[1][t] = b + c0;
-    s_cnt[2][t] = b + c0 + c1;
-    s_cnt[3][t] = b + c0 + c1 + c2;
+  // digit t: its waves' local bases (exclusive scan over digits of the block
+  // counts, then over waves) and the offset from local to global position
+  __shared__ uint32_t s_tot[R];
+  const uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t], c3 = s_cnt[3][t];
+  const uint32_t c = c0 + c1 + c2 + c3;
+  s_tot[t] = c;
+  __syncthreads();
+  for (int off = 1; off < R; off <<= 1) {
+    const uint32_t add = t >= off ? s_tot[t - off] : 0u;
+    __syncthreads();
+    s_tot[t] += add;
+    __syncthreads();
   }
+  const uint32_t lbase = s_tot[t] - c;
+  s_cnt[0][t] = lbase;
+  s_cnt[1][t] = lbase + c0;
+  s_cnt[2][t] = lbase + c0 + c1;
+  s_cnt[3][t] = lbase + c0 + c1 + c2;
+  s_scan[t] = s_scan[t] - lbase;  // global = s_scan[d] + local position
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const uint32_t idx = wbase + r * 64 + lane;
     if (idx < n) {
       const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
-      const uint32_t dst = s_cnt[w][d] + pos[r];
-      keys_out[dst] = key[r];
-      vals_out[dst] = val[r];
+      const uint32_t lp = s_cnt[w][d] + pos[r];
+      s_key[lp] = key[r];
+      s_val[lp] = val[r];
     }
+  }
+  __syncthreads();
+  const uint32_t nloc = min((uint32_t)TILE, n - start);
+  for (uint32_t i = t; i < nloc; i += kSortThreads) {
+    const K k = s_key[i];
+    const uint32_t d = (uint32_t)(k >> shift) & mask;
+    const uint32_t dst = s_scan[d] + i;
+    keys_out[dst] = k;
+    vals_out[dst] = s_val[i];
   }
 }
 
@@ -178,6 +205,16 @@ size_t sort_tmp_bytes(long long capacity) {
   const long long tile = sort_tile(capacity);
   const long long nb = (capacity + tile - 1) / tile;
   return align_up((size_t)kRadix * 4 * (size_t)(nb > 0 ? nb : 1), 256) + kRadix * 4;
+}
+
+template <typename K, int ITEMS, int R>
+void sort_pass(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, const uint32_t* n_dev, long long capacity,
+               int shift, int bits, uint32_t* hist, uint32_t* totals, int nb_max, int tile, hipStream_t s) {
+  hipLaunchKernelGGL((k_sort_upsweep<K, ITEMS, R>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity,
+                     shift, bits, hist, nb_max);
+  hipLaunchKernelGGL(k_sort_scan, dim3(R), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max, tile);
+  hipLaunchKernelGGL((k_sort_scatter<K, ITEMS, R>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout,
+                     n_dev, capacity, shift, bits, hist, totals, nb_max);
 }
 
 template <typename K>
@@ -195,21 +232,10 @@ hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
   K* kout = keys_b;
   uint32_t* vout = vals_b;
   for (int p = 0; p < passes; ++p) {
-    const int shift = 8 * p;
-    const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
-    if (tile == kSortThreads * 4) {
-      hipLaunchKernelGGL((k_sort_upsweep<K, 4>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity, shift,
-                         bits, hist, nb_max);
-      hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max, tile);
-      hipLaunchKernelGGL((k_sort_scatter<K, 4>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_dev,
-                         capacity, shift, bits, hist, totals, nb_max);
-    } else {
-      hipLaunchKernelGGL((k_sort_upsweep<K, 16>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity,
-                         shift, bits, hist, nb_max);
-      hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max, tile);
-      hipLaunchKernelGGL((k_sort_scatter<K, 16>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout,
-                         n_dev, capacity, shift, bits, hist, totals, nb_max);
-    }
+    const int shift = kRadixBits * p;
+    const int bits = end_bit - shift < kRadixBits ? end_bit - shift : kRadixBits;
+    auto pass = tile == kSortThreads * 4 ? sort_pass<K, 4, kRadix> : sort_pass<K, 16, kRadix>;
+    pass(kin, vin, kout, vout, n_dev, capacity, shift, bits, hist, totals, nb_max, tile, s);
     K* tk = kin;
     kin = kout;
     kout = tk;

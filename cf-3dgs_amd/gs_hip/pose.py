@@ -232,7 +232,8 @@ def solve_plain(inst: PoseInstance, iters: int = ITERS, lr: float = LR, lambda_d
     seconds = time.perf_counter() - t0
     with torch.no_grad():
         pose = se3_exp(xi, x0)
-    return dict(pose=pose.detach(), xi=xi.detach().clone(), loss=float(loss) if loss is not None else float("nan"),
+    return dict(pose=pose.detach(), xi=xi.detach().clone(),
+                loss=float(loss.detach()) if loss is not None else float("nan"),
                 seconds=seconds)
 
 

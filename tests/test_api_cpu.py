@@ -32,10 +32,13 @@ def test_library_exports_every_declared_symbol():
 def test_layouts():
     from gs_hip import _lib
     L = _lib.binning_layout(1000, 500, 960, 540, _lib.BINNING_SORT64)
-    assert L.sort_passes == 6 and L.key_bytes == 8  # 2040 tiles -> 11 bits -> 43-bit keys
-    assert _lib.binning_layout(10, 5, 256, 256, _lib.BINNING_SORT64).sort_passes == 5  # 256 tiles -> 8 bits
+    # 8-bit digits: 2040 tiles -> 11 bits -> 43-bit keys -> 6 passes
+    assert L.sort_passes == 6 and L.key_bytes == 8
+    assert _lib.binning_layout(10, 5, 256, 256, _lib.BINNING_SORT64).sort_passes == 5  # 8 + 32 bits
+    assert _lib.binning_layout(10, 5, 16, 16, _lib.BINNING_SORT64).sort_passes == 4  # 1 tile: 32 bits
     D = _lib.binning_layout(1000, 500, 960, 540, _lib.BINNING_DEPTH_FIRST)
     assert D.sort_passes == 2 and D.key_bytes == 4  # tile ids only: 11 bits
+    assert _lib.binning_layout(1000, 500, 1920, 1080, _lib.BINNING_DEPTH_FIRST).sort_passes == 2  # 13 bits
     assert D.dscan > D.dvals_b > D.dkeys_b > D.dvals_a > D.dkeys_a >= D.vals_alt
     G = _lib.geom_layout(1000)
     assert G.total >= 1000 * (48 + 24 + 8) and G.splat % 256 == 0 and G.cov3D % 256 == 0

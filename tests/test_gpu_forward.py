@@ -6,7 +6,6 @@ screen quantities and the colour / depth / alpha images are bit-exact too
 (both sides use the same explicit rounding steps), and in any case within the
 stated image tolerance max |error| <= 1e-5.
 """
-import math
 
 import numpy as np
 import pytest
@@ -14,7 +13,6 @@ import torch
 
 from gpu_helpers import DEV, np_, run_gpu, run_oracle, u64
 from gs_hip import scenes
-from gs_hip.camera import make_camera
 
 pytestmark = pytest.mark.gpu
 

@@ -11,7 +11,7 @@ from scipy import special
 from scipy.spatial import cKDTree
 from scipy.spatial.transform import Rotation
 
-from gs_hip.camera import fov2focal, make_camera, oracle_camera
+from gs_hip.camera import make_camera, oracle_camera
 from oracle import oracle as O
 
 
