@@ -125,14 +125,15 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
     t0 = time.perf_counter()
     cfg = autotune(insts)
     t_tune = time.perf_counter() - t0
-    solver = BatchedPoseSolver(insts, iters=iters, streams=cfg["streams"], binning=cfg["binning"])
+    solver = BatchedPoseSolver(insts, iters=iters, streams=cfg["streams"], binning=cfg["binning"],
+                               multi=cfg["multi"])
     r = solver.timed_run()
     batched_s = r["seconds"]
     _log(f"pose: batched {batched_s:.2f} s; plain one-at-a-time solves")
     if quick:
         return {"workload": f"{n} pose solves x {iters} Adam iterations (batched path only)",
                 "batched_solves_per_s": round(n / batched_s, 3), "batched_seconds": round(batched_s, 3),
-                "autotune": {"streams": cfg["streams"], "binning": cfg["binning"]}}
+                "autotune": {"streams": cfg["streams"], "binning": cfg["binning"], "multi": cfg["multi"]}}
     plain_t, diffs, same = [], [], 0
     for k, inst in enumerate(insts):
         o = solve_plain_native(inst, iters=iters)
@@ -175,7 +176,10 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
         "pose_err_vs_truth_median": [rot_gt[len(rot_gt) // 2], sorted(tr_gt)[len(tr_gt) // 2]],
         "pose_err_vs_truth_max": [max(rot_gt), max(tr_gt)],
         "autotune": {"streams": cfg["streams"], "binning": "depth_first" if cfg["binning"] == 0 else "sort64",
-                     "seconds": round(t_tune, 2)},
+                     "one_launch_blend_and_backward": cfg["multi"], "seconds": round(t_tune, 2),
+                     "candidates_ms_per_iter": {f"s{c['streams']}-{'df' if c['binning'] == 0 else 's64'}-"
+                                                f"{'multi' if c['multi'] else 'per-frame'}": round(c["ms_per_iter"], 2)
+                                                for c in cfg["candidates"]}},
         "regrows": solver.regrows,
         "c3_single_frame_s": {"plain_torch": round(c3_torch["seconds"], 3),
                               "plain_native": round(c3_native["seconds"], 3),

@@ -132,7 +132,7 @@ __device__ __forceinline__ void wave_sum12(const float v[12], float w[3]) {
 }
 
 template <bool DET>
-__global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
+__device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) {
   constexpr int kParts = DET ? 4 : 1;  // LDS partial sets: one per wave when deterministic
   __shared__ uint32_t s_id[256];
   __shared__ float4 s_a[256];
@@ -142,10 +142,6 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
   __shared__ uint8_t s_list[4][256];  // per wave: the staged splats that reach its pixels
   __shared__ float s_acc[10 * kParts][256];
   const uint32_t nitems = a.item_base[a.nt];
-  // The grid is an upper bound on the item count, so items are taken in
-  // dispatch order: live items then go round-robin over all 8 XCDs (a
-  // contiguous-per-XCD remap would put them all on the first few).
-  const uint32_t item = blockIdx.x;
   if (item >= nitems) return;
   // tile = last t with item_base[t] <= item (tiles without items share the next base)
   int lo = 0, hi = a.nt - 1;
@@ -321,6 +317,29 @@ __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
       }
     }
   }
+}
+
+// The grid is an upper bound on the item count, so items are taken in
+// dispatch order: live items then go round-robin over all 8 XCDs (a
+// contiguous-per-XCD remap would put them all on the first few).
+template <bool DET>
+__global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
+  render_bwd_item<DET>(a, blockIdx.x);
+}
+
+// Many frames in one launch: frame f owns blocks [prefix[f], prefix[f + 1]),
+// an upper bound on its work items.
+template <bool DET>
+__global__ __launch_bounds__(256) void k_render_bwd_multi(const uint32_t* prefix, const RBArgs* args, int n) {
+  const uint32_t b = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const RBArgs a = args[lo];
+  render_bwd_item<DET>(a, b - prefix[lo]);
 }
 
 // Deterministic mode: per Gaussian, the partials of its (Gaussian, tile)
@@ -676,10 +695,60 @@ extern "C" size_t gs_backward_scratch_bytes(int P, long long capacity, int deter
   return n;
 }
 
-extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
-                           const void* binning, long long capacity, void* image, const uint32_t* counters,
-                           const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
-                           const gs_grads_t* out, void* scratch, size_t scratch_bytes, void* stream) {
+// Render-backward arguments of one frame (workspaces of its forward).
+static RBArgs make_rb_args(const gs_frame_t* f, int P, const void* geom, long long capacity, const void* binning,
+                           void* image, const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
+                           void* scratch) {
+  gs_geom_layout_t LG;
+  gs_binning_layout_t LB;
+  gs_image_layout_t LI;
+  gs_geom_layout(P, &LG);
+  gs_binning_layout(capacity, P, f->width, f->height, f->binning, &LB);
+  gs_image_layout(f->width, f->height, &LI);
+  const int gx = (f->width + kBX - 1) / kBX, gy = (f->height + kBY - 1) / kBY;
+  const char* gb = (const char*)geom;
+  const char* bb = (const char*)binning;
+  char* ib = (char*)image;
+  float* g_render = (float*)scratch;
+  float* pose_part = (float*)((char*)scratch + align_up((size_t)P * kRG * sizeof(float), 256));
+  float* part = (float*)((char*)pose_part + align_up(pose_parts(P) * 12 * sizeof(float), 256));
+  RBArgs r;
+  r.W = f->width;
+  r.H = f->height;
+  r.gx = gx;
+  r.gy = gy;
+  r.nt = gx * gy;
+  r.ranges = (const uint32_t*)(ib + LI.ranges);
+  r.list = capacity > 0 ? (const uint32_t*)(bb + LB.vals_sorted) : nullptr;
+  r.splat = (const float4*)(gb + LG.splat);
+  r.bg = f->bg;
+  r.final_T = (const float*)(ib + LI.final_T);
+  r.n_contrib = (const uint32_t*)(ib + LI.n_contrib);
+  r.accum = (const float*)(ib + LI.accum);
+  r.ckpt = capacity > 0 ? (const float*)(bb + LB.ckpt) : nullptr;
+  r.ckpt_base = (const uint32_t*)(ib + LI.ckpt_base);
+  r.tile_maxc = (const uint32_t*)(ib + LI.tile_maxc);
+  r.item_base = (const uint32_t*)(ib + LI.item_base);
+  r.dL_dcolor = dL_dcolor;
+  r.dL_ddepth = dL_ddepth;
+  r.dL_dalpha = dL_dalpha;
+  r.g_render = g_render;
+  r.local_incl = (const uint32_t*)(gb + LG.offsets);
+  r.block_excl = (const uint32_t*)(gb + LG.scan_tmp);
+  r.part = part;
+  return r;
+}
+
+// Work-item upper bound of one frame's render backward: sum over tiles of
+// ceil(list length / 256) <= capacity / 256 + tiles.
+static long long rb_max_items(const gs_frame_t* f, long long capacity) {
+  const int gx = (f->width + kBX - 1) / kBX, gy = (f->height + kBY - 1) / kBY;
+  return capacity / 256 + (long long)gx * gy + 1;
+}
+
+static int bwd_check(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
+                     const void* binning, long long capacity, void* image, const uint32_t* counters,
+                     const float* dL_dcolor, const gs_grads_t* out, void* scratch, size_t scratch_bytes) {
   if (!f || !g || !out || !counters || !image || !dL_dcolor) return GS_E_ARG;
   if (f->width <= 0 || f->height <= 0 || !f->viewmatrix || !f->projmatrix || !f->bg) return GS_E_ARG;
   const int P = g->P;
@@ -692,23 +761,16 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
   if (!g->shs && out->dL_dsh) return GS_E_ARG;
   if (!has_sr && (out->dL_dscales || out->dL_drotations)) return GS_E_ARG;
   if (out->dL_dpose && !f->pose) return GS_E_ARG;
-  const bool det = f->deterministic != 0;
-  if (scratch_bytes < gs_backward_scratch_bytes(P, capacity, det) || !scratch) return GS_E_WORKSPACE;
-  gs_geom_layout_t LG;
+  if (capacity < 0 || capacity > 0x7fffffffLL || (capacity > 0 && !binning)) return GS_E_ARG;
+  if (scratch_bytes < gs_backward_scratch_bytes(P, capacity, f->deterministic != 0) || !scratch)
+    return GS_E_WORKSPACE;
   gs_binning_layout_t LB;
-  gs_image_layout_t LI;
-  gs_geom_layout(P, &LG);
   if (gs_binning_layout(capacity, P, f->width, f->height, f->binning, &LB) != GS_OK) return GS_E_ARG;
-  gs_image_layout(f->width, f->height, &LI);
-  hipStream_t s = (hipStream_t)stream;
-  const int gx = (f->width + kBX - 1) / kBX, gy = (f->height + kBY - 1) / kBY;
-  const int nt = gx * gy;
-  const char* gb = (const char*)geom;
-  const char* bb = (const char*)binning;
-  char* ib = (char*)image;
-  float* g_render = (float*)scratch;
-  float* pose_part = (float*)((char*)scratch + align_up((size_t)P * kRG * sizeof(float), 256));
-  float* part = (float*)((char*)pose_part + align_up(pose_parts(P) * 12 * sizeof(float), 256));
+  return GS_OK;
+}
+
+// Phase 1: clear the accumulators and build the work-item table.
+static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s) {
   auto zero = [&](float* p, long long n) {
     const long long nb = (n + 255) / 256;
     if (n > 0)
@@ -716,48 +778,38 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
   };
   // deterministic mode writes every record in k_det_gather; pairs past a
   // tile's last contributor are never visited, so their partials start at 0
-  if (det) zero(part, capacity * 10);
-  else zero(g_render, (long long)P * kRG);
-  if (capacity > 0 && P > 0) {
-    RBArgs r;
-    r.W = f->width;
-    r.H = f->height;
-    r.gx = gx;
-    r.gy = gy;
-    r.ranges = (const uint32_t*)(ib + LI.ranges);
-    r.list = (const uint32_t*)(bb + LB.vals_sorted);
-    r.splat = (const float4*)(gb + LG.splat);
-    r.bg = f->bg;
-    r.final_T = (const float*)(ib + LI.final_T);
-    r.n_contrib = (const uint32_t*)(ib + LI.n_contrib);
-    r.nt = nt;
-    r.accum = (const float*)(ib + LI.accum);
-    r.ckpt = (const float*)(bb + LB.ckpt);
-    r.ckpt_base = (const uint32_t*)(ib + LI.ckpt_base);
-    r.tile_maxc = (const uint32_t*)(ib + LI.tile_maxc);
-    r.item_base = (const uint32_t*)(ib + LI.item_base);
-    r.dL_dcolor = dL_dcolor;
-    r.dL_ddepth = dL_ddepth;
-    r.dL_dalpha = dL_dalpha;
-    r.g_render = g_render;
-    r.local_incl = (const uint32_t*)(gb + LG.offsets);
-    r.block_excl = (const uint32_t*)(gb + LG.scan_tmp);
-    r.part = part;
-    // (the item table lives in the image workspace, which the forward owns;
-    // the backward rewrites it from tile_maxc before use)
-    hipLaunchKernelGGL(k_bwd_items, dim3(1), dim3(1024), 0, s, r.tile_maxc, nt, (uint32_t*)(ib + LI.item_base));
-    // upper bound on items: sum over tiles of ceil(list length / 256)
-    const long long max_items = capacity / 256 + nt + 1;
-    timing::Scope ts(timing::kRenderBwd, s);
-    if (det)
-      hipLaunchKernelGGL(k_render_bwd<true>, dim3((unsigned)max_items), dim3(256), 0, s, r);
-    else
-      hipLaunchKernelGGL(k_render_bwd<false>, dim3((unsigned)max_items), dim3(256), 0, s, r);
-  }
-  if (det && P > 0)
+  if (f->deterministic) zero(r.part, capacity * 10);
+  else zero(r.g_render, (long long)P * kRG);
+  // (the item table lives in the image workspace, which the forward owns;
+  // the backward rewrites it from tile_maxc before use)
+  if (capacity > 0 && P > 0)
+    hipLaunchKernelGGL(k_bwd_items, dim3(1), dim3(1024), 0, s, r.tile_maxc, r.nt, (uint32_t*)r.item_base);
+}
+
+// Phase 2: the render backward of one frame.
+static void bwd_render(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s) {
+  if (capacity <= 0 || P <= 0) return;
+  const long long max_items = rb_max_items(f, capacity);
+  timing::Scope ts(timing::kRenderBwd, s);
+  if (f->deterministic)
+    hipLaunchKernelGGL(k_render_bwd<true>, dim3((unsigned)max_items), dim3(256), 0, s, r);
+  else
+    hipLaunchKernelGGL(k_render_bwd<false>, dim3((unsigned)max_items), dim3(256), 0, s, r);
+}
+
+// Phase 3: per-Gaussian gradients and the pose gradient.
+static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
+                       long long capacity, const uint32_t* counters, const gs_grads_t* out, const RBArgs& r,
+                       hipStream_t s) {
+  const int P = g->P;
+  gs_geom_layout_t LG;
+  gs_geom_layout(P, &LG);
+  const char* gb = (const char*)geom;
+  float* pose_part = (float*)((char*)r.g_render + align_up((size_t)P * kRG * sizeof(float), 256));
+  if (f->deterministic && P > 0)
     hipLaunchKernelGGL(k_det_gather, dim3((P + 255) / 256), dim3(256), 0, s, P,
                        (const uint32_t*)(gb + LG.tiles), (const uint32_t*)(gb + LG.offsets),
-                       (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, part, g_render);
+                       (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, (const float*)r.part, r.g_render);
   if (P > 0) {
     PBArgs b;
     b.P = P;
@@ -782,7 +834,7 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
     b.radii = radii;
     b.splat = (const float4*)(gb + LG.splat);
     b.cov3D = (const float*)(gb + LG.cov3D);
-    b.g_render = g_render;
+    b.g_render = r.g_render;
     b.dL_dmeans2D = out->dL_dmeans2D;
     b.dL_dcolors = out->dL_dcolors;
     b.dL_dopacity = out->dL_dopacity;
@@ -800,5 +852,94 @@ extern "C" int gs_backward(const gs_frame_t* f, const gs_gaussians_t* g, const i
   if (out->dL_dpose)
     hipLaunchKernelGGL(k_pose_finalize, dim3(1), dim3(256), 0, s, f->pose, pose_part, (int)pose_parts(P),
                        out->dL_dpose);
+}
+
+#define GS_BWD_ARGS                                                                                              \
+  const gs_frame_t *f, const gs_gaussians_t *g, const int *radii, const void *geom, const void *binning,       \
+      long long capacity, void *image, const uint32_t *counters, const float *dL_dcolor, const float *dL_ddepth, \
+      const float *dL_dalpha, const gs_grads_t *out, void *scratch, size_t scratch_bytes, void *stream
+
+extern "C" int gs_backward(GS_BWD_ARGS) {
+  const int rc = bwd_check(f, g, radii, geom, binning, capacity, image, counters, dL_dcolor, out, scratch,
+                           scratch_bytes);
+  if (rc != GS_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const RBArgs r = make_rb_args(f, g->P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
+  bwd_prepare(f, g->P, capacity, r, s);
+  bwd_render(f, g->P, capacity, r, s);
+  bwd_finish(f, g, radii, geom, capacity, counters, out, r, s);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_backward_prepare(GS_BWD_ARGS) {
+  const int rc = bwd_check(f, g, radii, geom, binning, capacity, image, counters, dL_dcolor, out, scratch,
+                           scratch_bytes);
+  if (rc != GS_OK) return rc;
+  const RBArgs r = make_rb_args(f, g->P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
+  bwd_prepare(f, g->P, capacity, r, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_backward_finish(GS_BWD_ARGS) {
+  const int rc = bwd_check(f, g, radii, geom, binning, capacity, image, counters, dL_dcolor, out, scratch,
+                           scratch_bytes);
+  if (rc != GS_OK) return rc;
+  const RBArgs r = make_rb_args(f, g->P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
+  bwd_finish(f, g, radii, geom, capacity, counters, out, r, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+// ---- many frames in one render-backward launch -----------------------------
+// Set layout: uint32 prefix[n + 1] of work-item bounds per frame, then
+// RBArgs[n] at a 256-byte boundary.
+static size_t multi_rb_args_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
+
+extern "C" size_t gs_multi_render_bwd_bytes(int n) {
+  return n <= 0 ? 0 : multi_rb_args_offset(n) + sizeof(RBArgs) * (size_t)n;
+}
+
+extern "C" int gs_multi_render_bwd_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
+                                         const long long* capacity, const void* const* binning, void* const* image,
+                                         const float* const* dL_dcolor, const float* const* dL_ddepth,
+                                         const float* const* dL_dalpha, void* const* scratch, void* host_set,
+                                         size_t set_bytes, unsigned* total_blocks) {
+  if (n <= 0 || !frames || !P || !geom || !capacity || !binning || !image || !dL_dcolor || !scratch || !host_set ||
+      !total_blocks)
+    return GS_E_ARG;
+  if (set_bytes < gs_multi_render_bwd_bytes(n)) return GS_E_WORKSPACE;
+  uint32_t* prefix = (uint32_t*)host_set;
+  RBArgs* args = (RBArgs*)((char*)host_set + multi_rb_args_offset(n));
+  unsigned long long total = 0;
+  const int det = frames[0].deterministic != 0;
+  for (int i = 0; i < n; ++i) {
+    const gs_frame_t* f = &frames[i];
+    if (f->width <= 0 || f->height <= 0 || !f->bg || P[i] < 0 || !image[i] || !dL_dcolor[i] || !scratch[i])
+      return GS_E_ARG;
+    if ((f->deterministic != 0) != det) return GS_E_ARG;  // one kernel variant per launch
+    if (capacity[i] < 0 || capacity[i] > 0x7fffffffLL || (P[i] > 0 && !geom[i]) || (capacity[i] > 0 && !binning[i]))
+      return GS_E_ARG;
+    prefix[i] = (uint32_t)total;
+    if (capacity[i] > 0 && P[i] > 0) total += (unsigned long long)rb_max_items(f, capacity[i]);
+    if (total > 0x7fffffffULL) return GS_E_ARG;
+    args[i] = make_rb_args(f, P[i], geom[i], capacity[i], binning[i], image[i], dL_dcolor[i],
+                           dL_ddepth ? dL_ddepth[i] : nullptr, dL_dalpha ? dL_dalpha[i] : nullptr, scratch[i]);
+  }
+  prefix[n] = (uint32_t)total;
+  *total_blocks = (unsigned)total;
+  return GS_OK;
+}
+
+extern "C" int gs_multi_render_bwd(int n, const void* dev_set, unsigned total_blocks, int deterministic,
+                                   void* stream) {
+  if (n <= 0 || !dev_set) return GS_E_ARG;
+  if (total_blocks == 0) return GS_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t* prefix = (const uint32_t*)dev_set;
+  const RBArgs* args = (const RBArgs*)((const char*)dev_set + multi_rb_args_offset(n));
+  timing::Scope ts(timing::kRenderBwd, s);
+  if (deterministic)
+    hipLaunchKernelGGL(k_render_bwd_multi<true>, dim3(total_blocks), dim3(256), 0, s, prefix, args, n);
+  else
+    hipLaunchKernelGGL(k_render_bwd_multi<false>, dim3(total_blocks), dim3(256), 0, s, prefix, args, n);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }

@@ -462,14 +462,13 @@ struct BlendArgs {
   float* accum;
 };
 
-__global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
+__device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
   __shared__ float4 s_a[256];  // x, y, conic a, conic b
   __shared__ float4 s_b[256];  // conic c, opacity, depth, -
   __shared__ float4 s_c[256];  // r, g, b, -
   __shared__ uint32_t s_mask[256];    // wave rows each staged splat can reach (splat_box)
   __shared__ uint8_t s_list[4][256];  // per wave: the staged splats that reach its 16 x 4 pixels
   __shared__ uint32_t s_maxc;
-  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = tile % a.gx, ty = tile / a.gx;
   const int px = tx * kBX + (threadIdx.x & 15), py = ty * kBY + (threadIdx.x >> 4);
   const bool inside = px < a.W && py < a.H;
@@ -565,6 +564,28 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
   }
   __syncthreads();
   if (threadIdx.x == 0) a.tile_maxc[tile] = s_maxc;
+}
+
+__global__ __launch_bounds__(256) void k_blend(BlendArgs a) { blend_tile(a, xcd_remap(blockIdx.x, gridDim.x)); }
+
+// Frame f of a set whose per-frame block counts have the exclusive prefix
+// `prefix` (n + 1 entries): the last f with prefix[f] <= b.
+__device__ __forceinline__ int frame_of(const uint32_t* prefix, int n, uint32_t b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Blend of many frames in one launch: one workgroup per tile of every frame.
+__global__ __launch_bounds__(256) void k_blend_multi(const uint32_t* prefix, const BlendArgs* args, int n) {
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int f = frame_of(prefix, n, b);
+  const BlendArgs a = args[f];
+  blend_tile(a, b - prefix[f]);
 }
 
 // One workgroup: checkpoint base of every tile = exclusive prefix over tiles
@@ -781,10 +802,11 @@ extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* 
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
-extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
-                         void* binning, size_t binning_bytes, void* image, size_t image_bytes, float* out_color,
-                         float* out_depth, float* out_alpha, void* stream) {
-  if (check_frame(f) != GS_OK || P < 0 || !counters || !image || !out_color || !out_depth) return GS_E_ARG;
+// Binning of one frame: everything of the forward but the blend (ranges
+// cleared, depth-first or 64-bit-key binning, ranges, checkpoint bases).
+static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
+                          void* binning, size_t binning_bytes, void* image, size_t image_bytes, hipStream_t s) {
+  if (check_frame(f) != GS_OK || P < 0 || !counters || !image) return GS_E_ARG;
   if (P > 0 && !geom) return GS_E_ARG;
   if (capacity < 0 || capacity > 0x7fffffffLL) return GS_E_ARG;
   gs_geom_layout_t LG;
@@ -795,7 +817,6 @@ extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t*
   gs_image_layout(f->width, f->height, &LI);
   if (binning_bytes < LB.total || image_bytes < LI.total) return GS_E_WORKSPACE;
   if (capacity > 0 && !binning) return GS_E_ARG;
-  hipStream_t s = (hipStream_t)stream;
   int gx, gy;
   const int nt = grid_tiles(f->width, f->height, gx, gy);
   const char* gb = (const char*)geom;
@@ -879,11 +900,31 @@ extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t*
     hipLaunchKernelGGL(k_ranges, dim3(nblocks256(capacity)), dim3(256), 0, s,
                        (const uint64_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, ranges);
   }
+  hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)ranges, nt,
+                     (uint32_t*)(ib + LI.ckpt_base));
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+// Blend arguments of one frame (its workspaces as binned by render_binning).
+static BlendArgs make_blend_args(const gs_frame_t* f, int P, const void* geom, long long capacity,
+                                 const void* binning, void* image, float* out_color, float* out_depth,
+                                 float* out_alpha) {
+  gs_geom_layout_t LG;
+  gs_binning_layout_t LB;
+  gs_image_layout_t LI;
+  gs_geom_layout(P, &LG);
+  gs_binning_layout(capacity, P, f->width, f->height, f->binning, &LB);
+  gs_image_layout(f->width, f->height, &LI);
+  int gx, gy;
+  grid_tiles(f->width, f->height, gx, gy);
+  const char* gb = (const char*)geom;
+  char* bb = (char*)binning;
+  char* ib = (char*)image;
   BlendArgs b;
   b.W = f->width;
   b.H = f->height;
   b.gx = gx;
-  b.ranges = ranges;
+  b.ranges = (const uint32_t*)(ib + LI.ranges);
   b.list = capacity > 0 ? (const uint32_t*)(bb + LB.vals_sorted) : nullptr;
   b.splat = (const float4*)(gb + LG.splat);
   b.bg = f->bg;
@@ -896,11 +937,78 @@ extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t*
   b.ckpt = (float*)(bb + LB.ckpt);
   b.tile_maxc = (uint32_t*)(ib + LI.tile_maxc);
   b.accum = (float*)(ib + LI.accum);
-  hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)ranges, nt,
-                     (uint32_t*)(ib + LI.ckpt_base));
+  return b;
+}
+
+extern "C" int gs_render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t* counters,
+                                 long long capacity, void* binning, size_t binning_bytes, void* image,
+                                 size_t image_bytes, void* stream) {
+  return render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                        (hipStream_t)stream);
+}
+
+extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
+                         void* binning, size_t binning_bytes, void* image, size_t image_bytes, float* out_color,
+                         float* out_depth, float* out_alpha, void* stream) {
+  if (!out_color || !out_depth) return GS_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int rc = render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes, s);
+  if (rc != GS_OK) return rc;
+  const BlendArgs b = make_blend_args(f, P, geom, capacity, binning, image, out_color, out_depth, out_alpha);
+  int gx, gy;
+  const int nt = grid_tiles(f->width, f->height, gx, gy);
   {
     timing::Scope ts(timing::kBlend, s);
     hipLaunchKernelGGL(k_blend, dim3(nt), dim3(256), 0, s, b);
+  }
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+// ---- many frames in one blend launch --------------------------------------
+// Set layout: uint32 prefix[n + 1] of tiles per frame, then BlendArgs[n] at a
+// 256-byte boundary.
+static size_t multi_blend_args_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
+
+extern "C" size_t gs_multi_blend_bytes(int n) {
+  return n <= 0 ? 0 : multi_blend_args_offset(n) + sizeof(BlendArgs) * (size_t)n;
+}
+
+extern "C" int gs_multi_blend_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
+                                    const long long* capacity, const void* const* binning, void* const* image,
+                                    float* const* out_color, float* const* out_depth, float* const* out_alpha,
+                                    void* host_set, size_t set_bytes, unsigned* total_tiles) {
+  if (n <= 0 || !frames || !P || !geom || !capacity || !binning || !image || !out_color || !out_depth || !host_set ||
+      !total_tiles)
+    return GS_E_ARG;
+  if (set_bytes < gs_multi_blend_bytes(n)) return GS_E_WORKSPACE;
+  uint32_t* prefix = (uint32_t*)host_set;
+  BlendArgs* args = (BlendArgs*)((char*)host_set + multi_blend_args_offset(n));
+  unsigned long long total = 0;
+  for (int i = 0; i < n; ++i) {
+    if (check_frame(&frames[i]) != GS_OK || P[i] < 0 || !image[i] || !out_color[i] || !out_depth[i]) return GS_E_ARG;
+    if (capacity[i] < 0 || capacity[i] > 0x7fffffffLL || (P[i] > 0 && !geom[i]) || (capacity[i] > 0 && !binning[i]))
+      return GS_E_ARG;
+    int gx, gy;
+    const int nt = grid_tiles(frames[i].width, frames[i].height, gx, gy);
+    prefix[i] = (uint32_t)total;
+    total += (unsigned long long)nt;
+    if (total > 0x7fffffffULL) return GS_E_ARG;
+    args[i] = make_blend_args(&frames[i], P[i], geom[i], capacity[i], binning[i], image[i], out_color[i],
+                              out_depth[i], out_alpha ? out_alpha[i] : nullptr);
+  }
+  prefix[n] = (uint32_t)total;
+  *total_tiles = (unsigned)total;
+  return GS_OK;
+}
+
+extern "C" int gs_multi_blend(int n, const void* dev_set, unsigned total_tiles, void* stream) {
+  if (n <= 0 || !dev_set || total_tiles == 0) return GS_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t* prefix = (const uint32_t*)dev_set;
+  const BlendArgs* args = (const BlendArgs*)((const char*)dev_set + multi_blend_args_offset(n));
+  {
+    timing::Scope ts(timing::kBlend, s);
+    hipLaunchKernelGGL(k_blend_multi, dim3(total_tiles), dim3(256), 0, s, prefix, args, n);
   }
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }

@@ -82,6 +82,28 @@ EXPORTS = {
     "gs_backward": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p, c_void_p,
                             c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(GsGrads),
                             c_void_p, c_size_t, c_void_p]),
+    "gs_render_binning": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong, c_void_p,
+                                  c_size_t, c_void_p, c_size_t, c_void_p]),
+    "gs_multi_blend_bytes": (c_size_t, [c_int]),
+    "gs_multi_blend_build": (c_int, [c_int, ctypes.POINTER(GsFrame), ctypes.POINTER(c_int), ctypes.POINTER(c_void_p),
+                                     ctypes.POINTER(c_longlong), ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                     ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                     c_void_p, c_size_t, ctypes.POINTER(ctypes.c_uint)]),
+    "gs_multi_blend": (c_int, [c_int, c_void_p, ctypes.c_uint, c_void_p]),
+    "gs_backward_prepare": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p,
+                                    c_void_p, c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    ctypes.POINTER(GsGrads), c_void_p, c_size_t, c_void_p]),
+    "gs_backward_finish": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p,
+                                   c_void_p, c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   ctypes.POINTER(GsGrads), c_void_p, c_size_t, c_void_p]),
+    "gs_multi_render_bwd_bytes": (c_size_t, [c_int]),
+    "gs_multi_render_bwd_build": (c_int, [c_int, ctypes.POINTER(GsFrame), ctypes.POINTER(c_int),
+                                          ctypes.POINTER(c_void_p), ctypes.POINTER(c_longlong),
+                                          ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                          ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                          ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_void_p, c_size_t,
+                                          ctypes.POINTER(ctypes.c_uint)]),
+    "gs_multi_render_bwd": (c_int, [c_int, c_void_p, ctypes.c_uint, c_int, c_void_p]),
     "gs_mark_visible": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gs_sort_tmp_bytes": (c_size_t, [c_longlong]),
     "gs_sort_pairs_u64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_longlong, c_int, c_void_p,

@@ -145,11 +145,44 @@ def make_slot(inst: PoseInstance, capacity: int, binning: int, pose_ptr: int, gr
     return s
 
 
+def _bwd_args(s: _Slot):
+    c = ctypes.c_void_p
+    return (ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()), c(s.geom.data_ptr()),
+            c(s.binning.data_ptr()), s.capacity, c(s.image.data_ptr()), c(s.counters.data_ptr()),
+            c(s.dcolor.data_ptr()), None, None, ctypes.byref(s.grads), c(s.scratch.data_ptr()), s.scratch.numel())
+
+
+def launch_slot_binning(s: _Slot, stream: ctypes.c_void_p) -> None:
+    """Preprocess (pose fused) and binning of one solve, enqueued on ``stream``."""
+    lib = _lib.load()
+    c = ctypes.c_void_p
+    _lib.check(lib.gs_preprocess(ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()),
+                                 c(s.geom.data_ptr()), s.geom.numel(), c(s.counters.data_ptr()), stream),
+               "gs_preprocess")
+    _lib.check(lib.gs_render_binning(ctypes.byref(s.frame), s.inst.P, c(s.geom.data_ptr()),
+                                     c(s.counters.data_ptr()), s.capacity, c(s.binning.data_ptr()),
+                                     s.binning.numel(), c(s.image.data_ptr()), s.image.numel(), stream),
+               "gs_render_binning")
+
+
+def launch_slot_loss(s: _Slot, out3_ptr: int, one_ptr: int, lam: float, stream: ctypes.c_void_p) -> None:
+    """Fused loss forward (stats to ``out3_ptr``) and backward (d loss scaled
+    by the float at ``one_ptr``) of one solve's rendered image."""
+    lib = _lib.load()
+    inst = s.inst
+    c = ctypes.c_void_p
+    _lib.check(lib.gs_ssim_l1_forward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
+                                      _taps(), lam, 1, c(out3_ptr), c(s.loss_ws.data_ptr()), s.loss_ws.numel(),
+                                      stream), "gs_ssim_l1_forward")
+    _lib.check(lib.gs_ssim_l1_backward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
+                                       _taps(), lam, 1, c(one_ptr), c(s.loss_ws.data_ptr()), s.loss_ws.numel(),
+                                       c(s.dcolor.data_ptr()), stream), "gs_ssim_l1_backward")
+
+
 def launch_slot(s: _Slot, out3_ptr: int, one_ptr: int, lam: float, stream: ctypes.c_void_p) -> None:
     """One solve iteration minus the optimiser step, enqueued on ``stream``:
-    preprocess (pose fused), binning + blend, fused loss forward (stats to
-    ``out3_ptr``) and backward (d loss scaled by the float at ``one_ptr``),
-    pose-only backward with fixed-order reductions."""
+    preprocess (pose fused), binning + blend, fused loss forward and
+    backward, pose-only backward with fixed-order reductions."""
     lib = _lib.load()
     inst = s.inst
     c = ctypes.c_void_p
@@ -160,16 +193,52 @@ def launch_slot(s: _Slot, out3_ptr: int, one_ptr: int, lam: float, stream: ctype
                              s.capacity, c(s.binning.data_ptr()), s.binning.numel(), c(s.image.data_ptr()),
                              s.image.numel(), c(s.color.data_ptr()), c(s.depth.data_ptr()), None, stream),
                "gs_render")
-    _lib.check(lib.gs_ssim_l1_forward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
-                                      _taps(), lam, 1, c(out3_ptr), c(s.loss_ws.data_ptr()), s.loss_ws.numel(),
-                                      stream), "gs_ssim_l1_forward")
-    _lib.check(lib.gs_ssim_l1_backward(c(s.color.data_ptr()), c(inst.target.data_ptr()), 3, inst.H, inst.W,
-                                       _taps(), lam, 1, c(one_ptr), c(s.loss_ws.data_ptr()), s.loss_ws.numel(),
-                                       c(s.dcolor.data_ptr()), stream), "gs_ssim_l1_backward")
-    _lib.check(lib.gs_backward(ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()),
-                               c(s.geom.data_ptr()), c(s.binning.data_ptr()), s.capacity, c(s.image.data_ptr()),
-                               c(s.counters.data_ptr()), c(s.dcolor.data_ptr()), None, None, ctypes.byref(s.grads),
-                               c(s.scratch.data_ptr()), s.scratch.numel(), stream), "gs_backward")
+    launch_slot_loss(s, out3_ptr, one_ptr, lam, stream)
+    _lib.check(lib.gs_backward(*_bwd_args(s), stream), "gs_backward")
+
+
+class FrameSets:
+    """Device descriptions of many solves' workspaces for the one-launch blend
+    and render backward (gs_multi_blend / gs_multi_render_bwd)."""
+
+    def __init__(self, slots: Sequence[_Slot], device):
+        lib = _lib.load()
+        n = len(slots)
+        self.n = n
+        vp = ctypes.c_void_p
+        frames = (_lib.GsFrame * n)(*[s.frame for s in slots])
+        P = (ctypes.c_int * n)(*[s.inst.P for s in slots])
+        cap = (ctypes.c_longlong * n)(*[s.capacity for s in slots])
+
+        def ptrs(attr):
+            return (vp * n)(*[getattr(s, attr).data_ptr() for s in slots])
+
+        nb = int(lib.gs_multi_blend_bytes(n))
+        host = (ctypes.c_uint8 * nb)()
+        tiles = ctypes.c_uint(0)
+        _lib.check(lib.gs_multi_blend_build(n, frames, P, ptrs("geom"), cap, ptrs("binning"), ptrs("image"),
+                                            ptrs("color"), ptrs("depth"), None, host, nb, ctypes.byref(tiles)),
+                   "gs_multi_blend_build")
+        self.blend = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+        self.blend_tiles = tiles.value
+        nr = int(lib.gs_multi_render_bwd_bytes(n))
+        host = (ctypes.c_uint8 * nr)()
+        blocks = ctypes.c_uint(0)
+        _lib.check(lib.gs_multi_render_bwd_build(n, frames, P, ptrs("geom"), cap, ptrs("binning"), ptrs("image"),
+                                                 ptrs("dcolor"), None, None, ptrs("scratch"), host, nr,
+                                                 ctypes.byref(blocks)), "gs_multi_render_bwd_build")
+        self.rbwd = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+        self.rbwd_blocks = blocks.value
+        self.deterministic = int(slots[0].frame.deterministic)
+
+    def blend_launch(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_blend(self.n, ctypes.c_void_p(self.blend.data_ptr()), self.blend_tiles,
+                                              stream), "gs_multi_blend")
+
+    def render_bwd_launch(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_render_bwd(self.n, ctypes.c_void_p(self.rbwd.data_ptr()),
+                                                   self.rbwd_blocks, self.deterministic, stream),
+                   "gs_multi_render_bwd")
 
 
 class BatchedPoseSolver:
@@ -179,7 +248,7 @@ class BatchedPoseSolver:
     def __init__(self, instances: Sequence[PoseInstance], iters: int = ITERS, lr: float = LR,
                  streams: int = 4, binning: int = _lib.BINNING_DEPTH_FIRST, iters_per_graph: int = 10,
                  headroom: float = 1.3, use_graph: bool = True, lambda_dssim: float = 0.2,
-                 X0: Optional[torch.Tensor] = None):
+                 X0: Optional[torch.Tensor] = None, multi: bool = True):
         if not instances:
             raise ValueError("no instances")
         self.instances = list(instances)
@@ -202,6 +271,8 @@ class BatchedPoseSolver:
         self.regrows = 0
         caps = start_capacities(self.instances, self.adam.pose, self.binning, self.headroom)
         self.slots = [self._make_slot(inst, cap, i) for i, (inst, cap) in enumerate(zip(self.instances, caps))]
+        self.multi = bool(multi)
+        self.sets = FrameSets(self.slots, self.device) if self.multi else None
 
     def _make_slot(self, inst: PoseInstance, capacity: int, i: int) -> _Slot:
         return make_slot(inst, capacity, self.binning, _vp(self.adam.pose, 28 * i), _vp(self.adam.grad, 28 * i))
@@ -210,16 +281,39 @@ class BatchedPoseSolver:
     def _launch_slot(self, s: _Slot, i: int, stream: ctypes.c_void_p):
         launch_slot(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, stream)
 
-    def _launch_iteration(self):
+    def _per_slot(self, fn):
+        """fn(i, slot, stream) for every slot, slot i on stream i mod S, between
+        a fork from and a join into the current stream."""
         cur = torch.cuda.current_stream(self.device)
         for st in self.streams:
             st.wait_stream(cur)
         for i, s in enumerate(self.slots):
             st = self.streams[i % self.nstreams]
             with torch.cuda.stream(st):
-                self._launch_slot(s, i, ctypes.c_void_p(st.cuda_stream))
+                fn(i, s, ctypes.c_void_p(st.cuda_stream))
         for st in self.streams:
             cur.wait_stream(st)
+
+    def _launch_iteration(self):
+        if not self.multi:
+            self._per_slot(lambda i, s, st: self._launch_slot(s, i, st))
+        else:
+            # per frame: preprocess + binning; all frames: one blend launch;
+            # per frame: loss and backward set-up; all frames: one render
+            # backward launch; per frame: per-Gaussian / pose gradients
+            lib = _lib.load()
+            cur = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+            self._per_slot(lambda i, s, st: launch_slot_binning(s, st))
+            self.sets.blend_launch(cur)
+
+            def loss_and_prepare(i, s, st):
+                launch_slot_loss(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, st)
+                _lib.check(lib.gs_backward_prepare(*_bwd_args(s), st), "gs_backward_prepare")
+
+            self._per_slot(loss_and_prepare)
+            self.sets.render_bwd_launch(cur)
+            self._per_slot(lambda i, s, st: _lib.check(lib.gs_backward_finish(*_bwd_args(s), st),
+                                                       "gs_backward_finish"))
         self.adam.step()
 
     def _reset(self):
@@ -271,6 +365,8 @@ class BatchedPoseSolver:
             for i in bad:
                 s = self.slots[i]
                 self.slots[i] = self._make_slot(s.inst, _round_capacity(int(s.capacity * 1.5), 1.0), i)
+            if self.multi:
+                self.sets = FrameSets(self.slots, self.device)
         raise RuntimeError(f"binning capacity still exceeded after {max_regrow} regrowths")
 
     def timed_run(self) -> Dict[str, float]:
@@ -299,33 +395,33 @@ def size_bucket(instances: Sequence[PoseInstance]) -> tuple:
 
 
 def autotune(instances: Sequence[PoseInstance], streams=(1, 2, 4),
-             binnings=(_lib.BINNING_DEPTH_FIRST, _lib.BINNING_SORT64), iters_per_graph: int = 5,
-             replays: int = 2) -> dict:
-    """Times ``replays`` graph replays of every (streams, binning) candidate
-    and returns (and caches per size bucket) the fastest configuration."""
+             binnings=(_lib.BINNING_DEPTH_FIRST, _lib.BINNING_SORT64), multis=(True, False),
+             iters_per_graph: int = 5, replays: int = 2) -> dict:
+    """Times ``replays`` graph replays of every (streams, binning, one-launch
+    blend/backward or not) candidate and returns (and caches per size bucket)
+    the fastest configuration."""
     key = size_bucket(instances)
     if key in _TUNED:
         return _TUNED[key]
     best = None
     results = []
-    for b in binnings:
-        for s in streams:
-            solver = BatchedPoseSolver(instances, iters=iters_per_graph * replays, streams=s, binning=b,
-                                       iters_per_graph=iters_per_graph)
-            solver._reset()
-            solver._capture()
-            solver._reset()
+    for b, s, m in [(b, s, m) for b in binnings for s in streams for m in multis]:
+        solver = BatchedPoseSolver(instances, iters=iters_per_graph * replays, streams=s, binning=b,
+                                   iters_per_graph=iters_per_graph, multi=m)
+        solver._reset()
+        solver._capture()
+        solver._reset()
+        solver.graph.replay()
+        torch.cuda.synchronize(solver.device)
+        t0 = time.perf_counter()
+        for _ in range(replays):
             solver.graph.replay()
-            torch.cuda.synchronize(solver.device)
-            t0 = time.perf_counter()
-            for _ in range(replays):
-                solver.graph.replay()
-            torch.cuda.synchronize(solver.device)
-            dt = (time.perf_counter() - t0) / (replays * iters_per_graph)
-            results.append(dict(streams=s, binning=b, ms_per_iter=dt * 1e3))
-            if best is None or dt < best[0]:
-                best = (dt, s, b)
-            del solver
-    cfg = dict(streams=best[1], binning=best[2], ms_per_iter=best[0] * 1e3, candidates=results)
+        torch.cuda.synchronize(solver.device)
+        dt = (time.perf_counter() - t0) / (replays * iters_per_graph)
+        results.append(dict(streams=s, binning=b, multi=m, ms_per_iter=dt * 1e3))
+        if best is None or dt < best[0]:
+            best = (dt, s, b, m)
+        del solver
+    cfg = dict(streams=best[1], binning=best[2], multi=best[3], ms_per_iter=best[0] * 1e3, candidates=results)
     _TUNED[key] = cfg
     return cfg

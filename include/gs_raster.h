@@ -174,6 +174,45 @@ int gs_backward(const gs_frame_t* frame, const gs_gaussians_t* g, const int* rad
                 const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha, const gs_grads_t* out,
                 void* scratch, size_t scratch_bytes, void* stream);
 
+/* ---- Many frames per launch (the batched pose solver, the serving loop) ----
+ * No single reference counterpart: these split gs_render / gs_backward so that
+ * the per-pixel kernels of many independent frames (each with its own
+ * workspaces, as above) run as ONE launch that fills the device.  A "set" is
+ * a device buffer describing every frame's workspaces; it is built on the host
+ * once (the *_build calls write a host buffer of *_bytes(n) bytes, which the
+ * caller copies to device memory) and stays valid while those workspaces do.
+ *
+ *   gs_render  = gs_render_binning (per frame)  + blend
+ *   gs_backward = gs_backward_prepare (per frame) + render backward
+ *                 + gs_backward_finish (per frame)
+ * with the middle step of every frame replaced by one gs_multi_blend /
+ * gs_multi_render_bwd launch over the set.  total_tiles / total_blocks are
+ * returned by the builds.  All frames of a render-backward set share one
+ * `deterministic` setting. */
+int gs_render_binning(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters, long long capacity,
+                      void* binning, size_t binning_bytes, void* image, size_t image_bytes, void* stream);
+size_t gs_multi_blend_bytes(int n);
+int gs_multi_blend_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
+                         const long long* capacity, const void* const* binning, void* const* image,
+                         float* const* out_color, float* const* out_depth, float* const* out_alpha,
+                         void* host_set, size_t set_bytes, unsigned* total_tiles);
+int gs_multi_blend(int n, const void* dev_set, unsigned total_tiles, void* stream);
+int gs_backward_prepare(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
+                        const void* binning, long long capacity, void* image, const uint32_t* counters,
+                        const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
+                        const gs_grads_t* out, void* scratch, size_t scratch_bytes, void* stream);
+int gs_backward_finish(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
+                       const void* binning, long long capacity, void* image, const uint32_t* counters,
+                       const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
+                       const gs_grads_t* out, void* scratch, size_t scratch_bytes, void* stream);
+size_t gs_multi_render_bwd_bytes(int n);
+int gs_multi_render_bwd_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
+                              const long long* capacity, const void* const* binning, void* const* image,
+                              const float* const* dL_dcolor, const float* const* dL_ddepth,
+                              const float* const* dL_dalpha, void* const* scratch, void* host_set,
+                              size_t set_bytes, unsigned* total_blocks);
+int gs_multi_render_bwd(int n, const void* dev_set, unsigned total_blocks, int deterministic, void* stream);
+
 /* Near-plane visibility (replaces diff_gaussian_rasterization._C.mark_visible,
  * GaussianRasterizer.markVisible). visible[P] = 1 if the view-space z > 0.2. */
 int gs_mark_visible(const float* means3D, int P, const float* viewmatrix, const float* projmatrix,

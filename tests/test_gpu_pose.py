@@ -140,16 +140,17 @@ def test_plain_solve_converges():
     assert rot < 0.1 * rot0 and trans < 0.1 * trans0
 
 
-@pytest.mark.parametrize("binning", [_lib.BINNING_DEPTH_FIRST, _lib.BINNING_SORT64])
-def test_batched_equals_plain_native(binning):
-    """Batched (graph, 2 streams) vs the plain one-solve-per-launch path on
-    the same kernels: every reduction is in a fixed order, so the final poses
+@pytest.mark.parametrize("binning,multi", [(_lib.BINNING_DEPTH_FIRST, True), (_lib.BINNING_SORT64, False)])
+def test_batched_equals_plain_native(binning, multi):
+    """Batched (graph, 2 streams; blend and render backward of all frames in
+    one launch or per frame) vs the plain one-solve-per-launch path on the
+    same kernels: every reduction is in a fixed order, so the final poses
     agree bit for bit (well inside 1e-4 rad / 1e-4)."""
-    print("batched vs plain native, binning", binning, flush=True)
+    print("batched vs plain native, binning", binning, "multi", multi, flush=True)
     insts = [build_instance(31 + k, W=W, H=H, P=P) for k, (W, H, P) in
              enumerate([(160, 90, 12000), (192, 108, 30000), (128, 72, 9216)])]
     plain = [solve_plain_native(i, iters=300)["pose"] for i in insts]
-    solver = BatchedPoseSolver(insts, iters=300, streams=2, binning=binning, iters_per_graph=10)
+    solver = BatchedPoseSolver(insts, iters=300, streams=2, binning=binning, iters_per_graph=10, multi=multi)
     poses, losses = solver.run()
     assert torch.isfinite(losses).all()
     for k, inst in enumerate(insts):
