@@ -157,7 +157,9 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
   const uint32_t cstart = chunk * 256;
   const uint32_t cend = min(cstart + 256, maxc);
   const int tx = tile % a.gx, ty = tile / a.gx;
-  const int px = tx * kBX + (threadIdx.x & 15), py = ty * kBY + (threadIdx.x >> 4);
+  int pdx, pdy;
+  tile_pixel(threadIdx.x, pdx, pdy);
+  const int px = tx * kBX + pdx, py = ty * kBY + pdy;
   const bool inside = px < a.W && py < a.H;
   const int pix = py * a.W + px;
   const int HW = a.W * a.H;
@@ -175,7 +177,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       b.w = skip_threshold(b.y);
       s_b[threadIdx.x] = b;
       s_c[threadIdx.x] = a.splat[3 * g + 2];
-      m = wave_rows_hit(splat_box(sa.x, sa.y, sa.z, sa.w, b.x, b.w), (float)(tx * kBX), (float)(ty * kBY));
+      m = wave_quads_hit(splat_box(sa.x, sa.y, sa.z, sa.w, b.x, b.w), (float)(tx * kBX), (float)(ty * kBY));
     }
     s_mask[threadIdx.x] = m;
 #pragma unroll
@@ -212,7 +214,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
   float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f, last_d = 0.f;
   const int lane = threadIdx.x & 63;
   __syncthreads();
-  // the wave's own list: splats that miss its 16 x 4 pixels are ones no lane would use
+  // the wave's own list: splats that miss its 8 x 8 pixels are ones no lane would use
   const int nw = compact_wave_list(s_mask, s_list[threadIdx.x >> 6], threadIdx.x >> 6, lane);
   for (int i = nw - 1; i >= 0; --i) {
     const int j = s_list[threadIdx.x >> 6][i];
@@ -236,7 +238,9 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       }
       if (active) {
         const float4 gc = s_c[j];
-        T = T / (1.0f - alpha);
+        // T before this splat; the gradient tolerates the 1-ulp reciprocal
+        const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
+        T = T * inv_1ma;
         const float w = alpha * T;
         float dL_da = 0.f;
         acc0 = last_alpha * lc0 + (1.0f - last_alpha) * acc0;
@@ -259,7 +263,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
         dL_da += (1.0f - acc_a) * dA;
         dL_da *= T;
         last_alpha = alpha;
-        dL_da += (-T_fin / (1.0f - alpha)) * bg_dot;
+        dL_da += (-T_fin * inv_1ma) * bg_dot;
         const float dL_dG = gb.y * dL_da;
         const float gdx = G * dx, gdy = G * dy;
         const float dG_dx = -gdx * ga.z - gdy * ga.w;

@@ -273,15 +273,23 @@ __device__ __forceinline__ float4 splat_box(float x, float y, float a, float b, 
   return make_float4(x - ex, x + ex, y - ey, y + ey);
 }
 
-// Bit r set when the box meets pixel rows [y0 + 4r, y0 + 4r + 3] of the tile
-// columns [x0, x0 + 15]: the 16 x 4 pixels of wave r of a 16 x 16 tile.
-__device__ __forceinline__ uint32_t wave_rows_hit(float4 bx, float x0, float y0) {
-  if (bx.y < x0 || bx.x > x0 + 15.0f) return 0u;
+// Pixel (dx, dy) of thread t inside a 16 x 16 tile: wave w = t / 64 takes the
+// 8 x 8 quadrant (w & 1, w >> 1), lane l the pixel (l & 7, l >> 3) in it.  A
+// small splat covers more lanes of an 8 x 8 block than of a 16 x 4 strip.
+__device__ __forceinline__ void tile_pixel(int t, int& dx, int& dy) {
+  const int w = t >> 6, l = t & 63;
+  dx = ((w & 1) << 3) + (l & 7);
+  dy = ((w >> 1) << 3) + (l >> 3);
+}
+
+// Bit w set when the box meets the 8 x 8 pixels of quadrant w of the tile whose
+// first pixel is (x0, y0).
+__device__ __forceinline__ uint32_t wave_quads_hit(float4 bx, float x0, float y0) {
   uint32_t m = 0u;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const float ya = y0 + 4.0f * (float)r;
-    if (!(bx.w < ya || bx.z > ya + 3.0f)) m |= 1u << r;
+  for (int w = 0; w < 4; ++w) {
+    const float qx = x0 + (float)((w & 1) << 3), qy = y0 + (float)((w >> 1) << 3);
+    if (!(bx.y < qx || bx.x > qx + 7.0f || bx.w < qy || bx.z > qy + 7.0f)) m |= 1u << w;
   }
   return m;
 }

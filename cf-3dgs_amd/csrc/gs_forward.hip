@@ -13,7 +13,7 @@
 //   k_ranges       [start, end) per tile in the sorted list
 //   k_blend        one 16x16 tile per 256-thread workgroup; splats staged in
 //                  LDS 256 at a time, each wave keeping its own list of the
-//                  staged splats whose alpha can reach 1/255 on its 16 x 4
+//                  staged splats whose alpha can reach 1/255 on its 8 x 8
 //                  pixels (splat_box); block-wide vote ends the tile early
 //                  once every pixel is saturated; XCD-aware tile order.
 #include "gs_common.hpp"
@@ -467,10 +467,12 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
   __shared__ float4 s_b[256];  // conic c, opacity, depth, -
   __shared__ float4 s_c[256];  // r, g, b, -
   __shared__ uint32_t s_mask[256];    // wave rows each staged splat can reach (splat_box)
-  __shared__ uint8_t s_list[4][256];  // per wave: the staged splats that reach its 16 x 4 pixels
+  __shared__ uint8_t s_list[4][256];  // per wave: the staged splats that reach its 8 x 8 pixels
   __shared__ uint32_t s_maxc;
   const int tx = tile % a.gx, ty = tile / a.gx;
-  const int px = tx * kBX + (threadIdx.x & 15), py = ty * kBY + (threadIdx.x >> 4);
+  int pdx, pdy;
+  tile_pixel(threadIdx.x, pdx, pdy);
+  const int px = tx * kBX + pdx, py = ty * kBY + pdy;
   const bool inside = px < a.W && py < a.H;
   const float pfx = (float)px, pfy = (float)py;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -501,7 +503,7 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
       b.w = skip_threshold(b.y);  // (the clamp bits in .w are not needed here)
       s_b[threadIdx.x] = b;
       s_c[threadIdx.x] = a.splat[3 * g + 2];
-      m = wave_rows_hit(splat_box(sa.x, sa.y, sa.z, sa.w, b.x, b.w), (float)(tx * kBX), (float)(ty * kBY));
+      m = wave_quads_hit(splat_box(sa.x, sa.y, sa.z, sa.w, b.x, b.w), (float)(tx * kBX), (float)(ty * kBY));
     }
     s_mask[threadIdx.x] = m;
     __syncthreads();
