@@ -215,7 +215,11 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
   const int lane = threadIdx.x & 63;
   __syncthreads();
   // the wave's own list: splats that miss its 8 x 8 pixels are ones no lane would use
-  const int nw = compact_wave_list(s_mask, s_list[threadIdx.x >> 6], threadIdx.x >> 6, lane);
+  // and entries at or past the last contributor of every pixel of the wave
+  // are inactive on every lane
+  const uint32_t wlast = wave_max_u32(last);
+  const int lim = wlast > cstart ? (int)min(wlast - cstart, 256u) : 0;
+  const int nw = compact_wave_list(s_mask, s_list[threadIdx.x >> 6], threadIdx.x >> 6, lane, lim);
   for (int i = nw - 1; i >= 0; --i) {
     const int j = s_list[threadIdx.x >> 6][i];
     const uint32_t q = cstart + j;  // 0-based list position; contributor index q + 1

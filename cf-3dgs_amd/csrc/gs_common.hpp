@@ -222,6 +222,16 @@ __device__ __forceinline__ float dpp(float v) {
 }
 
 // Sum over all 64 lanes; every lane must be active.  Result is wave-uniform.
+// Maximum of v over the 64 lanes of the wave (all lanes active).
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 // Tile rectangle [x0, x1) x [y0, y1) of a splat at (px, py) with radius r
 // (getRect of the reference rasteriser: ((p + r) + 16 - 1) / 16 for the end).
 __device__ __forceinline__ void tile_rect(float px, float py, int r, int gx, int gy, int& x0, int& y0, int& x1,
@@ -295,14 +305,15 @@ __device__ __forceinline__ uint32_t wave_quads_hit(float4 bx, float x0, float y0
 }
 
 // The wave's own list of the staged splats that meet its pixels: positions j
-// (ascending) with bit `wv` set in s_mask[j]; returns the count (wave-uniform).
-// Needs all 64 lanes active.
-__device__ __forceinline__ int compact_wave_list(const uint32_t* s_mask, uint8_t* s_list, int wv, int lane) {
+// < limit (ascending) with bit `wv` set in s_mask[j]; returns the count
+// (wave-uniform).  Needs all 64 lanes active.
+__device__ __forceinline__ int compact_wave_list(const uint32_t* s_mask, uint8_t* s_list, int wv, int lane,
+                                                 int limit = 256) {
   int n = 0;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int idx = c * 64 + lane;
-    const bool hit = (s_mask[idx] >> wv) & 1u;
+    const bool hit = idx < limit && ((s_mask[idx] >> wv) & 1u);
     const uint64_t bal = __ballot(hit);
     if (hit) s_list[n + __popcll(bal & lanemask_lt())] = (uint8_t)idx;
     n += __popcll(bal);
