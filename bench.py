@@ -99,6 +99,59 @@ def cpu_baseline(sc, budget_s: float = 12.0):
                       f"{dt:.1f} s"}
 
 
+def c1_bench(device, steps: int = 50, cpu: bool = True):
+    """The plain rasteriser case: 10,000 Gaussians, 256x256, SH degree 3,
+    L1 loss against a seeded target, forward + backward through the drop-in
+    autograd function, timed per iteration on the GPU; the CPU oracle on the
+    same scene beside it."""
+    import numpy as np
+    import torch
+    from diff_gaussian_rasterization import GaussianRasterizer
+    from gs_hip import scenes
+    from gs_hip.camera import oracle_camera, raster_settings
+    sc = scenes.plain_scene(seed=0)
+    cam = sc["cam"].to(device)
+    rs = raster_settings(cam, sc["bg"].to(device), 3)
+    leaves = {k: sc[k].to(device).requires_grad_(True) for k in ("means", "shs", "opacities", "scales", "rotations")}
+    target = sc["target"].to(device)
+    rast = GaussianRasterizer(raster_settings=rs)
+
+    def step():
+        means2D = torch.zeros_like(leaves["means"], requires_grad=True)
+        color, _, _ = rast(means3D=leaves["means"], means2D=means2D, opacities=leaves["opacities"],
+                           shs=leaves["shs"], scales=leaves["scales"], rotations=leaves["rotations"])
+        loss = (color - target).abs().mean()
+        loss.backward()
+        for t in leaves.values():
+            t.grad = None
+
+    for _ in range(5):
+        step()
+    torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize(device)
+    gpu = steps / (time.perf_counter() - t0)
+    out = {"workload": "10k Gaussians SH3, 256x256, fwd + bwd, L1 loss (drop-in autograd)",
+           "gpu_iters_per_s": round(gpu, 1)}
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        scene = {k: sc[k].numpy() for k in ("means", "shs", "opacities", "scales", "rotations")}
+        ocam = oracle_camera(sc["cam"], sc["bg"], 3)
+        tgt = sc["target"].numpy()
+        n, t0 = 0, time.perf_counter()
+        while n < 3 or (time.perf_counter() - t0 < 3.0 and n < 50):
+            f = O.forward(scene, ocam)
+            dc = (np.sign(f["color"] - tgt) / f["color"].size).astype(np.float32)
+            O.backward(scene, ocam, f, dc, np.zeros_like(f["depth"]))
+            n += 1
+        out["cpu_iters_per_s"] = round(n / (time.perf_counter() - t0), 2)
+        out["cpu_cores"] = O.get_threads()
+    return out
+
+
 def _log(msg: str) -> None:
     """Progress on stderr (stdout carries only the result line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -312,6 +365,7 @@ def main():
                     "ms_per_launch": round(d["ms_per_step"], 4)}
     _log(f"training step: {ms:.3f} ms")
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
+    c1 = c1_bench(device, cpu=not args.no_cpu_baseline)
     pose = None if args.no_pose else pose_bench(device, n=args.pose_instances, quick=args.pose_quick)
     serving = None if args.no_serve else serve_bench(device)
     res = {
@@ -324,6 +378,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "plain_10k_256": c1,
         "pose": pose,
         "serving": serving,
         "stages": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
