@@ -177,7 +177,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       b.w = skip_threshold(b.y);
       s_b[threadIdx.x] = b;
       s_c[threadIdx.x] = a.splat[3 * g + 2];
-      m = wave_quads_hit(splat_box(sa.x, sa.y, sa.z, sa.w, b.x, b.w), (float)(tx * kBX), (float)(ty * kBY));
+      m = splat_quads(sa.x, sa.y, sa.z, sa.w, b.x, b.w, (float)(tx * kBX), (float)(ty * kBY));
     }
     s_mask[threadIdx.x] = m;
 #pragma unroll

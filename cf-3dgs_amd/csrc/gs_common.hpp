@@ -304,6 +304,42 @@ __device__ __forceinline__ uint32_t wave_quads_hit(float4 bx, float x0, float y0
   return m;
 }
 
+// Quadrants of the tile at (x0, y0) that a splat at (x, y) with conic (a, b, c)
+// and pre-test threshold thr can reach: the box test of wave_quads_hit, then,
+// per quadrant, the exact minimum over its pixel rectangle of
+// Q = a dx^2 + 2 b dx dy + c dy^2 (dx = x - px, dy = y - py): 0 when the centre
+// lies inside, else the least of the four edge minima (a 1-D quadratic each,
+// its vertex clamped to the edge).  A quadrant whose minimum exceeds
+// L = -2 (thr - 0.01), padded by 1e-3 relative + 1e-3, has power below the
+// pre-test threshold at every pixel and is dropped.
+__device__ __forceinline__ float edge_min_q(float a, float b, float c, float u, float vlo, float vhi) {
+  // min over v in [vlo, vhi] of a u^2 + 2 b u v + c v^2 (c > 0)
+  const float v = fminf(vhi, fmaxf(vlo, -(b * u) / c));
+  return a * u * u + 2.0f * b * u * v + c * v * v;
+}
+__device__ __forceinline__ uint32_t splat_quads(float x, float y, float a, float b, float c, float thr, float x0,
+                                                float y0) {
+  uint32_t m = wave_quads_hit(splat_box(x, y, a, b, c, thr), x0, y0);
+  if (m == 0u || thr != thr) return m;
+  const float L = -2.0f * (thr - 0.01f);
+  if (!(a > 0.0f) || !(c > 0.0f) || !(a * c - b * b > 0.0f)) return m;
+  const float lim = L * 1.001f + 1e-3f;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (!((m >> w) & 1u)) continue;
+    const float qx = x0 + (float)((w & 1) << 3), qy = y0 + (float)((w >> 1) << 3);
+    const float dxlo = x - (qx + 7.0f), dxhi = x - qx;
+    const float dylo = y - (qy + 7.0f), dyhi = y - qy;
+    if (dxlo <= 0.0f && dxhi >= 0.0f && dylo <= 0.0f && dyhi >= 0.0f) continue;  // centre inside
+    float q = edge_min_q(a, b, c, dxlo, dylo, dyhi);
+    q = fminf(q, edge_min_q(a, b, c, dxhi, dylo, dyhi));
+    q = fminf(q, edge_min_q(c, b, a, dylo, dxlo, dxhi));
+    q = fminf(q, edge_min_q(c, b, a, dyhi, dxlo, dxhi));
+    if (q > lim) m &= ~(1u << w);
+  }
+  return m;
+}
+
 // The wave's own list of the staged splats that meet its pixels: positions j
 // < limit (ascending) with bit `wv` set in s_mask[j]; returns the count
 // (wave-uniform).  Needs all 64 lanes active.

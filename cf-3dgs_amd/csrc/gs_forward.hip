@@ -503,7 +503,7 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
       b.w = skip_threshold(b.y);  // (the clamp bits in .w are not needed here)
       s_b[threadIdx.x] = b;
       s_c[threadIdx.x] = a.splat[3 * g + 2];
-      m = wave_quads_hit(splat_box(sa.x, sa.y, sa.z, sa.w, b.x, b.w), (float)(tx * kBX), (float)(ty * kBY));
+      m = splat_quads(sa.x, sa.y, sa.z, sa.w, b.x, b.w, (float)(tx * kBX), (float)(ty * kBY));
     }
     s_mask[threadIdx.x] = m;
     __syncthreads();
