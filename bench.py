@@ -281,6 +281,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pose", action="store_true", help="skip the batched pose-solve measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-plain", action="store_true", help="skip the 10k-Gaussian 256x256 measurement")
     ap.add_argument("--pose-instances", type=int, default=64)
     ap.add_argument("--pose-quick", action="store_true", help="time only the batched pose solves (profiling runs)")
     ap.add_argument("--torch-loss", action="store_true", help="loss with torch conv2d windows instead of the fused kernel")
@@ -365,7 +366,7 @@ def main():
                     "ms_per_launch": round(d["ms_per_step"], 4)}
     _log(f"training step: {ms:.3f} ms")
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
-    c1 = c1_bench(device, cpu=not args.no_cpu_baseline)
+    c1 = None if args.no_plain else c1_bench(device, cpu=not args.no_cpu_baseline)
     pose = None if args.no_pose else pose_bench(device, n=args.pose_instances, quick=args.pose_quick)
     serving = None if args.no_serve else serve_bench(device)
     res = {
