@@ -143,3 +143,49 @@ def test_loss_matches_reference_formula():
                 C1, C2 = 1e-4, 9e-4
                 tot += ((2 * m1 * m2 + C1) * (2 * c12 + C2) / ((m1 * m1 + m2 * m2 + C1) * (v1 + v2 + C2))).item()
     assert float(s) == pytest.approx(tot / (3 * 20 * 24), rel=1e-5)
+
+
+def test_multi_frame_set_builders_on_the_host():
+    """The set builders only fill a host buffer: tile / work-item prefixes per
+    frame, and argument errors, checked without a device (the pointers are
+    never dereferenced on the host)."""
+    import ctypes
+    from gs_hip import _lib
+    lib = _lib.load()
+    n = 3
+    frames = (_lib.GsFrame * n)()
+    sizes = [(160, 90), (33, 17), (256, 256)]
+    dummy = ctypes.c_float(0.0)
+    for f, (W, H) in zip(frames, sizes):
+        f.width, f.height = W, H
+        f.tanfovx = f.tanfovy = 0.5
+        f.scale_modifier = 1.0
+        f.viewmatrix = f.projmatrix = f.bg = ctypes.addressof(dummy)
+        f.deterministic = 1
+    P = (ctypes.c_int * n)(100, 0, 50)
+    cap = (ctypes.c_longlong * n)(1000, 0, 70000)
+    ptr = (ctypes.c_void_p * n)(*[0x1000] * n)
+    tiles = [((W + 15) // 16) * ((H + 15) // 16) for W, H in sizes]
+    nb = lib.gs_multi_blend_bytes(n)
+    host = (ctypes.c_uint8 * nb)()
+    total = ctypes.c_uint(0)
+    assert lib.gs_multi_blend_build(n, frames, P, ptr, cap, ptr, ptr, ptr, ptr, None, host, nb,
+                                    ctypes.byref(total)) == 0
+    prefix = np.frombuffer(bytes(host)[:4 * (n + 1)], dtype=np.uint32)
+    assert list(prefix) == [0, tiles[0], tiles[0] + tiles[1], sum(tiles)] and total.value == sum(tiles)
+    assert lib.gs_multi_blend_build(n, frames, P, ptr, cap, ptr, ptr, ptr, ptr, None, host, nb - 1,
+                                    ctypes.byref(total)) == 2  # workspace too small
+    assert lib.gs_multi_blend_build(0, frames, P, ptr, cap, ptr, ptr, ptr, ptr, None, host, nb,
+                                    ctypes.byref(total)) == 1
+    nr = lib.gs_multi_render_bwd_bytes(n)
+    host2 = (ctypes.c_uint8 * nr)()
+    blocks = ctypes.c_uint(0)
+    assert lib.gs_multi_render_bwd_build(n, frames, P, ptr, cap, ptr, ptr, ptr, None, None, ptr, host2, nr,
+                                         ctypes.byref(blocks)) == 0
+    # work-item bound per frame: capacity / 256 + tiles + 1 (frames without Gaussians or capacity get none)
+    bounds = [cap[0] // 256 + tiles[0] + 1, 0, cap[2] // 256 + tiles[2] + 1]
+    prefix2 = np.frombuffer(bytes(host2)[:4 * (n + 1)], dtype=np.uint32)
+    assert list(prefix2) == [0, bounds[0], bounds[0], sum(bounds)] and blocks.value == sum(bounds)
+    frames[1].deterministic = 0  # one kernel variant per set
+    assert lib.gs_multi_render_bwd_build(n, frames, P, ptr, cap, ptr, ptr, ptr, None, None, ptr, host2, nr,
+                                         ctypes.byref(blocks)) == 1
