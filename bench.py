@@ -152,6 +152,26 @@ def c1_bench(device, steps: int = 50, cpu: bool = True):
     return out
 
 
+def knn_bench(device, reps: int = 5):
+    """distCUDA2 (gs_knn3_meandist): the lifted 480x270 frame (129,600
+    points) that every pose solve ingests, and a 2M-point cluster scene."""
+    import torch
+    from gs_hip import raster
+    from gs_hip.pose import lift_points
+    from gs_hip.serve import make_scene
+    out = {}
+    xyz, _, _, _ = lift_points(0, 480, 270, None)
+    for name, pts in (("frame_129600", xyz.to(device)), ("scene_2M", make_scene(99, 2_000_000, device).means)):
+        raster.knn3_meandist(pts)
+        torch.cuda.synchronize(device)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            raster.knn3_meandist(pts)
+        torch.cuda.synchronize(device)
+        out[name + "_ms"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+    return out
+
+
 def _log(msg: str) -> None:
     """Progress on stderr (stdout carries only the result line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -367,6 +387,7 @@ def main():
     _log(f"training step: {ms:.3f} ms")
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
     c1 = None if args.no_plain else c1_bench(device, cpu=not args.no_cpu_baseline)
+    knn = None if args.no_plain else knn_bench(device)
     pose = None if args.no_pose else pose_bench(device, n=args.pose_instances, quick=args.pose_quick)
     serving = None if args.no_serve else serve_bench(device)
     res = {
@@ -380,6 +401,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_10k_256": c1,
+        "distCUDA2": knn,
         "pose": pose,
         "serving": serving,
         "stages": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
