@@ -39,6 +39,16 @@ __device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, 
   p = p - step * (m / denom);
 }
 
+typedef float nt4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_nt(const float* p) {
+  const nt4 v = __builtin_nontemporal_load(reinterpret_cast<const nt4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_nt(float* p, const float4& v) {
+  const nt4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<nt4*>(p));
+}
+
 __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
   const int c = blockIdx.x;
   int t = 0;
@@ -51,20 +61,41 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
   float* M = a.m[t];
   float* V = a.v[t];
   const float st = a.step_size[t];
-  // vector part
+  // vector part: the chunk's two float4 per thread are all loaded before any
+  // is updated (8 loads in flight), streamed past the caches (each byte is
+  // touched once)
   const long long vend = base + ((end - base) & ~3LL);
-  for (long long i = base + 4LL * threadIdx.x; i < vend; i += 4LL * 256) {
-    float4 p = *reinterpret_cast<float4*>(P + i);
-    const float4 g = *reinterpret_cast<const float4*>(G + i);
-    float4 m = *reinterpret_cast<float4*>(M + i);
-    float4 v = *reinterpret_cast<float4*>(V + i);
+  auto upd4 = [&](float4& p, const float4& g, float4& m, float4& v) {
     adam_one(p.x, g.x, m.x, v.x, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, st);
     adam_one(p.y, g.y, m.y, v.y, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, st);
     adam_one(p.z, g.z, m.z, v.z, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, st);
     adam_one(p.w, g.w, m.w, v.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, st);
-    *reinterpret_cast<float4*>(P + i) = p;
-    *reinterpret_cast<float4*>(M + i) = m;
-    *reinterpret_cast<float4*>(V + i) = v;
+  };
+  const long long i0 = base + 4LL * threadIdx.x, i1 = i0 + 4LL * 256;
+  if (i1 + 4 <= vend) {
+    float4 p0 = ld_nt(P + i0), p1 = ld_nt(P + i1);
+    const float4 g0 = ld_nt(G + i0), g1 = ld_nt(G + i1);
+    float4 m0 = ld_nt(M + i0), m1 = ld_nt(M + i1);
+    float4 v0 = ld_nt(V + i0), v1 = ld_nt(V + i1);
+    upd4(p0, g0, m0, v0);
+    upd4(p1, g1, m1, v1);
+    st_nt(P + i0, p0);
+    st_nt(P + i1, p1);
+    st_nt(M + i0, m0);
+    st_nt(M + i1, m1);
+    st_nt(V + i0, v0);
+    st_nt(V + i1, v1);
+  } else {
+    for (long long i = i0; i < vend; i += 4LL * 256) {
+      float4 p = *reinterpret_cast<float4*>(P + i);
+      const float4 g = *reinterpret_cast<const float4*>(G + i);
+      float4 m = *reinterpret_cast<float4*>(M + i);
+      float4 v = *reinterpret_cast<float4*>(V + i);
+      upd4(p, g, m, v);
+      *reinterpret_cast<float4*>(P + i) = p;
+      *reinterpret_cast<float4*>(M + i) = m;
+      *reinterpret_cast<float4*>(V + i) = v;
+    }
   }
   for (long long i = vend + threadIdx.x; i < end; i += 256) {
     float p = P[i], m = M[i], v = V[i];
