@@ -332,11 +332,12 @@ def main():
     opt = g.optimizer
 
     def step():
-        out = render(g, cam, bg)
+        # the fused loss applies the image's [0, 1] clamp (and its gradient mask) itself
+        out = render(g, cam, bg, clamp=args.torch_loss)
         if args.torch_loss:
             loss = photometric_loss(out["image"], target)
         else:
-            loss, _ = fused_photometric_loss(out["image"], target)
+            loss, _ = fused_photometric_loss(out["image"], target, clamp01=True)
         loss.backward()
         opt.step()
         opt.zero_grad(set_to_none=True)
@@ -368,8 +369,8 @@ def main():
         from gs_hip import raster
         from gs_hip.camera import raster_settings
         rs = raster_settings(cam, bg, g.active_sh_degree)
-        _, _, _, _, st = raster.forward(rs, g.get_xyz, g.get_features, None, g.get_opacity, g.get_scaling,
-                                        g.get_rotation, None)
+        sc_, ro_, op_ = g.get_activated()
+        _, _, _, _, st = raster.forward(rs, g.get_xyz, g.get_features, None, op_, sc_, ro_, None)
         n_dup = st.num_rendered
     L = _lib.binning_layout(n_dup, args.P, args.width, args.height, raster.BINNING)
     for s, d in stages.items():

@@ -9,6 +9,10 @@
 //   m = m + (g - m) * (1 - b1)            (lerp)
 //   v = v * b2 + (1 - b2) * g * g
 //   p = p - step * m / (sqrt(v) / sqrt(bc2) + eps),  step = lr / bc1
+// A tensor may carry a second learning rate for the leading `lead` elements of
+// every `row` elements: the SH coefficients of all Gaussians live in one
+// [P,16,3] tensor whose first 3 floats per Gaussian are the reference's f_dc
+// group (lr feature_lr) and the other 45 its f_rest group (feature_lr / 20).
 #include <cmath>
 
 #include "gs_common.hpp"
@@ -27,6 +31,9 @@ struct AdamArgs {
   float* v[kAdamMax];
   long long n[kAdamMax];
   float step_size[kAdamMax];  // lr / (1 - b1^t)
+  float lead_step[kAdamMax];  // lead_lr / (1 - b1^t)
+  int lead[kAdamMax];         // elements [0, lead) of every row take lead_step
+  int row[kAdamMax];          // 0: no lead (row % 4 == 0, numel < 2^31 otherwise)
   int chunk0[kAdamMax + 1];   // first chunk of each tensor
   float b1, b2, eps, inv_sqrt_bc2;
 };
@@ -61,15 +68,29 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
   float* M = a.m[t];
   float* V = a.v[t];
   const float st = a.step_size[t];
+  const float lst = a.lead_step[t];
+  const unsigned row = (unsigned)a.row[t], lead = (unsigned)a.lead[t];
+  // step size of element i (row > 0 implies numel < 2^31)
+  auto step_of = [&](long long i) -> float {
+    return (row != 0u && (unsigned)i % row < lead) ? lst : st;
+  };
   // vector part: the chunk's two float4 per thread are all loaded before any
   // is updated (8 loads in flight), streamed past the caches (each byte is
-  // touched once)
+  // touched once).  row % 4 == 0, so the 4 elements of a float4 sit in one row.
   const long long vend = base + ((end - base) & ~3LL);
-  auto upd4 = [&](float4& p, const float4& g, float4& m, float4& v) {
-    adam_one(p.x, g.x, m.x, v.x, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, st);
-    adam_one(p.y, g.y, m.y, v.y, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, st);
-    adam_one(p.z, g.z, m.z, v.z, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, st);
-    adam_one(p.w, g.w, m.w, v.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, st);
+  auto upd4 = [&](float4& p, const float4& g, float4& m, float4& v, long long i) {
+    float s0 = st, s1 = st, s2 = st, s3 = st;
+    if (row != 0u) {
+      const unsigned r = (unsigned)i % row;
+      s0 = r < lead ? lst : st;
+      s1 = r + 1u < lead ? lst : st;
+      s2 = r + 2u < lead ? lst : st;
+      s3 = r + 3u < lead ? lst : st;
+    }
+    adam_one(p.x, g.x, m.x, v.x, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s0);
+    adam_one(p.y, g.y, m.y, v.y, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s1);
+    adam_one(p.z, g.z, m.z, v.z, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s2);
+    adam_one(p.w, g.w, m.w, v.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s3);
   };
   const long long i0 = base + 4LL * threadIdx.x, i1 = i0 + 4LL * 256;
   if (i1 + 4 <= vend) {
@@ -77,8 +98,8 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
     const float4 g0 = ld_nt(G + i0), g1 = ld_nt(G + i1);
     float4 m0 = ld_nt(M + i0), m1 = ld_nt(M + i1);
     float4 v0 = ld_nt(V + i0), v1 = ld_nt(V + i1);
-    upd4(p0, g0, m0, v0);
-    upd4(p1, g1, m1, v1);
+    upd4(p0, g0, m0, v0, i0);
+    upd4(p1, g1, m1, v1, i1);
     st_nt(P + i0, p0);
     st_nt(P + i1, p1);
     st_nt(M + i0, m0);
@@ -91,7 +112,7 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
       const float4 g = *reinterpret_cast<const float4*>(G + i);
       float4 m = *reinterpret_cast<float4*>(M + i);
       float4 v = *reinterpret_cast<float4*>(V + i);
-      upd4(p, g, m, v);
+      upd4(p, g, m, v, i);
       *reinterpret_cast<float4*>(P + i) = p;
       *reinterpret_cast<float4*>(M + i) = m;
       *reinterpret_cast<float4*>(V + i) = v;
@@ -99,7 +120,7 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
   }
   for (long long i = vend + threadIdx.x; i < end; i += 256) {
     float p = P[i], m = M[i], v = V[i];
-    adam_one(p, G[i], m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, st);
+    adam_one(p, G[i], m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, step_of(i));
     P[i] = p;
     M[i] = m;
     V[i] = v;
@@ -275,12 +296,14 @@ extern "C" int gs_se3_adam(int n, float* xi, const float* X0, const float* grad_
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
-extern "C" int gs_adam_step(int ntensors, float* const* params, const float* const* grads, float* const* exp_avg,
-                            float* const* exp_avg_sq, const long long* numel, const float* lr, float beta1,
-                            float beta2, float eps, int step, void* stream) {
+extern "C" int gs_adam_step_rows(int ntensors, float* const* params, const float* const* grads,
+                                 float* const* exp_avg, float* const* exp_avg_sq, const long long* numel,
+                                 const float* lr, const float* lead_lr, const int* lead, const int* row, float beta1,
+                                 float beta2, float eps, int step, void* stream) {
   if (ntensors <= 0 || ntensors > kAdamMax || step < 1 || !params || !grads || !exp_avg || !exp_avg_sq || !numel ||
       !lr)
     return GS_E_ARG;
+  if ((lead_lr || lead || row) && !(lead_lr && lead && row)) return GS_E_ARG;
   AdamArgs a;
   a.ntensors = ntensors;
   long long chunks = 0;
@@ -297,6 +320,12 @@ extern "C" int gs_adam_step(int ntensors, float* const* params, const float* con
     a.n[k] = numel[k];
     const double bc1 = 1.0 - std::pow((double)beta1, (double)step);
     a.step_size[k] = (float)(lr[k] / bc1);
+    a.row[k] = row ? row[k] : 0;
+    a.lead[k] = row ? lead[k] : 0;
+    a.lead_step[k] = row ? (float)(lead_lr[k] / bc1) : 0.0f;
+    if (a.row[k] < 0 || a.row[k] % 4 || a.lead[k] < 0 || a.lead[k] > a.row[k] ||
+        (a.row[k] > 0 && numel[k] >= (1LL << 31)))
+      return GS_E_ARG;
     a.chunk0[k] = (int)chunks;
     chunks += (numel[k] + kAdamChunk - 1) / kAdamChunk;
   }
@@ -310,4 +339,11 @@ extern "C" int gs_adam_step(int ntensors, float* const* params, const float* con
   if (chunks == 0) return GS_OK;
   hipLaunchKernelGGL(k_adam, dim3((unsigned)chunks), dim3(256), 0, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_adam_step(int ntensors, float* const* params, const float* const* grads, float* const* exp_avg,
+                            float* const* exp_avg_sq, const long long* numel, const float* lr, float beta1,
+                            float beta2, float eps, int step, void* stream) {
+  return gs_adam_step_rows(ntensors, params, grads, exp_avg, exp_avg_sq, numel, lr, nullptr, nullptr, nullptr, beta1,
+                           beta2, eps, step, stream);
 }

@@ -85,6 +85,7 @@ class _RasterizeGaussians(torch.autograd.Function):
                    po_ is not None)
         ctx.save_for_backward(means3D, sh, colors_precomp, opacities, scales, rotations, cov3Ds_precomp, pose)
         ctx.mark_non_differentiable(radii)
+        ctx.set_materialize_grads(False)  # an unused output's gradient stays None (treated as zero)
         if return_alpha:
             return color, radii, depth, alpha
         return color, radii, depth

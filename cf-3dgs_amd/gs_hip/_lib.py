@@ -124,6 +124,13 @@ EXPORTS = {
     "gs_adam_step": (c_int, [c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
                              ctypes.POINTER(c_void_p), ctypes.POINTER(c_longlong), ctypes.POINTER(c_float), c_float,
                              c_float, c_float, c_int, c_void_p]),
+    "gs_adam_step_rows": (c_int, [c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                  ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), ctypes.POINTER(c_longlong),
+                                  ctypes.POINTER(c_float), ctypes.POINTER(c_float), ctypes.POINTER(c_int),
+                                  ctypes.POINTER(c_int), c_float, c_float, c_float, c_int, c_void_p]),
+    "gs_activate_forward": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gs_activate_backward": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p]),
     "gs_timing_enable": (c_int, [c_int]),
     "gs_timing_reset": (c_int, []),
     "gs_timing_read": (c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int)]),

@@ -96,6 +96,7 @@ class _FusedPhotometric(torch.autograd.Function):
         ctx.lam = float(lambda_dssim)
         ctx.clamp01 = int(bool(clamp01))
         ctx.mark_non_differentiable(out)
+        ctx.set_materialize_grads(False)
         return out[0], out
 
     @staticmethod
@@ -103,6 +104,8 @@ class _FusedPhotometric(torch.autograd.Function):
         import ctypes
         from . import _lib
         lib = _lib.load()
+        if g_loss is None:
+            return None, None, None, None
         x, y, ws = ctx.saved_tensors
         C, H, W = x.shape
         go = g_loss.reshape(1).contiguous().float()

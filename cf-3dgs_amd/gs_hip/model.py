@@ -1,8 +1,8 @@
 """Gaussian parameters and the render call, as CF-3DGS's model layer uses the
 rasteriser.
 
-  * raw parameters and activations: scene/gaussian_model_cf.py:41-56,
-    115-172 (exp scales, normalised rotations, sigmoid opacities,
+  * raw parameters and activations: scene/gaussian_model_cf.py:48-56,
+    116-169 (exp scales, normalised rotations, sigmoid opacities,
     cat(features_dc, features_rest) -> [P, 16, 3]);
   * optimiser groups: ``training_setup`` (:242-268) with the defaults of
     arguments/__init__.py:103-111, Adam eps 1e-15;
@@ -10,8 +10,21 @@ rasteriser.
     with retained grad, keyword call of the rasteriser, clamp of the image.
 The rigid pose of the pose-solve loop is passed to the rasteriser (fused in
 its preprocess) instead of transforming the means in torch.
+
+Two layout choices differ from the reference's module, not its arithmetic:
+  * the SH coefficients live in ONE [P, 16, 3] parameter.  The reference
+    keeps f_dc [P,1,3] and f_rest [P,15,3] apart and concatenates them on
+    every ``get_features`` (a full copy of the SH forward and a split copy of
+    its gradient backward); ``_features_dc`` / ``_features_rest`` are views
+    here, and the fused Adam gives the leading 3 floats of every 48 the f_dc
+    learning rate and the rest the f_rest one;
+  * ``render`` evaluates the three activations in one HIP kernel each way
+    (``activate``) instead of ~17 torch kernels.  The single-tensor getters
+    stay torch expressions for callers outside the step.
 """
 from __future__ import annotations
+
+import ctypes
 
 import torch
 import torch.nn as nn
@@ -19,7 +32,67 @@ import torch.nn.functional as F
 
 from diff_gaussian_rasterization import GaussianRasterizer
 
+from . import _lib
 from .camera import Camera, raster_settings
+
+
+def _p(t: torch.Tensor):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _aligned(t: torch.Tensor) -> torch.Tensor:
+    t = t.contiguous()
+    return t if t.data_ptr() % 16 == 0 else t.clone()
+
+
+class _Activate(torch.autograd.Function):
+    """(exp(raw_scale), normalize(raw_rot), sigmoid(raw_opacity)) in one
+    launch forward (gs_activate_forward) and one backward
+    (gs_activate_backward)."""
+
+    @staticmethod
+    def forward(ctx, raw_scale, raw_rot, raw_opac):
+        dev = raw_scale.device
+        if dev.type != "cuda":
+            raise RuntimeError("the fused activations run on the GPU only")
+        for name, t in (("raw_scale", raw_scale), ("raw_rot", raw_rot), ("raw_opacity", raw_opac)):
+            if t.dtype != torch.float32:
+                raise RuntimeError(f"{name} must be float32")
+        rs, rr, ro = raw_scale.contiguous(), _aligned(raw_rot), raw_opac.contiguous()
+        P = rs.shape[0]
+        scales = torch.empty_like(rs)
+        rots = torch.empty_like(rr)
+        opac = torch.empty_like(ro)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        _lib.check(_lib.load().gs_activate_forward(P, _p(rs), _p(rr), _p(ro), _p(scales), _p(rots), _p(opac),
+                                                   stream), "gs_activate_forward")
+        ctx.set_materialize_grads(False)
+        ctx.save_for_backward(rr, scales, opac)
+        return scales, rots, opac
+
+    @staticmethod
+    def backward(ctx, g_s, g_r, g_o):
+        rr, scales, opac = ctx.saved_tensors
+        P = rr.shape[0]
+        # an output that took no part in the loss passes no gradient on (as in torch)
+        need = ctx.needs_input_grad
+        d_s = torch.empty_like(scales) if need[0] and g_s is not None else None
+        d_r = torch.empty_like(rr) if need[1] and g_r is not None else None
+        d_o = torch.empty_like(opac) if need[2] and g_o is not None else None
+        g_s = g_s.contiguous() if g_s is not None else None
+        g_r = _aligned(g_r) if g_r is not None else None
+        g_o = g_o.contiguous() if g_o is not None else None
+        stream = ctypes.c_void_p(torch.cuda.current_stream(rr.device).cuda_stream)
+        _lib.check(_lib.load().gs_activate_backward(P, _p(rr), _p(scales), _p(opac), _p(g_s), _p(g_r), _p(g_o),
+                                                    _p(d_s), _p(d_r), _p(d_o), stream), "gs_activate_backward")
+        return d_s, d_r, d_o
+
+
+def activate(raw_scale: torch.Tensor, raw_rot: torch.Tensor, raw_opacity: torch.Tensor):
+    """The reference's scaling / rotation / opacity activations
+    (torch.exp, F.normalize, torch.sigmoid) fused: returns
+    (scales [P,3], rotations [P,4], opacities [P,1])."""
+    return _Activate.apply(raw_scale, raw_rot, raw_opacity)
 
 
 class GaussianParams:
@@ -28,13 +101,20 @@ class GaussianParams:
         def p(t):
             return nn.Parameter(t.to(device).contiguous().requires_grad_(True))
         self._xyz = p(xyz)
-        self._features_dc = p(features_dc)
-        self._features_rest = p(features_rest)
+        self._features = p(torch.cat((features_dc, features_rest), dim=1))
         self._scaling = p(scaling)
         self._rotation = p(rotation)
         self._opacity = p(opacity)
         self.active_sh_degree = sh_degree
         self.optimizer = None
+
+    @property
+    def _features_dc(self):
+        return self._features[:, :1]
+
+    @property
+    def _features_rest(self):
+        return self._features[:, 1:]
 
     @property
     def get_xyz(self):
@@ -54,29 +134,36 @@ class GaussianParams:
 
     @property
     def get_features(self):
-        return torch.cat((self._features_dc, self._features_rest), dim=1)
+        return self._features
+
+    def get_activated(self):
+        """(scales, rotations, opacities) in one fused launch."""
+        return activate(self._scaling, self._rotation, self._opacity)
 
     def training_setup(self, spatial_lr_scale: float = 1.0, position_lr_init: float = 0.00016,
                        feature_lr: float = 0.0025, opacity_lr: float = 0.05, scaling_lr: float = 0.005,
-                       rotation_lr: float = 0.001, fused: bool = True):
+                       rotation_lr: float = 0.001):
+        from .optim import FusedAdam
+        row = self._features.shape[1] * self._features.shape[2]
         groups = [
             {"params": [self._xyz], "lr": position_lr_init * spatial_lr_scale, "name": "xyz"},
-            {"params": [self._features_dc], "lr": feature_lr, "name": "f_dc"},
-            {"params": [self._features_rest], "lr": feature_lr / 20.0, "name": "f_rest"},
+            # f_rest at feature_lr / 20, its leading 3 floats per Gaussian (f_dc) at feature_lr
+            {"params": [self._features], "lr": feature_lr / 20.0, "name": "f_rest",
+             "lead_lr": feature_lr, "lead": 3, "row": row},
             {"params": [self._opacity], "lr": opacity_lr, "name": "opacity"},
             {"params": [self._scaling], "lr": scaling_lr, "name": "scaling"},
             {"params": [self._rotation], "lr": rotation_lr, "name": "rotation"},
         ]
-        if fused:
-            from .optim import FusedAdam
-            self.optimizer = FusedAdam(groups, lr=0.0, eps=1e-15)
-        else:
-            self.optimizer = torch.optim.Adam(groups, lr=0.0, eps=1e-15)
+        self.optimizer = FusedAdam(groups, lr=0.0, eps=1e-15)
         return self.optimizer
 
 
-def render(gaussians: GaussianParams, cam: Camera, bg: torch.Tensor, pose=None, scaling_modifier: float = 1.0):
-    """CF3DGS_Render.render with the rasteriser's 3-tuple return."""
+def render(gaussians: GaussianParams, cam: Camera, bg: torch.Tensor, pose=None, scaling_modifier: float = 1.0,
+           clamp: bool = True):
+    """CF3DGS_Render.render with the rasteriser's 3-tuple return.  With
+    ``clamp=False`` "image" is the rasteriser's colour before the [0, 1]
+    clamp, for a loss that applies the clamp itself (the fused loss's
+    ``clamp01``)."""
     means3D = gaussians.get_xyz
     screenspace = torch.zeros_like(means3D, requires_grad=True) + 0
     try:
@@ -85,9 +172,9 @@ def render(gaussians: GaussianParams, cam: Camera, bg: torch.Tensor, pose=None, 
         pass
     rs = raster_settings(cam, bg, gaussians.active_sh_degree, scaling_modifier)
     rasterizer = GaussianRasterizer(raster_settings=rs)
+    scales, rotations, opacities = gaussians.get_activated()
     color, radii, depth = rasterizer(means3D=means3D, means2D=screenspace, shs=gaussians.get_features,
-                                     colors_precomp=None, opacities=gaussians.get_opacity,
-                                     scales=gaussians.get_scaling, rotations=gaussians.get_rotation,
+                                     colors_precomp=None, opacities=opacities, scales=scales, rotations=rotations,
                                      cov3D_precomp=None, pose=pose)
-    return {"image": color.clamp(0, 1), "depth": depth, "viewspace_points": screenspace,
+    return {"image": color.clamp(0, 1) if clamp else color, "depth": depth, "viewspace_points": screenspace,
             "visibility_filter": radii > 0, "radii": radii}

@@ -5,6 +5,12 @@ Same update and defaults as the torch.optim.Adam that CFGaussianModel builds
 (0.9, 0.999), eps 1e-15), with the same ``param_groups`` / ``zero_grad``
 surface so learning-rate schedules that edit ``param_groups`` keep working.
 Parameters without a gradient are skipped, as torch does.
+
+A group may also carry ``lead_lr``, ``lead`` and ``row``: element i of its
+tensors then steps with ``lead_lr`` when ``i % row < lead`` and with ``lr``
+otherwise.  ``GaussianParams`` keeps all SH coefficients in one [P,16,3]
+tensor and uses this for the reference's f_dc (the leading 3 floats of every
+48, lr feature_lr) and f_rest (feature_lr / 20) groups (:257-260).
 """
 from __future__ import annotations
 
@@ -39,8 +45,16 @@ class FusedAdam(torch.optim.Optimizer):
                     st["exp_avg_sq"] = torch.zeros_like(p)
                 st["step"] += 1
                 g = p.grad if p.grad.is_contiguous() else p.grad.contiguous()
+                if g.data_ptr() % 16:
+                    g = g.clone()  # the kernel reads 16-byte vectors
+                if p.data_ptr() % 16:
+                    raise RuntimeError("FusedAdam needs 16-byte aligned parameters (a fresh tensor, not an offset view)")
                 key = (float(b1), float(b2), float(group["eps"]), int(st["step"]), p.device)
-                by_hyper.setdefault(key, []).append((p, g, st["exp_avg"], st["exp_avg_sq"], float(group["lr"])))
+                row = int(group.get("row", 0))
+                lead = int(group.get("lead", 0)) if row else 0
+                lead_lr = float(group.get("lead_lr", group["lr"])) if row else 0.0
+                by_hyper.setdefault(key, []).append((p, g, st["exp_avg"], st["exp_avg_sq"], float(group["lr"]),
+                                                     lead_lr, lead, row))
         for (b1, b2, eps, step, dev), items in by_hyper.items():
             for i in range(0, len(items), 16):
                 chunk = items[i:i + 16]
@@ -51,6 +65,10 @@ class FusedAdam(torch.optim.Optimizer):
                 V = (ctypes.c_void_p * n)(*[c[3].data_ptr() for c in chunk])
                 N = (ctypes.c_longlong * n)(*[c[0].numel() for c in chunk])
                 LR = (ctypes.c_float * n)(*[c[4] for c in chunk])
+                LLR = (ctypes.c_float * n)(*[c[5] for c in chunk])
+                LEAD = (ctypes.c_int * n)(*[c[6] for c in chunk])
+                ROW = (ctypes.c_int * n)(*[c[7] for c in chunk])
                 stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
-                _lib.check(lib.gs_adam_step(n, P, G, M, V, N, LR, b1, b2, eps, step, stream), "gs_adam_step")
+                _lib.check(lib.gs_adam_step_rows(n, P, G, M, V, N, LR, LLR, LEAD, ROW, b1, b2, eps, step, stream),
+                           "gs_adam_step_rows")
         return loss

@@ -286,6 +286,30 @@ int gs_adam_step(int ntensors, float* const* params, const float* const* grads, 
                  float* const* exp_avg_sq, const long long* numel, const float* lr, float beta1, float beta2,
                  float eps, int step, void* stream);
 
+/* gs_adam_step with a second learning rate per tensor: element i of tensor k
+ * steps with lead_lr[k] when (i mod row[k]) < lead[k], with lr[k] otherwise
+ * (row[k] = 0: lr[k] throughout; row[k] % 4 == 0; numel[k] < 2^31 when
+ * row[k] > 0).  One [P,16,3] SH tensor then carries both of the reference's
+ * SH groups (scene/gaussian_model_cf.py:257-260: f_dc at feature_lr, the
+ * leading 3 floats of every 48, and f_rest at feature_lr / 20). */
+int gs_adam_step_rows(int ntensors, float* const* params, const float* const* grads, float* const* exp_avg,
+                      float* const* exp_avg_sq, const long long* numel, const float* lr, const float* lead_lr,
+                      const int* lead, const int* row, float beta1, float beta2, float eps, int step, void* stream);
+
+/* The Gaussian model's parameter activations (scene/gaussian_model_cf.py:48-56,
+ * :116-121, :168-169), one launch each way:
+ *   scales = exp(raw_scale) [P,3]; rotations = F.normalize(raw_rot) [P,4]
+ *   (x / max(||x||, 1e-12)); opacities = sigmoid(raw_opacity) [P,1].
+ * The backward is the chain rule of those torch ops (exp: g * scale; sigmoid:
+ * g * (1 - o) * o; normalize through norm, clamp_min and division).  An output
+ * pointer may be NULL (not computed); a NULL incoming gradient counts as zero.
+ * Rotation arrays are 16-byte aligned. */
+int gs_activate_forward(int P, const float* raw_scale, const float* raw_rot, const float* raw_opacity,
+                        float* scales, float* rotations, float* opacities, void* stream);
+int gs_activate_backward(int P, const float* raw_rot, const float* scales, const float* opacities,
+                         const float* dL_dscales, const float* dL_drotations, const float* dL_dopacities,
+                         float* dL_draw_scale, float* dL_draw_rot, float* dL_draw_opacity, void* stream);
+
 /* Opt-in per-stage timing with HIP events recorded on the launch stream (no
  * reference counterpart; the reference's debug mode only synchronises).
  * Stages: preprocess, scan, duplicate, sort, ranges, blend, render_bwd,
