@@ -211,7 +211,6 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
   const float bg_dot = (a.bg[0] * dpix0 + a.bg[1] * dpix1) + a.bg[2] * dpix2;
   const float ddelx = 0.5f * (float)a.W, ddely = 0.5f * (float)a.H;
   const float a_min = 1.0f / 255.0f;
-  float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f, last_d = 0.f;
   const int lane = threadIdx.x & 63;
   __syncthreads();
   // the wave's own list: splats that miss its 8 x 8 pixels are ones no lane would use
@@ -221,8 +220,14 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
   const int lim = wlast > cstart ? (int)min(wlast - cstart, 256u) : 0;
   const int nw = compact_wave_list(s_mask, s_list[threadIdx.x >> 6], threadIdx.x >> 6, lane, lim);
   // Branch-free per pair: masked lanes cost a SIMD as much as active ones,
-  // so every lane evaluates the pair and the pixel state is updated by
-  // select; the only branches are wave-uniform skips when no lane is active.
+  // so every lane evaluates the pair; the only branches are wave-uniform
+  // skips when no lane is active.  An inactive lane runs with alpha = G = 0,
+  // which leaves its state unchanged (T / (1 - 0) = T, 0 c + 1 acc = acc) and
+  // makes each of its gradient terms 0, so no per-lane selects are needed.
+  // The running sums acc = sum_{k > current} c_k alpha_k T_k / T_{current+1}
+  // are advanced right after each splat (the reference advances them
+  // lazily at the next one, from last_alpha / last_color; same expression,
+  // same rounding).
   for (int i = nw - 1; i >= 0; --i) {
     const int j = s_list[threadIdx.x >> 6][i];
     const uint32_t q = cstart + j;  // 0-based list position; contributor index q + 1
@@ -233,26 +238,22 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
     // gb.w: alpha < 1/255 for sure below it
     bool active = inside && q < last && !(power > 0.0f || power < gb.w);
     if (__ballot(active) == 0ull) continue;
-    const float G = spec_expf(power);
-    const float alpha = fminf(0.99f, gb.y * G);
-    active = active && !(alpha < a_min);
+    const float Graw = spec_expf(power);
+    const float alpha_raw = fminf(0.99f, gb.y * Graw);
+    active = active && !(alpha_raw < a_min);
     if (__ballot(active) == 0ull) continue;
+    const float G = active ? Graw : 0.0f;
+    const float alpha = active ? alpha_raw : 0.0f;
     const float4 gc = s_c[j];
     // T before this splat; the gradient tolerates the 1-ulp reciprocal
     const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
     const float Tn = T * inv_1ma;
     const float w = alpha * Tn;
-    const float n0 = last_alpha * lc0 + (1.0f - last_alpha) * acc0;
-    const float n1 = last_alpha * lc1 + (1.0f - last_alpha) * acc1;
-    const float n2 = last_alpha * lc2 + (1.0f - last_alpha) * acc2;
-    const float nd = last_alpha * last_d + (1.0f - last_alpha) * acc_d;
-    const float na = last_alpha + (1.0f - last_alpha) * acc_a;
-    float dL_da = 0.f;
-    dL_da += (gc.x - n0) * dpix0;
-    dL_da += (gc.y - n1) * dpix1;
-    dL_da += (gc.z - n2) * dpix2;
-    dL_da += (gb.z - nd) * dD;
-    dL_da += (1.0f - na) * dA;
+    float dL_da = (gc.x - acc0) * dpix0;
+    dL_da += (gc.y - acc1) * dpix1;
+    dL_da += (gc.z - acc2) * dpix2;
+    dL_da += (gb.z - acc_d) * dD;
+    dL_da += (1.0f - acc_a) * dA;
     dL_da *= Tn;
     dL_da += (-T_fin * inv_1ma) * bg_dot;
     const float dL_dG = gb.y * dL_da;
@@ -260,31 +261,25 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
     const float dG_dx = -gdx * ga.z - gdy * ga.w;
     const float dG_dy = -gdy * gb.x - gdx * ga.w;
     float v[12];
-    v[0] = active ? dL_dG * dG_dx * ddelx : 0.f;
-    v[1] = active ? dL_dG * dG_dy * ddely : 0.f;
-    v[2] = active ? -0.5f * gdx * dx * dL_dG : 0.f;
-    v[3] = active ? -gdx * dy * dL_dG : 0.f;
-    v[4] = active ? -0.5f * gdy * dy * dL_dG : 0.f;
-    v[5] = active ? G * dL_da : 0.f;
-    v[6] = active ? w * dpix0 : 0.f;
-    v[7] = active ? w * dpix1 : 0.f;
-    v[8] = active ? w * dpix2 : 0.f;
-    v[9] = active ? w * dD : 0.f;
+    v[0] = dL_dG * dG_dx * ddelx;
+    v[1] = dL_dG * dG_dy * ddely;
+    v[2] = -0.5f * gdx * dx * dL_dG;
+    v[3] = -gdx * dy * dL_dG;
+    v[4] = -0.5f * gdy * dy * dL_dG;
+    v[5] = G * dL_da;
+    v[6] = w * dpix0;
+    v[7] = w * dpix1;
+    v[8] = w * dpix2;
+    v[9] = w * dD;
     v[10] = 0.f;
     v[11] = 0.f;
-    if (active) {
-      T = Tn;
-      acc0 = n0;
-      acc1 = n1;
-      acc2 = n2;
-      acc_d = nd;
-      acc_a = na;
-      lc0 = gc.x;
-      lc1 = gc.y;
-      lc2 = gc.z;
-      last_d = gb.z;
-      last_alpha = alpha;
-    }
+    T = Tn;
+    const float oma = 1.0f - alpha;
+    acc0 = alpha * gc.x + oma * acc0;
+    acc1 = alpha * gc.y + oma * acc1;
+    acc2 = alpha * gc.z + oma * acc2;
+    acc_d = alpha * gb.z + oma * acc_d;
+    acc_a = alpha + oma * acc_a;
     {
       float w[3];
       wave_sum12(v, w);
