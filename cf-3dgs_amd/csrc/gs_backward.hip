@@ -54,7 +54,8 @@ struct RBArgs {
   const float* ckpt;          // [..][5][256] forward state at batch boundaries
   const uint32_t* ckpt_base;  // [nt + 1]
   const uint32_t* tile_maxc;  // [nt]
-  const uint32_t* item_base;  // [nt + 1]
+  const uint32_t* items;      // [0] count, then (tile, chunk) pairs, longest estimated first
+  const uint32_t* item_log;   // the blend's item records: [0] count, then (tile, chunk << 6 | class)
   const float* dL_dcolor;
   const float* dL_ddepth;
   const float* dL_dalpha;
@@ -65,32 +66,44 @@ struct RBArgs {
   float* part;                 // [capacity][10] partial per (Gaussian, tile) pair
 };
 
-// One workgroup: item_base = exclusive prefix over tiles of ceil(maxc / 256),
-// the number of 256-entry chunks the backward processes for each tile.
-__global__ __launch_bounds__(1024) void k_bwd_items(const uint32_t* tile_maxc, int nt, uint32_t* item_base) {
-  __shared__ uint32_t s[1024];
-  __shared__ uint32_t s_carry;
+// One workgroup orders the backward's work items.  A tile's items are its
+// 256-entry chunks up to its last contributor; the forward's blend logged
+// them with a cost class from the longest per-wave list it had in the same
+// batch, which the render backward's time per chunk follows closely.  Items
+// are handed out longest first (longest-processing-time order): a large frame
+// has more items than fit on the GPU at once, and an item that starts late
+// and runs long keeps the launch going after the rest have drained.  This is
+// a counting sort on the 64 classes.  The order within a class is whatever
+// the LDS atomics give: no result depends on the order of items
+// (deterministic mode writes one slot per pair; the atomic mode is
+// order-dependent in any case).
+__global__ __launch_bounds__(1024) void k_bwd_items(const uint32_t* item_log, uint32_t* items) {
+  __shared__ uint32_t s_cls[64];
   const int t = threadIdx.x;
-  if (t == 0) s_carry = 0;
+  const uint32_t n = item_log[0];
+  if (t < 64) s_cls[t] = 0;
   __syncthreads();
-  for (int b0 = 0; b0 < nt; b0 += 1024) {
-    const int b = b0 + t;
-    const uint32_t v = b < nt ? (tile_maxc[b] + 255) / 256 : 0u;
-    s[t] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const uint32_t add = t >= off ? s[t - off] : 0u;
-      __syncthreads();
-      s[t] += add;
-      __syncthreads();
+  for (uint32_t i = t; i < n; i += 1024) atomicAdd(&s_cls[item_log[2 + 2 * i] & 63u], 1u);
+  __syncthreads();
+  if (t < 64) {
+    // exclusive scan of the 64 class counts within wave 0
+    const uint32_t v = s_cls[t];
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t up = __shfl_up(incl, off, 64);
+      if (t >= off) incl += up;
     }
-    const uint32_t carry = s_carry;
-    if (b < nt) item_base[b] = carry + s[t] - v;
-    __syncthreads();
-    if (t == 1023) s_carry = carry + s[1023];
-    __syncthreads();
+    s_cls[t] = incl - v;
   }
-  if (t == 0) item_base[nt] = s_carry;
+  __syncthreads();
+  for (uint32_t i = t; i < n; i += 1024) {
+    const uint32_t tile = item_log[1 + 2 * i], cc = item_log[2 + 2 * i];
+    const uint32_t pos = atomicAdd(&s_cls[cc & 63u], 1u);
+    items[1 + 2 * pos] = tile;
+    items[2 + 2 * pos] = cc >> 6;
+  }
+  if (t == 0) items[0] = n;
 }
 
 // Sum over the 16 lanes of each DPP row; every lane of the row gets the sum.
@@ -141,17 +154,9 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
   __shared__ uint32_t s_mask[256];    // wave rows each staged splat can reach (splat_box)
   __shared__ uint8_t s_list[4][256];  // per wave: the staged splats that reach its pixels
   __shared__ float s_acc[10 * kParts][256];
-  const uint32_t nitems = a.item_base[a.nt];
-  if (item >= nitems) return;
-  // tile = last t with item_base[t] <= item (tiles without items share the next base)
-  int lo = 0, hi = a.nt - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (a.item_base[mid] <= item) lo = mid;
-    else hi = mid - 1;
-  }
-  const int tile = lo;
-  const uint32_t chunk = item - a.item_base[tile];
+  if (item >= a.items[0]) return;
+  const int tile = (int)a.items[1 + 2 * item];
+  const uint32_t chunk = a.items[2 + 2 * item];
   const uint32_t maxc = a.tile_maxc[tile];
   const uint32_t nchunk = (maxc + 255) / 256;
   const uint32_t cstart = chunk * 256;
@@ -704,8 +709,8 @@ extern "C" size_t gs_backward_scratch_bytes(int P, long long capacity, int deter
 }
 
 // Render-backward arguments of one frame (workspaces of its forward).
-static RBArgs make_rb_args(const gs_frame_t* f, int P, const void* geom, long long capacity, const void* binning,
-                           void* image, const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
+static RBArgs make_rb_args(const gs_frame_t* f, int P, const void* geom, long long capacity, void* binning,
+                           const void* image, const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
                            void* scratch) {
   gs_geom_layout_t LG;
   gs_binning_layout_t LB;
@@ -716,7 +721,7 @@ static RBArgs make_rb_args(const gs_frame_t* f, int P, const void* geom, long lo
   const int gx = (f->width + kBX - 1) / kBX, gy = (f->height + kBY - 1) / kBY;
   const char* gb = (const char*)geom;
   const char* bb = (const char*)binning;
-  char* ib = (char*)image;
+  const char* ib = (const char*)image;
   float* g_render = (float*)scratch;
   float* pose_part = (float*)((char*)scratch + align_up((size_t)P * kRG * sizeof(float), 256));
   float* part = (float*)((char*)pose_part + align_up(pose_parts(P) * 12 * sizeof(float), 256));
@@ -736,7 +741,8 @@ static RBArgs make_rb_args(const gs_frame_t* f, int P, const void* geom, long lo
   r.ckpt = capacity > 0 ? (const float*)(bb + LB.ckpt) : nullptr;
   r.ckpt_base = (const uint32_t*)(ib + LI.ckpt_base);
   r.tile_maxc = (const uint32_t*)(ib + LI.tile_maxc);
-  r.item_base = (const uint32_t*)(ib + LI.item_base);
+  r.items = (const uint32_t*)(bb + LB.items);
+  r.item_log = (const uint32_t*)(bb + LB.item_log);
   r.dL_dcolor = dL_dcolor;
   r.dL_ddepth = dL_ddepth;
   r.dL_dalpha = dL_dalpha;
@@ -755,7 +761,7 @@ static long long rb_max_items(const gs_frame_t* f, long long capacity) {
 }
 
 static int bwd_check(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
-                     const void* binning, long long capacity, void* image, const uint32_t* counters,
+                     void* binning, long long capacity, const void* image, const uint32_t* counters,
                      const float* dL_dcolor, const gs_grads_t* out, void* scratch, size_t scratch_bytes) {
   if (!f || !g || !out || !counters || !image || !dL_dcolor) return GS_E_ARG;
   if (f->width <= 0 || f->height <= 0 || !f->viewmatrix || !f->projmatrix || !f->bg) return GS_E_ARG;
@@ -788,10 +794,10 @@ static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const RB
   // tile's last contributor are never visited, so their partials start at 0
   if (f->deterministic) zero(r.part, capacity * 10);
   else zero(r.g_render, (long long)P * kRG);
-  // (the item table lives in the image workspace, which the forward owns;
-  // the backward rewrites it from tile_maxc before use)
+  // the work-item table, from the forward's item log (both live in the
+  // binning workspace)
   if (capacity > 0 && P > 0)
-    hipLaunchKernelGGL(k_bwd_items, dim3(1), dim3(1024), 0, s, r.tile_maxc, r.nt, (uint32_t*)r.item_base);
+    hipLaunchKernelGGL(k_bwd_items, dim3(1), dim3(1024), 0, s, r.item_log, (uint32_t*)r.items);
 }
 
 // Phase 2: the render backward of one frame.
@@ -863,8 +869,9 @@ static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* 
 }
 
 #define GS_BWD_ARGS                                                                                              \
-  const gs_frame_t *f, const gs_gaussians_t *g, const int *radii, const void *geom, const void *binning,       \
-      long long capacity, void *image, const uint32_t *counters, const float *dL_dcolor, const float *dL_ddepth, \
+  const gs_frame_t *f, const gs_gaussians_t *g, const int *radii, const void *geom, void *binning,             \
+      long long capacity, const void *image, const uint32_t *counters, const float *dL_dcolor,                  \
+      const float *dL_ddepth,                                                                                   \
       const float *dL_dalpha, const gs_grads_t *out, void *scratch, size_t scratch_bytes, void *stream
 
 extern "C" int gs_backward(GS_BWD_ARGS) {
@@ -907,7 +914,7 @@ extern "C" size_t gs_multi_render_bwd_bytes(int n) {
 }
 
 extern "C" int gs_multi_render_bwd_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
-                                         const long long* capacity, const void* const* binning, void* const* image,
+                                         const long long* capacity, void* const* binning, const void* const* image,
                                          const float* const* dL_dcolor, const float* const* dL_ddepth,
                                          const float* const* dL_dalpha, void* const* scratch, void* host_set,
                                          size_t set_bytes, unsigned* total_blocks) {

@@ -460,6 +460,12 @@ struct BlendArgs {
   float* ckpt;
   uint32_t* tile_maxc;
   float* accum;
+  // per (tile, batch) at ckpt_base[tile] + tile + batch: the longest
+  // per-wave list of the batch, the backward's cost estimate for that chunk
+  uint32_t* bweight;
+  // the backward's work items, appended as tiles finish: [0] count, then
+  // (tile, chunk << 6 | cost class) -- cleared by k_tile_ckpt_scan
+  uint32_t* item_log;
 };
 
 __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
@@ -467,8 +473,9 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
   __shared__ float4 s_b[256];  // conic c, opacity, depth, -
   __shared__ float4 s_c[256];  // r, g, b, -
   __shared__ uint32_t s_mask[256];    // wave rows each staged splat can reach (splat_box)
-  __shared__ uint8_t s_list[4][256];  // per wave: the staged splats that reach its 8 x 8 pixels
+  __shared__ __align__(4) uint8_t s_list[4][256];  // per wave: the staged splats that reach its 8 x 8 pixels
   __shared__ uint32_t s_maxc;
+  __shared__ uint32_t s_bw;  // longest per-wave list of the current batch
   const int tx = tile % a.gx, ty = tile / a.gx;
   int pdx, pdy;
   tile_pixel(threadIdx.x, pdx, pdy);
@@ -477,6 +484,15 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
   const float pfx = (float)px, pfy = (float)py;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t r0 = a.ranges[2 * tile], r1 = a.ranges[2 * tile + 1];
+  // All tiles of a frame are resident at once, so the launch lasts as long
+  // as its heaviest tiles; their waves get the SIMDs first (issue priority by
+  // list length) and the light tiles fill in around them.
+  {
+    const uint32_t len = r1 - r0;
+    if (len >= 8192u) __builtin_amdgcn_s_setprio(3);
+    else if (len >= 4096u) __builtin_amdgcn_s_setprio(2);
+    else if (len >= 2048u) __builtin_amdgcn_s_setprio(1);
+  }
   bool done = !inside;
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
   uint32_t last = 0;
@@ -493,6 +509,10 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
       ck[1024] = D;
     }
     if (__syncthreads_count(done) == 256) break;
+    if (threadIdx.x == 0) {
+      if (batch > 0) a.bweight[a.ckpt_base[tile] + tile + batch - 1] = s_bw;
+      s_bw = 0;
+    }
     const uint32_t k = base + threadIdx.x;
     uint32_t m = 0u;
     if (k < r1) {
@@ -509,6 +529,7 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
     __syncthreads();
     // a splat that misses the wave's pixels is one every lane would skip
     const int nw = compact_wave_list(s_mask, s_list[wv], wv, lane);
+    if (lane == 0) atomicMax(&s_bw, (uint32_t)nw);
     const uint32_t first = base - r0 + 1;  // contributor index of staged position 0
     // alpha of staged splat j at this pixel, 0 when it is skipped
     auto alpha_at = [&](int j) -> float {
@@ -536,16 +557,23 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
       T = test_T;
       last = first + (uint32_t)j;
     };
-    // two list entries per step: their alphas do not depend on each other, so
-    // both exponentials are in flight together; they are applied in order
-    for (int i = 0; !done && i < nw; i += 2) {
-      const int j0 = s_list[wv][i];
-      const bool two = i + 1 < nw;
-      const int j1 = two ? (int)s_list[wv][i + 1] : j0;
+    // four list entries per step: their alphas do not depend on each other,
+    // so the four exponentials are in flight together (the heaviest tiles run
+    // nearly alone at the end of the launch, where this latency is the time);
+    // they are applied in order.  The indices come in one 4-byte LDS read;
+    // slots past nw hold stale indices of staged splats and are masked.
+    for (int i = 0; !done && i < nw; i += 4) {
+      const uint32_t jj = *reinterpret_cast<const uint32_t*>(&s_list[wv][i]);
+      const int j0 = jj & 0xffu, j1 = (jj >> 8) & 0xffu, j2 = (jj >> 16) & 0xffu, j3 = jj >> 24;
+      const int left = nw - i;
       const float a0 = alpha_at(j0);
-      const float a1 = two ? alpha_at(j1) : 0.0f;
+      const float a1 = left > 1 ? alpha_at(j1) : 0.0f;
+      const float a2 = left > 2 ? alpha_at(j2) : 0.0f;
+      const float a3 = left > 3 ? alpha_at(j3) : 0.0f;
       apply(j0, a0);
       if (!done) apply(j1, a1);
+      if (!done) apply(j2, a2);
+      if (!done) apply(j3, a3);
     }
   }
   if (inside) {
@@ -565,7 +593,25 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
     if (last) atomicMax(&s_maxc, last);
   }
   __syncthreads();
-  if (threadIdx.x == 0) a.tile_maxc[tile] = s_maxc;
+  if (threadIdx.x == 0) {
+    a.tile_maxc[tile] = s_maxc;
+    if (batch > 0) a.bweight[a.ckpt_base[tile] + tile + batch - 1] = s_bw;  // the last batch run
+  }
+  __syncthreads();
+  // The backward's work items of this tile: its 256-entry chunks up to the
+  // last contributor, each with a cost class from the longest per-wave list
+  // the blend had in the same batch (class 0 = longest).
+  if (threadIdx.x < 64) {
+    const uint32_t nch = (s_maxc + 255u) / 256u;
+    const uint32_t wbase = a.ckpt_base[tile] + tile;
+    for (uint32_t c = lane; c < nch; c += 64) {
+      const uint32_t w = a.bweight[wbase + c];
+      const uint32_t cls = 63u - min(w >> 2, 63u);
+      const uint32_t slot = atomicAdd(&a.item_log[0], 1u);
+      a.item_log[1 + 2 * slot] = tile;
+      a.item_log[2 + 2 * slot] = (c << 6) | cls;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_blend(BlendArgs a) { blend_tile(a, xcd_remap(blockIdx.x, gridDim.x)); }
@@ -592,11 +638,16 @@ __global__ __launch_bounds__(256) void k_blend_multi(const uint32_t* prefix, con
 
 // One workgroup: checkpoint base of every tile = exclusive prefix over tiles
 // of (batches in its list - 1); ckpt_base[n_tiles] = total checkpoints.
-__global__ __launch_bounds__(1024) void k_tile_ckpt_scan(const uint32_t* ranges, int nt, uint32_t* ckpt_base) {
+// It also clears the blend's work-item log.
+__global__ __launch_bounds__(1024) void k_tile_ckpt_scan(const uint32_t* ranges, int nt, uint32_t* ckpt_base,
+                                                         uint32_t* item_log) {
   __shared__ uint32_t s[1024];
   __shared__ uint32_t s_carry;
   const int t = threadIdx.x;
-  if (t == 0) s_carry = 0;
+  if (t == 0) {
+    s_carry = 0;
+    item_log[0] = 0;
+  }
   __syncthreads();
   for (int b0 = 0; b0 < nt; b0 += 1024) {
     const int b = b0 + t;
@@ -704,6 +755,13 @@ extern "C" int gs_binning_layout(long long capacity, int P, int width, int heigh
   // blend checkpoints: at most capacity / 256 batch boundaries, 5 x 256 floats each
   out->ckpt = o;
   o = align_up(o + (size_t)(c / 256 + 1) * 5 * 256 * sizeof(float), 256);
+  // per (tile, batch) cost estimates: tile t's batches start at ckpt_base[t] + t
+  out->bweight = o;
+  o = align_up(o + 4 * (c / 256 + 2 * (size_t)nt + 2), 256);
+  out->items = o;
+  o = align_up(o + 4 * (1 + 2 * (c / 256 + (size_t)nt + 1)), 256);
+  out->item_log = o;
+  o = align_up(o + 4 * (1 + 2 * (c / 256 + (size_t)nt + 1)), 256);
   out->total = o;
   const int end_bit = (df ? 0 : 32) + bits_for((uint32_t)nt);
   out->sort_passes = sort_passes(end_bit > 0 ? end_bit : 1);
@@ -732,8 +790,6 @@ extern "C" int gs_image_layout(int width, int height, gs_image_layout_t* out) {
   o = align_up(o + 4 * ((size_t)nt + 1), 256);
   out->tile_maxc = o;
   o = align_up(o + 4 * (size_t)nt, 256);
-  out->item_base = o;
-  o = align_up(o + 4 * ((size_t)nt + 1), 256);
   out->total = o;
   return GS_OK;
 }
@@ -903,13 +959,13 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
                        (const uint64_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, ranges);
   }
   hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)ranges, nt,
-                     (uint32_t*)(ib + LI.ckpt_base));
+                     (uint32_t*)(ib + LI.ckpt_base), (uint32_t*)(bb + LB.item_log));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
 // Blend arguments of one frame (its workspaces as binned by render_binning).
 static BlendArgs make_blend_args(const gs_frame_t* f, int P, const void* geom, long long capacity,
-                                 const void* binning, void* image, float* out_color, float* out_depth,
+                                 void* binning, void* image, float* out_color, float* out_depth,
                                  float* out_alpha) {
   gs_geom_layout_t LG;
   gs_binning_layout_t LB;
@@ -938,6 +994,8 @@ static BlendArgs make_blend_args(const gs_frame_t* f, int P, const void* geom, l
   b.ckpt_base = (const uint32_t*)(ib + LI.ckpt_base);
   b.ckpt = (float*)(bb + LB.ckpt);
   b.tile_maxc = (uint32_t*)(ib + LI.tile_maxc);
+  b.bweight = (uint32_t*)(bb + LB.bweight);
+  b.item_log = (uint32_t*)(bb + LB.item_log);
   b.accum = (float*)(ib + LI.accum);
   return b;
 }
@@ -976,7 +1034,7 @@ extern "C" size_t gs_multi_blend_bytes(int n) {
 }
 
 extern "C" int gs_multi_blend_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
-                                    const long long* capacity, const void* const* binning, void* const* image,
+                                    const long long* capacity, void* const* binning, void* const* image,
                                     float* const* out_color, float* const* out_depth, float* const* out_alpha,
                                     void* host_set, size_t set_bytes, unsigned* total_tiles) {
   if (n <= 0 || !frames || !P || !geom || !capacity || !binning || !image || !out_color || !out_depth || !host_set ||

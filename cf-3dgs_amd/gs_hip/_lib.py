@@ -62,12 +62,12 @@ class GsBinningLayout(ctypes.Structure):
                 ("keys_sorted", c_size_t), ("vals_sorted", c_size_t), ("hist", c_size_t), ("sort_passes", c_int),
                 ("key_bytes", c_int), ("dkeys_a", c_size_t), ("dvals_a", c_size_t), ("dkeys_b", c_size_t),
                 ("dvals_b", c_size_t), ("dscan", c_size_t), ("dblocks", c_size_t), ("ckpt", c_size_t),
-                ("total", c_size_t)]
+                ("bweight", c_size_t), ("items", c_size_t), ("item_log", c_size_t), ("total", c_size_t)]
 
 
 class GsImageLayout(ctypes.Structure):
     _fields_ = [("ranges", c_size_t), ("final_T", c_size_t), ("n_contrib", c_size_t), ("accum", c_size_t),
-                ("ckpt_base", c_size_t), ("tile_maxc", c_size_t), ("item_base", c_size_t), ("total", c_size_t)]
+                ("ckpt_base", c_size_t), ("tile_maxc", c_size_t), ("total", c_size_t)]
 
 
 EXPORTS = {

@@ -126,6 +126,12 @@ typedef struct gs_binning_layout {
   size_t dkeys_a, dvals_a, dkeys_b, dvals_b; /* uint32[P] depth sort (DEPTH_FIRST) */
   size_t dscan, dblocks;       /* uint32[P] tiles prefix in depth order, per-256 block sums */
   size_t ckpt;                 /* float[capacity/256 + 1][5][256] blend state at list batch boundaries */
+  size_t bweight;              /* uint32[capacity/256 + 2 n_tiles + 2] per (tile, list batch): the blend's
+                                  longest per-wave list, the backward's estimate of that chunk's cost */
+  size_t items;                /* uint32[1 + 2 (capacity/256 + n_tiles + 1)] backward work items:
+                                  count, then (tile, chunk) pairs, longest estimated first */
+  size_t item_log;             /* same size: the blend's record of the work items, in completion
+                                  order: count, then (tile, chunk << 6 | cost class) */
   size_t total;
 } gs_binning_layout_t;
 
@@ -136,7 +142,6 @@ typedef struct gs_image_layout {
   size_t accum;     /* float[4][H*W] colour sums before the background term, depth */
   size_t ckpt_base; /* uint32[n_tiles+1] first checkpoint of each tile */
   size_t tile_maxc; /* uint32[n_tiles] largest contributor index in the tile */
-  size_t item_base; /* uint32[n_tiles+1] backward work items (256-entry chunks) per tile, prefix */
   size_t total;
 } gs_image_layout_t;
 
@@ -167,10 +172,11 @@ int gs_render(const gs_frame_t* frame, int P, const void* geom, uint32_t* counte
  * triggered by loss.backward() at trainer/cf3dgs_trainer.py:133).  dL_ddepth and
  * dL_dalpha may be NULL (treated as zero).  scratch: gs_backward_scratch_bytes(P,
  * capacity, frame->deterministic).
- * The image workspace is the forward's; the backward rewrites its work-item
- * table (item_base) and reads everything else. */
+ * The image workspace is the forward's and is only read; in the binning
+ * workspace the backward writes its work-item table (items) and reads the
+ * rest. */
 int gs_backward(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
-                const void* binning, long long capacity, void* image, const uint32_t* counters,
+                void* binning, long long capacity, const void* image, const uint32_t* counters,
                 const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha, const gs_grads_t* out,
                 void* scratch, size_t scratch_bytes, void* stream);
 
@@ -193,21 +199,21 @@ int gs_render_binning(const gs_frame_t* frame, int P, const void* geom, uint32_t
                       void* binning, size_t binning_bytes, void* image, size_t image_bytes, void* stream);
 size_t gs_multi_blend_bytes(int n);
 int gs_multi_blend_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
-                         const long long* capacity, const void* const* binning, void* const* image,
+                         const long long* capacity, void* const* binning, void* const* image,
                          float* const* out_color, float* const* out_depth, float* const* out_alpha,
                          void* host_set, size_t set_bytes, unsigned* total_tiles);
 int gs_multi_blend(int n, const void* dev_set, unsigned total_tiles, void* stream);
 int gs_backward_prepare(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
-                        const void* binning, long long capacity, void* image, const uint32_t* counters,
+                        void* binning, long long capacity, const void* image, const uint32_t* counters,
                         const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
                         const gs_grads_t* out, void* scratch, size_t scratch_bytes, void* stream);
 int gs_backward_finish(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
-                       const void* binning, long long capacity, void* image, const uint32_t* counters,
+                       void* binning, long long capacity, const void* image, const uint32_t* counters,
                        const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
                        const gs_grads_t* out, void* scratch, size_t scratch_bytes, void* stream);
 size_t gs_multi_render_bwd_bytes(int n);
 int gs_multi_render_bwd_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
-                              const long long* capacity, const void* const* binning, void* const* image,
+                              const long long* capacity, void* const* binning, const void* const* image,
                               const float* const* dL_dcolor, const float* const* dL_ddepth,
                               const float* const* dL_dalpha, void* const* scratch, void* host_set,
                               size_t set_bytes, unsigned* total_blocks);
