@@ -422,8 +422,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     const float* src = a.shs + (size_t)base * row;
     const int total = nloc * row;
     if ((a.M & 3) == 0) {
-      for (int k = threadIdx.x; k < (total >> 2); k += 256)
-        reinterpret_cast<float4*>(s_sh)[k] = reinterpret_cast<const float4*>(src)[k];
+      block_copy16(reinterpret_cast<float4*>(s_sh), reinterpret_cast<const float4*>(src), total >> 2);
     } else {
       for (int k = threadIdx.x; k < total; k += 256) s_sh[k] = src[k];
     }
@@ -543,17 +542,22 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
         const float* sh = staged ? s_sh + threadIdx.x * row : a.shs + (size_t)i * row;
         float* gsh = !a.dL_dsh ? nullptr : (staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row);
         float gdir0 = 0.f, gdir1 = 0.f, gdir2 = 0.f;
-        for (int k = 0; k < nc; ++k) {
-          const float s0 = sh[3 * k], s1 = sh[3 * k + 1], s2 = sh[3 * k + 2];
-          if (gsh) {
-            gsh[3 * k] = b[k] * gc[0];
-            gsh[3 * k + 1] = b[k] * gc[1];
-            gsh[3 * k + 2] = b[k] * gc[2];
+        // unrolled to the 16 coefficients of degree 3, so b / bg3 are indexed
+        // by constants and live in registers (a runtime bound puts them in scratch)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          if (k < nc) {
+            const float s0 = sh[3 * k], s1 = sh[3 * k + 1], s2 = sh[3 * k + 2];
+            if (gsh) {
+              gsh[3 * k] = b[k] * gc[0];
+              gsh[3 * k + 1] = b[k] * gc[1];
+              gsh[3 * k + 2] = b[k] * gc[2];
+            }
+            const float sd = s0 * gc[0] + s1 * gc[1] + s2 * gc[2];
+            gdir0 += sd * bg3[k][0];
+            gdir1 += sd * bg3[k][1];
+            gdir2 += sd * bg3[k][2];
           }
-          const float sd = s0 * gc[0] + s1 * gc[1] + s2 * gc[2];
-          gdir0 += sd * bg3[k][0];
-          gdir1 += sd * bg3[k][1];
-          gdir2 += sd * bg3[k][2];
         }
         if (gsh)
           for (int k = 3 * nc; k < 3 * a.M; ++k) gsh[k] = 0.f;
@@ -627,8 +631,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     float* dst = a.dL_dsh + (size_t)base * row;
     const int total = nloc * row;
     if ((a.M & 3) == 0) {
-      for (int k = threadIdx.x; k < (total >> 2); k += 256)
-        reinterpret_cast<float4*>(dst)[k] = reinterpret_cast<const float4*>(s_sh)[k];
+      block_copy16(reinterpret_cast<float4*>(dst), reinterpret_cast<const float4*>(s_sh), total >> 2);
     } else {
       for (int k = threadIdx.x; k < total; k += 256) dst[k] = s_sh[k];
     }

@@ -255,6 +255,26 @@ __device__ __forceinline__ float wave_sum(float v) {
   return (r0 + r1) + (r2 + r3);
 }
 
+// Block copies between global memory and LDS for 256-thread workgroups,
+// n16 16-byte vectors.  Every thread issues all of its (up to 12) loads
+// before its first store, so a workgroup has its whole block in flight at
+// once instead of one load-to-store round trip per vector (the SH rows of
+// 256 Gaussians at M = 16 are exactly 12 vectors per thread).
+// (The loads are unconditional, clamped to the last vector, so the staging
+// array is fully written and stays in registers.)  dst and src must not overlap.
+__device__ __forceinline__ void block_copy16(float4* __restrict__ dst, const float4* __restrict__ src, int n16) {
+  constexpr int kMax = 12;
+  if (n16 <= 0) return;
+  float4 v[kMax];
+#pragma unroll
+  for (int r = 0; r < kMax; ++r) v[r] = src[min((int)threadIdx.x + r * 256, n16 - 1)];
+  // stores clamped the same way: the duplicates write the last vector's own
+  // value to it again, and no store waits inside a branch for its load
+#pragma unroll
+  for (int r = 0; r < kMax; ++r) dst[min((int)threadIdx.x + r * 256, n16 - 1)] = v[r];
+  for (int k = (int)threadIdx.x + kMax * 256; k < n16; k += 256) dst[k] = src[k];
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const uint32_t lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));

@@ -77,9 +77,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
       const float* src = a.shs + (size_t)base * 3 * a.M;
       const int total = nloc * stride;
       if ((a.M & 3) == 0) {
-        const float4* s4 = reinterpret_cast<const float4*>(src);
-        float4* d4 = reinterpret_cast<float4*>(s_sh);
-        for (int k = threadIdx.x; k < (total >> 2); k += 256) d4[k] = s4[k];
+        block_copy16(reinterpret_cast<float4*>(s_sh), reinterpret_cast<const float4*>(src), total >> 2);
       } else {
         for (int k = threadIdx.x; k < total; k += 256) s_sh[k] = src[k];
       }

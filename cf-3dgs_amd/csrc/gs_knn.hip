@@ -218,19 +218,15 @@ __global__ __launch_bounds__(256) void k_knn_scatter(const float* pts, int P, co
   sorted[starts[c] + k] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], __int_as_float(i));
 }
 
+// Sorted insert into b0 <= b1 <= b2 (the three smallest so far), written as
+// min / max selects: the nested-branch form is lowered to a stack array with
+// a computed index, which puts the three values in scratch memory.  d >= b2
+// (and NaN) leaves them unchanged; ties give the same three values either way.
 __device__ __forceinline__ void best3_insert(float d, float& b0, float& b1, float& b2) {
   if (d < b2) {
-    if (d < b1) {
-      b2 = b1;
-      if (d < b0) {
-        b1 = b0;
-        b0 = d;
-      } else {
-        b1 = d;
-      }
-    } else {
-      b2 = d;
-    }
+    b2 = fminf(b2, fmaxf(b1, d));
+    b1 = fminf(b1, fmaxf(b0, d));
+    b0 = fminf(b0, d);
   }
 }
 
