@@ -482,15 +482,6 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
   const float pfx = (float)px, pfy = (float)py;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t r0 = a.ranges[2 * tile], r1 = a.ranges[2 * tile + 1];
-  // All tiles of a frame are resident at once, so the launch lasts as long
-  // as its heaviest tiles; their waves get the SIMDs first (issue priority by
-  // list length) and the light tiles fill in around them.
-  {
-    const uint32_t len = r1 - r0;
-    if (len >= 8192u) __builtin_amdgcn_s_setprio(3);
-    else if (len >= 4096u) __builtin_amdgcn_s_setprio(2);
-    else if (len >= 2048u) __builtin_amdgcn_s_setprio(1);
-  }
   bool done = !inside;
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
   uint32_t last = 0;

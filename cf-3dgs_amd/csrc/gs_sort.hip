@@ -8,10 +8,9 @@
 //
 // Per 8-bit digit pass:
 //   k_sort_upsweep  1024 or 4096 keys per workgroup (4 waves x 4 or 16 rounds
-//                   x 64 lanes);
-//                   wave64 ballot "match" groups equal digits so each group
-//                   adds its count once -> per-workgroup digit counts
-//                   (digit-major, so one row scan per digit orders blocks)
+//                   x 64 lanes), all loaded up front, counted with one LDS
+//                   atomic each -> per-workgroup digit counts (digit-major,
+//                   so one row scan per digit orders blocks)
 //   k_sort_scan     one workgroup per digit: exclusive scan of its row, total
 //   k_sort_scatter  recomputes the matches; rank = prior count of the digit in
 //                   this wave + lanes below with the same digit; waves are
@@ -37,6 +36,25 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid, int bits
   return m;
 }
 
+// Inclusive prefix sum over a 256-thread workgroup (thread t holds v); the
+// workgroup total in `total`.  Each wave scans by shuffles, then one barrier
+// combines the four wave totals (instead of a barrier pair per Hillis-Steele
+// step).  s_w (4 words of LDS) may be rewritten only after a later barrier.
+__device__ __forceinline__ uint32_t block_incl_scan256(uint32_t v, uint32_t* s_w, uint32_t& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  const uint32_t w0 = s_w[0], w1 = s_w[1], w2 = s_w[2], w3 = s_w[3];
+  total = (w0 + w1) + (w2 + w3);
+  return x + (w > 0 ? w0 : 0u) + (w > 1 ? w1 : 0u) + (w > 2 ? w2 : 0u);
+}
+
 __device__ __forceinline__ uint32_t n_eff(const uint32_t* n_dev, long long cap) {
   const uint32_t n = *n_dev;
   return ((long long)n > cap) ? 0u : n;
@@ -55,14 +73,40 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t mask = (1u << bits) - 1u;
   const uint32_t wbase = start + (uint32_t)w * (ITEMS * 64);
-#pragma unroll 4
-  for (int r = 0; r < ITEMS; ++r) {
-    const uint32_t idx = wbase + r * 64 + lane;
-    const bool valid = idx < n;
-    const uint32_t d = valid ? (uint32_t)(keys[idx] >> shift) & mask : 0u;
-    const uint64_t peers = match_digit(d, valid, bits);
-    const bool leader = valid && ((peers & lanemask_lt()) == 0);
-    if (leader) s_h[w][d] += (uint32_t)__popcll(peers);
+  // a histogram needs no ranks: one LDS atomic per key into the wave's own
+  // row, with every key of the thread loaded before the first atomic.  Order
+  // does not matter here, so 32-bit keys come four to a lane in 16-byte loads
+  // (the key arrays are 16-byte aligned).
+  if constexpr (sizeof(K) == 4 && ITEMS % 4 == 0) {
+    constexpr int V = ITEMS / 4;
+    uint4 kv[V];
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const uint32_t idx = wbase + 4u * (uint32_t)(r * 64 + lane);
+      kv[r] = idx + 3u < n ? *reinterpret_cast<const uint4*>(keys + idx)
+                           : make_uint4(idx < n ? (uint32_t)keys[idx] : 0u, idx + 1u < n ? (uint32_t)keys[idx + 1] : 0u,
+                                        idx + 2u < n ? (uint32_t)keys[idx + 2] : 0u, 0u);
+    }
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const uint32_t idx = wbase + 4u * (uint32_t)(r * 64 + lane);
+      const uint32_t e[4] = {kv[r].x, kv[r].y, kv[r].z, kv[r].w};
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (idx + (uint32_t)c < n) atomicAdd(&s_h[w][(e[c] >> shift) & mask], 1u);
+    }
+  } else {
+    K kk[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t idx = wbase + r * 64 + lane;
+      kk[r] = idx < n ? keys[idx] : (K)0;
+    }
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t idx = wbase + r * 64 + lane;
+      if (idx < n) atomicAdd(&s_h[w][(uint32_t)(kk[r] >> shift) & mask], 1u);
+    }
   }
   __syncthreads();
   for (int d = threadIdx.x; d < R; d += kSortThreads)
@@ -72,33 +116,23 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
 // One workgroup per digit: exclusive scan over the active blocks' counts.
 __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* totals, const uint32_t* n_dev,
                                                    long long cap, int nb_max, int tile) {
-  __shared__ uint32_t s[256];
-  __shared__ uint32_t s_carry;
+  __shared__ uint32_t s_w[4];
   const uint32_t n = n_eff(n_dev, cap);
   const int nb = (int)((n + tile - 1) / tile);
   const int d = blockIdx.x;
   uint32_t* row = hist + (size_t)d * nb_max;
   const int t = threadIdx.x;
-  if (t == 0) s_carry = 0;
-  __syncthreads();
+  uint32_t carry = 0;
   for (int b0 = 0; b0 < nb; b0 += 256) {
     const int b = b0 + t;
     const uint32_t v = b < nb ? row[b] : 0u;
-    s[t] = v;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-      const uint32_t add = t >= off ? s[t - off] : 0u;
-      __syncthreads();
-      s[t] += add;
-      __syncthreads();
-    }
-    const uint32_t carry = s_carry;
-    if (b < nb) row[b] = carry + s[t] - v;
-    __syncthreads();
-    if (t == 255) s_carry = carry + s[255];
-    __syncthreads();
+    uint32_t tot;
+    const uint32_t incl = block_incl_scan256(v, s_w, tot);
+    if (b < nb) row[b] = carry + incl - v;
+    carry += tot;
+    __syncthreads();  // s_w is rewritten by the next chunk
   }
-  if (t == 0) totals[d] = s_carry;
+  if (t == 0) totals[d] = carry;
 }
 
 template <typename K, int ITEMS, int R>
@@ -113,6 +147,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   constexpr int TILE = kSortThreads * ITEMS;
   __shared__ uint32_t s_cnt[4][R];
   __shared__ uint32_t s_scan[R];
+  __shared__ uint32_t s_w[4];
   __shared__ K s_key[TILE];
   __shared__ uint32_t s_val[TILE];
   const uint32_t n = n_eff(n_dev, cap);
@@ -122,17 +157,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   // global base of each digit for this block = exclusive scan of totals + row offset
   {
     const uint32_t v = totals[t];
-    s_scan[t] = v;
-    __syncthreads();
-    for (int off = 1; off < R; off <<= 1) {
-      const uint32_t add = t >= off ? s_scan[t - off] : 0u;
-      __syncthreads();
-      s_scan[t] += add;
-      __syncthreads();
-    }
-    const uint32_t excl = s_scan[t] - v + hist[(size_t)t * nb_max + blockIdx.x];
-    __syncthreads();
-    s_scan[t] = excl;
+    uint32_t tot;
+    s_scan[t] = block_incl_scan256(v, s_w, tot) - v + hist[(size_t)t * nb_max + blockIdx.x];
   }
   for (int k = t; k < 4 * R; k += kSortThreads) (&s_cnt[0][0])[k] = 0;
   __syncthreads();
@@ -158,18 +184,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   __syncthreads();
   // digit t: its waves' local bases (exclusive scan over digits of the block
   // counts, then over waves) and the offset from local to global position
-  __shared__ uint32_t s_tot[R];
   const uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t], c3 = s_cnt[3][t];
   const uint32_t c = c0 + c1 + c2 + c3;
-  s_tot[t] = c;
-  __syncthreads();
-  for (int off = 1; off < R; off <<= 1) {
-    const uint32_t add = t >= off ? s_tot[t - off] : 0u;
-    __syncthreads();
-    s_tot[t] += add;
-    __syncthreads();
-  }
-  const uint32_t lbase = s_tot[t] - c;
+  uint32_t tot;
+  const uint32_t lbase = block_incl_scan256(c, s_w, tot) - c;
   s_cnt[0][t] = lbase;
   s_cnt[1][t] = lbase + c0;
   s_cnt[2][t] = lbase + c0 + c1;
@@ -280,6 +298,7 @@ extern "C" int gs_sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* k
   if (capacity < 0 || capacity > 0x7fffffffLL || end_bit < 1 || end_bit > 32 || !n_dev || !result_in_b)
     return GS_E_ARG;
   if (capacity > 0 && (!keys_a || !vals_a || !keys_b || !vals_b || !tmp)) return GS_E_ARG;
+  if (((uintptr_t)keys_a | (uintptr_t)keys_b) & 15) return GS_E_ARG;  // 16-byte key loads
   if (tmp_bytes < gs::sort_tmp_bytes(capacity)) return GS_E_WORKSPACE;
   const hipError_t e = gs::sort_pairs_u32(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, end_bit, tmp,
                                           (hipStream_t)stream, result_in_b);
