@@ -275,6 +275,55 @@ __device__ __forceinline__ void block_copy16(float4* __restrict__ dst, const flo
   for (int k = (int)threadIdx.x + kMax * 256; k < n16; k += 256) dst[k] = src[k];
 }
 
+// Rows of `per4` 16-byte vectors, contiguous in global memory, into LDS rows
+// of `prow` floats.  A thread that then works on its own row reads
+// LDS at a stride of prow words across the wave: an odd prow puts the 64
+// lanes on 64 different banks (a stride of 48 puts them on 4).  The row /
+// column of vector k = threadIdx.x + 256 r advance by constant steps, so no
+// division per vector.  Loads come first, clamped to the last vector, as in
+// block_copy16.  256 threads.
+__device__ __forceinline__ void rows_to_lds(float* __restrict__ dst, const float4* __restrict__ src, int nrows,
+                                            int per4, int prow) {
+  constexpr int kMax = 12;
+  const int n16 = nrows * per4;
+  if (n16 <= 0) return;
+  float4 v[kMax];
+#pragma unroll
+  for (int r = 0; r < kMax; ++r) v[r] = src[min((int)threadIdx.x + r * 256, n16 - 1)];
+  const int q = 256 / per4, rem = 256 - q * per4;
+  const int lg = (n16 - 1) / per4, lj = (n16 - 1) - lg * per4;  // the last vector
+  int k = (int)threadIdx.x, g = k / per4, j = k - g * per4;
+#pragma unroll
+  for (int r = 0; r < kMax; ++r) {
+    const bool in = k < n16;
+    float* d = dst + (in ? g : lg) * prow + 4 * (in ? j : lj);
+    d[0] = v[r].x;
+    d[1] = v[r].y;
+    d[2] = v[r].z;
+    d[3] = v[r].w;
+    k += 256;
+    g += q;
+    j += rem;
+    if (j >= per4) {
+      j -= per4;
+      ++g;
+    }
+  }
+  for (; k < n16; k += 256) {
+    const float4 x = src[k];
+    float* d = dst + g * prow + 4 * j;
+    d[0] = x.x;
+    d[1] = x.y;
+    d[2] = x.z;
+    d[3] = x.w;
+    g += q;
+    j += rem;
+    if (j >= per4) {
+      j -= per4;
+      ++g;
+    }
+  }
+}
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const uint32_t lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));

@@ -62,29 +62,24 @@ __device__ __forceinline__ uint32_t block_incl_scan_256(uint32_t v, uint32_t* s_
 }
 
 __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
-  extern __shared__ float s_sh[];  // [256][3*nc] staged SH coefficients
+  extern __shared__ float s_sh[];  // [256][prow] staged SH coefficients (3 nc used per row)
   __shared__ uint32_t s_scan[256];
   __shared__ uint32_t s_flag;
   const int base = blockIdx.x * 256;
   const int i = base + threadIdx.x;
   const int nc = (a.D + 1) * (a.D + 1);
   const int stride = 3 * nc;
+  const int prow = stride | 1;  // odd LDS row stride: no bank conflicts on the per-thread row reads
   if (threadIdx.x == 0) s_flag = 0;
   if (a.shs) {
     const int nloc = min(256, a.P - base);
-    if (nc == a.M) {
-      // contiguous [nloc * 3M] floats; 3M*4 bytes per Gaussian is a multiple of 16 when M % 4 == 0
-      const float* src = a.shs + (size_t)base * 3 * a.M;
-      const int total = nloc * stride;
-      if ((a.M & 3) == 0) {
-        block_copy16(reinterpret_cast<float4*>(s_sh), reinterpret_cast<const float4*>(src), total >> 2);
-      } else {
-        for (int k = threadIdx.x; k < total; k += 256) s_sh[k] = src[k];
-      }
+    if (nc == a.M && (a.M & 3) == 0) {
+      // contiguous rows of 3M floats, a multiple of 16 bytes when M % 4 == 0
+      rows_to_lds(s_sh, reinterpret_cast<const float4*>(a.shs + (size_t)base * 3 * a.M), nloc, stride >> 2, prow);
     } else {
       for (int k = threadIdx.x; k < nloc * stride; k += 256) {
         const int g = k / stride, e = k - g * stride;
-        s_sh[k] = a.shs[(size_t)(base + g) * 3 * a.M + e];
+        s_sh[g * prow + e] = a.shs[(size_t)(base + g) * 3 * a.M + e];
       }
     }
   }
@@ -196,7 +191,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
         const float dx = dv.x / len, dy = dv.y / len, dz = dv.z / len;
         float b[16];
         sh_basis(a.D, dx, dy, dz, b);
-        const float* sh = s_sh + threadIdx.x * stride;
+        const float* sh = s_sh + threadIdx.x * prow;
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
           float acc = b[0] * sh[ch];
@@ -838,7 +833,7 @@ extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* 
   const int nb = nblocks256(P);
   if (nb > 0) {
     const int nc = (a.D + 1) * (a.D + 1);
-    const size_t shm = g->shs ? (size_t)256 * 3 * nc * sizeof(float) : 0;
+    const size_t shm = g->shs ? (size_t)256 * ((3 * nc) | 1) * sizeof(float) : 0;
     timing::Scope ts(timing::kPreprocess, s);
     hipLaunchKernelGGL(k_preprocess, dim3(nb), dim3(256), shm, s, a);
   }
