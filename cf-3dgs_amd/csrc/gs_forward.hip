@@ -327,14 +327,26 @@ __global__ __launch_bounds__(256) void k_ranges(const uint64_t* keys, const uint
 // ---- depth-first binning -------------------------------------------------
 // Keys for the depth sort: depth bits of visible Gaussians (positive floats
 // order as their bit patterns), 0xffffffff for the rest (sorted last, and they
-// touch no tile).  counters[2] = P is the sort's element count.
-__global__ __launch_bounds__(256) void k_df_keys(int P, const uint32_t* tiles, const float4* splat, uint32_t* keys,
-                                                 uint32_t* vals, uint32_t* counters) {
+// touch no tile).  counters[2] = P is the sort's element count.  The tile
+// rect of each visible Gaussian is packed into 8 bytes here, while the splat
+// records are read in order, so the duplicate (which visits Gaussians in
+// depth order, i.e. at random) reads one 8-byte record instead of the tile
+// count and two 16-byte splat vectors.
+__global__ __launch_bounds__(256) void k_df_keys(int P, const uint32_t* tiles, const float4* splat, int gx, int gy,
+                                                 uint32_t* keys, uint32_t* vals, uint2* rect, uint32_t* counters) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i == 0) counters[2] = (uint32_t)P;
   if (i >= P) return;
-  keys[i] = tiles[i] ? __float_as_uint(splat[3 * i + 1].z) : 0xffffffffu;
+  const bool vis = tiles[i] != 0u;
+  keys[i] = vis ? __float_as_uint(splat[3 * i + 1].z) : 0xffffffffu;
   vals[i] = (uint32_t)i;
+  if (vis) {
+    const float4 s0 = splat[3 * i];
+    const float4 s2 = splat[3 * i + 2];
+    int x0, y0, x1, y1;
+    tile_rect(s0.x, s0.y, __float_as_int(s2.w), gx, gy, x0, y0, x1, y1);
+    rect[i] = make_uint2((uint32_t)x0 | ((uint32_t)y0 << 16), (uint32_t)x1 | ((uint32_t)y1 << 16));
+  }
 }
 
 // Workgroup-local inclusive prefix of the tile counts in depth order.
@@ -378,7 +390,7 @@ __global__ __launch_bounds__(1024) void k_excl_scan_1wg(uint32_t* data, int n) {
 struct DfDupArgs {
   int P, gx, gy;
   long long capacity;
-  const float4* splat;
+  const uint2* rect;  // packed tile rect per Gaussian (k_df_keys)
   const uint32_t* tiles;
   const uint32_t* dvals;
   const uint32_t* dscan;
@@ -389,8 +401,17 @@ struct DfDupArgs {
 };
 
 // Duplicate tile ids in depth order (sorted position k -> Gaussian dvals[k]).
+// Each Gaussian's entries form one run at its prefix offset.  The wave
+// expands its 64 runs cooperatively: output j of the wave's concatenated runs
+// goes to lane j mod 64, which finds its run by binary search over the
+// wave's exclusive prefix, so consecutive lanes write consecutive entries of
+// a run (a run leaves as one contiguous segment, not as one word from each of
+// 64 places per store).  Entry order and values are those of a per-Gaussian
+// loop over the rect, row by row.
 __global__ __launch_bounds__(256) void k_df_duplicate(DfDupArgs a) {
+  __shared__ uint32_t s_excl[4][64], s_off[4][64], s_g[4][64], s_xy[4][64], s_w[4][64];
   const int k = blockIdx.x * 256 + threadIdx.x;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t n = a.counters[0];
   if ((long long)n > a.capacity) {
     if (k == 0) {
@@ -399,21 +420,46 @@ __global__ __launch_bounds__(256) void k_df_duplicate(DfDupArgs a) {
     }
     return;
   }
-  if (k >= a.P) return;
-  const uint32_t g = a.dvals[k];
-  const uint32_t nt = a.tiles[g];
-  if (nt == 0) return;
-  uint32_t off = a.dblocks[k >> 8] + a.dscan[k] - nt;
-  const float4 s0 = a.splat[3 * g];
-  const float4 s2 = a.splat[3 * g + 2];
-  int x0, y0, x1, y1;
-  tile_rect(s0.x, s0.y, __float_as_int(s2.w), a.gx, a.gy, x0, y0, x1, y1);
-  for (int y = y0; y < y1; ++y)
-    for (int x = x0; x < x1; ++x) {
-      a.keys[off] = (uint32_t)(y * a.gx + x);
-      a.vals[off] = g;
-      ++off;
+  uint32_t nt = 0, off = 0, g = 0, x0 = 0, y0 = 0, rw = 1;
+  if (k < a.P) {
+    g = a.dvals[k];
+    nt = a.tiles[g];
+    if (nt) {
+      off = a.dblocks[k >> 8] + a.dscan[k] - nt;
+      const uint2 r = a.rect[g];
+      x0 = r.x & 0xffffu;
+      y0 = r.x >> 16;
+      rw = (r.y & 0xffffu) - x0;
     }
+  }
+  // wave exclusive prefix of the run lengths
+  uint32_t incl = nt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  s_excl[w][lane] = incl - nt;
+  s_off[w][lane] = off;
+  s_g[w][lane] = g;
+  s_xy[w][lane] = x0 | (y0 << 16);
+  s_w[w][lane] = rw;
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t o = (uint32_t)lane; o < total; o += 64) {
+    // run j = last lane with excl <= o (empty runs share their successor's
+    // excl and lose to it)
+    int j = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+      if (s_excl[w][j + step] <= o) j += step;
+    const uint32_t r = o - s_excl[w][j];
+    const uint32_t wj = s_w[w][j], xy = s_xy[w][j];
+    const uint32_t ry = r / wj, rx = r - ry * wj;
+    const uint32_t dst = s_off[w][j] + r;
+    a.keys[dst] = ((xy >> 16) + ry) * (uint32_t)a.gx + (xy & 0xffffu) + rx;
+    a.vals[dst] = s_g[w][j];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_ranges32(const uint32_t* keys, const uint32_t* counters, long long capacity,
@@ -730,6 +776,8 @@ extern "C" int gs_binning_layout(long long capacity, int P, int width, int heigh
   o = align_up(o + 4 * p, 256);
   out->dblocks = o;
   o = align_up(o + 4 * (p / 256 + 2), 256);
+  out->drect = o;
+  o = align_up(o + 8 * p, 256);
   out->hist = o;
   {
     // histogram space for both sorts that share it (their tile sizes may differ)
@@ -867,6 +915,7 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
   uint32_t* ranges = (uint32_t*)(ib + LI.ranges);
   hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nblocks256(2LL * nt), 1024)), dim3(256), 0, s, ranges, 2LL * nt, 0u);
   if (capacity > 0 && P > 0 && f->binning == GS_BINNING_DEPTH_FIRST) {
+    if (gx > 0xffff || gy > 0xffff) return GS_E_ARG;  // tile rects are packed in 16-bit fields
     const float4* splat = (const float4*)(gb + LG.splat);
     const uint32_t* tiles = (const uint32_t*)(gb + LG.tiles);
     uint32_t* dka = (uint32_t*)(bb + LB.dkeys_a);
@@ -876,7 +925,8 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
     int in_b = 0;
     {
       timing::Scope ts(timing::kSort, s);
-      hipLaunchKernelGGL(k_df_keys, dim3(nbp), dim3(256), 0, s, P, tiles, splat, dka, dva, counters);
+      hipLaunchKernelGGL(k_df_keys, dim3(nbp), dim3(256), 0, s, P, tiles, splat, gx, gy, dka, dva,
+                         (uint2*)(bb + LB.drect), counters);
       e = sort_pairs_u32(dka, dva, (uint32_t*)(bb + LB.dkeys_b), (uint32_t*)(bb + LB.dvals_b), counters + 2, P, 32,
                          bb + LB.hist, s, &in_b);
     }
@@ -892,7 +942,7 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
       d.gx = gx;
       d.gy = gy;
       d.capacity = capacity;
-      d.splat = splat;
+      d.rect = (const uint2*)(bb + LB.drect);
       d.tiles = tiles;
       d.dvals = dvals;
       d.dscan = (const uint32_t*)(bb + LB.dscan);

@@ -125,6 +125,7 @@ typedef struct gs_binning_layout {
   int key_bytes;               /* 8 (SORT64) or 4 (DEPTH_FIRST) */
   size_t dkeys_a, dvals_a, dkeys_b, dvals_b; /* uint32[P] depth sort (DEPTH_FIRST) */
   size_t dscan, dblocks;       /* uint32[P] tiles prefix in depth order, per-256 block sums */
+  size_t drect;                /* uint32[2P] tile rect per Gaussian: x0 | y0 << 16, x1 | y1 << 16 (DEPTH_FIRST) */
   size_t ckpt;                 /* float[capacity/256 + 1][5][256] blend state at list batch boundaries */
   size_t bweight;              /* uint32[capacity/256 + 2 n_tiles + 2] per (tile, list batch): the blend's
                                   longest per-wave list, the backward's estimate of that chunk's cost */
