@@ -45,7 +45,8 @@ __global__ __launch_bounds__(256) void k_zero_f32(float* p, long long n) {
 struct RBArgs {
   int W, H, gx, gy, nt;
   const uint32_t* ranges;
-  const uint32_t* list;
+  const uint32_t* list;  // sorted entries; the Gaussian id is entry & id_mask
+  uint32_t id_mask;
   const float4* splat;
   const float* bg;
   const float* final_T;
@@ -174,7 +175,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
     const uint32_t q = cstart + threadIdx.x;
     uint32_t m = 0u;
     if (q < cend) {
-      const uint32_t g = a.list[r0 + q];
+      const uint32_t g = a.list[r0 + q] & a.id_mask;
       s_id[threadIdx.x] = g;
       const float4 sa = a.splat[3 * g];
       s_a[threadIdx.x] = sa;
@@ -736,6 +737,7 @@ static RBArgs make_rb_args(const gs_frame_t* f, int P, const void* geom, long lo
   r.nt = gx * gy;
   r.ranges = (const uint32_t*)(ib + LI.ranges);
   r.list = capacity > 0 ? (const uint32_t*)(bb + LB.vals_sorted) : nullptr;
+  r.id_mask = LB.id_bits ? (1u << LB.id_bits) - 1u : 0xffffffffu;
   r.splat = (const float4*)(gb + LG.splat);
   r.bg = f->bg;
   r.final_T = (const float*)(ib + LI.final_T);

@@ -396,8 +396,9 @@ struct DfDupArgs {
   const uint32_t* dscan;
   const uint32_t* dblocks;
   uint32_t* counters;
-  uint32_t* keys;  // tile ids
-  uint32_t* vals;
+  uint32_t* keys;  // tile ids, or tile << id_bits | id when id_bits > 0
+  uint32_t* vals;  // ids (unused when id_bits > 0)
+  int id_bits;
 };
 
 // Duplicate tile ids in depth order (sorted position k -> Gaussian dvals[k]).
@@ -457,22 +458,27 @@ __global__ __launch_bounds__(256) void k_df_duplicate(DfDupArgs a) {
     const uint32_t wj = s_w[w][j], xy = s_xy[w][j];
     const uint32_t ry = r / wj, rx = r - ry * wj;
     const uint32_t dst = s_off[w][j] + r;
-    a.keys[dst] = ((xy >> 16) + ry) * (uint32_t)a.gx + (xy & 0xffffu) + rx;
-    a.vals[dst] = s_g[w][j];
+    const uint32_t tile = ((xy >> 16) + ry) * (uint32_t)a.gx + (xy & 0xffffu) + rx;
+    if (a.id_bits) {
+      a.keys[dst] = (tile << a.id_bits) | s_g[w][j];
+    } else {
+      a.keys[dst] = tile;
+      a.vals[dst] = s_g[w][j];
+    }
   }
 }
 
 __global__ __launch_bounds__(256) void k_ranges32(const uint32_t* keys, const uint32_t* counters, long long capacity,
-                                                  uint32_t* ranges) {
+                                                  int shift, uint32_t* ranges) {
   const uint32_t n0 = counters[0];
   const uint32_t n = ((long long)n0 > capacity) ? 0u : n0;
   const uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n) return;
-  const uint32_t t = keys[k];
+  const uint32_t t = keys[k] >> shift;
   if (k == 0) {
     ranges[2 * t] = 0;
   } else {
-    const uint32_t tp = keys[k - 1];
+    const uint32_t tp = keys[k - 1] >> shift;
     if (t != tp) {
       ranges[2 * tp + 1] = k;
       ranges[2 * t] = k;
@@ -484,7 +490,8 @@ __global__ __launch_bounds__(256) void k_ranges32(const uint32_t* keys, const ui
 struct BlendArgs {
   int W, H, gx;
   const uint32_t* ranges;
-  const uint32_t* list;
+  const uint32_t* list;  // sorted entries; the Gaussian id is entry & id_mask
+  uint32_t id_mask;
   const float4* splat;
   const float* bg;
   float* out_color;
@@ -546,7 +553,7 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
     const uint32_t k = base + threadIdx.x;
     uint32_t m = 0u;
     if (k < r1) {
-      const uint32_t g = a.list[k];
+      const uint32_t g = a.list[k] & a.id_mask;
       const float4 sa = a.splat[3 * g];
       s_a[threadIdx.x] = sa;
       float4 b = a.splat[3 * g + 1];
@@ -801,6 +808,10 @@ extern "C" int gs_binning_layout(long long capacity, int P, int width, int heigh
   const bool in_b = (out->sort_passes & 1) != 0;
   out->keys_sorted = in_b ? out->keys_alt : out->keys_dup;
   out->vals_sorted = in_b ? out->vals_alt : out->vals_dup;
+  // one word per list entry when the Gaussian id and the tile id fit together
+  const int idb = bits_for((uint32_t)(P > 1 ? P : 2));
+  out->id_bits = (df && idb + bits_for((uint32_t)nt) <= 32) ? idb : 0;
+  if (out->id_bits) out->vals_sorted = out->keys_sorted;
   return GS_OK;
 }
 
@@ -950,18 +961,25 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
       d.counters = counters;
       d.keys = (uint32_t*)(bb + LB.keys_dup);
       d.vals = (uint32_t*)(bb + LB.vals_dup);
+      d.id_bits = LB.id_bits;
       hipLaunchKernelGGL(k_df_duplicate, dim3(nbp), dim3(256), 0, s, d);
     }
     {
       timing::Scope ts(timing::kSort, s);
-      const int tb = bits_for((uint32_t)nt);
-      e = sort_pairs_u32((uint32_t*)(bb + LB.keys_dup), (uint32_t*)(bb + LB.vals_dup), (uint32_t*)(bb + LB.keys_alt),
-                         (uint32_t*)(bb + LB.vals_alt), counters, capacity, tb > 0 ? tb : 1, bb + LB.hist, s, &in_b);
+      const int tb = bits_for((uint32_t)nt) > 0 ? bits_for((uint32_t)nt) : 1;
+      if (LB.id_bits)  // one word per entry: sort on its tile bits only
+        e = sort_keys_u32((uint32_t*)(bb + LB.keys_dup), (uint32_t*)(bb + LB.keys_alt), counters, capacity,
+                          LB.id_bits, LB.id_bits + tb, bb + LB.hist, s, &in_b);
+      else
+        e = sort_pairs_u32((uint32_t*)(bb + LB.keys_dup), (uint32_t*)(bb + LB.vals_dup),
+                           (uint32_t*)(bb + LB.keys_alt), (uint32_t*)(bb + LB.vals_alt), counters, capacity, tb,
+                           bb + LB.hist, s, &in_b);
     }
     if (e != hipSuccess) return GS_E_LAUNCH;
     timing::Scope ts(timing::kRanges, s);
     hipLaunchKernelGGL(k_ranges32, dim3(nblocks256(capacity)), dim3(256), 0, s,
-                       (const uint32_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, ranges);
+                       (const uint32_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, LB.id_bits,
+                       ranges);
   } else if (capacity > 0 && P > 0) {
     DupArgs d;
     d.P = P;
@@ -1018,6 +1036,7 @@ static BlendArgs make_blend_args(const gs_frame_t* f, int P, const void* geom, l
   b.gx = gx;
   b.ranges = (const uint32_t*)(ib + LI.ranges);
   b.list = capacity > 0 ? (const uint32_t*)(bb + LB.vals_sorted) : nullptr;
+  b.id_mask = LB.id_bits ? (1u << LB.id_bits) - 1u : 0xffffffffu;
   b.splat = (const float4*)(gb + LG.splat);
   b.bg = f->bg;
   b.out_color = out_color;

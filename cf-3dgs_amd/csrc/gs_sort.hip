@@ -6,6 +6,9 @@
 // needs the host to know num_rendered); grids are sized by capacity and
 // workgroups past the count exit at once.
 //
+// Keys-only sorts (32-bit keys that carry their payload in the low bits)
+// sort bits [start_bit, end_bit) and skip every value load and store.
+//
 // Per 8-bit digit pass:
 //   k_sort_upsweep  1024 or 4096 keys per workgroup (4 waves x 4 or 16 rounds
 //                   x 64 lanes), all loaded up front, counted with one LDS
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* tot
   if (t == 0) totals[d] = carry;
 }
 
-template <typename K, int ITEMS, int R>
+template <typename K, int ITEMS, int R, bool V>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restrict__ keys_in,
                                                                const uint32_t* __restrict__ vals_in,
                                                                K* __restrict__ keys_out,
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   __shared__ uint32_t s_scan[R];
   __shared__ uint32_t s_w[4];
   __shared__ K s_key[TILE];
-  __shared__ uint32_t s_val[TILE];
+  __shared__ uint32_t s_val[V ? TILE : 1];  // keys-only sorts carry no values
   const uint32_t n = n_eff(n_dev, cap);
   const uint32_t start = blockIdx.x * (uint32_t)TILE;
   if (start >= n) return;
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     const uint32_t idx = wbase + r * 64 + lane;
     const bool valid = idx < n;
     key[r] = valid ? keys_in[idx] : (K)0;
-    val[r] = valid ? vals_in[idx] : 0u;
+    if constexpr (V) val[r] = valid ? vals_in[idx] : 0u;
     const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
     const uint64_t peers = match_digit(d, valid, bits);
     const uint64_t below = peers & lanemask_lt();
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
       const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
       const uint32_t lp = s_cnt[w][d] + pos[r];
       s_key[lp] = key[r];
-      s_val[lp] = val[r];
+      if constexpr (V) s_val[lp] = val[r];
     }
   }
   __syncthreads();
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     const uint32_t d = (uint32_t)(k >> shift) & mask;
     const uint32_t dst = s_scan[d] + i;
     keys_out[dst] = k;
-    vals_out[dst] = s_val[i];
+    if constexpr (V) vals_out[dst] = s_val[i];
   }
 }
 
@@ -231,14 +234,20 @@ void sort_pass(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, const
   hipLaunchKernelGGL((k_sort_upsweep<K, ITEMS, R>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity,
                      shift, bits, hist, nb_max);
   hipLaunchKernelGGL(k_sort_scan, dim3(R), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max, tile);
-  hipLaunchKernelGGL((k_sort_scatter<K, ITEMS, R>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout, vout,
-                     n_dev, capacity, shift, bits, hist, totals, nb_max);
+  if (vin)
+    hipLaunchKernelGGL((k_sort_scatter<K, ITEMS, R, true>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout,
+                       vout, n_dev, capacity, shift, bits, hist, totals, nb_max);
+  else
+    hipLaunchKernelGGL((k_sort_scatter<K, ITEMS, R, false>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout,
+                       vout, n_dev, capacity, shift, bits, hist, totals, nb_max);
 }
 
+// Stable LSD sort on key bits [start_bit, end_bit); vals_a == nullptr sorts
+// the keys alone.
 template <typename K>
 hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, const uint32_t* n_dev,
-                      long long capacity, int end_bit, void* tmp, hipStream_t s, int* result_in_b) {
-  const int passes = sort_passes(end_bit);
+                      long long capacity, int start_bit, int end_bit, void* tmp, hipStream_t s, int* result_in_b) {
+  const int passes = sort_passes(end_bit - start_bit);
   *result_in_b = passes & 1;
   if (capacity <= 0) return hipSuccess;
   const int tile = sort_tile(capacity);
@@ -250,7 +259,7 @@ hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
   K* kout = keys_b;
   uint32_t* vout = vals_b;
   for (int p = 0; p < passes; ++p) {
-    const int shift = kRadixBits * p;
+    const int shift = start_bit + kRadixBits * p;
     const int bits = end_bit - shift < kRadixBits ? end_bit - shift : kRadixBits;
     auto pass = tile == kSortThreads * 4 ? sort_pass<K, 4, kRadix> : sort_pass<K, 16, kRadix>;
     pass(kin, vin, kout, vout, n_dev, capacity, shift, bits, hist, totals, nb_max, tile, s);
@@ -267,13 +276,19 @@ hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
 hipError_t sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b,
                           const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
                           int* result_in_b) {
-  return sort_pairs<uint64_t>(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, end_bit, tmp, s, result_in_b);
+  return sort_pairs<uint64_t>(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, 0, end_bit, tmp, s, result_in_b);
 }
 
 hipError_t sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, uint32_t* vals_b,
                           const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
                           int* result_in_b) {
-  return sort_pairs<uint32_t>(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, end_bit, tmp, s, result_in_b);
+  return sort_pairs<uint32_t>(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, 0, end_bit, tmp, s, result_in_b);
+}
+
+hipError_t sort_keys_u32(uint32_t* keys_a, uint32_t* keys_b, const uint32_t* n_dev, long long capacity, int start_bit,
+                         int end_bit, void* tmp, hipStream_t s, int* result_in_b) {
+  return sort_pairs<uint32_t>(keys_a, nullptr, keys_b, nullptr, n_dev, capacity, start_bit, end_bit, tmp, s,
+                              result_in_b);
 }
 
 }  // namespace gs

@@ -60,7 +60,7 @@ class GsGeomLayout(ctypes.Structure):
 class GsBinningLayout(ctypes.Structure):
     _fields_ = [("keys_dup", c_size_t), ("vals_dup", c_size_t), ("keys_alt", c_size_t), ("vals_alt", c_size_t),
                 ("keys_sorted", c_size_t), ("vals_sorted", c_size_t), ("hist", c_size_t), ("sort_passes", c_int),
-                ("key_bytes", c_int), ("dkeys_a", c_size_t), ("dvals_a", c_size_t), ("dkeys_b", c_size_t),
+                ("key_bytes", c_int), ("id_bits", c_int), ("dkeys_a", c_size_t), ("dvals_a", c_size_t), ("dkeys_b", c_size_t),
                 ("dvals_b", c_size_t), ("dscan", c_size_t), ("dblocks", c_size_t), ("drect", c_size_t),
                 ("ckpt", c_size_t),
                 ("bweight", c_size_t), ("items", c_size_t), ("item_log", c_size_t), ("total", c_size_t)]

@@ -358,12 +358,18 @@ def views(st: ForwardState, P: int, W: int, H: int) -> dict:
         return buf[off:off + nbytes].view(dtype)
 
     splat = sl(st.geom, LG.splat, 48 * P, torch.float32).view(P, 3, 4)
-    vals = sl(st.binning, LB.vals_sorted, 4 * n, torch.int32)
+    if LB.id_bits:  # one word per entry: tile << id_bits | id
+        words = sl(st.binning, LB.keys_sorted, 4 * n, torch.int32).to(torch.int64) & 0xFFFFFFFF
+        vals = (words & ((1 << LB.id_bits) - 1)).to(torch.int32)
+        tiles_words = words >> LB.id_bits
+    else:
+        vals = sl(st.binning, LB.vals_sorted, 4 * n, torch.int32)
     if LB.key_bytes == 8:
         keys = sl(st.binning, LB.keys_sorted, 8 * n, torch.int64)
     else:
         # (tile << 32 | depth bits) rebuilt from the sorted tile ids and the ids' depths
-        tiles_sorted = sl(st.binning, LB.keys_sorted, 4 * n, torch.int32).to(torch.int64)
+        tiles_sorted = (tiles_words if LB.id_bits else
+                        sl(st.binning, LB.keys_sorted, 4 * n, torch.int32).to(torch.int64))
         dbits = splat[:, 1, 2].contiguous().view(torch.int32).to(torch.int64) & 0xFFFFFFFF
         keys = (tiles_sorted << 32) | dbits[vals.to(torch.int64)] if n else torch.zeros(0, dtype=torch.int64)
     return dict(

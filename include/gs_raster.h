@@ -123,6 +123,9 @@ typedef struct gs_binning_layout {
   size_t hist;                 /* sort digit histograms */
   int sort_passes;             /* 8-bit LSD passes of the list sort */
   int key_bytes;               /* 8 (SORT64) or 4 (DEPTH_FIRST) */
+  int id_bits;                 /* DEPTH_FIRST with bits(P) + bits(n_tiles) <= 32: the list entries are
+                                  single words tile << id_bits | Gaussian id, sorted in place of the keys
+                                  (vals_sorted == keys_sorted); 0: ids in their own array */
   size_t dkeys_a, dvals_a, dkeys_b, dvals_b; /* uint32[P] depth sort (DEPTH_FIRST) */
   size_t dscan, dblocks;       /* uint32[P] tiles prefix in depth order, per-256 block sums */
   size_t drect;                /* uint32[2P] tile rect per Gaussian: x0 | y0 << 16, x1 | y1 << 16 (DEPTH_FIRST) */

@@ -39,6 +39,12 @@ def test_layouts():
     D = _lib.binning_layout(1000, 500, 960, 540, _lib.BINNING_DEPTH_FIRST)
     assert D.sort_passes == 2 and D.key_bytes == 4  # tile ids only: 11 bits
     assert _lib.binning_layout(1000, 500, 1920, 1080, _lib.BINNING_DEPTH_FIRST).sort_passes == 2  # 13 bits
+    # one word per list entry while bits(P) + bits(tiles) <= 32, ids in their own array otherwise
+    Dp = _lib.binning_layout(1000, 500_000, 960, 540, _lib.BINNING_DEPTH_FIRST)
+    assert Dp.id_bits == 19 and Dp.vals_sorted == Dp.keys_sorted and Dp.sort_passes == 2  # 19 + 11 bits
+    assert _lib.binning_layout(1000, 524_288, 1920, 1080, _lib.BINNING_DEPTH_FIRST).id_bits == 19  # 19 + 13
+    assert _lib.binning_layout(1000, 524_289, 1920, 1080, _lib.BINNING_DEPTH_FIRST).id_bits == 0   # 20 + 13
+    assert _lib.binning_layout(1000, 500_000, 960, 540, _lib.BINNING_SORT64).id_bits == 0
     assert D.dscan > D.dvals_b > D.dkeys_b > D.dvals_a > D.dkeys_a >= D.vals_alt
     G = _lib.geom_layout(1000)
     assert G.total >= 1000 * (48 + 24 + 8) and G.splat % 256 == 0 and G.cov3D % 256 == 0
