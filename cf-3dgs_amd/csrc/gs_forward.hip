@@ -568,15 +568,19 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
     const int nw = compact_wave_list(s_mask, s_list[wv], wv, lane);
     if (lane == 0) atomicMax(&s_bw, (uint32_t)nw);
     const uint32_t first = base - r0 + 1;  // contributor index of staged position 0
-    // alpha of staged splat j at this pixel, 0 when it is skipped
+    // alpha of staged splat j at this pixel, 0 when it is skipped.  Branch
+    // free (the exponential is evaluated and then discarded for skipped
+    // pairs), so the loads and exponentials of the entries of one step
+    // overlap instead of each waiting behind its own branch.
     auto alpha_at = [&](int j) -> float {
       const float4 ga = s_a[j];
       const float4 gb = s_b[j];
       const float dx = ga.x - pfx, dy = ga.y - pfy;
       const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
-      if (power > 0.0f || power < gb.w) return 0.0f;  // gb.w: alpha < 1/255 for sure below it
       const float alpha = fminf(0.99f, gb.y * spec_expf(power));
-      return alpha < a_min ? 0.0f : alpha;
+      // gb.w: alpha < 1/255 for sure below it
+      const bool skip = power > 0.0f || power < gb.w || alpha < a_min;
+      return skip ? 0.0f : alpha;
     };
     auto apply = [&](int j, float alpha) {
       if (alpha == 0.0f) return;
