@@ -909,8 +909,17 @@ extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* 
 
 // Binning of one frame: everything of the forward but the blend (ranges
 // cleared, depth-first or 64-bit-key binning, ranges, checkpoint bases).
+// Phases of the binning: kPrepare is everything that does not read
+// num_rendered (clearing the tile ranges; in the depth-first schedule also
+// the depth keys, the 32-bit depth sort and the prefix of the tile counts in
+// depth order), so a caller can queue it before it waits for the readback;
+// kResume is the rest, after kPrepare with the same arguments.
+enum BinPhase { kBinAll = 0, kPrepare = 1, kResume = 2 };
+
 static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
-                          void* binning, size_t binning_bytes, void* image, size_t image_bytes, hipStream_t s) {
+                          void* binning, size_t binning_bytes, void* image, size_t image_bytes, hipStream_t s,
+                          int phase = kBinAll) {
+  const bool pre = phase != kResume, post = phase != kPrepare;
   if (check_frame(f) != GS_OK || P < 0 || !counters || !image) return GS_E_ARG;
   if (P > 0 && !geom) return GS_E_ARG;
   if (capacity < 0 || capacity > 0x7fffffffLL) return GS_E_ARG;
@@ -928,7 +937,9 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
   char* bb = (char*)binning;
   char* ib = (char*)image;
   uint32_t* ranges = (uint32_t*)(ib + LI.ranges);
-  hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nblocks256(2LL * nt), 1024)), dim3(256), 0, s, ranges, 2LL * nt, 0u);
+  if (pre)
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nblocks256(2LL * nt), 1024)), dim3(256), 0, s, ranges, 2LL * nt,
+                       0u);
   if (capacity > 0 && P > 0 && f->binning == GS_BINNING_DEPTH_FIRST) {
     if (gx > 0xffff || gy > 0xffff) return GS_E_ARG;  // tile rects are packed in 16-bit fields
     const float4* splat = (const float4*)(gb + LG.splat);
@@ -936,22 +947,28 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
     uint32_t* dka = (uint32_t*)(bb + LB.dkeys_a);
     uint32_t* dva = (uint32_t*)(bb + LB.dvals_a);
     const int nbp = nblocks256(P);
-    hipError_t e;
-    int in_b = 0;
-    {
-      timing::Scope ts(timing::kSort, s);
-      hipLaunchKernelGGL(k_df_keys, dim3(nbp), dim3(256), 0, s, P, tiles, splat, gx, gy, dka, dva,
-                         (uint2*)(bb + LB.drect), counters);
-      e = sort_pairs_u32(dka, dva, (uint32_t*)(bb + LB.dkeys_b), (uint32_t*)(bb + LB.dvals_b), counters + 2, P, 32,
-                         bb + LB.hist, s, &in_b);
+    hipError_t e = hipSuccess;
+    int in_b = sort_passes(32) & 1;  // where the depth sort leaves its result
+    if (pre) {
+      {
+        timing::Scope ts(timing::kSort, s);
+        hipLaunchKernelGGL(k_df_keys, dim3(nbp), dim3(256), 0, s, P, tiles, splat, gx, gy, dka, dva,
+                           (uint2*)(bb + LB.drect), counters);
+        e = sort_pairs_u32(dka, dva, (uint32_t*)(bb + LB.dkeys_b), (uint32_t*)(bb + LB.dvals_b), counters + 2, P,
+                           32, bb + LB.hist, s, &in_b);
+      }
+      if (e != hipSuccess) return GS_E_LAUNCH;
     }
-    if (e != hipSuccess) return GS_E_LAUNCH;
     const uint32_t* dvals = (const uint32_t*)(bb + (in_b ? LB.dvals_b : LB.dvals_a));
-    {
+    if (pre) {
       timing::Scope ts(timing::kDuplicate, s);
       hipLaunchKernelGGL(k_df_scan, dim3(nbp), dim3(256), 0, s, P, dvals, tiles, (uint32_t*)(bb + LB.dscan),
                          (uint32_t*)(bb + LB.dblocks));
       hipLaunchKernelGGL(k_excl_scan_1wg, dim3(1), dim3(1024), 0, s, (uint32_t*)(bb + LB.dblocks), nbp);
+    }
+    if (!post) return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+    {
+      timing::Scope ts(timing::kDuplicate, s);
       DfDupArgs d;
       d.P = P;
       d.gx = gx;
@@ -985,6 +1002,7 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
                        (const uint32_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, LB.id_bits,
                        ranges);
   } else if (capacity > 0 && P > 0) {
+    if (!post) return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
     DupArgs d;
     d.P = P;
     d.gx = gx;
@@ -1014,8 +1032,9 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
     hipLaunchKernelGGL(k_ranges, dim3(nblocks256(capacity)), dim3(256), 0, s,
                        (const uint64_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, ranges);
   }
-  hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)ranges, nt,
-                     (uint32_t*)(ib + LI.ckpt_base), (uint32_t*)(bb + LB.item_log));
+  if (post)
+    hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)ranges, nt,
+                       (uint32_t*)(ib + LI.ckpt_base), (uint32_t*)(bb + LB.item_log));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
@@ -1064,12 +1083,11 @@ extern "C" int gs_render_binning(const gs_frame_t* f, int P, const void* geom, u
                         (hipStream_t)stream);
 }
 
-extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
-                         void* binning, size_t binning_bytes, void* image, size_t image_bytes, float* out_color,
-                         float* out_depth, float* out_alpha, void* stream) {
+static int render_blend_from(int phase, const gs_frame_t* f, int P, const void* geom, uint32_t* counters,
+                             long long capacity, void* binning, size_t binning_bytes, void* image, size_t image_bytes,
+                             float* out_color, float* out_depth, float* out_alpha, hipStream_t s) {
   if (!out_color || !out_depth) return GS_E_ARG;
-  hipStream_t s = (hipStream_t)stream;
-  const int rc = render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes, s);
+  const int rc = render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes, s, phase);
   if (rc != GS_OK) return rc;
   const BlendArgs b = make_blend_args(f, P, geom, capacity, binning, image, out_color, out_depth, out_alpha);
   int gx, gy;
@@ -1079,6 +1097,26 @@ extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t*
     hipLaunchKernelGGL(k_blend, dim3(nt), dim3(256), 0, s, b);
   }
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_render(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
+                         void* binning, size_t binning_bytes, void* image, size_t image_bytes, float* out_color,
+                         float* out_depth, float* out_alpha, void* stream) {
+  return render_blend_from(kBinAll, f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                           out_color, out_depth, out_alpha, (hipStream_t)stream);
+}
+
+extern "C" int gs_render_prepare(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
+                                 void* binning, size_t binning_bytes, void* image, size_t image_bytes, void* stream) {
+  return render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                        (hipStream_t)stream, kPrepare);
+}
+
+extern "C" int gs_render_resume(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
+                                void* binning, size_t binning_bytes, void* image, size_t image_bytes,
+                                float* out_color, float* out_depth, float* out_alpha, void* stream) {
+  return render_blend_from(kResume, f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                           out_color, out_depth, out_alpha, (hipStream_t)stream);
 }
 
 // ---- many frames in one blend launch --------------------------------------

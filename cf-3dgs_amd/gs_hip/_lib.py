@@ -80,6 +80,10 @@ EXPORTS = {
                               c_void_p, c_void_p]),
     "gs_render": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong, c_void_p, c_size_t,
                           c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gs_render_prepare": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong, c_void_p,
+                                  c_size_t, c_void_p, c_size_t, c_void_p]),
+    "gs_render_resume": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong, c_void_p,
+                                 c_size_t, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gs_backward": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p, c_void_p,
                             c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(GsGrads),
                             c_void_p, c_size_t, c_void_p]),

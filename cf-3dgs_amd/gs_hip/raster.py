@@ -161,6 +161,11 @@ def make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, c
 
 _CAPACITY_QUANTUM = 1 << 16
 _PINNED = {}
+_EVENTS = {}
+# capacity of the last forward per (P, W, H, schedule): the guess under which
+# the part of the binning that does not need num_rendered is queued before the
+# readback
+_LAST_CAPACITY = {}
 
 
 def _pinned_counters(device: torch.device) -> torch.Tensor:
@@ -222,21 +227,43 @@ def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations
     alpha = torch.empty((1, H, W), dtype=torch.float32, device=device) if want_alpha else None
     _lib.check(lib.gs_preprocess(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(radii), _ptr(geom),
                                  geom.numel(), _ptr(counters), stream), "gs_preprocess")
-    # the one host synchronisation of the forward (the reference's num_rendered readback)
+    # the one host synchronisation of the forward (the reference's num_rendered
+    # readback): only the copy is waited for.  Meanwhile the binning work that
+    # does not need the count runs under the last capacity used for this size.
     host = _pinned_counters(device)
     host.copy_(counters, non_blocking=True)
-    torch.cuda.current_stream(device).synchronize()
+    ev = _EVENTS.get(device)
+    if ev is None:
+        ev = _EVENTS[device] = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream(device))
+    key = (P, W, H, fa.frame.binning)
+    guess = _LAST_CAPACITY.get(key, 0)
+    binning = None
+    if guess:
+        LB = _binning_layout(guess, P, W, H, fa.frame.binning)
+        binning = torch.empty(max(LB.total, 256), dtype=torch.uint8, device=device)
+        _lib.check(lib.gs_render_prepare(ctypes.byref(fa.frame), P, _ptr(geom), _ptr(counters), guess,
+                                         _ptr(binning), binning.numel(), _ptr(image), image.numel(), stream),
+                   "gs_render_prepare")
+    ev.synchronize()
     num_rendered = int(host[0]) & 0xFFFFFFFF
     status = int(host[1])
     if status & _lib.GS_STATUS_PREFILTERED:
         raise RuntimeError("a point was culled by the near plane although prefiltered=True")
     # capacity rounded up so the caching allocator can reuse the binning block
     capacity = min(-(-num_rendered // _CAPACITY_QUANTUM) * _CAPACITY_QUANTUM, 0x7FFFFFFF)
-    LB = _binning_layout(capacity, P, W, H, fa.frame.binning)
-    binning = torch.empty(max(LB.total, 256), dtype=torch.uint8, device=device)
-    _lib.check(lib.gs_render(ctypes.byref(fa.frame), P, _ptr(geom), _ptr(counters), capacity, _ptr(binning),
-                             binning.numel(), _ptr(image), image.numel(), _ptr(color), _ptr(depth), _ptr(alpha),
-                             stream), "gs_render")
+    if guess and num_rendered <= guess and capacity * 2 > guess:
+        capacity = guess
+        _lib.check(lib.gs_render_resume(ctypes.byref(fa.frame), P, _ptr(geom), _ptr(counters), capacity,
+                                        _ptr(binning), binning.numel(), _ptr(image), image.numel(), _ptr(color),
+                                        _ptr(depth), _ptr(alpha), stream), "gs_render_resume")
+    else:  # first frame of this size, or the list grew past (or shrank well below) the guess
+        LB = _binning_layout(capacity, P, W, H, fa.frame.binning)
+        binning = torch.empty(max(LB.total, 256), dtype=torch.uint8, device=device)
+        _lib.check(lib.gs_render(ctypes.byref(fa.frame), P, _ptr(geom), _ptr(counters), capacity, _ptr(binning),
+                                 binning.numel(), _ptr(image), image.numel(), _ptr(color), _ptr(depth), _ptr(alpha),
+                                 stream), "gs_render")
+    _LAST_CAPACITY[key] = capacity
     st = ForwardState(geom, binning, image, counters, radii, capacity, num_rendered, fa.frame.binning)
     return color, radii, depth, alpha, st
 

@@ -172,6 +172,23 @@ int gs_render(const gs_frame_t* frame, int P, const void* geom, uint32_t* counte
               void* binning, size_t binning_bytes, void* image, size_t image_bytes, float* out_color,
               float* out_depth, float* out_alpha, void* stream);
 
+/* gs_render split at the num_rendered readback (no reference counterpart; the
+ * reference waits for the readback before it binns anything).
+ * gs_render_prepare queues everything that does not read num_rendered: the
+ * tile ranges are cleared and, in the depth-first schedule, the Gaussians are
+ * put in depth order and their tile counts prefix-summed in that order.  A
+ * caller that guesses `capacity` (e.g. from the previous frame) queues it
+ * before it waits for counters[0], so the GPU has work while the host reads
+ * the count.  gs_render_resume, with the same arguments, then does the rest
+ * of gs_render; if counters[0] turned out larger than the guess, the caller
+ * calls gs_render with a larger capacity and a workspace of that size instead.
+ * prepare + resume == gs_render. */
+int gs_render_prepare(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters, long long capacity,
+                      void* binning, size_t binning_bytes, void* image, size_t image_bytes, void* stream);
+int gs_render_resume(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters, long long capacity,
+                     void* binning, size_t binning_bytes, void* image, size_t image_bytes, float* out_color,
+                     float* out_depth, float* out_alpha, void* stream);
+
 /* Backward (replaces diff_gaussian_rasterization._C.rasterize_gaussians_backward,
  * triggered by loss.backward() at trainer/cf3dgs_trainer.py:133).  dL_ddepth and
  * dL_dalpha may be NULL (treated as zero).  scratch: gs_backward_scratch_bytes(P,

@@ -153,3 +153,29 @@ def test_prefiltered_flag_raises():
     with pytest.raises(RuntimeError, match="prefiltered"):
         raster.forward(rs, sc["means"].to(DEV), sc["shs"].to(DEV), None, sc["opacities"].to(DEV),
                        sc["scales"].to(DEV), sc["rotations"].to(DEV), None)
+
+
+@pytest.mark.parametrize("binning", [0, 1])
+def test_prepare_resume_equals_one_call(binning):
+    """The forward queues the binning work that does not need num_rendered
+    under the previous frame's capacity (gs_render_prepare) and resumes after
+    the readback (gs_render_resume); a list that outgrew that capacity takes
+    the one-call path with a larger workspace.  Both equal a first call, which
+    has no previous capacity, bit for bit."""
+    from gs_hip import raster
+    sc = scenes.plain_scene(seed=11, P=3000, W=200, H=136)
+    key = (3000, 200, 136, binning)
+    raster._LAST_CAPACITY.pop(key, None)
+    first = run_gpu(sc, 3, binning=binning)  # one call: no previous capacity for this size
+    assert raster._LAST_CAPACITY[key] == first["st"].capacity
+    again = run_gpu(sc, 3, binning=binning)  # prepare + resume under the same capacity
+    for k in ("color", "depth", "alpha", "radii"):
+        assert torch.equal(first[k], again[k]), k
+    for k in ("keys", "vals", "ranges", "n_contrib", "final_T"):
+        assert torch.equal(first["views"][k], again["views"][k]), k
+    # the previous capacity is too small for a larger scene of the same size: falls back
+    raster._LAST_CAPACITY[key] = 256
+    grown = run_gpu(sc, 3, binning=binning)
+    for k in ("color", "depth", "radii"):
+        assert torch.equal(first[k], grown[k]), k
+    assert grown["st"].capacity >= grown["st"].num_rendered > 256
