@@ -90,13 +90,34 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
                            : make_uint4(idx < n ? (uint32_t)keys[idx] : 0u, idx + 1u < n ? (uint32_t)keys[idx + 1] : 0u,
                                         idx + 2u < n ? (uint32_t)keys[idx + 2] : 0u, 0u);
     }
+    if (bits <= 4) {
+      // few bins (a top digit): the wave's lanes would pile onto a handful of
+      // LDS words, so count each bin with a ballot instead; lane b keeps bin b
+      uint32_t cnt = 0;
 #pragma unroll
-    for (int r = 0; r < V; ++r) {
-      const uint32_t idx = wbase + 4u * (uint32_t)(r * 64 + lane);
-      const uint32_t e[4] = {kv[r].x, kv[r].y, kv[r].z, kv[r].w};
+      for (int r = 0; r < V; ++r) {
+        const uint32_t idx = wbase + 4u * (uint32_t)(r * 64 + lane);
+        const uint32_t e[4] = {kv[r].x, kv[r].y, kv[r].z, kv[r].w};
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (idx + (uint32_t)c < n) atomicAdd(&s_h[w][(e[c] >> shift) & mask], 1u);
+        for (int c = 0; c < 4; ++c) {
+          const bool valid = idx + (uint32_t)c < n;
+          const uint32_t d = (e[c] >> shift) & mask;
+          for (uint32_t b = 0; b <= mask; ++b) {
+            const uint32_t m = (uint32_t)__popcll(__ballot(valid && d == b));
+            if ((uint32_t)lane == b) cnt += m;
+          }
+        }
+      }
+      if ((uint32_t)lane <= mask) s_h[w][lane] = cnt;
+    } else {
+#pragma unroll
+      for (int r = 0; r < V; ++r) {
+        const uint32_t idx = wbase + 4u * (uint32_t)(r * 64 + lane);
+        const uint32_t e[4] = {kv[r].x, kv[r].y, kv[r].z, kv[r].w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (idx + (uint32_t)c < n) atomicAdd(&s_h[w][(e[c] >> shift) & mask], 1u);
+      }
     }
   } else {
     K kk[ITEMS];
