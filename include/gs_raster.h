@@ -173,7 +173,7 @@ int gs_render(const gs_frame_t* frame, int P, const void* geom, uint32_t* counte
               float* out_depth, float* out_alpha, void* stream);
 
 /* gs_render split at the num_rendered readback (no reference counterpart; the
- * reference waits for the readback before it binns anything).
+ * reference waits for the readback before it bins anything).
  * gs_render_prepare queues everything that does not read num_rendered: the
  * tile ranges are cleared and, in the depth-first schedule, the Gaussians are
  * put in depth order and their tile counts prefix-summed in that order.  A
