@@ -148,13 +148,17 @@ __device__ __forceinline__ void wave_sum12(const float v[12], float w[3]) {
 template <bool DET>
 __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) {
   constexpr int kParts = DET ? 4 : 1;  // LDS partial sets: one per wave when deterministic
-  __shared__ uint32_t s_id[256];
-  __shared__ float4 s_a[256];
-  __shared__ float4 s_b[256];
-  __shared__ float4 s_c[256];
-  __shared__ uint32_t s_mask[256];    // wave rows each staged splat can reach (splat_box)
-  __shared__ uint8_t s_list[4][256];  // per wave: the staged splats that reach its pixels
-  __shared__ float s_acc[10 * kParts][256];
+  // The item's 256-entry chunk is staged 128 entries at a time (back half
+  // first): half the LDS per workgroup, so twice as many workgroups fit on a
+  // CU (deterministic mode: 5 instead of 2).  The pixel state carries over.
+  constexpr int kSub = 128;
+  __shared__ uint32_t s_id[kSub];
+  __shared__ float4 s_a[kSub];
+  __shared__ float4 s_b[kSub];
+  __shared__ float4 s_c[kSub];
+  __shared__ uint32_t s_mask[kSub];    // quadrants each staged splat can reach (splat_quads)
+  __shared__ uint8_t s_list[4][kSub];  // per wave: the staged splats that reach its pixels
+  __shared__ float s_acc[10 * kParts][kSub];
   if (item >= a.items[0]) return;
   const int tile = (int)a.items[1 + 2 * item];
   const uint32_t chunk = a.items[2 + 2 * item];
@@ -171,24 +175,6 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
   const int HW = a.W * a.H;
   const float pfx = (float)px, pfy = (float)py;
   const uint32_t r0 = a.ranges[2 * tile];
-  {
-    const uint32_t q = cstart + threadIdx.x;
-    uint32_t m = 0u;
-    if (q < cend) {
-      const uint32_t g = a.list[r0 + q] & a.id_mask;
-      s_id[threadIdx.x] = g;
-      const float4 sa = a.splat[3 * g];
-      s_a[threadIdx.x] = sa;
-      float4 b = a.splat[3 * g + 1];
-      b.w = skip_threshold(b.y);
-      s_b[threadIdx.x] = b;
-      s_c[threadIdx.x] = a.splat[3 * g + 2];
-      m = splat_quads(sa.x, sa.y, sa.z, sa.w, b.x, b.w, (float)(tx * kBX), (float)(ty * kBY));
-    }
-    s_mask[threadIdx.x] = m;
-#pragma unroll
-    for (int k = 0; k < 10 * kParts; ++k) s_acc[k][threadIdx.x] = 0.0f;
-  }
   const float T_fin = inside ? a.final_T[pix] : 0.0f;
   const uint32_t last = inside ? a.n_contrib[pix] : 0u;
   float dpix0 = 0.f, dpix1 = 0.f, dpix2 = 0.f, dD = 0.f, dA = 0.f;
@@ -218,116 +204,143 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
   const float ddelx = 0.5f * (float)a.W, ddely = 0.5f * (float)a.H;
   const float a_min = 1.0f / 255.0f;
   const int lane = threadIdx.x & 63;
-  __syncthreads();
-  // the wave's own list: splats that miss its 8 x 8 pixels are ones no lane would use
-  // and entries at or past the last contributor of every pixel of the wave
-  // are inactive on every lane
+  const int wv = threadIdx.x >> 6;
+  // entries at or past the last contributor of every pixel of the wave are
+  // inactive on every lane
   const uint32_t wlast = wave_max_u32(last);
-  const int lim = wlast > cstart ? (int)min(wlast - cstart, 256u) : 0;
-  const int nw = compact_wave_list(s_mask, s_list[threadIdx.x >> 6], threadIdx.x >> 6, lane, lim);
-  // Branch-free per pair: masked lanes cost a SIMD as much as active ones,
-  // so every lane evaluates the pair; the only branches are wave-uniform
-  // skips when no lane is active.  An inactive lane runs with alpha = G = 0,
-  // which leaves its state unchanged (T / (1 - 0) = T, 0 c + 1 acc = acc) and
-  // makes each of its gradient terms 0, so no per-lane selects are needed.
-  // The running sums acc = sum_{k > current} c_k alpha_k T_k / T_{current+1}
-  // are advanced right after each splat (the reference advances them
-  // lazily at the next one, from last_alpha / last_color; same expression,
-  // same rounding).
-  for (int i = nw - 1; i >= 0; --i) {
-    const int j = s_list[threadIdx.x >> 6][i];
-    const uint32_t q = cstart + j;  // 0-based list position; contributor index q + 1
-    const float4 ga = s_a[j];
-    const float4 gb = s_b[j];
-    const float dx = ga.x - pfx, dy = ga.y - pfy;
-    const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
-    // gb.w: alpha < 1/255 for sure below it
-    bool active = inside && q < last && !(power > 0.0f || power < gb.w);
-    if (__ballot(active) == 0ull) continue;
-    const float Graw = spec_expf(power);
-    const float alpha_raw = fminf(0.99f, gb.y * Graw);
-    active = active && !(alpha_raw < a_min);
-    if (__ballot(active) == 0ull) continue;
-    const float G = active ? Graw : 0.0f;
-    const float alpha = active ? alpha_raw : 0.0f;
-    const float4 gc = s_c[j];
-    // T before this splat; the gradient tolerates the 1-ulp reciprocal
-    const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
-    const float Tn = T * inv_1ma;
-    const float w = alpha * Tn;
-    float dL_da = (gc.x - acc0) * dpix0;
-    dL_da += (gc.y - acc1) * dpix1;
-    dL_da += (gc.z - acc2) * dpix2;
-    dL_da += (gb.z - acc_d) * dD;
-    dL_da += (1.0f - acc_a) * dA;
-    dL_da *= Tn;
-    dL_da += (-T_fin * inv_1ma) * bg_dot;
-    const float dL_dG = gb.y * dL_da;
-    const float gdx = G * dx, gdy = G * dy;
-    const float dG_dx = -gdx * ga.z - gdy * ga.w;
-    const float dG_dy = -gdy * gb.x - gdx * ga.w;
-    float v[12];
-    v[0] = dL_dG * dG_dx * ddelx;
-    v[1] = dL_dG * dG_dy * ddely;
-    v[2] = -0.5f * gdx * dx * dL_dG;
-    v[3] = -gdx * dy * dL_dG;
-    v[4] = -0.5f * gdy * dy * dL_dG;
-    v[5] = G * dL_da;
-    v[6] = w * dpix0;
-    v[7] = w * dpix1;
-    v[8] = w * dpix2;
-    v[9] = w * dD;
-    v[10] = 0.f;
-    v[11] = 0.f;
-    T = Tn;
-    const float oma = 1.0f - alpha;
-    acc0 = alpha * gc.x + oma * acc0;
-    acc1 = alpha * gc.y + oma * acc1;
-    acc2 = alpha * gc.z + oma * acc2;
-    acc_d = alpha * gb.z + oma * acc_d;
-    acc_a = alpha + oma * acc_a;
-    {
-      float w[3];
-      wave_sum12(v, w);
-      if ((lane & 15) == 0) {
-        const int q = lane >> 4;
-        const int o = q == 1 ? 2 : (q == 2 ? 1 : q);  // value of row q: 4m + {0, 2, 1, 3}[q]
+#pragma unroll 1
+  for (int sb = 1; sb >= 0; --sb) {
+    const uint32_t sbase = cstart + (uint32_t)(sb * kSub);
+    if (sbase >= cend) continue;  // (uniform) a chunk of at most 128 entries
+    const uint32_t send = min(sbase + (uint32_t)kSub, cend);
+    __syncthreads();  // the previous half's staged data has been flushed
+    if (threadIdx.x < kSub) {
+      const uint32_t q = sbase + threadIdx.x;
+      uint32_t m = 0u;
+      if (q < send) {
+        const uint32_t g = a.list[r0 + q] & a.id_mask;
+        s_id[threadIdx.x] = g;
+        const float4 sa = a.splat[3 * g];
+        s_a[threadIdx.x] = sa;
+        float4 b = a.splat[3 * g + 1];
+        b.w = skip_threshold(b.y);
+        s_b[threadIdx.x] = b;
+        s_c[threadIdx.x] = a.splat[3 * g + 2];
+        m = splat_quads(sa.x, sa.y, sa.z, sa.w, b.x, b.w, (float)(tx * kBX), (float)(ty * kBY));
+      }
+      s_mask[threadIdx.x] = m;
+    }
+    for (int k = threadIdx.x; k < 10 * kParts * kSub; k += 256) (&s_acc[0][0])[k] = 0.0f;
+    __syncthreads();
+    // the wave's own list: splats that miss its 8 x 8 pixels are ones no lane would use
+    const int lim = wlast > sbase ? (int)min(wlast - sbase, (uint32_t)kSub) : 0;
+    const int nw = compact_wave_list<kSub / 64>(s_mask, s_list[wv], wv, lane, lim);
+    // Branch-free per pair: masked lanes cost a SIMD as much as active ones,
+    // so every lane evaluates the pair; the only branches are wave-uniform
+    // skips when no lane is active.  An inactive lane runs with alpha = G = 0,
+    // which leaves its state unchanged (T / (1 - 0) = T, 0 c + 1 acc = acc) and
+    // makes each of its gradient terms 0, so no per-lane selects are needed.
+    // The running sums acc = sum_{k > current} c_k alpha_k T_k / T_{current+1}
+    // are advanced right after each splat (the reference advances them
+    // lazily at the next one, from last_alpha / last_color; same expression,
+    // same rounding).
+    for (int i = nw - 1; i >= 0; --i) {
+      const int j = s_list[wv][i];
+      const uint32_t q = sbase + j;  // 0-based list position; contributor index q + 1
+      const float4 ga = s_a[j];
+      const float4 gb = s_b[j];
+      const float dx = ga.x - pfx, dy = ga.y - pfy;
+      const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
+      // gb.w: alpha < 1/255 for sure below it
+      bool active = inside && q < last && !(power > 0.0f || power < gb.w);
+      if (__ballot(active) == 0ull) continue;
+      const float Graw = spec_expf(power);
+      const float alpha_raw = fminf(0.99f, gb.y * Graw);
+      active = active && !(alpha_raw < a_min);
+      if (__ballot(active) == 0ull) continue;
+      const float G = active ? Graw : 0.0f;
+      const float alpha = active ? alpha_raw : 0.0f;
+      const float4 gc = s_c[j];
+      // T before this splat; the gradient tolerates the 1-ulp reciprocal
+      const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
+      const float Tn = T * inv_1ma;
+      const float w = alpha * Tn;
+      float dL_da = (gc.x - acc0) * dpix0;
+      dL_da += (gc.y - acc1) * dpix1;
+      dL_da += (gc.z - acc2) * dpix2;
+      dL_da += (gb.z - acc_d) * dD;
+      dL_da += (1.0f - acc_a) * dA;
+      dL_da *= Tn;
+      dL_da += (-T_fin * inv_1ma) * bg_dot;
+      const float dL_dG = gb.y * dL_da;
+      const float gdx = G * dx, gdy = G * dy;
+      const float dG_dx = -gdx * ga.z - gdy * ga.w;
+      const float dG_dy = -gdy * gb.x - gdx * ga.w;
+      float v[12];
+      v[0] = dL_dG * dG_dx * ddelx;
+      v[1] = dL_dG * dG_dy * ddely;
+      v[2] = -0.5f * gdx * dx * dL_dG;
+      v[3] = -gdx * dy * dL_dG;
+      v[4] = -0.5f * gdy * dy * dL_dG;
+      v[5] = G * dL_da;
+      v[6] = w * dpix0;
+      v[7] = w * dpix1;
+      v[8] = w * dpix2;
+      v[9] = w * dD;
+      v[10] = 0.f;
+      v[11] = 0.f;
+      T = Tn;
+      const float oma = 1.0f - alpha;
+      acc0 = alpha * gc.x + oma * acc0;
+      acc1 = alpha * gc.y + oma * acc1;
+      acc2 = alpha * gc.z + oma * acc2;
+      acc_d = alpha * gb.z + oma * acc_d;
+      acc_a = alpha + oma * acc_a;
+      {
+        float w[3];
+        wave_sum12(v, w);
+        if ((lane & 15) == 0) {
+          const int q = lane >> 4;
+          const int o = q == 1 ? 2 : (q == 2 ? 1 : q);  // value of row q: 4m + {0, 2, 1, 3}[q]
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          const int k = 4 * m + o;
-          if (k < 10) {
-            if constexpr (DET) {
-              s_acc[(threadIdx.x >> 6) * 10 + k][j] = w[m];
-            } else {
-              if (w[m] != 0.0f) atomicAdd(&s_acc[k][j], w[m]);
+          for (int m = 0; m < 3; ++m) {
+            const int k = 4 * m + o;
+            if (k < 10) {
+              if constexpr (DET) {
+                s_acc[wv * 10 + k][j] = w[m];
+              } else {
+                if (w[m] != 0.0f) atomicAdd(&s_acc[k][j], w[m]);
+              }
             }
           }
         }
       }
     }
-  }
-  __syncthreads();
-  const uint32_t t = threadIdx.x;
-  if (t < cend - cstart) {
-    if constexpr (DET) {
-      // slot of this (Gaussian, tile) pair: the Gaussian's first slot in the
-      // Gaussian-order prefix of tiles touched + the tile's row-major rank in
-      // its rect (every pair of the list has exactly one chunk)
-      const uint32_t g = s_id[t];
-      int x0, y0, x1, y1;
-      tile_rect(s_a[t].x, s_a[t].y, __float_as_int(s_c[t].w), a.gx, a.gy, x0, y0, x1, y1);
-      const uint32_t w = (uint32_t)(x1 - x0);
-      const uint32_t ntg = w * (uint32_t)(y1 - y0);
-      const uint32_t slot = a.block_excl[g >> 8] + a.local_incl[g] - ntg + (uint32_t)(ty - y0) * w + (uint32_t)(tx - x0);
-      float* dst = a.part + (size_t)slot * 10;
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    if (t < send - sbase) {
+      if constexpr (DET) {
+        // slot of this (Gaussian, tile) pair: the Gaussian's first slot in the
+        // Gaussian-order prefix of tiles touched + the tile's row-major rank in
+        // its rect (every pair of the list has exactly one chunk)
+        const uint32_t g = s_id[t];
+        int x0, y0, x1, y1;
+        tile_rect(s_a[t].x, s_a[t].y, __float_as_int(s_c[t].w), a.gx, a.gy, x0, y0, x1, y1);
+        const uint32_t w = (uint32_t)(x1 - x0);
+        const uint32_t ntg = w * (uint32_t)(y1 - y0);
+        const uint32_t slot =
+            a.block_excl[g >> 8] + a.local_incl[g] - ntg + (uint32_t)(ty - y0) * w + (uint32_t)(tx - x0);
+        float* dst = a.part + (size_t)slot * 10;
 #pragma unroll
-      for (int k = 0; k < 10; ++k) dst[k] = ((s_acc[k][t] + s_acc[10 + k][t]) + s_acc[20 + k][t]) + s_acc[30 + k][t];
-    } else {
-      float* dst = a.g_render + (size_t)s_id[t] * kRG;
+        for (int k = 0; k < 10; ++k)
+          dst[k] = ((s_acc[k][t] + s_acc[10 + k][t]) + s_acc[20 + k][t]) + s_acc[30 + k][t];
+      } else {
+        float* dst = a.g_render + (size_t)s_id[t] * kRG;
 #pragma unroll
-      for (int k = 0; k < 10; ++k) {
-        const float s = s_acc[k][t];
-        if (s != 0.0f) unsafeAtomicAdd(dst + k, s);
+        for (int k = 0; k < 10; ++k) {
+          const float s = s_acc[k][t];
+          if (s != 0.0f) unsafeAtomicAdd(dst + k, s);
+        }
       }
     }
   }

@@ -412,11 +412,12 @@ __device__ __forceinline__ uint32_t splat_quads(float x, float y, float a, float
 // The wave's own list of the staged splats that meet its pixels: positions j
 // < limit (ascending) with bit `wv` set in s_mask[j]; returns the count
 // (wave-uniform).  Needs all 64 lanes active.
+template <int NC = 4>  // staged entries: NC * 64
 __device__ __forceinline__ int compact_wave_list(const uint32_t* s_mask, uint8_t* s_list, int wv, int lane,
-                                                 int limit = 256) {
+                                                 int limit = NC * 64) {
   int n = 0;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < NC; ++c) {
     const int idx = c * 64 + lane;
     const bool hit = idx < limit && ((s_mask[idx] >> wv) & 1u);
     const uint64_t bal = __ballot(hit);
