@@ -20,6 +20,7 @@ namespace gs {
 constexpr int kLTX = 32, kLTY = 16, kLR = 5;
 constexpr int kLIW = kLTX + 2 * kLR, kLIH = kLTY + 2 * kLR;
 constexpr float kC1 = 0.01f * 0.01f, kC2 = 0.03f * 0.03f;
+static_assert(kLTX * kLTY == 2 * 256, "the vertical passes give each of 256 threads two outputs");
 
 struct Win {
   float g[11];
@@ -36,89 +37,153 @@ struct LossArgs {
   float* partial;  // [2 * nblocks]: sum S, sum |x - y|
 };
 
-__device__ __forceinline__ float at_or_zero(const float* p, int H, int W, int r, int c) {
-  return (r >= 0 && r < H && c >= 0 && c < W) ? p[r * W + c] : 0.0f;
+// The zero-padded (kLIH x kLIW) window of a plane whose first pixel is (r0,
+// c0), for 256 threads: every load of a thread is issued before its first
+// LDS store (addresses clamped into the plane, out-of-plane values zeroed),
+// instead of one load-to-store round trip per element.
+template <int NQ>
+__device__ __forceinline__ void stage_halo(float (*dst[NQ])[kLIW], const float* const src[NQ], int H, int W, int r0,
+                                           int c0, int clamp_first) {
+  constexpr int kPer = (kLIH * kLIW + 255) / 256;
+  float v[NQ][kPer];
+  int idx[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int k = (int)threadIdx.x + 256 * i;
+    const int kk = k < kLIH * kLIW ? k : kLIH * kLIW - 1;
+    const int r = kk / kLIW, c = kk - r * kLIW;
+    const int gr = r0 + r, gc = c0 + c;
+    const bool in = gr >= 0 && gr < H && gc >= 0 && gc < W;
+    const int o = min(max(gr, 0), H - 1) * W + min(max(gc, 0), W - 1);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const float x = src[q][o];
+      v[q][i] = in ? x : 0.0f;
+    }
+    idx[i] = kk;
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int r = idx[i] / kLIW, c = idx[i] - r * kLIW;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const float x = v[q][i];
+      dst[q][r][c] = (q == 0 && clamp_first) ? fminf(fmaxf(x, 0.0f), 1.0f) : x;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_ssim_fwd(LossArgs a) {
   __shared__ float sx[kLIH][kLIW];
   __shared__ float sy[kLIH][kLIW];
   __shared__ float sh[5][kLIH][kLTX];
-  __shared__ float red[2][256];
+  __shared__ float red[2][4];
   const int ch = blockIdx.z;
   const int r0 = blockIdx.y * kLTY - kLR, c0 = blockIdx.x * kLTX - kLR;
   const float* X = a.x + (size_t)ch * a.H * a.W;
   const float* Y = a.y + (size_t)ch * a.H * a.W;
   const int t = threadIdx.x;
-  for (int k = t; k < kLIH * kLIW; k += 256) {
-    const int r = k / kLIW, c = k - r * kLIW;
-    const float xr = at_or_zero(X, a.H, a.W, r0 + r, c0 + c);
-    sx[r][c] = a.clamp01 ? fminf(fmaxf(xr, 0.0f), 1.0f) : xr;
-    sy[r][c] = at_or_zero(Y, a.H, a.W, r0 + r, c0 + c);
+  {
+    float (*dst[2])[kLIW] = {sx, sy};
+    const float* const src[2] = {X, Y};
+    stage_halo<2>(dst, src, a.H, a.W, r0, c0, a.clamp01);
   }
   __syncthreads();
-  for (int k = t; k < kLIH * kLTX; k += 256) {
-    const int r = k / kLTX, c = k - r * kLTX;
-    float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f, h4 = 0.f;
+  // horizontal 11-tap pass, two neighbouring outputs per thread: they share
+  // ten of their twelve inputs (the per-output sum order is unchanged)
+  for (int k = t; k < kLIH * (kLTX / 2); k += 256) {
+    const int r = k / (kLTX / 2), c = 2 * (k - r * (kLTX / 2));
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f;
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f, b4 = 0.f;
+    float xp = sx[r][c], yp = sy[r][c];
 #pragma unroll
     for (int j = 0; j < 11; ++j) {
       const float g = a.win.g[j];
-      const float xv = sx[r][c + j], yv = sy[r][c + j];
-      h0 = fmaf(g, xv, h0);
-      h1 = fmaf(g, yv, h1);
-      h2 = fmaf(g, xv * xv, h2);
-      h3 = fmaf(g, yv * yv, h3);
-      h4 = fmaf(g, xv * yv, h4);
+      const float xn = sx[r][c + j + 1], yn = sy[r][c + j + 1];
+      a0 = fmaf(g, xp, a0);
+      a1 = fmaf(g, yp, a1);
+      a2 = fmaf(g, xp * xp, a2);
+      a3 = fmaf(g, yp * yp, a3);
+      a4 = fmaf(g, xp * yp, a4);
+      b0 = fmaf(g, xn, b0);
+      b1 = fmaf(g, yn, b1);
+      b2 = fmaf(g, xn * xn, b2);
+      b3 = fmaf(g, yn * yn, b3);
+      b4 = fmaf(g, xn * yn, b4);
+      xp = xn;
+      yp = yn;
     }
-    sh[0][r][c] = h0;
-    sh[1][r][c] = h1;
-    sh[2][r][c] = h2;
-    sh[3][r][c] = h3;
-    sh[4][r][c] = h4;
+    sh[0][r][c] = a0;
+    sh[1][r][c] = a1;
+    sh[2][r][c] = a2;
+    sh[3][r][c] = a3;
+    sh[4][r][c] = a4;
+    sh[0][r][c + 1] = b0;
+    sh[1][r][c + 1] = b1;
+    sh[2][r][c + 1] = b2;
+    sh[3][r][c + 1] = b3;
+    sh[4][r][c + 1] = b4;
   }
   __syncthreads();
   float sumS = 0.f, sumL1 = 0.f;
   const size_t HW = (size_t)a.H * a.W;
-  for (int k = t; k < kLTY * kLTX; k += 256) {
-    const int r = k / kLTX, c = k - r * kLTX;
-    const int gr = blockIdx.y * kLTY + r, gc = blockIdx.x * kLTX + c;
-    if (gr >= a.H || gc >= a.W) continue;
-    float m1 = 0.f, m2 = 0.f, e11 = 0.f, e22 = 0.f, e12 = 0.f;
+  // vertical pass, two vertically neighbouring outputs per thread (256 pairs)
+  {
+    const int r = 2 * (t / kLTX), c = t - (t / kLTX) * kLTX;
+    float m1[2] = {0.f, 0.f}, m2[2] = {0.f, 0.f}, e11[2] = {0.f, 0.f}, e22[2] = {0.f, 0.f}, e12[2] = {0.f, 0.f};
 #pragma unroll
-    for (int j = 0; j < 11; ++j) {
-      const float g = a.win.g[j];
-      m1 = fmaf(g, sh[0][r + j][c], m1);
-      m2 = fmaf(g, sh[1][r + j][c], m2);
-      e11 = fmaf(g, sh[2][r + j][c], e11);
-      e22 = fmaf(g, sh[3][r + j][c], e22);
-      e12 = fmaf(g, sh[4][r + j][c], e12);
+    for (int j = 0; j < 12; ++j) {
+      const float v0 = sh[0][r + j][c], v1 = sh[1][r + j][c], v2 = sh[2][r + j][c], v3 = sh[3][r + j][c],
+                  v4 = sh[4][r + j][c];
+      if (j < 11) {
+        const float g = a.win.g[j];
+        m1[0] = fmaf(g, v0, m1[0]);
+        m2[0] = fmaf(g, v1, m2[0]);
+        e11[0] = fmaf(g, v2, e11[0]);
+        e22[0] = fmaf(g, v3, e22[0]);
+        e12[0] = fmaf(g, v4, e12[0]);
+      }
+      if (j > 0) {
+        const float g = a.win.g[j - 1];
+        m1[1] = fmaf(g, v0, m1[1]);
+        m2[1] = fmaf(g, v1, m2[1]);
+        e11[1] = fmaf(g, v2, e11[1]);
+        e22[1] = fmaf(g, v3, e22[1]);
+        e12[1] = fmaf(g, v4, e12[1]);
+      }
     }
-    const float s1 = e11 - m1 * m1, s2 = e22 - m2 * m2, s12 = e12 - m1 * m2;
-    const float A1 = 2.f * m1 * m2 + kC1, A2 = 2.f * s12 + kC2;
-    const float B1 = m1 * m1 + m2 * m2 + kC1, B2 = s1 + s2 + kC2;
-    const float inv = 1.0f / (B1 * B2);
-    const float S = A1 * A2 * inv;
-    const size_t o = (size_t)ch * HW + (size_t)gr * a.W + gc;
-    a.maps[o] = (2.f * m2 * (A2 - A1) - 2.f * m1 * S * (B2 - B1)) * inv;  // dS/dmu_x
-    a.maps[(size_t)a.C * HW + o] = -S / B2;                             // dS/dE[x^2]
-    a.maps[2 * (size_t)a.C * HW + o] = 2.f * A1 * inv;                  // dS/dE[xy]
-    sumS += S;
-    sumL1 += fabsf(sx[r + kLR][c + kLR] - sy[r + kLR][c + kLR]);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int gr = blockIdx.y * kLTY + r + u, gc = blockIdx.x * kLTX + c;
+      if (gr >= a.H || gc >= a.W) continue;
+      const float s1 = e11[u] - m1[u] * m1[u], s2 = e22[u] - m2[u] * m2[u], s12 = e12[u] - m1[u] * m2[u];
+      const float A1 = 2.f * m1[u] * m2[u] + kC1, A2 = 2.f * s12 + kC2;
+      const float B1 = m1[u] * m1[u] + m2[u] * m2[u] + kC1, B2 = s1 + s2 + kC2;
+      const float inv = 1.0f / (B1 * B2);
+      const float S = A1 * A2 * inv;
+      const size_t o = (size_t)ch * HW + (size_t)gr * a.W + gc;
+      a.maps[o] = (2.f * m2[u] * (A2 - A1) - 2.f * m1[u] * S * (B2 - B1)) * inv;  // dS/dmu_x
+      a.maps[(size_t)a.C * HW + o] = -S / B2;                                      // dS/dE[x^2]
+      a.maps[2 * (size_t)a.C * HW + o] = 2.f * A1 * inv;                           // dS/dE[xy]
+      sumS += S;
+      sumL1 += fabsf(sx[r + u + kLR][c + kLR] - sy[r + u + kLR][c + kLR]);
+    }
   }
-  red[0][t] = sumS;
-  red[1][t] = sumL1;
+  // block sums: wave shuffles, then the four wave partials in order
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sumS += __shfl_down(sumS, off, 64);
+    sumL1 += __shfl_down(sumL1, off, 64);
+  }
+  if ((t & 63) == 0) {
+    red[0][t >> 6] = sumS;
+    red[1][t >> 6] = sumL1;
+  }
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (t < off) {
-      red[0][t] += red[0][t + off];
-      red[1][t] += red[1][t + off];
-    }
-    __syncthreads();
-  }
   if (t == 0) {
     const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-    a.partial[2 * b] = red[0][0];
-    a.partial[2 * b + 1] = red[1][0];
+    a.partial[2 * b] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+    a.partial[2 * b + 1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
   }
 }
 
@@ -169,48 +234,70 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(LossBwdArgs a) {
   const size_t HW = (size_t)a.H * a.W;
   const int r0 = blockIdx.y * kLTY - kLR, c0 = blockIdx.x * kLTX - kLR;
   const int t = threadIdx.x;
-  for (int q = 0; q < 3; ++q) {
-    const float* Mq = a.maps + (size_t)q * a.C * HW + (size_t)ch * HW;
-    for (int k = t; k < kLIH * kLIW; k += 256) {
-      const int r = k / kLIW, c = k - r * kLIW;
-      sm[q][r][c] = at_or_zero(Mq, a.H, a.W, r0 + r, c0 + c);
-    }
+  {
+    float (*dst[3])[kLIW] = {sm[0], sm[1], sm[2]};
+    const float* const src[3] = {a.maps + (size_t)ch * HW, a.maps + (size_t)a.C * HW + (size_t)ch * HW,
+                                 a.maps + 2 * (size_t)a.C * HW + (size_t)ch * HW};
+    stage_halo<3>(dst, src, a.H, a.W, r0, c0, 0);
   }
   __syncthreads();
-  for (int k = t; k < kLIH * kLTX; k += 256) {
-    const int r = k / kLTX, c = k - r * kLTX;
-    float h0 = 0.f, h1 = 0.f, h2 = 0.f;
+  // horizontal pass of the three maps, two neighbouring outputs per thread
+  for (int k = t; k < kLIH * (kLTX / 2); k += 256) {
+    const int r = k / (kLTX / 2), c = 2 * (k - r * (kLTX / 2));
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, b0 = 0.f, b1 = 0.f, b2 = 0.f;
+    float p0 = sm[0][r][c], p1 = sm[1][r][c], p2 = sm[2][r][c];
 #pragma unroll
     for (int j = 0; j < 11; ++j) {
       const float g = a.win.g[j];
-      h0 = fmaf(g, sm[0][r][c + j], h0);
-      h1 = fmaf(g, sm[1][r][c + j], h1);
-      h2 = fmaf(g, sm[2][r][c + j], h2);
+      const float n0 = sm[0][r][c + j + 1], n1 = sm[1][r][c + j + 1], n2 = sm[2][r][c + j + 1];
+      a0 = fmaf(g, p0, a0);
+      a1 = fmaf(g, p1, a1);
+      a2 = fmaf(g, p2, a2);
+      b0 = fmaf(g, n0, b0);
+      b1 = fmaf(g, n1, b1);
+      b2 = fmaf(g, n2, b2);
+      p0 = n0;
+      p1 = n1;
+      p2 = n2;
     }
-    sh[0][r][c] = h0;
-    sh[1][r][c] = h1;
-    sh[2][r][c] = h2;
+    sh[0][r][c] = a0;
+    sh[1][r][c] = a1;
+    sh[2][r][c] = a2;
+    sh[0][r][c + 1] = b0;
+    sh[1][r][c + 1] = b1;
+    sh[2][r][c + 1] = b2;
   }
   __syncthreads();
   const float go = *a.grad_out;
-  for (int k = t; k < kLTY * kLTX; k += 256) {
-    const int r = k / kLTX, c = k - r * kLTX;
-    const int gr = blockIdx.y * kLTY + r, gc = blockIdx.x * kLTX + c;
-    if (gr >= a.H || gc >= a.W) continue;
-    float ga = 0.f, gb = 0.f, gc2 = 0.f;
+  // vertical pass, two vertically neighbouring outputs per thread
+  const int r = 2 * (t / kLTX), c = t - (t / kLTX) * kLTX;
+  float ga[2] = {0.f, 0.f}, gb[2] = {0.f, 0.f}, gc2[2] = {0.f, 0.f};
 #pragma unroll
-    for (int j = 0; j < 11; ++j) {
+  for (int j = 0; j < 12; ++j) {
+    const float v0 = sh[0][r + j][c], v1 = sh[1][r + j][c], v2 = sh[2][r + j][c];
+    if (j < 11) {
       const float g = a.win.g[j];
-      ga = fmaf(g, sh[0][r + j][c], ga);
-      gb = fmaf(g, sh[1][r + j][c], gb);
-      gc2 = fmaf(g, sh[2][r + j][c], gc2);
+      ga[0] = fmaf(g, v0, ga[0]);
+      gb[0] = fmaf(g, v1, gb[0]);
+      gc2[0] = fmaf(g, v2, gc2[0]);
     }
-    const size_t o = (size_t)ch * HW + (size_t)gr * a.W + gc;
+    if (j > 0) {
+      const float g = a.win.g[j - 1];
+      ga[1] = fmaf(g, v0, ga[1]);
+      gb[1] = fmaf(g, v1, gb[1]);
+      gc2[1] = fmaf(g, v2, gc2[1]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int gr = blockIdx.y * kLTY + r + u, gcol = blockIdx.x * kLTX + c;
+    if (gr >= a.H || gcol >= a.W) continue;
+    const size_t o = (size_t)ch * HW + (size_t)gr * a.W + gcol;
     const float xraw = a.x[o], yv = a.y[o];
     const float xv = a.clamp01 ? fminf(fmaxf(xraw, 0.0f), 1.0f) : xraw;
     // clamp's gradient passes where 0 <= x <= 1 (torch's clamp backward)
     const float pass = (!a.clamp01 || (xraw >= 0.0f && xraw <= 1.0f)) ? 1.0f : 0.0f;
-    const float dssim = (ga + 2.f * xv * gb + yv * gc2) * a.inv_n;
+    const float dssim = (ga[u] + 2.f * xv * gb[u] + yv * gc2[u]) * a.inv_n;
     const float d = xv - yv;
     const float sgn = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
     a.dx[o] = pass * (go * ((1.0f - a.lambda) * sgn * a.inv_n - a.lambda * dssim));
