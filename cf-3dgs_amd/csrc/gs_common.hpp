@@ -324,6 +324,34 @@ __device__ __forceinline__ void rows_to_lds(float* __restrict__ dst, const float
     }
   }
 }
+// Inclusive prefix sum over a workgroup of NT threads (a multiple of 64, at
+// most 1024); `total` receives the workgroup sum.  Each wave scans by
+// shuffles and one barrier combines the wave totals (a Hillis-Steele scan
+// needs two barriers per step).  s_w (NT / 64 words of LDS) may be rewritten
+// only after a later barrier.
+template <int NT>
+__device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* s_w, uint32_t& total) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    const uint32_t sw = s_w[i];
+    before += i < w ? sw : 0u;
+    tot += sw;
+  }
+  total = tot;
+  return x + before;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const uint32_t lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));

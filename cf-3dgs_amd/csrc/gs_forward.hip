@@ -44,21 +44,8 @@ struct PreArgs {
 };
 
 // Inclusive prefix sum over the 256 threads of a workgroup (uint32).
-__device__ __forceinline__ uint32_t block_incl_scan_256(uint32_t v, uint32_t* s_tmp /*[256]*/, uint32_t& total) {
-  const int t = threadIdx.x;
-  s_tmp[t] = v;
-  __syncthreads();
-#pragma unroll
-  for (int off = 1; off < 256; off <<= 1) {
-    const uint32_t add = t >= off ? s_tmp[t - off] : 0u;
-    __syncthreads();
-    s_tmp[t] += add;
-    __syncthreads();
-  }
-  total = s_tmp[255];
-  const uint32_t r = s_tmp[t];
-  __syncthreads();
-  return r;
+__device__ __forceinline__ uint32_t block_incl_scan_256(uint32_t v, uint32_t* s_tmp /*[>= 4]*/, uint32_t& total) {
+  return block_scan_incl<256>(v, s_tmp, total);
 }
 
 __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
@@ -239,18 +226,12 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* block_sum, int nblo
     const int b = b0 + t;
     const uint32_t v = b < nblocks ? block_sum[b] : 0u;
     if (b < nblocks) myflags |= block_sum[nblocks + b];
-    s[t] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const uint32_t add = t >= off ? s[t - off] : 0u;
-      __syncthreads();
-      s[t] += add;
-      __syncthreads();
-    }
+    uint32_t tot;
+    const uint32_t incl = block_scan_incl<1024>(v, s, tot);
     const uint32_t carry = s_carry;
-    if (b < nblocks) block_sum[b] = carry + s[t] - v;
+    if (b < nblocks) block_sum[b] = carry + incl - v;
     __syncthreads();
-    if (t == 1023) s_carry = carry + s[1023];
+    if (t == 0) s_carry = carry + tot;
     __syncthreads();
   }
   if (myflags) atomicOr(&s_flags, myflags);
@@ -371,18 +352,12 @@ __global__ __launch_bounds__(1024) void k_excl_scan_1wg(uint32_t* data, int n) {
   for (int b0 = 0; b0 < n; b0 += 1024) {
     const int b = b0 + t;
     const uint32_t v = b < n ? data[b] : 0u;
-    s[t] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const uint32_t add = t >= off ? s[t - off] : 0u;
-      __syncthreads();
-      s[t] += add;
-      __syncthreads();
-    }
+    uint32_t tot;
+    const uint32_t incl = block_scan_incl<1024>(v, s, tot);
     const uint32_t carry = s_carry;
-    if (b < n) data[b] = carry + s[t] - v;
+    if (b < n) data[b] = carry + incl - v;
     __syncthreads();
-    if (t == 1023) s_carry = carry + s[1023];
+    if (t == 0) s_carry = carry + tot;
     __syncthreads();
   }
 }
@@ -698,18 +673,12 @@ __global__ __launch_bounds__(1024) void k_tile_ckpt_scan(const uint32_t* ranges,
       const uint32_t nb = (len + 255) / 256;
       v = nb > 0 ? nb - 1 : 0;
     }
-    s[t] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const uint32_t add = t >= off ? s[t - off] : 0u;
-      __syncthreads();
-      s[t] += add;
-      __syncthreads();
-    }
+    uint32_t tot;
+    const uint32_t incl = block_scan_incl<1024>(v, s, tot);
     const uint32_t carry = s_carry;
-    if (b < nt) ckpt_base[b] = carry + s[t] - v;
+    if (b < nt) ckpt_base[b] = carry + incl - v;
     __syncthreads();
-    if (t == 1023) s_carry = carry + s[1023];
+    if (t == 0) s_carry = carry + tot;
     __syncthreads();
   }
   if (t == 0) ckpt_base[nt] = s_carry;

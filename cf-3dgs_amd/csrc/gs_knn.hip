@@ -164,16 +164,10 @@ __global__ __launch_bounds__(1024) void k_knn_scan_local(const uint32_t* in, uin
   const int i = blockIdx.x * 1024 + threadIdx.x;
   if ((int)blockIdx.x * 1024 >= n) return;
   const uint32_t v = i < n ? in[i] : 0u;
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint32_t add = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += add;
-    __syncthreads();
-  }
-  if (i < n) out[i] = s[threadIdx.x] - v;
-  if (threadIdx.x == 1023) block_sums[blockIdx.x] = s[1023];
+  uint32_t tot;
+  const uint32_t incl = block_scan_incl<1024>(v, s, tot);
+  if (i < n) out[i] = incl - v;
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
 __global__ __launch_bounds__(1024) void k_knn_scan_top(uint32_t* block_sums, const KnnParams* prm) {
@@ -186,18 +180,12 @@ __global__ __launch_bounds__(1024) void k_knn_scan_top(uint32_t* block_sums, con
   for (int b0 = 0; b0 < nb; b0 += 1024) {
     const int b = b0 + t;
     const uint32_t v = b < nb ? block_sums[b] : 0u;
-    s[t] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const uint32_t add = t >= off ? s[t - off] : 0u;
-      __syncthreads();
-      s[t] += add;
-      __syncthreads();
-    }
+    uint32_t tot;
+    const uint32_t incl = block_scan_incl<1024>(v, s, tot);
     const uint32_t carry = s_carry;
-    if (b < nb) block_sums[b] = carry + s[t] - v;
+    if (b < nb) block_sums[b] = carry + incl - v;
     __syncthreads();
-    if (t == 1023) s_carry = carry + s[1023];
+    if (t == 0) s_carry = carry + tot;
     __syncthreads();
   }
 }
