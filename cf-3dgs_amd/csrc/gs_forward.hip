@@ -1157,4 +1157,4 @@ extern "C" const char* gs_error_string(int code) {
   }
 }
 
-extern "C" int gs_abi_version(void) { return 1; }
+extern "C" int gs_abi_version(void) { return 2; }

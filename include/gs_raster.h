@@ -346,6 +346,8 @@ int gs_timing_reset(void);
 int gs_timing_read(const char* stage, double* total_ms, int* launches);
 
 const char* gs_error_string(int code);
+/* 2: binning layout with the backward work items, one-word list entries and
+ * packed tile rects; split forward (gs_render_prepare / gs_render_resume). */
 int gs_abi_version(void);
 
 #ifdef __cplusplus

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 15
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.gs_abi_version() == 1
+    assert lib.gs_abi_version() == 2
 
 
 def test_layouts():
