@@ -33,19 +33,21 @@ HBM_PEAK_GBS = 8000.0
 
 
 def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, passes: int = 6,
-                depth_first: bool = True) -> float:
+                depth_first: bool = True, one_word: bool = False) -> float:
     """Algorithmic HBM bytes of one step's worth of a stage (the bytes the
-    algorithm must move, independent of how the kernels move them)."""
+    algorithm must move, independent of how the kernels move them).
+    one_word: depth-first list entries are single words tile << id_bits | id."""
     hw = W * H
+    entry = 4 if one_word else 8  # bytes per list entry (tile id + Gaussian id)
     if stage == "preprocess":   # in: xyz 12, scale 12, rot 16, opac 4, SH 12M; out: radii 4, splat 48, cov 24, tiles 4, scan 4
         return P * (44 + 12 * M + 84)
     if stage == "duplicate":
-        if depth_first:         # scan of tiles in depth order (id 4, tiles 4, prefix 4) + dup (id, tiles, prefix, 2 splat records)
-            return P * 12 + P * (12 + 32) + 8 * n_dup
+        if depth_first:         # scan of tiles in depth order (id 4, tiles 4, prefix 4) + dup (id, tiles, prefix, rect 8)
+            return P * 12 + P * (12 + 8) + entry * n_dup
         return P * 56 + 12 * n_dup  # splat 48 + tiles 4 + scan 4; out: key 8 + id 4 per duplicated entry
     if stage == "sort":
-        if depth_first:         # 4 passes over P (32-bit key 4, id 4, in + out) + tile passes over the list
-            return 4 * 16 * P + passes * 16 * n_dup
+        if depth_first:         # keys pass (tiles 4 + splat 48 in; key 4, id 4, rect 8 out), 4 passes over P
+            return 68 * P + 4 * 16 * P + passes * 2 * entry * n_dup  # (key 4, id 4, in + out) + the list passes
         return passes * 24 * n_dup  # each 8-bit pass reads and writes (key 8, value 4)
     if stage == "ranges":
         return 8 * n_dup + 8 * ((W + 15) // 16) * ((H + 15) // 16)
@@ -385,7 +387,7 @@ def main():
     L = _lib.binning_layout(n_dup, args.P, args.width, args.height, raster.BINNING)
     for s, d in stages.items():
         b = stage_bytes(s, args.P, n_dup, args.width, args.height, passes=L.sort_passes,
-                        depth_first=raster.BINNING == _lib.BINNING_DEPTH_FIRST)
+                        depth_first=raster.BINNING == _lib.BINNING_DEPTH_FIRST, one_word=L.id_bits > 0)
         d["alg_bytes"] = b
         d["GB_s"] = b / (d["ms_per_step"] * 1e-3) / 1e9 if d["ms_per_step"] > 0 else 0.0
     dom = max(stages, key=lambda s: stages[s]["ms_per_step"]) if stages else None
