@@ -180,6 +180,47 @@ def knn_bench(device, reps: int = 5):
     return out
 
 
+def ingest_bench(device, reps: int = 5, cpu: bool = True):
+    """Frame ingest (depth lift, voxel downsample 0.01, create_from_pcd with
+    distCUDA2) of the synthetic 480x270 and 960x540 frames with 8-bit
+    images; wall time per frame including the two host readbacks.  The CPU
+    figure is the oracle's numpy lift + voxel downsample, the part the
+    reference runs on the host (open3d)."""
+    import math
+
+    import numpy as np
+    import torch
+    from gs_hip import ingest
+    from gs_hip.camera import fov2focal
+    from gs_hip.scenes import synthetic_depth
+    out = {}
+    for W, H in ((480, 270), (960, 540)):
+        g = torch.Generator().manual_seed(W)
+        d = synthetic_depth(W, H, g)
+        img = torch.randint(0, 256, (H, W, 3), generator=g, dtype=torch.uint8)
+        fx = fov2focal(math.radians(60.0), W)
+        K = (fx, fx, W / 2.0, H / 2.0)
+        dd, ii = d.to(device), img.to(device)
+        r = ingest.ingest_frame(dd, ii, K)
+        torch.cuda.synchronize(device)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = ingest.ingest_frame(dd, ii, K)
+            torch.cuda.synchronize(device)
+            ts.append(time.perf_counter() - t0)
+        rec = {"points_in": W * H, "points_out": int(r["xyz"].shape[0]),
+               "gpu_ms": round(float(np.median(ts)) * 1e3, 3)}
+        if cpu:
+            from oracle import ingest as OI
+            t0 = time.perf_counter()
+            pts = OI.depth_to_3d(d.numpy(), *[float(np.float32(x)) for x in K])
+            OI.voxel_down_sample(pts, OI.colors_u8(img.numpy()), 0.01)
+            rec["cpu_oracle_lift_voxel_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        out[f"{W}x{H}"] = rec
+    return out
+
+
 def _log(msg: str) -> None:
     """Progress on stderr (stdout carries only the result line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -401,6 +442,7 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
     c1 = None if args.no_plain else c1_bench(device, cpu=not args.no_cpu_baseline)
     knn = None if args.no_plain else knn_bench(device)
+    ing = None if args.no_plain else ingest_bench(device, cpu=not args.no_cpu_baseline)
     pose = None if args.no_pose else pose_bench(device, n=args.pose_instances, quick=args.pose_quick)
     serving = None if args.no_serve else serve_bench(device)
     res = {
@@ -415,6 +457,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_10k_256": c1,
         "distCUDA2": knn,
+        "ingest": ing,
         "pose": pose,
         "serving": serving,
         "stages": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}

@@ -17,6 +17,7 @@ c_float = ctypes.c_float
 c_size_t = ctypes.c_size_t
 c_void_p = ctypes.c_void_p
 c_longlong = ctypes.c_longlong
+c_double = ctypes.c_double
 
 GS_OK = 0
 GS_STATUS_CAPACITY = 1
@@ -136,6 +137,16 @@ EXPORTS = {
     "gs_activate_forward": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gs_activate_backward": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p]),
+    "gs_depth_to_points": (c_int, [c_int, c_int, c_void_p, c_float, c_float, c_float, c_float, c_void_p, c_void_p,
+                                   c_int, c_longlong, c_longlong, c_void_p, c_void_p]),
+    "gs_bounds_workspace_bytes": (c_size_t, []),
+    "gs_points_bounds": (c_int, [c_void_p, c_longlong, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "gs_voxel_workspace_bytes": (c_size_t, [c_longlong]),
+    "gs_voxel_down_sample": (c_int, [c_void_p, c_longlong, c_void_p, c_int, c_longlong, c_longlong,
+                                     ctypes.POINTER(c_double), c_double, c_int, c_int, c_int, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_size_t, c_void_p]),
+    "gs_create_from_pcd": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p]),
     "gs_timing_enable": (c_int, [c_int]),
     "gs_timing_reset": (c_int, []),
     "gs_timing_read": (c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int)]),

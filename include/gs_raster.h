@@ -263,6 +263,51 @@ int gs_sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, uint
 size_t gs_knn_workspace_bytes(int P);
 int gs_knn3_meandist(const float* points, int P, float* out, void* ws, size_t ws_bytes, void* stream);
 
+/* Frame ingest: a depth map and its image become a frame's initial Gaussians
+ * on the device.  The reference does this on the host: kornia depth_to_3d,
+ * numpy, open3d voxel_down_sample(0.01), then create_from_pcd
+ * (trainer/trainer.py:389-423,476-512; scene/gaussian_model_cf.py:178-212).
+ *
+ * Colour sources are per point (pixel i = point i of a lifted frame).  Value
+ * (point i, channel c) sits at element i*color_pix_stride + c*color_chan_stride
+ * of a uint8 array (colors_u8 = 1; v becomes v / 255.0 in double, as
+ * np.asarray(image) / 255.0) or of a float32 array (colors_u8 = 0).
+ *
+ * gs_depth_to_points:   kornia depth_to_3d(normalize_points=False) over an
+ *                       [H, W] depth.  points[i = v*W + u] =
+ *                       (((u - cx) / fx) * d, ((v - cy) / fy) * d, d) in
+ *                       float32.  When `colors` is non-NULL it also gets the
+ *                       source colours as float32 [H*W, 3].
+ * gs_points_bounds:     bounds6 (device) = min xyz, max xyz of [N, 3] points;
+ *                       all NaN when a coordinate is not finite.
+ *                       ws: gs_bounds_workspace_bytes().
+ * gs_voxel_down_sample: open3d VoxelDownSample.  voxel_min3 (HOST, double) is
+ *                       min - voxel_size / 2.  The index per axis is
+ *                       floor((p - voxel_min) / voxel_size) in double, and
+ *                       must fit bits_x / bits_y / bits_z bits (total <= 64).
+ *                       Each occupied voxel's mean point and mean colour
+ *                       (double sums in point order, then float32) go to
+ *                       out_points / out_colors ([N, 3] capacity), in
+ *                       ascending (ix, iy, iz) order; *n_out (device) = the
+ *                       number of voxels.  ws: gs_voxel_workspace_bytes(N).
+ * gs_create_from_pcd:   create_from_pcd's raw parameters from points, colours
+ *                       and gs_knn3_meandist's dist2: xyz; features [P, n_coef,
+ *                       3] (row 0 = RGB2SH(colour), the rest 0); scaling =
+ *                       log(sqrt(max(dist2, 1e-7))) x3; rotation (1, 0, 0, 0);
+ *                       opacity = inverse_sigmoid(0.1). */
+int gs_depth_to_points(int W, int H, const float* depth, float fx, float fy, float cx, float cy, float* points,
+                       const void* colors_src, int colors_u8, long long color_pix_stride,
+                       long long color_chan_stride, float* colors, void* stream);
+size_t gs_bounds_workspace_bytes(void);
+int gs_points_bounds(const float* points, long long N, float* bounds6, void* ws, size_t ws_bytes, void* stream);
+size_t gs_voxel_workspace_bytes(long long N);
+int gs_voxel_down_sample(const float* points, long long N, const void* colors_src, int colors_u8,
+                         long long color_pix_stride, long long color_chan_stride, const double* voxel_min3,
+                         double voxel_size, int bits_x, int bits_y, int bits_z, float* out_points,
+                         float* out_colors, uint32_t* n_out, void* ws, size_t ws_bytes, void* stream);
+int gs_create_from_pcd(int P, int n_coef, const float* points, const float* colors, const float* dist2, float* xyz,
+                       float* features, float* scaling, float* rotation, float* opacity, void* stream);
+
 /* Fused photometric loss of the training step (replaces Loss.forward's
  * (1-l)*l1_loss + l*(1 - SSIM_V2) in torch, trainer/losses.py:98-136 and
  * :164-251; l = lambda_dssim = 0.2 at arguments/__init__.py:113).  pred and
