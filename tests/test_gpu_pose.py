@@ -106,6 +106,34 @@ def test_pose_tangent_gradient_through_autograd():
     torch.testing.assert_close(xr.grad.reshape(1, 6), via_kernel, rtol=2e-3, atol=2e-3 * float(via_kernel.abs().max()))
 
 
+def test_lietorch_compat_path_matches_fused_pose():
+    """The reference's unchanged code shape -- a LieGroupParameter and
+    ``P.retr().act(xyz)`` in torch feeding the drop-in rasteriser
+    (scene/gaussian_model_cf.py:123-133) -- renders the image of the fused-pose
+    path (pose applied inside preprocess) and delivers the same tangent
+    gradient."""
+    from diff_gaussian_rasterization import GaussianRasterizer
+    from lietorch import SE3, LieGroupParameter
+    print("lietorch compat vs fused pose", flush=True)
+    inst = build_instance(11, W=160, H=90)
+    xi = torch.tensor([0.01, -0.02, 0.015, 0.02, -0.01, 0.005], dtype=torch.float32)
+    P = LieGroupParameter(SE3(torch.tensor([0.0, 0, 0, 0, 0, 0, 1.0], device=DEV)[None]))
+    with torch.no_grad():
+        P.copy_(xi[None].to(DEV))
+    means = P.retr().act(inst.means.clone())
+    r = GaussianRasterizer(raster_settings=inst.settings)
+    color_a, _, _ = r(means3D=means, means2D=torch.zeros_like(means), opacities=inst.opacities, shs=inst.shs,
+                      scales=inst.scales, rotations=inst.rotations)
+    fused_photometric_loss(color_a, inst.target, 0.2, clamp01=True)[0].backward()
+    xr = xi.clone().to(DEV).requires_grad_(True)
+    color_b, _, _ = r(means3D=inst.means, means2D=torch.zeros_like(inst.means), opacities=inst.opacities,
+                      shs=inst.shs, scales=inst.scales, rotations=inst.rotations, pose=se3_exp(xr))
+    fused_photometric_loss(color_b, inst.target, 0.2, clamp01=True)[0].backward()
+    torch.testing.assert_close(color_a, color_b, rtol=0, atol=1e-5)
+    ga, gb = P.grad.reshape(6), xr.grad
+    torch.testing.assert_close(ga, gb, rtol=2e-3, atol=2e-3 * float(gb.abs().max()))
+
+
 def test_pose_tangent_gradient_direction_matches_fd():
     """Central differences of the rendered loss in the tangent (h = 1e-3).
     The renderer truncates every splat (alpha < 1/255, 3-sigma rect), and the
