@@ -313,10 +313,14 @@ __global__ __launch_bounds__(256) void k_ranges(const uint64_t* keys, const uint
 // records are read in order, so the duplicate (which visits Gaussians in
 // depth order, i.e. at random) reads one 8-byte record instead of the tile
 // count and two 16-byte splat vectors.
+// Also clears the n_ranges words of the tile ranges (the grid covers
+// max(P, n_ranges) threads).
 __global__ __launch_bounds__(256) void k_df_keys(int P, const uint32_t* tiles, const float4* splat, int gx, int gy,
-                                                 uint32_t* keys, uint32_t* vals, uint2* rect, uint32_t* counters) {
+                                                 uint32_t* keys, uint32_t* vals, uint2* rect, uint32_t* counters,
+                                                 uint32_t* ranges, int n_ranges) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i == 0) counters[2] = (uint32_t)P;
+  if (i < n_ranges) ranges[i] = 0u;
   if (i >= P) return;
   const bool vis = tiles[i] != 0u;
   keys[i] = vis ? __float_as_uint(splat[3 * i + 1].z) : 0xffffffffu;
@@ -906,10 +910,11 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
   char* bb = (char*)binning;
   char* ib = (char*)image;
   uint32_t* ranges = (uint32_t*)(ib + LI.ranges);
-  if (pre)
+  const bool df = capacity > 0 && P > 0 && f->binning == GS_BINNING_DEPTH_FIRST;
+  if (pre && !df)  // the depth-first schedule clears them in k_df_keys
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nblocks256(2LL * nt), 1024)), dim3(256), 0, s, ranges, 2LL * nt,
                        0u);
-  if (capacity > 0 && P > 0 && f->binning == GS_BINNING_DEPTH_FIRST) {
+  if (df) {
     if (gx > 0xffff || gy > 0xffff) return GS_E_ARG;  // tile rects are packed in 16-bit fields
     const float4* splat = (const float4*)(gb + LG.splat);
     const uint32_t* tiles = (const uint32_t*)(gb + LG.tiles);
@@ -921,8 +926,8 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
     if (pre) {
       {
         timing::Scope ts(timing::kSort, s);
-        hipLaunchKernelGGL(k_df_keys, dim3(nbp), dim3(256), 0, s, P, tiles, splat, gx, gy, dka, dva,
-                           (uint2*)(bb + LB.drect), counters);
+        hipLaunchKernelGGL(k_df_keys, dim3(std::max(nbp, nblocks256(2LL * nt))), dim3(256), 0, s, P, tiles, splat,
+                           gx, gy, dka, dva, (uint2*)(bb + LB.drect), counters, ranges, 2 * nt);
         e = sort_pairs_u32(dka, dva, (uint32_t*)(bb + LB.dkeys_b), (uint32_t*)(bb + LB.dvals_b), counters + 2, P,
                            32, bb + LB.hist, s, &in_b);
       }

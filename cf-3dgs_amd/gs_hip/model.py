@@ -107,6 +107,7 @@ class GaussianParams:
         self._opacity = p(opacity)
         self.active_sh_degree = sh_degree
         self.optimizer = None
+        self._screen_zeros = None
 
     @property
     def _features_dc(self):
@@ -135,6 +136,20 @@ class GaussianParams:
     @property
     def get_features(self):
         return self._features
+
+    def screenspace_leaf(self) -> torch.Tensor:
+        """A fresh zero [P, 3] leaf for the rasteriser's means2D input, whose
+        .grad receives dL/dmean2D.  The reference makes it as
+        ``zeros_like(xyz, requires_grad=True) + 0`` plus ``retain_grad()``
+        (scene/gaussian_model_cf.py:793-805).  That costs a fill, an add and
+        a clone of the gradient every step.  Here every step gets a new leaf
+        over one zero buffer, which is never written: the rasteriser only
+        routes the gradient through it."""
+        xyz = self._xyz
+        z = self._screen_zeros
+        if z is None or z.shape != xyz.shape or z.device != xyz.device:
+            z = self._screen_zeros = torch.zeros_like(xyz)
+        return z.detach().requires_grad_(True)
 
     def get_activated(self):
         """(scales, rotations, opacities) in one fused launch."""
@@ -165,16 +180,25 @@ def render(gaussians: GaussianParams, cam: Camera, bg: torch.Tensor, pose=None, 
     clamp, for a loss that applies the clamp itself (the fused loss's
     ``clamp01``)."""
     means3D = gaussians.get_xyz
-    screenspace = torch.zeros_like(means3D, requires_grad=True) + 0
-    try:
-        screenspace.retain_grad()
-    except RuntimeError:
-        pass
+    screenspace = gaussians.screenspace_leaf()
     rs = raster_settings(cam, bg, gaussians.active_sh_degree, scaling_modifier)
     rasterizer = GaussianRasterizer(raster_settings=rs)
     scales, rotations, opacities = gaussians.get_activated()
     color, radii, depth = rasterizer(means3D=means3D, means2D=screenspace, shs=gaussians.get_features,
                                      colors_precomp=None, opacities=opacities, scales=scales, rotations=rotations,
                                      cov3D_precomp=None, pose=pose)
-    return {"image": color.clamp(0, 1) if clamp else color, "depth": depth, "viewspace_points": screenspace,
-            "visibility_filter": radii > 0, "radii": radii}
+    return _RenderOut(image=color.clamp(0, 1) if clamp else color, depth=depth, viewspace_points=screenspace,
+                      radii=radii)
+
+
+class _RenderOut(dict):
+    """The render dict of CF3DGS_Render.render; "visibility_filter"
+    (radii > 0) is computed on first access, so a step that never reads it
+    launches no compare kernel."""
+
+    def __missing__(self, key):
+        if key == "visibility_filter":
+            v = self["radii"] > 0
+            self[key] = v
+            return v
+        raise KeyError(key)

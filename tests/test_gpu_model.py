@@ -146,6 +146,7 @@ def test_training_step_gradients_match_reference_composition():
         from diff_gaussian_rasterization import GaussianRasterizer
         from gs_hip.camera import raster_settings
         means2D = torch.zeros_like(leaves["xyz"], requires_grad=True) + 0
+        means2D.retain_grad()  # scene/gaussian_model_cf.py:793-805
         rast = GaussianRasterizer(raster_settings=raster_settings(cam, bg, sc["sh_degree"]))
         color, _, _ = rast(means3D=leaves["xyz"], means2D=means2D,
                            shs=torch.cat((leaves["features_dc"], leaves["features_rest"]), dim=1),
@@ -168,4 +169,10 @@ def test_training_step_gradients_match_reference_composition():
     close(gp._scaling.grad, leaves["scaling"].grad, "scaling")
     close(gp._rotation.grad, leaves["rotation"].grad, "rotation")
     close(gp._opacity.grad, leaves["opacity"].grad, "opacity")
+    # the screen-space leaf carries dL/dmean2D as the reference's retained grad does,
+    # and its zero buffer is left untouched for the next step
+    close(out["viewspace_points"].grad, means2D.grad, "means2D")
+    assert out["viewspace_points"].is_leaf and not torch.any(gp._screen_zeros)
+    assert torch.equal(out["visibility_filter"], out["radii"] > 0)
+    assert gp.screenspace_leaf() is not out["viewspace_points"]
     assert torch.equal(gp._features_dc, gp._features[:, :1]) and gp._features_rest.shape[1] == 15
