@@ -33,10 +33,12 @@ HBM_PEAK_GBS = 8000.0
 
 
 def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, passes: int = 6,
-                depth_first: bool = True, one_word: bool = False) -> float:
+                depth_first: bool = True, one_word: bool = False, adam_in_backward: bool = False) -> float:
     """Algorithmic HBM bytes of one step's worth of a stage (the bytes the
     algorithm must move, independent of how the kernels move them).
-    one_word: depth-first list entries are single words tile << id_bits | id."""
+    one_word: depth-first list entries are single words tile << id_bits | id.
+    adam_in_backward: the preprocess backward applies the means' and SH
+    coefficients' Adam step instead of writing their gradients."""
     hw = W * H
     entry = 4 if one_word else 8  # bytes per list entry (tile id + Gaussian id)
     if stage == "preprocess":   # in: xyz 12, scale 12, rot 16, opac 4, SH 12M; out: radii 4, splat 48, cov 24, tiles 4, scan 4
@@ -56,6 +58,8 @@ def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, pas
     if stage == "render_bwd":   # per list entry: id 4 + splat 48 + 40 B of gradient; per pixel in: 5 cotangents + T + n
         return 92 * n_dup + 28 * hw
     if stage == "preprocess_bwd":  # in: render grads 48, splat 16, cov 24, radii 4, xyz 12, scale 12, rot 16, SH 12M
+        if adam_in_backward:       # no xyz / SH gradients out; their moments in, parameters and moments out
+            return P * (132 + 12 * M + 92 - 12) + P * (3 + 3 * M) * 4 * (2 + 3)
         return P * (132 + 12 * M + 92 + 12 * M)
     return 0.0
 
@@ -354,6 +358,8 @@ def main():
     ap.add_argument("--pose-instances", type=int, default=64)
     ap.add_argument("--pose-quick", action="store_true", help="time only the batched pose solves (profiling runs)")
     ap.add_argument("--torch-loss", action="store_true", help="loss with torch conv2d windows instead of the fused kernel")
+    ap.add_argument("--no-adam-in-backward", action="store_true",
+                    help="write the means' and SH gradients and step them in the optimiser launch")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy 5 times (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -382,7 +388,7 @@ def main():
 
     def step():
         # the fused loss applies the image's [0, 1] clamp (and its gradient mask) itself
-        out = render(g, cam, bg, clamp=args.torch_loss)
+        out = render(g, cam, bg, clamp=args.torch_loss, adam_in_backward=not args.no_adam_in_backward)
         if args.torch_loss:
             loss = photometric_loss(out["image"], target)
         else:
@@ -428,7 +434,8 @@ def main():
     L = _lib.binning_layout(n_dup, args.P, args.width, args.height, raster.BINNING)
     for s, d in stages.items():
         b = stage_bytes(s, args.P, n_dup, args.width, args.height, passes=L.sort_passes,
-                        depth_first=raster.BINNING == _lib.BINNING_DEPTH_FIRST, one_word=L.id_bits > 0)
+                        depth_first=raster.BINNING == _lib.BINNING_DEPTH_FIRST, one_word=L.id_bits > 0,
+                        adam_in_backward=not args.no_adam_in_backward)
         d["alg_bytes"] = b
         d["GB_s"] = b / (d["ms_per_step"] * 1e-3) / 1e9 if d["ms_per_step"] > 0 else 0.0
     dom = max(stages, key=lambda s: stages[s]["ms_per_step"]) if stages else None
@@ -452,7 +459,7 @@ def main():
         "config": {"workload": "full-scene training step: 500k Gaussians SH3, 32 clusters, 960x540, "
                                "fwd colour+depth, 0.8 L1 + 0.2 (1-SSIM), full bwd, Adam",
                    "P": args.P, "width": args.width, "height": args.height, "n_dup": n_dup,
-                   "parallelism": "single GPU"},
+                   "adam_in_backward": not args.no_adam_in_backward, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_10k_256": c1,

@@ -38,24 +38,6 @@ struct AdamArgs {
   float b1, b2, eps, inv_sqrt_bc2;
 };
 
-__device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, float b1, float b2, float eps,
-                                         float inv_sqrt_bc2, float step) {
-  m = fmaf(g - m, 1.0f - b1, m);
-  v = fmaf(v, b2, (1.0f - b2) * g * g);
-  const float denom = sqrtf(v) * inv_sqrt_bc2 + eps;
-  p = p - step * (m / denom);
-}
-
-typedef float nt4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ float4 ld_nt(const float* p) {
-  const nt4 v = __builtin_nontemporal_load(reinterpret_cast<const nt4*>(p));
-  return make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ void st_nt(float* p, const float4& v) {
-  const nt4 w = {v.x, v.y, v.z, v.w};
-  __builtin_nontemporal_store(w, reinterpret_cast<nt4*>(p));
-}
-
 __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
   const int c = blockIdx.x;
   int t = 0;

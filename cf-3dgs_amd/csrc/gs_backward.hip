@@ -31,6 +31,8 @@
 //   k_pose_finalize  sums the wave partials in a fixed tree order, then
 //                    d/dR, d/dt -> d/d(q, t) through the quaternion
 //                    normalisation.
+#include <cmath>
+
 #include "gs_common.hpp"
 #include "gs_internal.hpp"
 
@@ -420,7 +422,89 @@ struct PBArgs {
   float* dL_dscales;
   float* dL_drots;
   float* pose_part;  // [4 * gridDim.x][12] per-wave sums: dL/dt (3), dL/dR row-major (9)
+  // gs_backward_adam: one Adam step of the means and SH coefficients instead
+  // of writing their gradients (adam_on == 0 otherwise)
+  int adam_on;
+  float* means_p;
+  float* means_m;
+  float* means_v;
+  float* shs_p;
+  float* shs_m;
+  float* shs_v;
+  float means_step, shs_step, shs_lead_step;
+  int shs_lead;
+  float b1, b2, eps, inv_sqrt_bc2;
 };
+
+// The fused Adam step of one Gaussian's mean (gradient gp).
+__device__ __forceinline__ void means_adam(const PBArgs& a, int i, const float gp[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t e = 3 * (size_t)i + k;
+    float p = a.means_p[e], m = a.means_m[e], v = a.means_v[e];
+    adam_one(p, gp[k], m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.means_step);
+    a.means_p[e] = p;
+    a.means_m[e] = m;
+    a.means_v[e] = v;
+  }
+}
+
+// The fused Adam step of the workgroup's SH rows [base, base + nloc): the
+// gradients are in LDS (s_g, row floats per Gaussian), parameters and moments
+// in global memory.  Element r of a row steps with shs_lead_step when
+// r < shs_lead.  Two float4 per thread are loaded before either is updated.
+__device__ __forceinline__ void block_adam_rows(const PBArgs& a, const float* s_g, int base, int nloc, int row) {
+  const size_t off = (size_t)base * row;
+  float* P = a.shs_p + off;
+  float* M = a.shs_m + off;
+  float* V = a.shs_v + off;
+  const int total = nloc * row;
+  auto upd4 = [&](float4& p, const float4& g, float4& m, float4& v, int e) {
+    const int r = e % row;
+    const float s0 = r < a.shs_lead ? a.shs_lead_step : a.shs_step;
+    const float s1 = r + 1 < a.shs_lead ? a.shs_lead_step : a.shs_step;
+    const float s2 = r + 2 < a.shs_lead ? a.shs_lead_step : a.shs_step;
+    const float s3 = r + 3 < a.shs_lead ? a.shs_lead_step : a.shs_step;
+    adam_one(p.x, g.x, m.x, v.x, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s0);
+    adam_one(p.y, g.y, m.y, v.y, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s1);
+    adam_one(p.z, g.z, m.z, v.z, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s2);
+    adam_one(p.w, g.w, m.w, v.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s3);
+  };
+  if ((row & 3) == 0) {
+    const int n4 = total >> 2;
+    const float4* g4 = reinterpret_cast<const float4*>(s_g);
+    int k = threadIdx.x;
+    for (; k + 256 < n4; k += 512) {
+      float4 p0 = ld_nt(P + 4 * k), p1 = ld_nt(P + 4 * (k + 256));
+      float4 m0 = ld_nt(M + 4 * k), m1 = ld_nt(M + 4 * (k + 256));
+      float4 v0 = ld_nt(V + 4 * k), v1 = ld_nt(V + 4 * (k + 256));
+      upd4(p0, g4[k], m0, v0, 4 * k);
+      upd4(p1, g4[k + 256], m1, v1, 4 * (k + 256));
+      st_nt(P + 4 * k, p0);
+      st_nt(P + 4 * (k + 256), p1);
+      st_nt(M + 4 * k, m0);
+      st_nt(M + 4 * (k + 256), m1);
+      st_nt(V + 4 * k, v0);
+      st_nt(V + 4 * (k + 256), v1);
+    }
+    if (k < n4) {
+      float4 p = ld_nt(P + 4 * k), m = ld_nt(M + 4 * k), v = ld_nt(V + 4 * k);
+      upd4(p, g4[k], m, v, 4 * k);
+      st_nt(P + 4 * k, p);
+      st_nt(M + 4 * k, m);
+      st_nt(V + 4 * k, v);
+    }
+  } else {
+    for (int k = threadIdx.x; k < total; k += 256) {
+      float p = P[k], m = M[k], v = V[k];
+      adam_one(p, s_g[k], m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2,
+               k % row < a.shs_lead ? a.shs_lead_step : a.shs_step);
+      P[k] = p;
+      M[k] = m;
+      V[k] = v;
+    }
+  }
+}
 
 __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
   // [256][3M] SH rows of this workgroup: coefficients are read from and their
@@ -430,7 +514,8 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int base = blockIdx.x * 256;
   const int nloc = min(256, a.P - base);
-  const bool staged = a.shs && a.dL_dsh && a.M <= 16;
+  const bool sh_grad = a.shs && (a.dL_dsh || a.adam_on);
+  const bool staged = sh_grad && a.M <= 16;
   const int row = 3 * a.M;
   if (staged) {
     const float* src = a.shs + (size_t)base * row;
@@ -543,7 +628,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) gp[k] += a.view[4 * k + 2] * rC.y;
       // ---- SH colour (at degree 0 the colour does not depend on the mean)
-      if (a.shs && (a.dL_dsh || a.D > 0)) {
+      if (a.shs && (sh_grad || a.D > 0)) {
         const float cbits = a.splat[3 * i + 1].w;
         const uint32_t cm = __float_as_uint(cbits);
         const float gc[3] = {(cm & 1u) ? 0.f : rB.z, (cm & 2u) ? 0.f : rB.w, (cm & 4u) ? 0.f : rC.x};
@@ -554,7 +639,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
         sh_basis(a.D, dx, dy, dz, b);
         sh_basis_grad(a.D, dx, dy, dz, bg3);
         const float* sh = staged ? s_sh + threadIdx.x * row : a.shs + (size_t)i * row;
-        float* gsh = !a.dL_dsh ? nullptr : (staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row);
+        float* gsh = !sh_grad ? nullptr : (staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row);
         float gdir0 = 0.f, gdir1 = 0.f, gdir2 = 0.f;
         // unrolled to the 16 coefficients of degree 3, so b / bg3 are indexed
         // by constants and live in registers (a runtime bound puts them in scratch)
@@ -622,14 +707,21 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
 #pragma unroll
           for (int k = 0; k < 3; ++k)
             a.dL_dmeans3D[3 * i + k] = ps.R[k] * gp[0] + ps.R[3 + k] * gp[1] + ps.R[6 + k] * gp[2];
+      } else if (a.adam_on) {
+        means_adam(a, i, gp);
       } else if (a.dL_dmeans3D) {
         a.dL_dmeans3D[3 * i] = gp[0];
         a.dL_dmeans3D[3 * i + 1] = gp[1];
         a.dL_dmeans3D[3 * i + 2] = gp[2];
       }
     } else {
-      if (a.dL_dmeans3D) a.dL_dmeans3D[3 * i] = a.dL_dmeans3D[3 * i + 1] = a.dL_dmeans3D[3 * i + 2] = 0.f;
-      if (a.shs && a.dL_dsh) {
+      if (a.adam_on) {
+        const float g0[3] = {0.f, 0.f, 0.f};
+        means_adam(a, i, g0);
+      } else if (a.dL_dmeans3D) {
+        a.dL_dmeans3D[3 * i] = a.dL_dmeans3D[3 * i + 1] = a.dL_dmeans3D[3 * i + 2] = 0.f;
+      }
+      if (sh_grad) {
         float* gsh = staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row;
         for (int k = 0; k < row; ++k) gsh[k] = 0.f;
       }
@@ -640,7 +732,10 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * i + k] = gS[k];
   }
-  if (staged) {
+  if (staged && a.adam_on) {
+    __syncthreads();
+    block_adam_rows(a, s_sh, base, nloc, row);
+  } else if (staged) {
     __syncthreads();
     float* dst = a.dL_dsh + (size_t)base * row;
     const int total = nloc * row;
@@ -830,9 +925,17 @@ static void bwd_render(const gs_frame_t* f, int P, long long capacity, const RBA
 }
 
 // Phase 3: per-Gaussian gradients and the pose gradient.
+// Host-side values of a fused Adam step (gs_backward_adam), checked.
+struct AdamFusedHost {
+  float* means_p;
+  float* shs_p;
+  const gs_adam_fused_t* in;
+  float means_step, shs_step, shs_lead_step, inv_sqrt_bc2;
+};
+
 static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
                        long long capacity, const uint32_t* counters, const gs_grads_t* out, const RBArgs& r,
-                       hipStream_t s) {
+                       hipStream_t s, const AdamFusedHost* ad = nullptr) {
   const int P = g->P;
   gs_geom_layout_t LG;
   gs_geom_layout(P, &LG);
@@ -876,8 +979,24 @@ static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* 
     b.dL_dscales = out->dL_dscales;
     b.dL_drots = out->dL_drotations;
     b.pose_part = pose_part;
+    b.adam_on = ad ? 1 : 0;
+    b.means_p = ad ? ad->means_p : nullptr;
+    b.means_m = ad ? ad->in->means_m : nullptr;
+    b.means_v = ad ? ad->in->means_v : nullptr;
+    b.shs_p = ad ? ad->shs_p : nullptr;
+    b.shs_m = ad ? ad->in->shs_m : nullptr;
+    b.shs_v = ad ? ad->in->shs_v : nullptr;
+    b.means_step = ad ? ad->means_step : 0.f;
+    b.shs_step = ad ? ad->shs_step : 0.f;
+    b.shs_lead_step = ad ? ad->shs_lead_step : 0.f;
+    b.shs_lead = ad ? ad->in->shs_lead : 0;
+    b.b1 = ad ? ad->in->beta1 : 0.f;
+    b.b2 = ad ? ad->in->beta2 : 0.f;
+    b.eps = ad ? ad->in->eps : 0.f;
+    b.inv_sqrt_bc2 = ad ? ad->inv_sqrt_bc2 : 0.f;
     timing::Scope ts(timing::kPreprocessBwd, s);
-    const size_t shm = (g->shs && out->dL_dsh && g->M <= 16) ? (size_t)256 * 3 * g->M * sizeof(float) : 0;
+    const bool sh_grad = g->shs && (out->dL_dsh || ad);
+    const size_t shm = (sh_grad && g->M <= 16) ? (size_t)256 * 3 * g->M * sizeof(float) : 0;
     hipLaunchKernelGGL(k_preprocess_bwd, dim3((P + 255) / 256), dim3(256), shm, s, b);
   }
   // (P = 0: no wave partials, the pose gradient is zero)
@@ -901,6 +1020,43 @@ extern "C" int gs_backward(GS_BWD_ARGS) {
   bwd_prepare(f, g->P, capacity, r, s);
   bwd_render(f, g->P, capacity, r, s);
   bwd_finish(f, g, radii, geom, capacity, counters, out, r, s);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_backward_adam(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
+                                void* binning, long long capacity, const void* image, const uint32_t* counters,
+                                const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
+                                const gs_grads_t* out, const gs_adam_fused_t* adam, void* scratch,
+                                size_t scratch_bytes, void* stream) {
+  const int rc = bwd_check(f, g, radii, geom, binning, capacity, image, counters, dL_dcolor, out, scratch,
+                           scratch_bytes);
+  if (rc != GS_OK) return rc;
+  if (!adam || f->pose || out->dL_dmeans3D || out->dL_dsh || adam->step < 1) return GS_E_ARG;
+  const int P = g->P;
+  if (P > 0 && (!g->means3D || !adam->means_m || !adam->means_v)) return GS_E_ARG;
+  if (P > 0 && g->shs && (g->M > 16 || !adam->shs_m || !adam->shs_v || adam->shs_lead < 0 ||
+                          adam->shs_lead > 3 * g->M))
+    return GS_E_ARG;
+  const uintptr_t al = (uintptr_t)g->means3D | (uintptr_t)adam->means_m | (uintptr_t)adam->means_v |
+                       (uintptr_t)(g->shs ? g->shs : nullptr) | (uintptr_t)(g->shs ? adam->shs_m : nullptr) |
+                       (uintptr_t)(g->shs ? adam->shs_v : nullptr);
+  if (al & 15) return GS_E_ARG;
+  // the bias corrections as gs_adam_step_rows computes them
+  const double bc1 = 1.0 - std::pow((double)adam->beta1, (double)adam->step);
+  const double bc2 = 1.0 - std::pow((double)adam->beta2, (double)adam->step);
+  AdamFusedHost ad;
+  ad.in = adam;
+  ad.means_p = const_cast<float*>(g->means3D);  // updated in place
+  ad.shs_p = const_cast<float*>(g->shs);
+  ad.means_step = (float)(adam->means_lr / bc1);
+  ad.shs_step = (float)(adam->shs_lr / bc1);
+  ad.shs_lead_step = (float)(adam->shs_lead_lr / bc1);
+  ad.inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
+  hipStream_t s = (hipStream_t)stream;
+  const RBArgs r = make_rb_args(f, P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
+  bwd_prepare(f, P, capacity, r, s);
+  bwd_render(f, P, capacity, r, s);
+  bwd_finish(f, g, radii, geom, capacity, counters, out, r, s, &ad);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 

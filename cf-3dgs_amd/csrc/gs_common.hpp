@@ -352,6 +352,30 @@ __device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* s_w, u
   return x + before;
 }
 
+// One Adam element update in torch.optim.Adam's order (shared by the Adam
+// kernel and the backward's fused update, so both round identically):
+//   m = m + (g - m) * (1 - b1);  v = v * b2 + (1 - b2) * g * g;
+//   p = p - step * m / (sqrt(v) / sqrt(bc2) + eps),  step = lr / bc1.
+__device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, float b1, float b2, float eps,
+                                         float inv_sqrt_bc2, float step) {
+  m = fmaf(g - m, 1.0f - b1, m);
+  v = fmaf(v, b2, (1.0f - b2) * g * g);
+  const float denom = sqrtf(v) * inv_sqrt_bc2 + eps;
+  p = p - step * (m / denom);
+}
+
+// 16-byte loads and stores that bypass cache retention (each byte of an
+// optimiser pass is touched once).
+typedef float nt4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_nt(const float* p) {
+  const nt4 v = __builtin_nontemporal_load(reinterpret_cast<const nt4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_nt(float* p, const float4& v) {
+  const nt4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<nt4*>(p));
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const uint32_t lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));

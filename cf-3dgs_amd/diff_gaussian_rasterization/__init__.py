@@ -20,7 +20,10 @@ densification, :716-719), shs / colors_precomp, opacities, scales /
 rotations / cov3D_precomp.  The optional ``pose`` keyword ([tx, ty, tz, qx,
 qy, qz, qw]) applies the rigid transform that CF-3DGS performs with
 ``P[0].retr().act(xyz)`` (:123-133) inside the kernel and returns its
-gradient.
+gradient.  The optional ``adam_in_backward`` keyword
+(``gs_hip.optim.FusedAdam.in_backward(xyz, shs)``) makes the backward apply
+the optimiser's Adam step to means3D and shs itself, in place, instead of
+returning their gradients (the step after it then skips them).
 
 All work runs in libgs_hip.so on the GPU; there is no CPU path.
 """
@@ -68,7 +71,7 @@ def _none_if_empty(t):
 class _RasterizeGaussians(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means3D, means2D, sh, colors_precomp, opacities, scales, rotations, cov3Ds_precomp, pose,
-                raster_settings, return_alpha):
+                raster_settings, return_alpha, adam_in_backward=None):
         sh_ = _none_if_empty(sh)
         col_ = _none_if_empty(colors_precomp)
         sc_ = _none_if_empty(scales)
@@ -81,6 +84,7 @@ class _RasterizeGaussians(torch.autograd.Function):
             torch.cuda.synchronize(means3D.device)
         ctx.raster_settings = raster_settings
         ctx.state = st
+        ctx.adam = adam_in_backward
         ctx.has = (sh_ is not None, col_ is not None, sc_ is not None, ro_ is not None, cv_ is not None,
                    po_ is not None)
         ctx.save_for_backward(means3D, sh, colors_precomp, opacities, scales, rotations, cov3Ds_precomp, pose)
@@ -101,11 +105,14 @@ class _RasterizeGaussians(torch.autograd.Function):
         need = ctx.needs_input_grad
         names = ("means3D", "means2D", "shs", "colors", "opacities", "scales", "rotations", "cov3D", "pose")
         want = {n for n, k in zip(names, need[:9]) if k}
+        # adam_in_backward: the means and SH coefficients take their optimiser
+        # step here and pass no gradient on
+        adam = ctx.adam.take() if ctx.adam is not None and grad_color is not None else None
         g = _raster.backward(
             ctx.raster_settings, st, means3D,
             sh if has_sh else None, colors_precomp if has_col else None, opacities,
             scales if has_sc else None, rotations if has_ro else None, cov3Ds_precomp if has_cv else None,
-            pose if has_pose else None, grad_color, grad_depth, grad_alpha, want=want)
+            pose if has_pose else None, grad_color, grad_depth, grad_alpha, want=want, adam=adam)
         if ctx.raster_settings.debug:
             torch.cuda.synchronize(means3D.device)
 
@@ -123,11 +130,11 @@ class _RasterizeGaussians(torch.autograd.Function):
         grad_cv = shaped("cov3D", cov3Ds_precomp) if has_cv else None
         grad_pose = shaped("pose", pose) if has_pose else None
         return (grad_means3D, grad_means2D, grad_sh, grad_col, grad_opac, grad_sc, grad_ro, grad_cv, grad_pose,
-                None, None)
+                None, None, None)
 
 
 def rasterize_gaussians(means3D, means2D, sh, colors_precomp, opacities, scales, rotations, cov3Ds_precomp,
-                        raster_settings, pose=None, return_alpha=False):
+                        raster_settings, pose=None, return_alpha=False, adam_in_backward=None):
     empty = torch.empty(0, device=means3D.device)
     return _RasterizeGaussians.apply(
         means3D, means2D,
@@ -138,7 +145,7 @@ def rasterize_gaussians(means3D, means2D, sh, colors_precomp, opacities, scales,
         rotations if rotations is not None else empty,
         cov3Ds_precomp if cov3Ds_precomp is not None else empty,
         pose if pose is not None else empty,
-        raster_settings, bool(return_alpha))
+        raster_settings, bool(return_alpha), adam_in_backward)
 
 
 class GaussianRasterizer(nn.Module):
@@ -153,7 +160,7 @@ class GaussianRasterizer(nn.Module):
             return _raster.mark_visible(positions, rs.viewmatrix, rs.projmatrix)
 
     def forward(self, means3D, means2D, opacities, shs=None, colors_precomp=None, scales=None, rotations=None,
-                cov3D_precomp=None, pose=None):
+                cov3D_precomp=None, pose=None, adam_in_backward=None):
         rs = self.raster_settings
         if (shs is None and colors_precomp is None) or (shs is not None and colors_precomp is not None):
             raise Exception('Please provide excatly one of either SHs or precomputed colors!')
@@ -162,5 +169,8 @@ class GaussianRasterizer(nn.Module):
             raise Exception('Please provide exactly one of either scale/rotation pair or precomputed 3D covariance!')
         if means3D.dim() != 2 or means3D.shape[1] != 3:
             raise Exception('means3D must have dimensions (num_points, 3)')
+        if adam_in_backward is not None and (pose is not None or shs is None):
+            raise Exception('adam_in_backward needs SH colours and no pose')
         return rasterize_gaussians(means3D, means2D, shs, colors_precomp, opacities, scales, rotations,
-                                   cov3D_precomp, rs, pose=pose, return_alpha=self.return_alpha)
+                                   cov3D_precomp, rs, pose=pose, return_alpha=self.return_alpha,
+                                   adam_in_backward=adam_in_backward)

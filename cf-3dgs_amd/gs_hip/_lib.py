@@ -53,6 +53,14 @@ class GsGrads(ctypes.Structure):
     ]
 
 
+class GsAdamFused(ctypes.Structure):
+    _fields_ = [
+        ("means_m", c_void_p), ("means_v", c_void_p), ("means_lr", c_float),
+        ("shs_m", c_void_p), ("shs_v", c_void_p), ("shs_lr", c_float), ("shs_lead_lr", c_float),
+        ("shs_lead", c_int), ("beta1", c_float), ("beta2", c_float), ("eps", c_float), ("step", c_int),
+    ]
+
+
 class GsGeomLayout(ctypes.Structure):
     _fields_ = [("splat", c_size_t), ("cov3D", c_size_t), ("tiles", c_size_t), ("offsets", c_size_t),
                 ("scan_tmp", c_size_t), ("total", c_size_t)]
@@ -88,6 +96,10 @@ EXPORTS = {
     "gs_backward": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p, c_void_p,
                             c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(GsGrads),
                             c_void_p, c_size_t, c_void_p]),
+    "gs_backward_adam": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians), c_void_p, c_void_p,
+                                 c_void_p, c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 ctypes.POINTER(GsGrads), ctypes.POINTER(GsAdamFused), c_void_p, c_size_t,
+                                 c_void_p]),
     "gs_render_binning": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong, c_void_p,
                                   c_size_t, c_void_p, c_size_t, c_void_p]),
     "gs_multi_blend_bytes": (c_size_t, [c_int]),

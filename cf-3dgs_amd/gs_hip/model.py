@@ -174,19 +174,27 @@ class GaussianParams:
 
 
 def render(gaussians: GaussianParams, cam: Camera, bg: torch.Tensor, pose=None, scaling_modifier: float = 1.0,
-           clamp: bool = True):
+           clamp: bool = True, adam_in_backward: bool = False):
     """CF3DGS_Render.render with the rasteriser's 3-tuple return.  With
     ``clamp=False`` "image" is the rasteriser's colour before the [0, 1]
     clamp, for a loss that applies the clamp itself (the fused loss's
-    ``clamp01``)."""
+    ``clamp01``).  With ``adam_in_backward`` the backward applies the
+    ``training_setup`` optimiser's step to the means and SH coefficients
+    itself (their gradients are never written); ``optimizer.step()`` then
+    updates the rest."""
     means3D = gaussians.get_xyz
     screenspace = gaussians.screenspace_leaf()
     rs = raster_settings(cam, bg, gaussians.active_sh_degree, scaling_modifier)
     rasterizer = GaussianRasterizer(raster_settings=rs)
     scales, rotations, opacities = gaussians.get_activated()
+    fused = None
+    if adam_in_backward:
+        if gaussians.optimizer is None:
+            raise RuntimeError("adam_in_backward needs training_setup() first")
+        fused = gaussians.optimizer.in_backward(gaussians._xyz, gaussians._features)
     color, radii, depth = rasterizer(means3D=means3D, means2D=screenspace, shs=gaussians.get_features,
                                      colors_precomp=None, opacities=opacities, scales=scales, rotations=rotations,
-                                     cov3D_precomp=None, pose=pose)
+                                     cov3D_precomp=None, pose=pose, adam_in_backward=fused)
     return _RenderOut(image=color.clamp(0, 1) if clamp else color, depth=depth, viewspace_points=screenspace,
                       radii=radii)
 

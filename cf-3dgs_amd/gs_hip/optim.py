@@ -26,6 +26,28 @@ class FusedAdam(torch.optim.Optimizer):
         super().__init__(params, dict(lr=lr, betas=betas, eps=eps))
         self._steps = 0
 
+    def _group_of(self, p):
+        for group in self.param_groups:
+            if any(q is p for q in group["params"]):
+                return group
+        raise ValueError("parameter is not managed by this optimiser")
+
+    def _state_of(self, p):
+        st = self.state[p]
+        if not st:
+            st["step"] = 0
+            st["exp_avg"] = torch.zeros_like(p)
+            st["exp_avg_sq"] = torch.zeros_like(p)
+        return st
+
+    def in_backward(self, means: torch.Tensor, shs: torch.Tensor) -> "_InBackward":
+        """The Adam step of ``means`` [P,3] and ``shs`` [P,M,3] for the
+        rasteriser's backward to apply itself (``adam_in_backward=``): the
+        backward then never writes their gradients.  The update is this
+        optimiser's, bit for bit; ``step()`` afterwards skips both, since they
+        have no gradient."""
+        return _InBackward(self, means, shs)
+
     @torch.no_grad()
     def step(self, closure=None):
         loss = closure() if closure is not None else None
@@ -72,3 +94,34 @@ class FusedAdam(torch.optim.Optimizer):
                 _lib.check(lib.gs_adam_step_rows(n, P, G, M, V, N, LR, LLR, LEAD, ROW, b1, b2, eps, step, stream),
                            "gs_adam_step_rows")
         return loss
+
+
+class _InBackward:
+    """One pending fused step of (means, shs); ``take()`` advances their step
+    counts and returns the gs_adam_fused_t fields."""
+
+    def __init__(self, opt: FusedAdam, means: torch.Tensor, shs: torch.Tensor):
+        self.opt, self.means, self.shs = opt, means, shs
+
+    def take(self) -> dict:
+        opt = self.opt
+        gm, gs = opt._group_of(self.means), opt._group_of(self.shs)
+        for p in (self.means, self.shs):
+            if p.device.type != "cuda" or p.dtype != torch.float32 or not p.is_contiguous() or p.data_ptr() % 16:
+                raise RuntimeError("adam in backward needs contiguous, 16-byte aligned float32 GPU parameters")
+        if gm["betas"] != gs["betas"] or gm["eps"] != gs["eps"]:
+            raise RuntimeError("adam in backward: means and shs must share betas and eps")
+        sm, ss = opt._state_of(self.means), opt._state_of(self.shs)
+        if sm["step"] != ss["step"]:
+            raise RuntimeError("adam in backward: means and shs have taken different numbers of steps")
+        sm["step"] += 1
+        ss["step"] += 1
+        row = int(gs.get("row", 0))
+        lead = int(gs.get("lead", 0)) if row else 0
+        if row and row != self.shs[0].numel():
+            raise RuntimeError("adam in backward: the SH group's row must be one Gaussian's coefficients")
+        b1, b2 = gm["betas"]
+        return dict(means_m=sm["exp_avg"].data_ptr(), means_v=sm["exp_avg_sq"].data_ptr(), means_lr=float(gm["lr"]),
+                    shs_m=ss["exp_avg"].data_ptr(), shs_v=ss["exp_avg_sq"].data_ptr(), shs_lr=float(gs["lr"]),
+                    shs_lead_lr=float(gs.get("lead_lr", gs["lr"])) if row else float(gs["lr"]),
+                    shs_lead=lead, beta1=float(b1), beta2=float(b2), eps=float(gm["eps"]), step=int(sm["step"]))

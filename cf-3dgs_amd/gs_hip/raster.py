@@ -270,9 +270,12 @@ def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations
 
 def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities, scales, rotations,
              cov3D_precomp, pose, grad_color, grad_depth=None, grad_alpha=None, want=None,
-             deterministic: Optional[bool] = None):
+             deterministic: Optional[bool] = None, adam=None):
     """Returns a dict of gradients (None where the input was absent or, when
-    `want` (a set of names) is given, not wanted)."""
+    `want` (a set of names) is given, not wanted).  With ``adam`` (a dict of
+    the gs_adam_fused_t fields, ``FusedAdam.in_backward(...).take()``) the
+    means and SH coefficients take one Adam step inside the backward
+    (gs_backward_adam, updated in place) and get no gradient."""
     lib = _lib.load()
     device = require_device(means3D)
     fa = make_frame(settings, device, pose, st.binning_mode, deterministic)
@@ -280,6 +283,9 @@ def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities
     P = ga.g.P
     M = ga.g.M
     stream = _stream(device)
+    if adam is not None:  # these two take their step in place and get no gradient array
+        every = {"means2D", "colors", "opacities", "means3D", "cov3D", "shs", "scales", "rotations", "pose"}
+        want = (set(want) if want is not None else every) - {"means3D", "shs"}
     dc = _dev_f32(grad_color, device, "grad_color")
     dd = _dev_f32(grad_depth, device, "grad_depth")
     da = _dev_f32(grad_alpha, device, "grad_alpha")
@@ -320,9 +326,24 @@ def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities
     gr.dL_dpose = ptr("pose")
     scratch = torch.empty(lib.gs_backward_scratch_bytes(P, st.capacity, fa.frame.deterministic), dtype=torch.uint8,
                           device=device)
-    _lib.check(lib.gs_backward(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(st.radii), _ptr(st.geom),
-                               _ptr(st.binning), st.capacity, _ptr(st.image), _ptr(st.counters), _ptr(dc), _ptr(dd),
-                               _ptr(da), ctypes.byref(gr), _ptr(scratch), scratch.numel(), stream), "gs_backward")
+    if adam is None:
+        _lib.check(lib.gs_backward(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(st.radii), _ptr(st.geom),
+                                   _ptr(st.binning), st.capacity, _ptr(st.image), _ptr(st.counters), _ptr(dc),
+                                   _ptr(dd), _ptr(da), ctypes.byref(gr), _ptr(scratch), scratch.numel(), stream),
+                   "gs_backward")
+        return out
+    # the parameters are updated in place: the kernel must see the caller's
+    # storage, not a converted copy
+    for name, t, k in (("means3D", means3D, 0), ("shs", shs, 1)):
+        if t is not None and (ga.tensors[k] is None or ga.tensors[k].data_ptr() != t.data_ptr()):
+            raise RuntimeError(f"adam in backward: {name} must be a contiguous float32 device tensor")
+    ad = _lib.GsAdamFused()
+    for k, v in adam.items():
+        setattr(ad, k, v)
+    _lib.check(lib.gs_backward_adam(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(st.radii), _ptr(st.geom),
+                                    _ptr(st.binning), st.capacity, _ptr(st.image), _ptr(st.counters), _ptr(dc),
+                                    _ptr(dd), _ptr(da), ctypes.byref(gr), ctypes.byref(ad), _ptr(scratch),
+                                    scratch.numel(), stream), "gs_backward_adam")
     return out
 
 

@@ -201,6 +201,35 @@ int gs_backward(const gs_frame_t* frame, const gs_gaussians_t* g, const int* rad
                 const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha, const gs_grads_t* out,
                 void* scratch, size_t scratch_bytes, void* stream);
 
+/* Adam for the means and the SH coefficients applied inside the backward
+ * (no reference counterpart: the reference runs loss.backward() and then
+ * optimizer.step(), scene/gaussian_model_cf.py:256-268, trainer/cf3dgs_trainer.py:81-173).
+ * gs_backward_adam is gs_backward followed by one Adam step of the means and
+ * the SH coefficients (torch.optim.Adam, the update of gs_adam_step_rows, bit
+ * for bit), except that those two gradients are never written: each
+ * Gaussian's gradient goes straight into its Adam step.  g->means3D and g->shs
+ * are updated in place; out->dL_dmeans3D and out->dL_dsh must be NULL.  The
+ * frame must have no pose, and M <= 16.  Step count `step` >= 1 (bias
+ * corrections 1 - beta^step, computed as gs_adam_step_rows does).  SH element
+ * e of the [P, M, 3] tensor steps with shs_lead_lr when e % (3 M) < shs_lead,
+ * else with shs_lr.  All pointers 16-byte aligned. */
+typedef struct gs_adam_fused {
+  float* means_m;
+  float* means_v;
+  float means_lr;
+  float* shs_m;
+  float* shs_v;
+  float shs_lr;
+  float shs_lead_lr;
+  int shs_lead;
+  float beta1, beta2, eps;
+  int step;
+} gs_adam_fused_t;
+int gs_backward_adam(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
+                     void* binning, long long capacity, const void* image, const uint32_t* counters,
+                     const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha, const gs_grads_t* out,
+                     const gs_adam_fused_t* adam, void* scratch, size_t scratch_bytes, void* stream);
+
 /* ---- Many frames per launch (the batched pose solver, the serving loop) ----
  * No single reference counterpart: these split gs_render / gs_backward so that
  * the per-pixel kernels of many independent frames (each with its own

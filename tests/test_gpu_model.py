@@ -176,3 +176,52 @@ def test_training_step_gradients_match_reference_composition():
     assert torch.equal(out["visibility_filter"], out["radii"] > 0)
     assert gp.screenspace_leaf() is not out["viewspace_points"]
     assert torch.equal(gp._features_dc, gp._features[:, :1]) and gp._features_rest.shape[1] == 15
+
+
+def test_adam_in_backward_equals_the_separate_step():
+    """render(adam_in_backward=True): the backward applies the means' and SH
+    coefficients' Adam step itself (gs_backward_adam) and writes no gradient
+    for them; then optimizer.step() updates the rest.  Over three steps of the
+    deterministic backward this equals, bit for bit, backward + FusedAdam over
+    every parameter (parameters and both moments)."""
+    sc = scenes.training_scene(seed=6, P=30_000, W=320, H=180)
+    cam, bg, target = sc["cam"].to(DEV), sc["bg"].to(DEV), sc["target"].to(DEV)
+    names = ("_xyz", "_features", "_scaling", "_rotation", "_opacity")
+    raster.set_deterministic(True)
+    runs = []
+    try:
+        for fused in (False, True):
+            gp = GaussianParams(sc["xyz"], sc["features_dc"], sc["features_rest"], sc["scaling"], sc["rotation"],
+                                sc["opacity"], sc["sh_degree"], device=DEV)
+            opt = gp.training_setup()
+            for _ in range(3):
+                out = render(gp, cam, bg, clamp=False, adam_in_backward=fused)
+                loss, _ = fused_photometric_loss(out["image"], target, clamp01=True)
+                loss.backward()
+                if fused:
+                    assert gp._xyz.grad is None and gp._features.grad is None
+                    assert gp._scaling.grad is not None and out["viewspace_points"].grad is not None
+                opt.step()
+                opt.zero_grad(set_to_none=True)
+            runs.append((gp, opt))
+    finally:
+        raster.set_deterministic(False)
+    (a, oa), (b, ob) = runs
+    for n in names:
+        pa, pb = getattr(a, n), getattr(b, n)
+        assert torch.equal(pa, pb), n
+        sa, sb = oa.state[pa], ob.state[pb]
+        assert sa["step"] == sb["step"] == 3, n
+        assert torch.equal(sa["exp_avg"], sb["exp_avg"]) and torch.equal(sa["exp_avg_sq"], sb["exp_avg_sq"]), n
+    # the parameters did move
+    assert not torch.equal(a._xyz, sc["xyz"].to(DEV))
+
+
+def test_adam_in_backward_refuses_a_pose():
+    sc = scenes.training_scene(seed=7, P=2_000, W=64, H=48)
+    gp = GaussianParams(sc["xyz"], sc["features_dc"], sc["features_rest"], sc["scaling"], sc["rotation"],
+                        sc["opacity"], sc["sh_degree"], device=DEV)
+    gp.training_setup()
+    pose = torch.tensor([0.0, 0, 0, 0, 0, 0, 1.0], device=DEV)
+    with pytest.raises(Exception, match="adam_in_backward"):
+        render(gp, sc["cam"].to(DEV), sc["bg"].to(DEV), pose=pose, adam_in_backward=True)
