@@ -49,6 +49,20 @@ __global__ __launch_bounds__(256) void k_activate_fwd(int P, const float* __rest
   if (opac) opac[i] = 1.0f / (1.0f + expf(-raw_opac[i]));
 }
 
+// The three backward rules, shared by the plain backward and the one that
+// applies Adam (so both round identically).
+__device__ __forceinline__ float act_bwd_exp(float g, float s) { return g * s; }
+__device__ __forceinline__ float act_bwd_sigmoid(float g, float o) { return (g * (1.0f - o)) * o; }
+__device__ __forceinline__ float4 act_bwd_normalize(float4 x, float4 g) {
+  const float n = sqrtf(((x.x * x.x + x.y * x.y) + x.z * x.z) + x.w * x.w);
+  const float c = fmaxf(n, kNormalizeEps);
+  const float cc = c * c;
+  const float dc = (((-g.x * x.x) / cc + (-g.y * x.y) / cc) + (-g.z * x.z) / cc) + (-g.w * x.w) / cc;
+  const float dn = n >= kNormalizeEps ? dc : 0.0f;
+  const float k = n > 0.0f ? dn / n : 0.0f;
+  return make_float4(g.x / c + x.x * k, g.y / c + x.y * k, g.z / c + x.z * k, g.w / c + x.w * k);
+}
+
 __global__ __launch_bounds__(256) void k_activate_bwd(int P, const float4* __restrict__ raw_rot,
                                                       const float* __restrict__ scales,
                                                       const float* __restrict__ opac, const float* __restrict__ g_s,
@@ -62,41 +76,120 @@ __global__ __launch_bounds__(256) void k_activate_bwd(int P, const float4* __res
     if (g_s) {
       const float* g = g_s + 3LL * i;
       const float* s = scales + 3LL * i;
-      o[0] = g[0] * s[0];
-      o[1] = g[1] * s[1];
-      o[2] = g[2] * s[2];
+      o[0] = act_bwd_exp(g[0], s[0]);
+      o[1] = act_bwd_exp(g[1], s[1]);
+      o[2] = act_bwd_exp(g[2], s[2]);
     } else {
       o[0] = o[1] = o[2] = 0.0f;
     }
   }
   if (d_r) {
     if (g_r) {
-      const float4 x = raw_rot[i];
-      const float4 g = g_r[i];
-      const float n = sqrtf(((x.x * x.x + x.y * x.y) + x.z * x.z) + x.w * x.w);
-      const float c = fmaxf(n, kNormalizeEps);
-      const float cc = c * c;
-      const float dc = (((-g.x * x.x) / cc + (-g.y * x.y) / cc) + (-g.z * x.z) / cc) + (-g.w * x.w) / cc;
-      const float dn = n >= kNormalizeEps ? dc : 0.0f;
-      const float k = n > 0.0f ? dn / n : 0.0f;
-      d_r[i] = make_float4(g.x / c + x.x * k, g.y / c + x.y * k, g.z / c + x.z * k, g.w / c + x.w * k);
+      d_r[i] = act_bwd_normalize(raw_rot[i], g_r[i]);
     } else {
       d_r[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
   }
   if (d_o) {
     if (g_o) {
-      const float o = opac[i];
-      d_o[i] = (g_o[i] * (1.0f - o)) * o;
+      d_o[i] = act_bwd_sigmoid(g_o[i], opac[i]);
     } else {
       d_o[i] = 0.0f;
     }
   }
 }
 
+struct ActAdam {
+  float* raw_scale;  // parameters, updated in place
+  float4* raw_rot;
+  float* raw_opac;
+  float* sm;
+  float* sv;
+  float4* rm;
+  float4* rv;
+  float* om;
+  float* ov;
+  float s_step, r_step, o_step, b1, b2, eps, inv_sqrt_bc2;
+};
+
+// The backward of the three activations followed by one Adam step of the raw
+// parameters: the raw gradients are never written.
+__global__ __launch_bounds__(256) void k_activate_bwd_adam(int P, const float* __restrict__ scales,
+                                                           const float* __restrict__ opac,
+                                                           const float* __restrict__ g_s,
+                                                           const float4* __restrict__ g_r,
+                                                           const float* __restrict__ g_o, ActAdam a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P) return;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const long long e = 3LL * i + k;
+    float p = a.raw_scale[e], m = a.sm[e], v = a.sv[e];
+    adam_one(p, act_bwd_exp(g_s[e], scales[e]), m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.s_step);
+    a.raw_scale[e] = p;
+    a.sm[e] = m;
+    a.sv[e] = v;
+  }
+  {
+    float4 p = a.raw_rot[i], m = a.rm[i], v = a.rv[i];
+    const float4 d = act_bwd_normalize(p, g_r[i]);
+    adam_one(p.x, d.x, m.x, v.x, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
+    adam_one(p.y, d.y, m.y, v.y, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
+    adam_one(p.z, d.z, m.z, v.z, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
+    adam_one(p.w, d.w, m.w, v.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
+    a.raw_rot[i] = p;
+    a.rm[i] = m;
+    a.rv[i] = v;
+  }
+  {
+    float p = a.raw_opac[i], m = a.om[i], v = a.ov[i];
+    adam_one(p, act_bwd_sigmoid(g_o[i], opac[i]), m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.o_step);
+    a.raw_opac[i] = p;
+    a.om[i] = m;
+    a.ov[i] = v;
+  }
+}
+
 }  // namespace gs
 
 using namespace gs;
+
+extern "C" int gs_activate_backward_adam(int P, float* raw_scale, float* raw_rot, float* raw_opacity,
+                                         const float* scales, const float* opacities, const float* dL_dscales,
+                                         const float* dL_drotations, const float* dL_dopacities,
+                                         const gs_adam_act_t* adam, void* stream) {
+  if (P < 0 || !adam || adam->step < 1) return GS_E_ARG;
+  if (P == 0) return GS_OK;
+  if (!raw_scale || !raw_rot || !raw_opacity || !scales || !opacities || !dL_dscales || !dL_drotations ||
+      !dL_dopacities || !adam->scale_m || !adam->scale_v || !adam->rot_m || !adam->rot_v || !adam->opacity_m ||
+      !adam->opacity_v)
+    return GS_E_ARG;
+  if (((uintptr_t)raw_rot | (uintptr_t)dL_drotations | (uintptr_t)adam->rot_m | (uintptr_t)adam->rot_v) & 15)
+    return GS_E_ARG;
+  // bias corrections as gs_adam_step_rows computes them
+  const double bc1 = 1.0 - std::pow((double)adam->beta1, (double)adam->step);
+  const double bc2 = 1.0 - std::pow((double)adam->beta2, (double)adam->step);
+  ActAdam a;
+  a.raw_scale = raw_scale;
+  a.raw_rot = reinterpret_cast<float4*>(raw_rot);
+  a.raw_opac = raw_opacity;
+  a.sm = adam->scale_m;
+  a.sv = adam->scale_v;
+  a.rm = reinterpret_cast<float4*>(adam->rot_m);
+  a.rv = reinterpret_cast<float4*>(adam->rot_v);
+  a.om = adam->opacity_m;
+  a.ov = adam->opacity_v;
+  a.s_step = (float)(adam->scale_lr / bc1);
+  a.r_step = (float)(adam->rot_lr / bc1);
+  a.o_step = (float)(adam->opacity_lr / bc1);
+  a.b1 = adam->beta1;
+  a.b2 = adam->beta2;
+  a.eps = adam->eps;
+  a.inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
+  hipLaunchKernelGGL(k_activate_bwd_adam, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, (hipStream_t)stream, P,
+                     scales, opacities, dL_dscales, reinterpret_cast<const float4*>(dL_drotations), dL_dopacities, a);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
 
 extern "C" int gs_activate_forward(int P, const float* raw_scale, const float* raw_rot, const float* raw_opacity,
                                    float* scales, float* rotations, float* opacities, void* stream) {

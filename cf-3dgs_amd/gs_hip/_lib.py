@@ -61,6 +61,15 @@ class GsAdamFused(ctypes.Structure):
     ]
 
 
+class GsAdamAct(ctypes.Structure):
+    _fields_ = [
+        ("scale_m", c_void_p), ("scale_v", c_void_p), ("scale_lr", c_float),
+        ("rot_m", c_void_p), ("rot_v", c_void_p), ("rot_lr", c_float),
+        ("opacity_m", c_void_p), ("opacity_v", c_void_p), ("opacity_lr", c_float),
+        ("beta1", c_float), ("beta2", c_float), ("eps", c_float), ("step", c_int),
+    ]
+
+
 class GsGeomLayout(ctypes.Structure):
     _fields_ = [("splat", c_size_t), ("cov3D", c_size_t), ("tiles", c_size_t), ("offsets", c_size_t),
                 ("scan_tmp", c_size_t), ("total", c_size_t)]
@@ -159,6 +168,8 @@ EXPORTS = {
                                      c_void_p, c_void_p, c_size_t, c_void_p]),
     "gs_create_from_pcd": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p]),
+    "gs_activate_backward_adam": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, ctypes.POINTER(GsAdamAct), c_void_p]),
     "gs_timing_enable": (c_int, [c_int]),
     "gs_timing_reset": (c_int, []),
     "gs_timing_read": (c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int)]),

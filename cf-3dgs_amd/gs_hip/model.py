@@ -51,7 +51,7 @@ class _Activate(torch.autograd.Function):
     (gs_activate_backward)."""
 
     @staticmethod
-    def forward(ctx, raw_scale, raw_rot, raw_opac):
+    def forward(ctx, raw_scale, raw_rot, raw_opac, adam=None):
         dev = raw_scale.device
         if dev.type != "cuda":
             raise RuntimeError("the fused activations run on the GPU only")
@@ -68,6 +68,8 @@ class _Activate(torch.autograd.Function):
                                                    stream), "gs_activate_forward")
         ctx.set_materialize_grads(False)
         ctx.save_for_backward(rr, scales, opac)
+        # adam in backward updates the caller's raw tensors, not converted copies
+        ctx.adam = adam if adam is not None and rs is raw_scale and rr is raw_rot and ro is raw_opac else None
         return scales, rots, opac
 
     @staticmethod
@@ -76,23 +78,34 @@ class _Activate(torch.autograd.Function):
         P = rr.shape[0]
         # an output that took no part in the loss passes no gradient on (as in torch)
         need = ctx.needs_input_grad
+        stream = ctypes.c_void_p(torch.cuda.current_stream(rr.device).cuda_stream)
+        if ctx.adam is not None and all(need[:3]) and g_s is not None and g_r is not None and g_o is not None:
+            # one Adam step of the raw parameters here; they get no gradient
+            ad = _lib.GsAdamAct(**ctx.adam.take())
+            rs_p, rr_p, ro_p = ctx.adam.params
+            _lib.check(_lib.load().gs_activate_backward_adam(
+                P, _p(rs_p), _p(rr_p), _p(ro_p), _p(scales), _p(opac), _p(g_s.contiguous()), _p(_aligned(g_r)),
+                _p(g_o.contiguous()), ctypes.byref(ad), stream), "gs_activate_backward_adam")
+            return None, None, None, None
         d_s = torch.empty_like(scales) if need[0] and g_s is not None else None
         d_r = torch.empty_like(rr) if need[1] and g_r is not None else None
         d_o = torch.empty_like(opac) if need[2] and g_o is not None else None
         g_s = g_s.contiguous() if g_s is not None else None
         g_r = _aligned(g_r) if g_r is not None else None
         g_o = g_o.contiguous() if g_o is not None else None
-        stream = ctypes.c_void_p(torch.cuda.current_stream(rr.device).cuda_stream)
         _lib.check(_lib.load().gs_activate_backward(P, _p(rr), _p(scales), _p(opac), _p(g_s), _p(g_r), _p(g_o),
                                                     _p(d_s), _p(d_r), _p(d_o), stream), "gs_activate_backward")
-        return d_s, d_r, d_o
+        return d_s, d_r, d_o, None
 
 
-def activate(raw_scale: torch.Tensor, raw_rot: torch.Tensor, raw_opacity: torch.Tensor):
+def activate(raw_scale: torch.Tensor, raw_rot: torch.Tensor, raw_opacity: torch.Tensor, adam=None):
     """The reference's scaling / rotation / opacity activations
     (torch.exp, F.normalize, torch.sigmoid) fused: returns
-    (scales [P,3], rotations [P,4], opacities [P,1])."""
-    return _Activate.apply(raw_scale, raw_rot, raw_opacity)
+    (scales [P,3], rotations [P,4], opacities [P,1]).  With ``adam``
+    (``FusedAdam.in_backward_activations(raw_scale, raw_rot, raw_opacity)``)
+    the backward applies their Adam step in place instead of returning the
+    raw gradients."""
+    return _Activate.apply(raw_scale, raw_rot, raw_opacity, adam)
 
 
 class GaussianParams:
@@ -151,9 +164,9 @@ class GaussianParams:
             z = self._screen_zeros = torch.zeros_like(xyz)
         return z.detach().requires_grad_(True)
 
-    def get_activated(self):
+    def get_activated(self, adam=None):
         """(scales, rotations, opacities) in one fused launch."""
-        return activate(self._scaling, self._rotation, self._opacity)
+        return activate(self._scaling, self._rotation, self._opacity, adam)
 
     def training_setup(self, spatial_lr_scale: float = 1.0, position_lr_init: float = 0.00016,
                        feature_lr: float = 0.0025, opacity_lr: float = 0.05, scaling_lr: float = 0.005,
@@ -179,19 +192,23 @@ def render(gaussians: GaussianParams, cam: Camera, bg: torch.Tensor, pose=None, 
     ``clamp=False`` "image" is the rasteriser's colour before the [0, 1]
     clamp, for a loss that applies the clamp itself (the fused loss's
     ``clamp01``).  With ``adam_in_backward`` the backward applies the
-    ``training_setup`` optimiser's step to the means and SH coefficients
-    itself (their gradients are never written); ``optimizer.step()`` then
-    updates the rest."""
+    ``training_setup`` optimiser's step to every Gaussian parameter itself:
+    the rasteriser's backward steps the means and SH coefficients, the
+    activations' backward steps the raw scaling, rotation and opacity, and no
+    gradient of theirs is written.  ``optimizer.step()`` then has nothing
+    left to do for them."""
     means3D = gaussians.get_xyz
     screenspace = gaussians.screenspace_leaf()
     rs = raster_settings(cam, bg, gaussians.active_sh_degree, scaling_modifier)
     rasterizer = GaussianRasterizer(raster_settings=rs)
-    scales, rotations, opacities = gaussians.get_activated()
-    fused = None
+    fused = fused_act = None
     if adam_in_backward:
         if gaussians.optimizer is None:
             raise RuntimeError("adam_in_backward needs training_setup() first")
-        fused = gaussians.optimizer.in_backward(gaussians._xyz, gaussians._features)
+        opt = gaussians.optimizer
+        fused = opt.in_backward(gaussians._xyz, gaussians._features)
+        fused_act = opt.in_backward_activations(gaussians._scaling, gaussians._rotation, gaussians._opacity)
+    scales, rotations, opacities = gaussians.get_activated(fused_act)
     color, radii, depth = rasterizer(means3D=means3D, means2D=screenspace, shs=gaussians.get_features,
                                      colors_precomp=None, opacities=opacities, scales=scales, rotations=rotations,
                                      cov3D_precomp=None, pose=pose, adam_in_backward=fused)

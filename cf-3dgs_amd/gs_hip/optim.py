@@ -48,6 +48,13 @@ class FusedAdam(torch.optim.Optimizer):
         have no gradient."""
         return _InBackward(self, means, shs)
 
+    def in_backward_activations(self, raw_scale: torch.Tensor, raw_rot: torch.Tensor,
+                                raw_opacity: torch.Tensor) -> "_ActInBackward":
+        """The same for the raw scaling, rotation and opacity parameters: the
+        activations' backward (``gs_hip.model.activate(..., adam=...)``)
+        applies their Adam step itself."""
+        return _ActInBackward(self, raw_scale, raw_rot, raw_opacity)
+
     @torch.no_grad()
     def step(self, closure=None):
         loss = closure() if closure is not None else None
@@ -96,6 +103,42 @@ class FusedAdam(torch.optim.Optimizer):
         return loss
 
 
+def _check_fused(opt: FusedAdam, params) -> tuple:
+    """(groups, states) of parameters stepped together inside a backward:
+    contiguous aligned float32 GPU tensors, one betas / eps, one step count."""
+    groups = [opt._group_of(p) for p in params]
+    for p in params:
+        if p.device.type != "cuda" or p.dtype != torch.float32 or not p.is_contiguous() or p.data_ptr() % 16:
+            raise RuntimeError("adam in backward needs contiguous, 16-byte aligned float32 GPU parameters")
+    if len({(tuple(g["betas"]), g["eps"]) for g in groups}) != 1:
+        raise RuntimeError("adam in backward: the parameters must share betas and eps")
+    states = [opt._state_of(p) for p in params]
+    if len({st["step"] for st in states}) != 1:
+        raise RuntimeError("adam in backward: the parameters have taken different numbers of steps")
+    return groups, states
+
+
+class _ActInBackward:
+    """One pending fused step of the raw (scaling, rotation, opacity);
+    ``take()`` advances their step counts and returns the gs_adam_act_t
+    fields."""
+
+    def __init__(self, opt: FusedAdam, raw_scale, raw_rot, raw_opacity):
+        self.opt, self.params = opt, (raw_scale, raw_rot, raw_opacity)
+
+    def take(self) -> dict:
+        (gs, gr, go), states = _check_fused(self.opt, self.params)
+        for st in states:
+            st["step"] += 1
+        ss, sr, so = states
+        b1, b2 = gs["betas"]
+        return dict(scale_m=ss["exp_avg"].data_ptr(), scale_v=ss["exp_avg_sq"].data_ptr(), scale_lr=float(gs["lr"]),
+                    rot_m=sr["exp_avg"].data_ptr(), rot_v=sr["exp_avg_sq"].data_ptr(), rot_lr=float(gr["lr"]),
+                    opacity_m=so["exp_avg"].data_ptr(), opacity_v=so["exp_avg_sq"].data_ptr(),
+                    opacity_lr=float(go["lr"]), beta1=float(b1), beta2=float(b2), eps=float(gs["eps"]),
+                    step=int(ss["step"]))
+
+
 class _InBackward:
     """One pending fused step of (means, shs); ``take()`` advances their step
     counts and returns the gs_adam_fused_t fields."""
@@ -104,16 +147,7 @@ class _InBackward:
         self.opt, self.means, self.shs = opt, means, shs
 
     def take(self) -> dict:
-        opt = self.opt
-        gm, gs = opt._group_of(self.means), opt._group_of(self.shs)
-        for p in (self.means, self.shs):
-            if p.device.type != "cuda" or p.dtype != torch.float32 or not p.is_contiguous() or p.data_ptr() % 16:
-                raise RuntimeError("adam in backward needs contiguous, 16-byte aligned float32 GPU parameters")
-        if gm["betas"] != gs["betas"] or gm["eps"] != gs["eps"]:
-            raise RuntimeError("adam in backward: means and shs must share betas and eps")
-        sm, ss = opt._state_of(self.means), opt._state_of(self.shs)
-        if sm["step"] != ss["step"]:
-            raise RuntimeError("adam in backward: means and shs have taken different numbers of steps")
+        (gm, gs), (sm, ss) = _check_fused(self.opt, (self.means, self.shs))
         sm["step"] += 1
         ss["step"] += 1
         row = int(gs.get("row", 0))

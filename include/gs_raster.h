@@ -411,6 +411,30 @@ int gs_activate_backward(int P, const float* raw_rot, const float* scales, const
                          const float* dL_dscales, const float* dL_drotations, const float* dL_dopacities,
                          float* dL_draw_scale, float* dL_draw_rot, float* dL_draw_opacity, void* stream);
 
+/* The activation backward followed by one Adam step of the raw parameters
+ * (no reference counterpart; the caller side of gs_backward_adam): the raw
+ * gradients gs_activate_backward would write go straight into Adam, and
+ * raw_scale / raw_rot / raw_opacity are updated in place.  All three incoming
+ * gradients are required.  Same update, bit for bit, as gs_activate_backward
+ * followed by gs_adam_step_rows (bias corrections 1 - beta^step).  raw_rot,
+ * dL_drotations and the rotation moments 16-byte aligned. */
+typedef struct gs_adam_act {
+  float* scale_m;
+  float* scale_v;
+  float scale_lr;
+  float* rot_m;
+  float* rot_v;
+  float rot_lr;
+  float* opacity_m;
+  float* opacity_v;
+  float opacity_lr;
+  float beta1, beta2, eps;
+  int step;
+} gs_adam_act_t;
+int gs_activate_backward_adam(int P, float* raw_scale, float* raw_rot, float* raw_opacity, const float* scales,
+                              const float* opacities, const float* dL_dscales, const float* dL_drotations,
+                              const float* dL_dopacities, const gs_adam_act_t* adam, void* stream);
+
 /* Opt-in per-stage timing with HIP events recorded on the launch stream (no
  * reference counterpart; the reference's debug mode only synchronises).
  * Stages: preprocess, scan, duplicate, sort, ranges, blend, render_bwd,

@@ -179,11 +179,12 @@ def test_training_step_gradients_match_reference_composition():
 
 
 def test_adam_in_backward_equals_the_separate_step():
-    """render(adam_in_backward=True): the backward applies the means' and SH
-    coefficients' Adam step itself (gs_backward_adam) and writes no gradient
-    for them; then optimizer.step() updates the rest.  Over three steps of the
-    deterministic backward this equals, bit for bit, backward + FusedAdam over
-    every parameter (parameters and both moments)."""
+    """render(adam_in_backward=True): the rasteriser's backward applies the
+    means' and SH coefficients' Adam step itself (gs_backward_adam) and the
+    activations' backward that of the raw scaling, rotation and opacity
+    (gs_activate_backward_adam); no parameter gets a gradient.  Over three
+    steps of the deterministic backward this equals, bit for bit, backward +
+    FusedAdam over every parameter (parameters and both moments)."""
     sc = scenes.training_scene(seed=6, P=30_000, W=320, H=180)
     cam, bg, target = sc["cam"].to(DEV), sc["bg"].to(DEV), sc["target"].to(DEV)
     names = ("_xyz", "_features", "_scaling", "_rotation", "_opacity")
@@ -199,8 +200,8 @@ def test_adam_in_backward_equals_the_separate_step():
                 loss, _ = fused_photometric_loss(out["image"], target, clamp01=True)
                 loss.backward()
                 if fused:
-                    assert gp._xyz.grad is None and gp._features.grad is None
-                    assert gp._scaling.grad is not None and out["viewspace_points"].grad is not None
+                    assert all(getattr(gp, n).grad is None for n in names)
+                    assert out["viewspace_points"].grad is not None
                 opt.step()
                 opt.zero_grad(set_to_none=True)
             runs.append((gp, opt))
