@@ -90,7 +90,38 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
                            : make_uint4(idx < n ? (uint32_t)keys[idx] : 0u, idx + 1u < n ? (uint32_t)keys[idx + 1] : 0u,
                                         idx + 2u < n ? (uint32_t)keys[idx + 2] : 0u, 0u);
     }
-    if (bits <= 4) {
+    if (bits <= 3) {
+      // at most 8 bins (a short top digit): LDS atomics would pile onto a few
+      // words, so each lane counts in registers, four 8-bit fields per word
+      // (a lane sees at most 4 V <= 16 keys), and the wave adds the counts
+      // after widening them to 16-bit fields (a wave sees at most 1024 keys)
+      uint32_t lo = 0, hi = 0;
+#pragma unroll
+      for (int r = 0; r < V; ++r) {
+        const uint32_t idx = wbase + 4u * (uint32_t)(r * 64 + lane);
+        const uint32_t e[4] = {kv[r].x, kv[r].y, kv[r].z, kv[r].w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t d = (e[c] >> shift) & mask;
+          const uint32_t inc = idx + (uint32_t)c < n ? 1u << ((d & 3u) * 8u) : 0u;
+          lo += d < 4u ? inc : 0u;
+          hi += d < 4u ? 0u : inc;
+        }
+      }
+      // 16-bit fields: w0 = bins (0, 2), w1 = (1, 3), w2 = (4, 6), w3 = (5, 7)
+      uint32_t cw[4] = {lo & 0x00ff00ffu, (lo >> 8) & 0x00ff00ffu, hi & 0x00ff00ffu, (hi >> 8) & 0x00ff00ffu};
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cw[k] += __shfl_xor(cw[k], off, 64);
+      }
+      if (lane < 8) {
+        const int k = (lane >> 2) * 2 + (lane & 1);
+        const uint32_t word = k == 0 ? cw[0] : (k == 1 ? cw[1] : (k == 2 ? cw[2] : cw[3]));
+        const uint32_t cnt = ((lane >> 1) & 1) ? word >> 16 : word & 0xffffu;
+        if ((uint32_t)lane <= mask) s_h[w][lane] = cnt;
+      }
+    } else if (bits <= 4) {
       // few bins (a top digit): the wave's lanes would pile onto a handful of
       // LDS words, so count each bin with a ballot instead; lane b keeps bin b
       uint32_t cnt = 0;

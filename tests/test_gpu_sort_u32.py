@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("n,end_bit,hi", [(1, 32, 1 << 32), (5000, 32, 1 << 32), (300_001, 11, 2040),
-                                          (2_000_000, 13, 8160), (70_000, 1, 1)])
+                                          (2_000_000, 13, 8160), (70_000, 1, 1), (3_000_001, 11, 2040),
+                                          (700_001, 10, 1024)])
 def test_sort_pairs_u32_stable(n, end_bit, hi):
     rng = np.random.default_rng(n)
     keys = rng.integers(0, hi, size=n, dtype=np.uint64).astype(np.uint32)
