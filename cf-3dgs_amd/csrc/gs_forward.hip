@@ -11,6 +11,9 @@
 //   k_duplicate    per Gaussian: (tile << 32 | depth bits, id) pairs
 //   radix sort     gs_sort.hip, 8-bit LSD passes over bits [0, 32 + bits(T))
 //   k_ranges       [start, end) per tile in the sorted list
+//   (the depth-first schedule instead sorts the Gaussians by depth, expands
+//   them into tile ids in that order (k_df_*), sorts on the tile bits alone,
+//   and records the tile ranges in that sort's last pass)
 //   k_blend        one 16x16 tile per 256-thread workgroup; splats staged in
 //                  LDS 256 at a time, each wave keeping its own list of the
 //                  staged splats whose alpha can reach 1/255 on its 8 x 8
@@ -313,14 +316,15 @@ __global__ __launch_bounds__(256) void k_ranges(const uint64_t* keys, const uint
 // records are read in order, so the duplicate (which visits Gaussians in
 // depth order, i.e. at random) reads one 8-byte record instead of the tile
 // count and two 16-byte splat vectors.
-// Also clears the n_ranges words of the tile ranges (the grid covers
+// Also sets the n_ranges words of the tile ranges to [UINT_MAX, 0) per tile,
+// the start the list sort's atomic min / max needs (the grid covers
 // max(P, n_ranges) threads).
 __global__ __launch_bounds__(256) void k_df_keys(int P, const uint32_t* tiles, const float4* splat, int gx, int gy,
                                                  uint32_t* keys, uint32_t* vals, uint2* rect, uint32_t* counters,
                                                  uint32_t* ranges, int n_ranges) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i == 0) counters[2] = (uint32_t)P;
-  if (i < n_ranges) ranges[i] = 0u;
+  if (i < n_ranges) ranges[i] = (i & 1) ? 0u : 0xffffffffu;
   if (i >= P) return;
   const bool vis = tiles[i] != 0u;
   keys[i] = vis ? __float_as_uint(splat[3 * i + 1].z) : 0xffffffffu;
@@ -445,25 +449,6 @@ __global__ __launch_bounds__(256) void k_df_duplicate(DfDupArgs a) {
       a.vals[dst] = s_g[w][j];
     }
   }
-}
-
-__global__ __launch_bounds__(256) void k_ranges32(const uint32_t* keys, const uint32_t* counters, long long capacity,
-                                                  int shift, uint32_t* ranges) {
-  const uint32_t n0 = counters[0];
-  const uint32_t n = ((long long)n0 > capacity) ? 0u : n0;
-  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t t = keys[k] >> shift;
-  if (k == 0) {
-    ranges[2 * t] = 0;
-  } else {
-    const uint32_t tp = keys[k - 1] >> shift;
-    if (t != tp) {
-      ranges[2 * tp + 1] = k;
-      ranges[2 * t] = k;
-    }
-  }
-  if (k == n - 1) ranges[2 * t + 1] = n;
 }
 
 struct BlendArgs {
@@ -659,7 +644,9 @@ __global__ __launch_bounds__(256) void k_blend_multi(const uint32_t* prefix, con
 // One workgroup: checkpoint base of every tile = exclusive prefix over tiles
 // of (batches in its list - 1); ckpt_base[n_tiles] = total checkpoints.
 // It also clears the blend's work-item log.
-__global__ __launch_bounds__(1024) void k_tile_ckpt_scan(const uint32_t* ranges, int nt, uint32_t* ckpt_base,
+// Also turns the ranges of empty tiles, left at [UINT_MAX, 0) by the
+// depth-first list sort, into [0, 0).
+__global__ __launch_bounds__(1024) void k_tile_ckpt_scan(uint32_t* ranges, int nt, uint32_t* ckpt_base,
                                                          uint32_t* item_log) {
   __shared__ uint32_t s[1024];
   __shared__ uint32_t s_carry;
@@ -673,7 +660,13 @@ __global__ __launch_bounds__(1024) void k_tile_ckpt_scan(const uint32_t* ranges,
     const int b = b0 + t;
     uint32_t v = 0;
     if (b < nt) {
-      const uint32_t len = ranges[2 * b + 1] - ranges[2 * b];
+      uint32_t r0 = ranges[2 * b];
+      const uint32_t r1 = ranges[2 * b + 1];
+      if (r0 > r1) {
+        r0 = r1;  // r1 = 0: an empty tile
+        ranges[2 * b] = 0u;
+      }
+      const uint32_t len = r1 - r0;
       const uint32_t nb = (len + 255) / 256;
       v = nb > 0 ? nb - 1 : 0;
     }
@@ -962,19 +955,16 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
     {
       timing::Scope ts(timing::kSort, s);
       const int tb = bits_for((uint32_t)nt) > 0 ? bits_for((uint32_t)nt) : 1;
+      // the last pass also records the tile ranges
       if (LB.id_bits)  // one word per entry: sort on its tile bits only
         e = sort_keys_u32((uint32_t*)(bb + LB.keys_dup), (uint32_t*)(bb + LB.keys_alt), counters, capacity,
-                          LB.id_bits, LB.id_bits + tb, bb + LB.hist, s, &in_b);
+                          LB.id_bits, LB.id_bits + tb, bb + LB.hist, s, &in_b, ranges);
       else
         e = sort_pairs_u32((uint32_t*)(bb + LB.keys_dup), (uint32_t*)(bb + LB.vals_dup),
                            (uint32_t*)(bb + LB.keys_alt), (uint32_t*)(bb + LB.vals_alt), counters, capacity, tb,
-                           bb + LB.hist, s, &in_b);
+                           bb + LB.hist, s, &in_b, ranges);
     }
     if (e != hipSuccess) return GS_E_LAUNCH;
-    timing::Scope ts(timing::kRanges, s);
-    hipLaunchKernelGGL(k_ranges32, dim3(nblocks256(capacity)), dim3(256), 0, s,
-                       (const uint32_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, LB.id_bits,
-                       ranges);
   } else if (capacity > 0 && P > 0) {
     if (!post) return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
     DupArgs d;
@@ -1007,8 +997,8 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
                        (const uint64_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, ranges);
   }
   if (post)
-    hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)ranges, nt,
-                       (uint32_t*)(ib + LI.ckpt_base), (uint32_t*)(bb + LB.item_log));
+    hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, ranges, nt, (uint32_t*)(ib + LI.ckpt_base),
+                       (uint32_t*)(bb + LB.item_log));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 

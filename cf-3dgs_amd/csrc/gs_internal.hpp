@@ -48,11 +48,14 @@ size_t sort_tmp_bytes(long long capacity);
 hipError_t sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b,
                           const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
                           int* result_in_b);
-// keys only, on key bits [start_bit, end_bit)
+// keys only, on key bits [start_bit, end_bit).  The 32-bit sorts of tile
+// lists may also record each tile's [start, end) of the sorted list in
+// `ranges` (key >> start_bit is the tile id; the caller sets every range to
+// [UINT_MAX, 0) beforehand, and empty tiles keep that).
 hipError_t sort_keys_u32(uint32_t* keys_a, uint32_t* keys_b, const uint32_t* n_dev, long long capacity, int start_bit,
-                         int end_bit, void* tmp, hipStream_t s, int* result_in_b);
+                         int end_bit, void* tmp, hipStream_t s, int* result_in_b, uint32_t* ranges = nullptr);
 hipError_t sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, uint32_t* vals_b,
                           const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
-                          int* result_in_b);
+                          int* result_in_b, uint32_t* ranges = nullptr);
 
 }  // namespace gs

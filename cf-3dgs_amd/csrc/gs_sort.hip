@@ -197,7 +197,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
                                                                uint32_t* __restrict__ vals_out,
                                                                const uint32_t* n_dev, long long cap, int shift,
                                                                int bits, const uint32_t* hist,
-                                                               const uint32_t* totals, int nb_max) {
+                                                               const uint32_t* totals, int nb_max, uint32_t* ranges,
+                                                               int tshift) {
   static_assert(R == kSortThreads, "one digit per thread");
   constexpr int TILE = kSortThreads * ITEMS;
   __shared__ uint32_t s_cnt[4][R];
@@ -267,6 +268,15 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     const uint32_t dst = s_scan[d] + i;
     keys_out[dst] = k;
     if constexpr (V) vals_out[dst] = s_val[i];
+    if (ranges) {
+      // last pass of a tile-list sort: within a digit run the block's keys are
+      // in final order, so a run of one tile id is contiguous here and in the
+      // output; its ends give the tile's [start, end) across blocks by
+      // atomic min / max (ranges start as [UINT_MAX, 0))
+      const uint32_t tile = (uint32_t)(k >> tshift);
+      if (i == 0 || (uint32_t)(s_key[i - 1] >> tshift) != tile) atomicMin(&ranges[2 * tile], dst);
+      if (i + 1 == nloc || (uint32_t)(s_key[i + 1] >> tshift) != tile) atomicMax(&ranges[2 * tile + 1], dst + 1);
+    }
   }
 }
 
@@ -282,23 +292,27 @@ size_t sort_tmp_bytes(long long capacity) {
 
 template <typename K, int ITEMS, int R>
 void sort_pass(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, const uint32_t* n_dev, long long capacity,
-               int shift, int bits, uint32_t* hist, uint32_t* totals, int nb_max, int tile, hipStream_t s) {
+               int shift, int bits, uint32_t* hist, uint32_t* totals, int nb_max, int tile, hipStream_t s,
+               uint32_t* ranges, int tshift) {
   hipLaunchKernelGGL((k_sort_upsweep<K, ITEMS, R>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, n_dev, capacity,
                      shift, bits, hist, nb_max);
   hipLaunchKernelGGL(k_sort_scan, dim3(R), dim3(256), 0, s, hist, totals, n_dev, capacity, nb_max, tile);
   if (vin)
     hipLaunchKernelGGL((k_sort_scatter<K, ITEMS, R, true>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout,
-                       vout, n_dev, capacity, shift, bits, hist, totals, nb_max);
+                       vout, n_dev, capacity, shift, bits, hist, totals, nb_max, ranges, tshift);
   else
     hipLaunchKernelGGL((k_sort_scatter<K, ITEMS, R, false>), dim3(nb_max), dim3(kSortThreads), 0, s, kin, vin, kout,
-                       vout, n_dev, capacity, shift, bits, hist, totals, nb_max);
+                       vout, n_dev, capacity, shift, bits, hist, totals, nb_max, ranges, tshift);
 }
 
 // Stable LSD sort on key bits [start_bit, end_bit); vals_a == nullptr sorts
-// the keys alone.
+// the keys alone.  With `ranges` (tile lists: key >> start_bit is a tile id),
+// the last pass also records every tile's [start, end) into ranges, which the
+// caller initialises to [UINT_MAX, 0) per tile (empty tiles keep that).
 template <typename K>
 hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, const uint32_t* n_dev,
-                      long long capacity, int start_bit, int end_bit, void* tmp, hipStream_t s, int* result_in_b) {
+                      long long capacity, int start_bit, int end_bit, void* tmp, hipStream_t s, int* result_in_b,
+                      uint32_t* ranges = nullptr) {
   const int passes = sort_passes(end_bit - start_bit);
   *result_in_b = passes & 1;
   if (capacity <= 0) return hipSuccess;
@@ -314,7 +328,8 @@ hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
     const int shift = start_bit + kRadixBits * p;
     const int bits = end_bit - shift < kRadixBits ? end_bit - shift : kRadixBits;
     auto pass = tile == kSortThreads * 4 ? sort_pass<K, 4, kRadix> : sort_pass<K, 16, kRadix>;
-    pass(kin, vin, kout, vout, n_dev, capacity, shift, bits, hist, totals, nb_max, tile, s);
+    pass(kin, vin, kout, vout, n_dev, capacity, shift, bits, hist, totals, nb_max, tile, s,
+         p + 1 == passes ? ranges : nullptr, start_bit);
     K* tk = kin;
     kin = kout;
     kout = tk;
@@ -333,14 +348,15 @@ hipError_t sort_pairs_u64(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, 
 
 hipError_t sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, uint32_t* vals_b,
                           const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
-                          int* result_in_b) {
-  return sort_pairs<uint32_t>(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, 0, end_bit, tmp, s, result_in_b);
+                          int* result_in_b, uint32_t* ranges) {
+  return sort_pairs<uint32_t>(keys_a, vals_a, keys_b, vals_b, n_dev, capacity, 0, end_bit, tmp, s, result_in_b,
+                              ranges);
 }
 
 hipError_t sort_keys_u32(uint32_t* keys_a, uint32_t* keys_b, const uint32_t* n_dev, long long capacity, int start_bit,
-                         int end_bit, void* tmp, hipStream_t s, int* result_in_b) {
+                         int end_bit, void* tmp, hipStream_t s, int* result_in_b, uint32_t* ranges) {
   return sort_pairs<uint32_t>(keys_a, nullptr, keys_b, nullptr, n_dev, capacity, start_bit, end_bit, tmp, s,
-                              result_in_b);
+                              result_in_b, ranges);
 }
 
 }  // namespace gs
