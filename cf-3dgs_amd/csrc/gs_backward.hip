@@ -1,11 +1,11 @@
 // gs_backward.hip -- backward rasteriser for gfx950.
 //
-//   k_zero_f32       clears the per-Gaussian render-gradient accumulators and
-//                    the pose accumulator (a kernel, not a memset, so the
-//                    sequence can be captured in a graph)
-//   k_bwd_items      one workgroup: work items = 256-entry chunks of each
-//                    tile's list up to the tile's largest contributor count
-//                    (entries past it never contributed)
+//   k_bwd_prepare    one launch, two jobs: workgroup 0 builds the work items
+//                    (256-entry chunks of each tile's list up to the tile's
+//                    largest contributor count, longest first, from the
+//                    forward's item log); the others clear the per-Gaussian
+//                    render-gradient accumulators (a kernel, not a memset, so
+//                    the sequence can be captured in a graph)
 //   k_render_bwd     one chunk per workgroup (16x16 pixels), back to front.
 //                    A chunk that is not the tile's last starts from the
 //                    forward's checkpoint at its end: T_b, and the suffix sums
@@ -39,10 +39,6 @@
 namespace gs {
 
 constexpr int kRG = 12;  // render-gradient record: m2x m2y ca cb cc opac r g b depth - -
-
-__global__ __launch_bounds__(256) void k_zero_f32(float* p, long long n) {
-  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256) p[k] = 0.0f;
-}
 
 struct RBArgs {
   int W, H, gx, gy, nt;
@@ -80,7 +76,7 @@ struct RBArgs {
 // the LDS atomics give: no result depends on the order of items
 // (deterministic mode writes one slot per pair; the atomic mode is
 // order-dependent in any case).
-__global__ __launch_bounds__(1024) void k_bwd_items(const uint32_t* item_log, uint32_t* items) {
+__device__ __forceinline__ void bwd_items(const uint32_t* item_log, uint32_t* items) {
   __shared__ uint32_t s_cls[64];
   const int t = threadIdx.x;
   const uint32_t n = item_log[0];
@@ -107,6 +103,20 @@ __global__ __launch_bounds__(1024) void k_bwd_items(const uint32_t* item_log, ui
     items[2 + 2 * pos] = cc >> 6;
   }
   if (t == 0) items[0] = n;
+}
+
+// The backward's set-up in one launch: workgroup 0 builds the work-item table
+// (bwd_items; none when items is null), the others clear the zn accumulator
+// floats at zp.
+__global__ __launch_bounds__(1024) void k_bwd_prepare(float* zp, long long zn, const uint32_t* item_log,
+                                                      uint32_t* items) {
+  const int first = items ? 1 : 0;
+  if (first && blockIdx.x == 0) {
+    bwd_items(item_log, items);
+    return;
+  }
+  const long long nb = (long long)gridDim.x - first;
+  for (long long k = ((long long)blockIdx.x - first) * 1024 + threadIdx.x; k < zn; k += nb * 1024) zp[k] = 0.0f;
 }
 
 // Sum over the 16 lanes of each DPP row; every lane of the row gets the sum.
@@ -898,19 +908,16 @@ static int bwd_check(const gs_frame_t* f, const gs_gaussians_t* g, const int* ra
 
 // Phase 1: clear the accumulators and build the work-item table.
 static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s) {
-  auto zero = [&](float* p, long long n) {
-    const long long nb = (n + 255) / 256;
-    if (n > 0)
-      hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)(nb < 4096 ? nb : 4096)), dim3(256), 0, s, p, n);
-  };
   // deterministic mode writes every record in k_det_gather; pairs past a
   // tile's last contributor are never visited, so their partials start at 0
-  if (f->deterministic) zero(r.part, capacity * 10);
-  else zero(r.g_render, (long long)P * kRG);
+  float* zp = f->deterministic ? r.part : r.g_render;
+  const long long zn = f->deterministic ? capacity * 10 : (long long)P * kRG;
   // the work-item table, from the forward's item log (both live in the
   // binning workspace)
-  if (capacity > 0 && P > 0)
-    hipLaunchKernelGGL(k_bwd_items, dim3(1), dim3(1024), 0, s, r.item_log, (uint32_t*)r.items);
+  uint32_t* items = capacity > 0 && P > 0 ? (uint32_t*)r.items : nullptr;
+  const long long nzb = (zn + 1023) / 1024;
+  const unsigned grid = (unsigned)((nzb < 2048 ? nzb : 2048) + (items ? 1 : 0));
+  if (grid > 0) hipLaunchKernelGGL(k_bwd_prepare, dim3(grid), dim3(1024), 0, s, zp, zn, r.item_log, items);
 }
 
 // Phase 2: the render backward of one frame.
