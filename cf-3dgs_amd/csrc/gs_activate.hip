@@ -113,7 +113,9 @@ struct ActAdam {
 };
 
 // The backward of the three activations followed by one Adam step of the raw
-// parameters: the raw gradients are never written.
+// parameters: the raw gradients are never written.  Every load is issued
+// before the first store (the arrays may not alias, but the compiler cannot
+// know that), so each thread waits for memory once.
 __global__ __launch_bounds__(256) void k_activate_bwd_adam(int P, const float* __restrict__ scales,
                                                            const float* __restrict__ opac,
                                                            const float* __restrict__ g_s,
@@ -121,33 +123,42 @@ __global__ __launch_bounds__(256) void k_activate_bwd_adam(int P, const float* _
                                                            const float* __restrict__ g_o, ActAdam a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P) return;
+  float ps[3], ms[3], vs[3], gs[3], sc[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const long long e = 3LL * i + k;
-    float p = a.raw_scale[e], m = a.sm[e], v = a.sv[e];
-    adam_one(p, act_bwd_exp(g_s[e], scales[e]), m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.s_step);
-    a.raw_scale[e] = p;
-    a.sm[e] = m;
-    a.sv[e] = v;
+    ps[k] = a.raw_scale[e];
+    ms[k] = a.sm[e];
+    vs[k] = a.sv[e];
+    gs[k] = g_s[e];
+    sc[k] = scales[e];
   }
-  {
-    float4 p = a.raw_rot[i], m = a.rm[i], v = a.rv[i];
-    const float4 d = act_bwd_normalize(p, g_r[i]);
-    adam_one(p.x, d.x, m.x, v.x, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
-    adam_one(p.y, d.y, m.y, v.y, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
-    adam_one(p.z, d.z, m.z, v.z, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
-    adam_one(p.w, d.w, m.w, v.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
-    a.raw_rot[i] = p;
-    a.rm[i] = m;
-    a.rv[i] = v;
+  float4 pr = a.raw_rot[i], mr = a.rm[i], vr = a.rv[i];
+  const float4 gr = g_r[i];
+  float po = a.raw_opac[i], mo = a.om[i], vo = a.ov[i];
+  const float go = g_o[i], oo = opac[i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    adam_one(ps[k], act_bwd_exp(gs[k], sc[k]), ms[k], vs[k], a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.s_step);
+  const float4 d = act_bwd_normalize(pr, gr);
+  adam_one(pr.x, d.x, mr.x, vr.x, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
+  adam_one(pr.y, d.y, mr.y, vr.y, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
+  adam_one(pr.z, d.z, mr.z, vr.z, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
+  adam_one(pr.w, d.w, mr.w, vr.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.r_step);
+  adam_one(po, act_bwd_sigmoid(go, oo), mo, vo, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.o_step);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const long long e = 3LL * i + k;
+    a.raw_scale[e] = ps[k];
+    a.sm[e] = ms[k];
+    a.sv[e] = vs[k];
   }
-  {
-    float p = a.raw_opac[i], m = a.om[i], v = a.ov[i];
-    adam_one(p, act_bwd_sigmoid(g_o[i], opac[i]), m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.o_step);
-    a.raw_opac[i] = p;
-    a.om[i] = m;
-    a.ov[i] = v;
-  }
+  a.raw_rot[i] = pr;
+  a.rm[i] = mr;
+  a.rv[i] = vr;
+  a.raw_opac[i] = po;
+  a.om[i] = mo;
+  a.ov[i] = vo;
 }
 
 }  // namespace gs

@@ -446,23 +446,32 @@ struct PBArgs {
   float b1, b2, eps, inv_sqrt_bc2;
 };
 
-// The fused Adam step of one Gaussian's mean (gradient gp).
+// The fused Adam step of one Gaussian's mean (gradient gp); all nine loads
+// before the first store.
 __device__ __forceinline__ void means_adam(const PBArgs& a, int i, const float gp[3]) {
+  float p[3], m[3], v[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const size_t e = 3 * (size_t)i + k;
-    float p = a.means_p[e], m = a.means_m[e], v = a.means_v[e];
-    adam_one(p, gp[k], m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.means_step);
-    a.means_p[e] = p;
-    a.means_m[e] = m;
-    a.means_v[e] = v;
+    p[k] = a.means_p[e];
+    m[k] = a.means_m[e];
+    v[k] = a.means_v[e];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t e = 3 * (size_t)i + k;
+    adam_one(p[k], gp[k], m[k], v[k], a.b1, a.b2, a.eps, a.inv_sqrt_bc2, a.means_step);
+    a.means_p[e] = p[k];
+    a.means_m[e] = m[k];
+    a.means_v[e] = v[k];
   }
 }
 
 // The fused Adam step of the workgroup's SH rows [base, base + nloc): the
 // gradients are in LDS (s_g, row floats per Gaussian), parameters and moments
 // in global memory.  Element r of a row steps with shs_lead_step when
-// r < shs_lead.  Two float4 per thread are loaded before either is updated.
+// r < shs_lead.  Four float4 of each array per thread are loaded before any
+// is updated.
 __device__ __forceinline__ void block_adam_rows(const PBArgs& a, const float* s_g, int base, int nloc, int row) {
   const size_t off = (size_t)base * row;
   float* P = a.shs_p + off;
@@ -484,6 +493,24 @@ __device__ __forceinline__ void block_adam_rows(const PBArgs& a, const float* s_
     const int n4 = total >> 2;
     const float4* g4 = reinterpret_cast<const float4*>(s_g);
     int k = threadIdx.x;
+    for (; k + 768 < n4; k += 1024) {
+      float4 p[4], m[4], v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = 4 * (k + 256 * u);
+        p[u] = ld_nt(P + q);
+        m[u] = ld_nt(M + q);
+        v[u] = ld_nt(V + q);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = 4 * (k + 256 * u);
+        upd4(p[u], g4[k + 256 * u], m[u], v[u], q);
+        st_nt(P + q, p[u]);
+        st_nt(M + q, m[u]);
+        st_nt(V + q, v[u]);
+      }
+    }
     for (; k + 256 < n4; k += 512) {
       float4 p0 = ld_nt(P + 4 * k), p1 = ld_nt(P + 4 * (k + 256));
       float4 m0 = ld_nt(M + 4 * k), m1 = ld_nt(M + 4 * (k + 256));

@@ -81,7 +81,7 @@ class _Activate(torch.autograd.Function):
         stream = ctypes.c_void_p(torch.cuda.current_stream(rr.device).cuda_stream)
         if ctx.adam is not None and all(need[:3]) and g_s is not None and g_r is not None and g_o is not None:
             # one Adam step of the raw parameters here; they get no gradient
-            ad = _lib.GsAdamAct(**ctx.adam.take())
+            ad = ctx.adam.take()
             rs_p, rr_p, ro_p = ctx.adam.params
             _lib.check(_lib.load().gs_activate_backward_adam(
                 P, _p(rs_p), _p(rr_p), _p(ro_p), _p(scales), _p(opac), _p(g_s.contiguous()), _p(_aligned(g_r)),

@@ -46,14 +46,22 @@ class FusedAdam(torch.optim.Optimizer):
         backward then never writes their gradients.  The update is this
         optimiser's, bit for bit; ``step()`` afterwards skips both, since they
         have no gradient."""
-        return _InBackward(self, means, shs)
+        return self._fused_step(_InBackward, (means, shs))
 
     def in_backward_activations(self, raw_scale: torch.Tensor, raw_rot: torch.Tensor,
                                 raw_opacity: torch.Tensor) -> "_ActInBackward":
         """The same for the raw scaling, rotation and opacity parameters: the
         activations' backward (``gs_hip.model.activate(..., adam=...)``)
         applies their Adam step itself."""
-        return _ActInBackward(self, raw_scale, raw_rot, raw_opacity)
+        return self._fused_step(_ActInBackward, (raw_scale, raw_rot, raw_opacity))
+
+    def _fused_step(self, cls, params):
+        cache = self.__dict__.setdefault("_fused_cache", {})
+        key = (cls, tuple(id(p) for p in params))
+        fs = cache.get(key)
+        if fs is None or any(a is not b for a, b in zip(fs.params, params)):
+            fs = cache[key] = cls(self, params)
+        return fs
 
     @torch.no_grad()
     def step(self, closure=None):
@@ -118,44 +126,77 @@ def _check_fused(opt: FusedAdam, params) -> tuple:
     return groups, states
 
 
-class _ActInBackward:
-    """One pending fused step of the raw (scaling, rotation, opacity);
-    ``take()`` advances their step counts and returns the gs_adam_act_t
-    fields."""
+class _FusedStep:
+    """Parameters whose Adam step a backward kernel applies.  The checks and
+    the argument struct are built once and reused (only the step count and
+    the learning rates are refreshed per step) while the parameters' and
+    moments' storage stays the same; ``take()`` advances the step counts and
+    returns the struct."""
 
-    def __init__(self, opt: FusedAdam, raw_scale, raw_rot, raw_opacity):
-        self.opt, self.params = opt, (raw_scale, raw_rot, raw_opacity)
+    def __init__(self, opt: FusedAdam, params):
+        self.opt, self.params = opt, tuple(params)
+        self._key = None
+        self._struct = None
 
-    def take(self) -> dict:
-        (gs, gr, go), states = _check_fused(self.opt, self.params)
+    def _storage_key(self):
+        key = []
+        for p in self.params:
+            st = self.opt.state.get(p)
+            key.append((p.data_ptr(), st["exp_avg"].data_ptr() if st else 0, st["exp_avg_sq"].data_ptr() if st else 0))
+        return tuple(key)
+
+    def take(self):
+        if self._storage_key() != self._key:
+            groups, states = _check_fused(self.opt, self.params)
+            self._groups, self._states = groups, states
+            self._struct = self._build(groups, states)
+            self._key = self._storage_key()
+        states = self._states
+        step = states[0]["step"]
+        if any(st["step"] != step for st in states):
+            raise RuntimeError("adam in backward: the parameters have taken different numbers of steps")
         for st in states:
-            st["step"] += 1
+            st["step"] = step + 1
+        self._refresh(self._struct, self._groups, step + 1)
+        return self._struct
+
+
+class _ActInBackward(_FusedStep):
+    """The raw (scaling, rotation, opacity): a gs_adam_act_t."""
+
+    def _build(self, groups, states):
         ss, sr, so = states
-        b1, b2 = gs["betas"]
-        return dict(scale_m=ss["exp_avg"].data_ptr(), scale_v=ss["exp_avg_sq"].data_ptr(), scale_lr=float(gs["lr"]),
-                    rot_m=sr["exp_avg"].data_ptr(), rot_v=sr["exp_avg_sq"].data_ptr(), rot_lr=float(gr["lr"]),
-                    opacity_m=so["exp_avg"].data_ptr(), opacity_v=so["exp_avg_sq"].data_ptr(),
-                    opacity_lr=float(go["lr"]), beta1=float(b1), beta2=float(b2), eps=float(gs["eps"]),
-                    step=int(ss["step"]))
+        b1, b2 = groups[0]["betas"]
+        return _lib.GsAdamAct(scale_m=ss["exp_avg"].data_ptr(), scale_v=ss["exp_avg_sq"].data_ptr(),
+                              rot_m=sr["exp_avg"].data_ptr(), rot_v=sr["exp_avg_sq"].data_ptr(),
+                              opacity_m=so["exp_avg"].data_ptr(), opacity_v=so["exp_avg_sq"].data_ptr(),
+                              beta1=float(b1), beta2=float(b2), eps=float(groups[0]["eps"]))
+
+    @staticmethod
+    def _refresh(a, groups, step):
+        gs, gr, go = groups
+        a.scale_lr, a.rot_lr, a.opacity_lr = float(gs["lr"]), float(gr["lr"]), float(go["lr"])
+        a.step = step
 
 
-class _InBackward:
-    """One pending fused step of (means, shs); ``take()`` advances their step
-    counts and returns the gs_adam_fused_t fields."""
+class _InBackward(_FusedStep):
+    """(means, shs): a gs_adam_fused_t."""
 
-    def __init__(self, opt: FusedAdam, means: torch.Tensor, shs: torch.Tensor):
-        self.opt, self.means, self.shs = opt, means, shs
-
-    def take(self) -> dict:
-        (gm, gs), (sm, ss) = _check_fused(self.opt, (self.means, self.shs))
-        sm["step"] += 1
-        ss["step"] += 1
+    def _build(self, groups, states):
+        gm, gs = groups
+        sm, ss = states
         row = int(gs.get("row", 0))
-        lead = int(gs.get("lead", 0)) if row else 0
-        if row and row != self.shs[0].numel():
+        if row and row != self.params[1][0].numel():
             raise RuntimeError("adam in backward: the SH group's row must be one Gaussian's coefficients")
         b1, b2 = gm["betas"]
-        return dict(means_m=sm["exp_avg"].data_ptr(), means_v=sm["exp_avg_sq"].data_ptr(), means_lr=float(gm["lr"]),
-                    shs_m=ss["exp_avg"].data_ptr(), shs_v=ss["exp_avg_sq"].data_ptr(), shs_lr=float(gs["lr"]),
-                    shs_lead_lr=float(gs.get("lead_lr", gs["lr"])) if row else float(gs["lr"]),
-                    shs_lead=lead, beta1=float(b1), beta2=float(b2), eps=float(gm["eps"]), step=int(sm["step"]))
+        return _lib.GsAdamFused(means_m=sm["exp_avg"].data_ptr(), means_v=sm["exp_avg_sq"].data_ptr(),
+                                shs_m=ss["exp_avg"].data_ptr(), shs_v=ss["exp_avg_sq"].data_ptr(),
+                                shs_lead=int(gs.get("lead", 0)) if row else 0, beta1=float(b1), beta2=float(b2),
+                                eps=float(gm["eps"]))
+
+    @staticmethod
+    def _refresh(a, groups, step):
+        gm, gs = groups
+        a.means_lr, a.shs_lr = float(gm["lr"]), float(gs["lr"])
+        a.shs_lead_lr = float(gs.get("lead_lr", gs["lr"])) if gs.get("row", 0) else float(gs["lr"])
+        a.step = step

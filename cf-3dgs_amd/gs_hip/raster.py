@@ -272,8 +272,9 @@ def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities
              cov3D_precomp, pose, grad_color, grad_depth=None, grad_alpha=None, want=None,
              deterministic: Optional[bool] = None, adam=None):
     """Returns a dict of gradients (None where the input was absent or, when
-    `want` (a set of names) is given, not wanted).  With ``adam`` (a dict of
-    the gs_adam_fused_t fields, ``FusedAdam.in_backward(...).take()``) the
+    `want` (a set of names) is given, not wanted).  With ``adam`` (a
+    gs_adam_fused_t, ``FusedAdam.in_backward(...).take()``, or a dict of its
+    fields) the
     means and SH coefficients take one Adam step inside the backward
     (gs_backward_adam, updated in place) and get no gradient."""
     lib = _lib.load()
@@ -337,9 +338,7 @@ def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities
     for name, t, k in (("means3D", means3D, 0), ("shs", shs, 1)):
         if t is not None and (ga.tensors[k] is None or ga.tensors[k].data_ptr() != t.data_ptr()):
             raise RuntimeError(f"adam in backward: {name} must be a contiguous float32 device tensor")
-    ad = _lib.GsAdamFused()
-    for k, v in adam.items():
-        setattr(ad, k, v)
+    ad = adam if isinstance(adam, _lib.GsAdamFused) else _lib.GsAdamFused(**adam)
     _lib.check(lib.gs_backward_adam(ctypes.byref(fa.frame), ctypes.byref(ga.g), _ptr(st.radii), _ptr(st.geom),
                                     _ptr(st.binning), st.capacity, _ptr(st.image), _ptr(st.counters), _ptr(dc),
                                     _ptr(dd), _ptr(da), ctypes.byref(gr), ctypes.byref(ad), _ptr(scratch),
