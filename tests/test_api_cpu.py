@@ -66,6 +66,19 @@ def test_c_abi_rejects_bad_arguments_without_device():
     assert lib.gs_preprocess(ctypes.byref(f), ctypes.byref(g), None, None, 0, None, None) == 1
     assert lib.gs_knn3_meandist(None, -1, None, None, 0, None) == 1
     assert lib.gs_error_string(1) == b"invalid argument"
+    # the optimiser-in-backward entry points refuse missing arguments before any launch
+    ad = _lib.GsAdamAct(step=1)
+    assert lib.gs_activate_backward_adam(-1, None, None, None, None, None, None, None, None, ctypes.byref(ad),
+                                         None) == 1
+    assert lib.gs_activate_backward_adam(10, None, None, None, None, None, None, None, None, None, None) == 1
+    assert lib.gs_activate_backward_adam(10, None, None, None, None, None, None, None, None, ctypes.byref(ad),
+                                         None) == 1
+    assert lib.gs_activate_backward_adam(0, None, None, None, None, None, None, None, None, ctypes.byref(ad),
+                                         None) == 0
+    gr = _lib.GsGrads()
+    fa = _lib.GsAdamFused(step=1)
+    assert lib.gs_backward_adam(ctypes.byref(f), ctypes.byref(g), None, None, None, 0, None, None, None, None, None,
+                                ctypes.byref(gr), ctypes.byref(fa), None, 0, None) == 1
 
 
 def _settings():
