@@ -64,6 +64,29 @@ def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, pas
     return 0.0
 
 
+TRAFFIC_PROFILE = os.path.join("profiles", "training_step_500k_960x540_hbm_traffic.md")
+STAGE_KERNEL = {"render_bwd": "k_render_bwd", "blend": "k_blend", "preprocess_bwd": "k_preprocess_bwd",
+                "preprocess": "k_preprocess"}
+
+
+def measured_traffic(stage: str):
+    """HBM bytes per launch of the stage's kernel as the committed rocprofv3
+    summary gives them (FETCH_SIZE and WRITE_SIZE runs, calibrated on a copy
+    kernel), or None when the summary has no row for it."""
+    kernel = STAGE_KERNEL.get(stage)
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    if kernel is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].strip("`").split("::")[-1].split("<")[0] == kernel:
+            try:
+                return round(float(cells[4]) * 1e6)
+            except ValueError:
+                return None
+    return None
+
+
 def build_state(device, P, W, H):
     import torch
     from gs_hip import scenes
@@ -442,9 +465,11 @@ def main():
     roofline = None
     if dom:
         d = stages[dom]
+        traffic = measured_traffic(dom)
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(d["GB_s"], 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(d["GB_s"] / HBM_PEAK_GBS, 4), "traffic": None,
-                    "ms_per_launch": round(d["ms_per_step"], 4)}
+                    "unit": "GB/s", "frac": round(d["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "traffic_unit": "bytes per launch", "traffic_source": TRAFFIC_PROFILE if traffic else None,
+                    "alg_bytes": round(d["alg_bytes"]), "ms_per_launch": round(d["ms_per_step"], 4)}
     _log(f"training step: {ms:.3f} ms")
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
     c1 = None if args.no_plain else c1_bench(device, cpu=not args.no_cpu_baseline)
