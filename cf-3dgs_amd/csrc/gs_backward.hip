@@ -257,6 +257,12 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
     // lazily at the next one, from last_alpha / last_color; same expression,
     // same rounding).
     for (int i = nw - 1; i >= 0; --i) {
+      // Gradient arithmetic may fuse multiply-adds (the library is built
+      // without contraction for the forward's bit-exactness).  What decides
+      // which pairs contribute -- blend_power, spec_expf and the single
+      // product alpha = o G -- is outside this scope or has no add, so it
+      // stays exactly the forward's.
+#pragma clang fp contract(fast)
       const int j = s_list[wv][i];
       const uint32_t q = sbase + j;  // 0-based list position; contributor index q + 1
       const float4 ga = s_a[j];
