@@ -327,7 +327,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
               if constexpr (DET) {
                 s_acc[wv * 10 + k][j] = w[m];
               } else {
-                if (w[m] != 0.0f) atomicAdd(&s_acc[k][j], w[m]);
+                atomicAdd(&s_acc[k][j], w[m]);  // (a zero adds nothing; testing for it costs more)
               }
             }
           }
