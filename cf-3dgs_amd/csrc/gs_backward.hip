@@ -157,6 +157,20 @@ __device__ __forceinline__ void wave_sum12(const float v[12], float w[3]) {
   }
 }
 
+// wave_sum12 of v[0..9] with v[10] = v[11] = 0 (the render backward's 10
+// terms): the last stage-1 sum is the literal 0, not a swap of two zeros.
+__device__ __forceinline__ void wave_sum10(const float v[10], float w[3]) {
+  float u[6];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) swap_add32(v[2 * i], v[2 * i + 1], u[i]);
+  u[5] = 0.0f;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    swap_add16(u[2 * m], u[2 * m + 1], w[m]);
+    w[m] = row_sum16(w[m]);
+  }
+}
+
 template <bool DET>
 __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) {
   constexpr int kParts = DET ? 4 : 1;  // LDS partial sets: one per wave when deterministic
@@ -271,11 +285,11 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
       // gb.w: alpha < 1/255 for sure below it
       bool active = inside && q < last && !(power > 0.0f || power < gb.w);
-      if (__ballot(active) == 0ull) continue;
+      if (__builtin_amdgcn_ballot_w64(active) == 0ull) continue;
       const float Graw = spec_expf(power);
       const float alpha_raw = fminf(0.99f, gb.y * Graw);
       active = active && !(alpha_raw < a_min);
-      if (__ballot(active) == 0ull) continue;
+      if (__builtin_amdgcn_ballot_w64(active) == 0ull) continue;
       const float G = active ? Graw : 0.0f;
       const float alpha = active ? alpha_raw : 0.0f;
       const float4 gc = s_c[j];
@@ -294,7 +308,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       const float gdx = G * dx, gdy = G * dy;
       const float dG_dx = -gdx * ga.z - gdy * ga.w;
       const float dG_dy = -gdy * gb.x - gdx * ga.w;
-      float v[12];
+      float v[10];
       v[0] = dL_dG * dG_dx * ddelx;
       v[1] = dL_dG * dG_dy * ddely;
       v[2] = -0.5f * gdx * dx * dL_dG;
@@ -305,8 +319,6 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       v[7] = w * dpix1;
       v[8] = w * dpix2;
       v[9] = w * dD;
-      v[10] = 0.f;
-      v[11] = 0.f;
       T = Tn;
       const float oma = 1.0f - alpha;
       acc0 = alpha * gc.x + oma * acc0;
@@ -316,7 +328,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       acc_a = alpha + oma * acc_a;
       {
         float w[3];
-        wave_sum12(v, w);
+        wave_sum10(v, w);
         if ((lane & 15) == 0) {
           const int q = lane >> 4;
           const int o = q == 1 ? 2 : (q == 2 ? 1 : q);  // value of row q: 4m + {0, 2, 1, 3}[q]
