@@ -546,21 +546,24 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
       const bool skip = power > 0.0f || power < gb.w || alpha < a_min;
       return skip ? 0.0f : alpha;
     };
+    // in list order; selects instead of branches (the same arithmetic: a
+    // skipped or saturating pair leaves the state as it was)
     auto apply = [&](int j, float alpha) {
-      if (alpha == 0.0f) return;
       const float test_T = T * (1.0f - alpha);
-      if (test_T < 0.0001f) {
-        done = true;
-        return;
-      }
+      const bool live = alpha != 0.0f && !done;
+      const bool stop = live && test_T < 0.0001f;
+      const bool upd = live && !stop;
       const float w = alpha * T;
       const float4 gc = s_c[j];
-      C0 = fmaf(gc.x, w, C0);
-      C1 = fmaf(gc.y, w, C1);
-      C2 = fmaf(gc.z, w, C2);
-      D = fmaf(s_b[j].z, w, D);
-      T = test_T;
-      last = first + (uint32_t)j;
+      const float dz = s_b[j].z;
+      const float n0 = fmaf(gc.x, w, C0), n1 = fmaf(gc.y, w, C1), n2 = fmaf(gc.z, w, C2), nd = fmaf(dz, w, D);
+      C0 = upd ? n0 : C0;
+      C1 = upd ? n1 : C1;
+      C2 = upd ? n2 : C2;
+      D = upd ? nd : D;
+      T = upd ? test_T : T;
+      last = upd ? first + (uint32_t)j : last;
+      done = done || stop;
     };
     // four list entries per step: their alphas do not depend on each other,
     // so the four exponentials are in flight together (the heaviest tiles run
@@ -576,9 +579,9 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
       const float a2 = left > 2 ? alpha_at(j2) : 0.0f;
       const float a3 = left > 3 ? alpha_at(j3) : 0.0f;
       apply(j0, a0);
-      if (!done) apply(j1, a1);
-      if (!done) apply(j2, a2);
-      if (!done) apply(j3, a3);
+      apply(j1, a1);
+      apply(j2, a2);
+      apply(j3, a3);
     }
   }
   if (inside) {
