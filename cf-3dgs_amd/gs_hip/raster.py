@@ -203,6 +203,20 @@ class ForwardState:
     capacity: int
     num_rendered: int
     binning_mode: int = 0
+    frame_args: Optional["FrameArgs"] = None        # the forward's, reused by the backward
+    gaussian_args: Optional["GaussianArgs"] = None
+
+
+def _same_tensors(held, given) -> bool:
+    """Whether the tensors an argument struct holds are the given ones (absent
+    or empty on both sides, or the same storage)."""
+    for h, g in zip(held, given):
+        if g is None or (isinstance(g, torch.Tensor) and g.numel() == 0):
+            if h is not None:
+                return False
+        elif h is None or not isinstance(g, torch.Tensor) or h.data_ptr() != g.data_ptr() or h.device != g.device:
+            return False
+    return True
 
 
 def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp,
@@ -264,7 +278,7 @@ def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations
                                  binning.numel(), _ptr(image), image.numel(), _ptr(color), _ptr(depth), _ptr(alpha),
                                  stream), "gs_render")
     _LAST_CAPACITY[key] = capacity
-    st = ForwardState(geom, binning, image, counters, radii, capacity, num_rendered, fa.frame.binning)
+    st = ForwardState(geom, binning, image, counters, radii, capacity, num_rendered, fa.frame.binning, fa, ga)
     return color, radii, depth, alpha, st
 
 
@@ -279,8 +293,16 @@ def backward(settings, st: ForwardState, means3D, shs, colors_precomp, opacities
     (gs_backward_adam, updated in place) and get no gradient."""
     lib = _lib.load()
     device = require_device(means3D)
-    fa = make_frame(settings, device, pose, st.binning_mode, deterministic)
-    ga = make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp, device)
+    # the forward's argument structs when they describe the same call (the
+    # autograd function passes the tensors it saved, so the pointers match)
+    want_det = int(DETERMINISTIC if deterministic is None else bool(deterministic))
+    fa = st.frame_args
+    if fa is None or fa.frame.deterministic != want_det or not _same_tensors((fa.pose,), (pose,)):
+        fa = make_frame(settings, device, pose, st.binning_mode, deterministic)
+    ga = st.gaussian_args
+    if ga is None or not _same_tensors(ga.tensors, (means3D, shs, colors_precomp, opacities, scales, rotations,
+                                                    cov3D_precomp)):
+        ga = make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, cov3D_precomp, device)
     P = ga.g.P
     M = ga.g.M
     stream = _stream(device)
