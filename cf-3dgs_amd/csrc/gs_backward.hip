@@ -485,76 +485,63 @@ __device__ __forceinline__ void means_adam(const PBArgs& a, int i, const float g
   }
 }
 
-// The fused Adam step of the workgroup's SH rows [base, base + nloc): the
-// gradients are in LDS (s_g, row floats per Gaussian), parameters and moments
-// in global memory.  Element r of a row steps with shs_lead_step when
-// r < shs_lead.  Four float4 of each array per thread are loaded before any
-// is updated.
-__device__ __forceinline__ void block_adam_rows(const PBArgs& a, const float* s_g, int base, int nloc, int row) {
+// The fused Adam step's moments for the workgroup's SH rows [base, base +
+// nloc): the gradients are in LDS (s_g, row floats per Gaussian) and each is
+// replaced there by its element's decrement step * m / denom; the moments are
+// in global memory and the parameters are not touched.  Element r of a row
+// steps with shs_lead_step when r < shs_lead.  Four float4 of each moment
+// array per thread are loaded before any is updated.
+__device__ __forceinline__ void block_adam_moments(const PBArgs& a, float* s_g, int base, int nloc, int row) {
   const size_t off = (size_t)base * row;
-  float* P = a.shs_p + off;
   float* M = a.shs_m + off;
   float* V = a.shs_v + off;
   const int total = nloc * row;
-  auto upd4 = [&](float4& p, const float4& g, float4& m, float4& v, int e) {
+  auto upd4 = [&](float4& g, float4& m, float4& v, int e) {
     const int r = e % row;
     const float s0 = r < a.shs_lead ? a.shs_lead_step : a.shs_step;
     const float s1 = r + 1 < a.shs_lead ? a.shs_lead_step : a.shs_step;
     const float s2 = r + 2 < a.shs_lead ? a.shs_lead_step : a.shs_step;
     const float s3 = r + 3 < a.shs_lead ? a.shs_lead_step : a.shs_step;
-    adam_one(p.x, g.x, m.x, v.x, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s0);
-    adam_one(p.y, g.y, m.y, v.y, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s1);
-    adam_one(p.z, g.z, m.z, v.z, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s2);
-    adam_one(p.w, g.w, m.w, v.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s3);
+    g.x = adam_delta(g.x, m.x, v.x, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s0);
+    g.y = adam_delta(g.y, m.y, v.y, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s1);
+    g.z = adam_delta(g.z, m.z, v.z, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s2);
+    g.w = adam_delta(g.w, m.w, v.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s3);
   };
   if ((row & 3) == 0) {
     const int n4 = total >> 2;
-    const float4* g4 = reinterpret_cast<const float4*>(s_g);
+    float4* g4 = reinterpret_cast<float4*>(s_g);
     int k = threadIdx.x;
     for (; k + 768 < n4; k += 1024) {
-      float4 p[4], m[4], v[4];
+      float4 m[4], v[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int q = 4 * (k + 256 * u);
-        p[u] = ld_nt(P + q);
         m[u] = ld_nt(M + q);
         v[u] = ld_nt(V + q);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int q = 4 * (k + 256 * u);
-        upd4(p[u], g4[k + 256 * u], m[u], v[u], q);
-        st_nt(P + q, p[u]);
+        float4 g = g4[k + 256 * u];
+        upd4(g, m[u], v[u], q);
+        g4[k + 256 * u] = g;
         st_nt(M + q, m[u]);
         st_nt(V + q, v[u]);
       }
     }
-    for (; k + 256 < n4; k += 512) {
-      float4 p0 = ld_nt(P + 4 * k), p1 = ld_nt(P + 4 * (k + 256));
-      float4 m0 = ld_nt(M + 4 * k), m1 = ld_nt(M + 4 * (k + 256));
-      float4 v0 = ld_nt(V + 4 * k), v1 = ld_nt(V + 4 * (k + 256));
-      upd4(p0, g4[k], m0, v0, 4 * k);
-      upd4(p1, g4[k + 256], m1, v1, 4 * (k + 256));
-      st_nt(P + 4 * k, p0);
-      st_nt(P + 4 * (k + 256), p1);
-      st_nt(M + 4 * k, m0);
-      st_nt(M + 4 * (k + 256), m1);
-      st_nt(V + 4 * k, v0);
-      st_nt(V + 4 * (k + 256), v1);
-    }
-    if (k < n4) {
-      float4 p = ld_nt(P + 4 * k), m = ld_nt(M + 4 * k), v = ld_nt(V + 4 * k);
-      upd4(p, g4[k], m, v, 4 * k);
-      st_nt(P + 4 * k, p);
+    for (; k < n4; k += 256) {
+      float4 m = ld_nt(M + 4 * k), v = ld_nt(V + 4 * k);
+      float4 g = g4[k];
+      upd4(g, m, v, 4 * k);
+      g4[k] = g;
       st_nt(M + 4 * k, m);
       st_nt(V + 4 * k, v);
     }
   } else {
     for (int k = threadIdx.x; k < total; k += 256) {
-      float p = P[k], m = M[k], v = V[k];
-      adam_one(p, s_g[k], m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2,
-               k % row < a.shs_lead ? a.shs_lead_step : a.shs_step);
-      P[k] = p;
+      float m = M[k], v = V[k];
+      s_g[k] = adam_delta(s_g[k], m, v, a.b1, a.b2, a.eps, a.inv_sqrt_bc2,
+                          k % row < a.shs_lead ? a.shs_lead_step : a.shs_step);
       M[k] = m;
       V[k] = v;
     }
@@ -572,6 +559,15 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
   const bool sh_grad = a.shs && (a.dL_dsh || a.adam_on);
   const bool staged = sh_grad && a.M <= 16;
   const int row = 3 * a.M;
+  // With the fused Adam the thread's SH gradient row (b[k] * gc[c] for the
+  // nc coefficients of degree D, zero beyond) stays in registers until the
+  // parameter row it is computed from has been read back out of LDS, so the
+  // parameters are read from global memory once.
+  const bool swap_rows = staged && a.adam_on;
+  float sb[16], sgc[3] = {0.f, 0.f, 0.f};
+  int row_nc = 0;  // coefficients whose gradient row was set (0: none)
+#pragma unroll
+  for (int k = 0; k < 16; ++k) sb[k] = 0.f;
   if (staged) {
     const float* src = a.shs + (size_t)base * row;
     const int total = nloc * row;
@@ -694,7 +690,8 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
         sh_basis(a.D, dx, dy, dz, b);
         sh_basis_grad(a.D, dx, dy, dz, bg3);
         const float* sh = staged ? s_sh + threadIdx.x * row : a.shs + (size_t)i * row;
-        float* gsh = !sh_grad ? nullptr : (staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row);
+        float* gsh = (!sh_grad || swap_rows) ? nullptr
+                                             : (staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row);
         float gdir0 = 0.f, gdir1 = 0.f, gdir2 = 0.f;
         // unrolled to the 16 coefficients of degree 3, so b / bg3 are indexed
         // by constants and live in registers (a runtime bound puts them in scratch)
@@ -715,6 +712,14 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
         }
         if (gsh)
           for (int k = 3 * nc; k < 3 * a.M; ++k) gsh[k] = 0.f;
+        if (swap_rows) {
+#pragma unroll
+          for (int k = 0; k < 16; ++k) sb[k] = k < nc ? b[k] : 0.f;
+          sgc[0] = gc[0];
+          sgc[1] = gc[1];
+          sgc[2] = gc[2];
+          row_nc = nc;
+        }
         const float ndot = dx * gdir0 + dy * gdir1 + dz * gdir2;
         gp[0] += (gdir0 - dx * ndot) / len;
         gp[1] += (gdir1 - dy * ndot) / len;
@@ -776,7 +781,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
       } else if (a.dL_dmeans3D) {
         a.dL_dmeans3D[3 * i] = a.dL_dmeans3D[3 * i + 1] = a.dL_dmeans3D[3 * i + 2] = 0.f;
       }
-      if (sh_grad) {
+      if (sh_grad && !swap_rows) {
         float* gsh = staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row;
         for (int k = 0; k < row; ++k) gsh[k] = 0.f;
       }
@@ -787,9 +792,66 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * i + k] = gS[k];
   }
-  if (staged && a.adam_on) {
+  if (swap_rows) {
+    // parameter row out of LDS, gradient row in; the moments' pass replaces
+    // the gradients by the decrements; p - decrement goes back to LDS and the
+    // block is stored with coalesced 16-byte writes.  The own-row passes use
+    // 16-byte accesses when the row is whole vectors (at a 48-word row stride
+    // a wave's scalar accesses share 4 banks).
+    float prow[48];
+    float* own = s_sh + threadIdx.x * row;
+    const bool mine = (int)threadIdx.x < nloc;
+    auto gval = [&](int e) { return e / 3 < row_nc ? sb[e / 3] * sgc[e % 3] : 0.f; };
+    if (mine) {
+      if ((row & 3) == 0) {
+        float4* own4 = reinterpret_cast<float4*>(own);
+#pragma unroll
+        for (int q = 0; q < 12; ++q)
+          if (4 * q < row) {
+            const float4 v = own4[q];
+            prow[4 * q] = v.x;
+            prow[4 * q + 1] = v.y;
+            prow[4 * q + 2] = v.z;
+            prow[4 * q + 3] = v.w;
+            own4[q] = make_float4(gval(4 * q), gval(4 * q + 1), gval(4 * q + 2), gval(4 * q + 3));
+          }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 48; ++k)
+          if (k < row) {
+            prow[k] = own[k];
+            own[k] = gval(k);
+          }
+      }
+    }
     __syncthreads();
-    block_adam_rows(a, s_sh, base, nloc, row);
+    block_adam_moments(a, s_sh, base, nloc, row);
+    __syncthreads();
+    if (mine) {
+      if ((row & 3) == 0) {
+        float4* own4 = reinterpret_cast<float4*>(own);
+#pragma unroll
+        for (int q = 0; q < 12; ++q)
+          if (4 * q < row) {
+            const float4 d = own4[q];
+            own4[q] = make_float4(prow[4 * q] - d.x, prow[4 * q + 1] - d.y, prow[4 * q + 2] - d.z,
+                                  prow[4 * q + 3] - d.w);
+          }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 48; ++k)
+          if (k < row) own[k] = prow[k] - own[k];
+      }
+    }
+    __syncthreads();
+    float* P = a.shs_p + (size_t)base * row;
+    const int total = nloc * row;
+    if ((row & 3) == 0) {
+      const float4* s4 = reinterpret_cast<const float4*>(s_sh);
+      for (int k = threadIdx.x; k < (total >> 2); k += 256) st_nt(P + 4 * k, s4[k]);
+    } else {
+      for (int k = threadIdx.x; k < total; k += 256) P[k] = s_sh[k];
+    }
   } else if (staged) {
     __syncthreads();
     float* dst = a.dL_dsh + (size_t)base * row;

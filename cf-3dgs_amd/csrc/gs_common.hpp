@@ -356,12 +356,20 @@ __device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* s_w, u
 // kernel and the backward's fused update, so both round identically):
 //   m = m + (g - m) * (1 - b1);  v = v * b2 + (1 - b2) * g * g;
 //   p = p - step * m / (sqrt(v) / sqrt(bc2) + eps),  step = lr / bc1.
-__device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, float b1, float b2, float eps,
-                                         float inv_sqrt_bc2, float step) {
+// The moments' update and the parameter's decrement step * m / denom, which
+// does not depend on the parameter (with contraction off, p - adam_delta(..)
+// is adam_one's result bit for bit).
+__device__ __forceinline__ float adam_delta(float g, float& m, float& v, float b1, float b2, float eps,
+                                            float inv_sqrt_bc2, float step) {
   m = fmaf(g - m, 1.0f - b1, m);
   v = fmaf(v, b2, (1.0f - b2) * g * g);
   const float denom = sqrtf(v) * inv_sqrt_bc2 + eps;
-  p = p - step * (m / denom);
+  return step * (m / denom);
+}
+
+__device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, float b1, float b2, float eps,
+                                         float inv_sqrt_bc2, float step) {
+  p = p - adam_delta(g, m, v, b1, b2, eps, inv_sqrt_bc2, step);
 }
 
 // 16-byte loads and stores that bypass cache retention (each byte of an
