@@ -169,6 +169,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
 }
 
 // One workgroup per digit: exclusive scan over the active blocks' counts.
+// Each thread takes kScanRun consecutive counts per round (all loaded before
+// any is used) and scans them in registers around one workgroup scan of the
+// run sums, so a row of up to 256 * kScanRun blocks costs one workgroup scan.
+constexpr int kScanRun = 8;
 __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* totals, const uint32_t* n_dev,
                                                    long long cap, int nb_max, int tile) {
   __shared__ uint32_t s_w[4];
@@ -178,14 +182,23 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* tot
   uint32_t* row = hist + (size_t)d * nb_max;
   const int t = threadIdx.x;
   uint32_t carry = 0;
-  for (int b0 = 0; b0 < nb; b0 += 256) {
-    const int b = b0 + t;
-    const uint32_t v = b < nb ? row[b] : 0u;
+  for (int b0 = 0; b0 < nb; b0 += 256 * kScanRun) {
+    const int first = b0 + t * kScanRun;
+    uint32_t v[kScanRun], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanRun; ++j) v[j] = first + j < nb ? row[first + j] : 0u;
+#pragma unroll
+    for (int j = 0; j < kScanRun; ++j) sum += v[j];
     uint32_t tot;
-    const uint32_t incl = block_incl_scan256(v, s_w, tot);
-    if (b < nb) row[b] = carry + incl - v;
+    const uint32_t incl = block_incl_scan256(sum, s_w, tot);
+    uint32_t run = carry + incl - sum;
+#pragma unroll
+    for (int j = 0; j < kScanRun; ++j) {
+      if (first + j < nb) row[first + j] = run;
+      run += v[j];
+    }
     carry += tot;
-    __syncthreads();  // s_w is rewritten by the next chunk
+    __syncthreads();  // s_w is rewritten by the next round
   }
   if (t == 0) totals[d] = carry;
 }
