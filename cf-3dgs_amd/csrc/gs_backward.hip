@@ -486,12 +486,14 @@ __device__ __forceinline__ void means_adam(const PBArgs& a, int i, const float g
 }
 
 // The fused Adam step's moments for the workgroup's SH rows [base, base +
-// nloc): the gradients are in LDS (s_g, row floats per Gaussian) and each is
+// nloc): the gradients are in LDS (s_g, rows of ldsrow >= row floats; ldsrow
+// == row unless row is a whole number of 16-byte vectors) and each is
 // replaced there by its element's decrement step * m / denom; the moments are
 // in global memory and the parameters are not touched.  Element r of a row
 // steps with shs_lead_step when r < shs_lead.  Four float4 of each moment
 // array per thread are loaded before any is updated.
-__device__ __forceinline__ void block_adam_moments(const PBArgs& a, float* s_g, int base, int nloc, int row) {
+__device__ __forceinline__ void block_adam_moments(const PBArgs& a, float* s_g, int base, int nloc, int row,
+                                                   int ldsrow) {
   const size_t off = (size_t)base * row;
   float* M = a.shs_m + off;
   float* V = a.shs_v + off;
@@ -508,8 +510,9 @@ __device__ __forceinline__ void block_adam_moments(const PBArgs& a, float* s_g, 
     g.w = adam_delta(g.w, m.w, v.w, a.b1, a.b2, a.eps, a.inv_sqrt_bc2, s3);
   };
   if ((row & 3) == 0) {
-    const int n4 = total >> 2;
+    const int n4 = total >> 2, per4 = row >> 2, lper4 = ldsrow >> 2;
     float4* g4 = reinterpret_cast<float4*>(s_g);
+    auto lds4 = [&](int k) { return (k / per4) * lper4 + k % per4; };  // vector k of the rows, in LDS
     int k = threadIdx.x;
     for (; k + 768 < n4; k += 1024) {
       float4 m[4], v[4];
@@ -522,18 +525,20 @@ __device__ __forceinline__ void block_adam_moments(const PBArgs& a, float* s_g, 
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int q = 4 * (k + 256 * u);
-        float4 g = g4[k + 256 * u];
+        const int l = lds4(k + 256 * u);
+        float4 g = g4[l];
         upd4(g, m[u], v[u], q);
-        g4[k + 256 * u] = g;
+        g4[l] = g;
         st_nt(M + q, m[u]);
         st_nt(V + q, v[u]);
       }
     }
     for (; k < n4; k += 256) {
       float4 m = ld_nt(M + 4 * k), v = ld_nt(V + 4 * k);
-      float4 g = g4[k];
+      const int l = lds4(k);
+      float4 g = g4[l];
       upd4(g, m, v, 4 * k);
-      g4[k] = g;
+      g4[l] = g;
       st_nt(M + 4 * k, m);
       st_nt(V + 4 * k, v);
     }
@@ -559,6 +564,10 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
   const bool sh_grad = a.shs && (a.dL_dsh || a.adam_on);
   const bool staged = sh_grad && a.M <= 16;
   const int row = 3 * a.M;
+  // LDS row stride: a row of whole 16-byte vectors gets one more vector, so
+  // a thread's own-row accesses, at a stride of an odd number of vectors,
+  // spread a wave over the banks (a 48-word stride puts 16 lanes on a bank)
+  const int ldsrow = (row & 3) == 0 ? row + 4 : row;
   // With the fused Adam the thread's SH gradient row (b[k] * gc[c] for the
   // nc coefficients of degree D, zero beyond) stays in registers until the
   // parameter row it is computed from has been read back out of LDS, so the
@@ -572,7 +581,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     const float* src = a.shs + (size_t)base * row;
     const int total = nloc * row;
     if ((a.M & 3) == 0) {
-      block_copy16(reinterpret_cast<float4*>(s_sh), reinterpret_cast<const float4*>(src), total >> 2);
+      rows_to_lds(s_sh, reinterpret_cast<const float4*>(src), nloc, row >> 2, ldsrow);
     } else {
       for (int k = threadIdx.x; k < total; k += 256) s_sh[k] = src[k];
     }
@@ -689,9 +698,9 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
         float b[16], bg3[16][3];
         sh_basis(a.D, dx, dy, dz, b);
         sh_basis_grad(a.D, dx, dy, dz, bg3);
-        const float* sh = staged ? s_sh + threadIdx.x * row : a.shs + (size_t)i * row;
+        const float* sh = staged ? s_sh + threadIdx.x * ldsrow : a.shs + (size_t)i * row;
         float* gsh = (!sh_grad || swap_rows) ? nullptr
-                                             : (staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row);
+                                             : (staged ? s_sh + threadIdx.x * ldsrow : a.dL_dsh + (size_t)i * row);
         float gdir0 = 0.f, gdir1 = 0.f, gdir2 = 0.f;
         // unrolled to the 16 coefficients of degree 3, so b / bg3 are indexed
         // by constants and live in registers (a runtime bound puts them in scratch)
@@ -782,7 +791,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
         a.dL_dmeans3D[3 * i] = a.dL_dmeans3D[3 * i + 1] = a.dL_dmeans3D[3 * i + 2] = 0.f;
       }
       if (sh_grad && !swap_rows) {
-        float* gsh = staged ? s_sh + threadIdx.x * row : a.dL_dsh + (size_t)i * row;
+        float* gsh = staged ? s_sh + threadIdx.x * ldsrow : a.dL_dsh + (size_t)i * row;
         for (int k = 0; k < row; ++k) gsh[k] = 0.f;
       }
       if (a.dL_dscales) a.dL_dscales[3 * i] = a.dL_dscales[3 * i + 1] = a.dL_dscales[3 * i + 2] = 0.f;
@@ -796,10 +805,9 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     // parameter row out of LDS, gradient row in; the moments' pass replaces
     // the gradients by the decrements; p - decrement goes back to LDS and the
     // block is stored with coalesced 16-byte writes.  The own-row passes use
-    // 16-byte accesses when the row is whole vectors (at a 48-word row stride
-    // a wave's scalar accesses share 4 banks).
+    // 16-byte accesses when the row is whole vectors.
     float prow[48];
-    float* own = s_sh + threadIdx.x * row;
+    float* own = s_sh + threadIdx.x * ldsrow;
     const bool mine = (int)threadIdx.x < nloc;
     auto gval = [&](int e) { return e / 3 < row_nc ? sb[e / 3] * sgc[e % 3] : 0.f; };
     if (mine) {
@@ -825,7 +833,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
       }
     }
     __syncthreads();
-    block_adam_moments(a, s_sh, base, nloc, row);
+    block_adam_moments(a, s_sh, base, nloc, row, ldsrow);
     __syncthreads();
     if (mine) {
       if ((row & 3) == 0) {
@@ -848,7 +856,8 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     const int total = nloc * row;
     if ((row & 3) == 0) {
       const float4* s4 = reinterpret_cast<const float4*>(s_sh);
-      for (int k = threadIdx.x; k < (total >> 2); k += 256) st_nt(P + 4 * k, s4[k]);
+      const int per4 = row >> 2, lper4 = ldsrow >> 2;
+      for (int k = threadIdx.x; k < (total >> 2); k += 256) st_nt(P + 4 * k, s4[(k / per4) * lper4 + k % per4]);
     } else {
       for (int k = threadIdx.x; k < total; k += 256) P[k] = s_sh[k];
     }
@@ -857,7 +866,10 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
     float* dst = a.dL_dsh + (size_t)base * row;
     const int total = nloc * row;
     if ((a.M & 3) == 0) {
-      block_copy16(reinterpret_cast<float4*>(dst), reinterpret_cast<const float4*>(s_sh), total >> 2);
+      float4* d4 = reinterpret_cast<float4*>(dst);
+      const float4* s4 = reinterpret_cast<const float4*>(s_sh);
+      const int per4 = row >> 2, lper4 = ldsrow >> 2;
+      for (int k = threadIdx.x; k < (total >> 2); k += 256) d4[k] = s4[(k / per4) * lper4 + k % per4];
     } else {
       for (int k = threadIdx.x; k < total; k += 256) dst[k] = s_sh[k];
     }
@@ -1110,7 +1122,10 @@ static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* 
     b.inv_sqrt_bc2 = ad ? ad->inv_sqrt_bc2 : 0.f;
     timing::Scope ts(timing::kPreprocessBwd, s);
     const bool sh_grad = g->shs && (out->dL_dsh || ad);
-    const size_t shm = (sh_grad && g->M <= 16) ? (size_t)256 * 3 * g->M * sizeof(float) : 0;
+    // LDS rows as the kernel lays them out: 3M floats, plus 4 when 3M is a
+    // multiple of 4
+    const size_t shm = (sh_grad && g->M <= 16) ? (size_t)256 * (3 * g->M + ((g->M & 3) == 0 ? 4 : 0)) * sizeof(float)
+                                               : 0;
     hipLaunchKernelGGL(k_preprocess_bwd, dim3((P + 255) / 256), dim3(256), shm, s, b);
   }
   // (P = 0: no wave partials, the pose gradient is zero)
