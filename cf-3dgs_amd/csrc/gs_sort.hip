@@ -10,7 +10,7 @@
 // sort bits [start_bit, end_bit) and skip every value load and store.
 //
 // Per 8-bit digit pass:
-//   k_sort_upsweep  1024 or 4096 keys per workgroup (4 waves x 4 or 16 rounds
+//   k_sort_upsweep  1024 or 2048 keys per workgroup (4 waves x 4 or 8 rounds
 //                   x 64 lanes), all loaded up front, counted with one LDS
 //                   atomic each -> per-workgroup digit counts (digit-major,
 //                   so one row scan per digit orders blocks)
@@ -294,8 +294,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
 }
 
 // Keys per workgroup: short sorts use 1024 (4 per thread) so that they still
-// fill the 256 CUs; long ones 4096 (16 per thread) to amortise the scans.
-int sort_tile(long long capacity) { return capacity <= (1LL << 21) ? kSortThreads * 4 : kSortThreads * 16; }
+// fill the 256 CUs; long ones 2048 (8 per thread).  Measured at 5.7M keys,
+// 8 per thread sorts faster than 16 or 12: the scans are cheap enough that
+// more, smaller workgroups pay.
+int sort_tile(long long capacity) { return capacity <= (1LL << 21) ? kSortThreads * 4 : kSortThreads * 8; }
 
 size_t sort_tmp_bytes(long long capacity) {
   const long long tile = sort_tile(capacity);
@@ -340,7 +342,7 @@ hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
   for (int p = 0; p < passes; ++p) {
     const int shift = start_bit + kRadixBits * p;
     const int bits = end_bit - shift < kRadixBits ? end_bit - shift : kRadixBits;
-    auto pass = tile == kSortThreads * 4 ? sort_pass<K, 4, kRadix> : sort_pass<K, 16, kRadix>;
+    auto pass = tile == kSortThreads * 4 ? sort_pass<K, 4, kRadix> : sort_pass<K, 8, kRadix>;
     pass(kin, vin, kout, vout, n_dev, capacity, shift, bits, hist, totals, nb_max, tile, s,
          p + 1 == passes ? ranges : nullptr, start_bit);
     K* tk = kin;
