@@ -294,9 +294,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
 }
 
 // Keys per workgroup: short sorts use 1024 (4 per thread) so that they still
-// fill the 256 CUs; long ones 2048 (8 per thread).  Measured at 5.7M keys,
-// 8 per thread sorts faster than 16 or 12: the scans are cheap enough that
-// more, smaller workgroups pay.
+// fill the 256 CUs (at 500k keys, 2048 per workgroup sorts slower); long
+// ones 2048 (8 per thread).  Measured at 5.7M keys, 8 per thread sorts faster
+// than 16 or 12: the scans are cheap enough that more, smaller workgroups pay.
 int sort_tile(long long capacity) { return capacity <= (1LL << 21) ? kSortThreads * 4 : kSortThreads * 8; }
 
 size_t sort_tmp_bytes(long long capacity) {
