@@ -3,6 +3,11 @@
   * world-to-view ``getWorld2View2`` (utils/graphics_utils.py:57-68): R is the
     camera-to-world rotation, t the world-to-camera translation;
   * ``getProjectionMatrix`` (utils/graphics_utils.py:85-105);
+  * the CF pipeline's camera, ``Camera(..., is_co3d=True)``
+    (scene/cameras.py:57-66, built at trainer/trainer.py:709-719):
+    ``getWorld2View3`` (utils/graphics_utils.py:71-82, R is the
+    world-to-camera rotation) and a projection from the intrinsics with the
+    principal point (cx, cy);
   * ``Camera`` (scene/cameras.py:51-74): world_view_transform = W2V^T,
     projection_matrix = P^T, full_proj_transform = W2V^T @ P^T,
     camera_center = inverse(W2V^T)[3, :3]; znear 0.01, zfar 100;
@@ -54,6 +59,33 @@ def projection_matrix(znear: float, zfar: float, fovx: float, fovy: float) -> to
     return P
 
 
+def world_to_view3(R: np.ndarray, t: np.ndarray, translate=(0.0, 0.0, 0.0), scale: float = 1.0) -> np.ndarray:
+    """getWorld2View3: as world_to_view, but R enters untransposed (it is the
+    world-to-camera rotation)."""
+    Rt = np.zeros((4, 4))
+    Rt[:3, :3] = np.asarray(R)
+    Rt[:3, 3] = t
+    Rt[3, 3] = 1.0
+    C2W = np.linalg.inv(Rt)
+    C2W[:3, 3] = (C2W[:3, 3] + np.asarray(translate)) * scale
+    return np.float32(np.linalg.inv(C2W))
+
+
+def intrinsics_projection(K, width: int, height: int, znear: float = 0.01, zfar: float = 100.0) -> torch.Tensor:
+    """The is_co3d projection (scene/cameras.py:60-66), untransposed: from the
+    float32 intrinsics fx = K[0,0], fy = K[1,1], cx = K[0,2], cy = K[1,2].
+    A point (X, Y, Z) in the camera frame lands on the pixel centre
+    (fx X / Z + cx - 0.5, fy Y / Z + cy - 0.5) of the rasteriser."""
+    K = np.asarray(K, np.float32)
+    fx, fy, cx, cy = K[0, 0], K[1, 1], K[0, 2], K[1, 2]
+    w, h = width, height
+    far, near = zfar, znear
+    return torch.tensor([[2 * fx / w, 0.0, -(w - 2 * cx) / w, 0.0],
+                         [0.0, 2 * fy / h, -(h - 2 * cy) / h, 0.0],
+                         [0.0, 0.0, far / (far - near), -(far * near) / (far - near)],
+                         [0.0, 0.0, 1.0, 0.0]]).float()
+
+
 @dataclass
 class Camera:
     image_width: int
@@ -85,6 +117,23 @@ def make_camera(R, t, fovx: float, fovy: float, width: int, height: int, znear: 
     pm = projection_matrix(znear, zfar, fovx, fovy).transpose(0, 1)
     full = (wv.unsqueeze(0).bmm(pm.unsqueeze(0))).squeeze(0)
     center = wv.inverse()[3, :3]
+    return Camera(width, height, fovx, fovy, wv.contiguous(), pm.contiguous(), full.contiguous(),
+                  center.contiguous()).to(device)
+
+
+def make_camera_co3d(R, t, K, width: int, height: int, znear: float = 0.01, zfar: float = 100.0,
+                     device="cpu") -> Camera:
+    """Camera(..., intrinsics=K, is_co3d=True) of the CF pipeline: R the
+    world-to-camera rotation (identity and t = 0 for a new frame's pose solve,
+    trainer/trainer.py:709-714), FoVx / FoVy = focal2fov of fx / fy as the
+    data loaders set them (trainer/trainer.py:108-109)."""
+    K = np.asarray(K, np.float32)
+    wv = torch.tensor(world_to_view3(np.asarray(R, np.float64), np.asarray(t, np.float64))).transpose(0, 1)
+    pm = intrinsics_projection(K, width, height, znear, zfar).transpose(0, 1)
+    full = (wv.unsqueeze(0).bmm(pm.unsqueeze(0))).squeeze(0)
+    center = wv.inverse()[3, :3]
+    fovx = focal2fov(float(K[0, 0]), width)
+    fovy = focal2fov(float(K[1, 1]), height)
     return Camera(width, height, fovx, fovy, wv.contiguous(), pm.contiguous(), full.contiguous(),
                   center.contiguous()).to(device)
 

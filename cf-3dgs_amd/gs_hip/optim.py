@@ -130,8 +130,8 @@ class _FusedStep:
     """Parameters whose Adam step a backward kernel applies.  The checks and
     the argument struct are built once and reused (only the step count and
     the learning rates are refreshed per step) while the parameters' and
-    moments' storage stays the same; ``take()`` advances the step counts and
-    returns the struct."""
+    moments' storage and the groups' betas and eps stay the same;
+    ``take()`` advances the step counts and returns the struct."""
 
     def __init__(self, opt: FusedAdam, params):
         self.opt, self.params = opt, tuple(params)
@@ -139,10 +139,14 @@ class _FusedStep:
         self._struct = None
 
     def _storage_key(self):
+        """Storage of the parameters and moments, and the groups' betas and
+        eps: a change in any of them rebuilds (and re-checks) the struct."""
         key = []
         for p in self.params:
             st = self.opt.state.get(p)
-            key.append((p.data_ptr(), st["exp_avg"].data_ptr() if st else 0, st["exp_avg_sq"].data_ptr() if st else 0))
+            g = self.opt._group_of(p)
+            key.append((p.data_ptr(), st["exp_avg"].data_ptr() if st else 0, st["exp_avg_sq"].data_ptr() if st else 0,
+                        tuple(float(b) for b in g["betas"]), float(g["eps"])))
         return tuple(key)
 
     def take(self):
