@@ -1151,6 +1151,8 @@ extern "C" const char* gs_error_string(int code) {
     case GS_E_ARG: return "invalid argument";
     case GS_E_WORKSPACE: return "workspace too small";
     case GS_E_LAUNCH: return "kernel launch failed";
+    case GS_E_STATIC_LIMIT: return "input larger than the entry point's static scratch allows";
+    case GS_E_SCRATCH: return "static scratch address lookup failed";
     default: return "unknown error";
   }
 }

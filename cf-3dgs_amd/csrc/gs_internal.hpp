@@ -14,7 +14,7 @@ constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort workgroup
 constexpr int kRadixBits = 8;  // digit width of the LSD passes: 256 buckets
 constexpr int kRadix = 1 << kRadixBits;
 
-inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 inline int bits_for(uint32_t n) {  // number of bits needed to hold values < n
   int b = 0;
   while (b < 32 && (1ull << b) < (uint64_t)n) ++b;

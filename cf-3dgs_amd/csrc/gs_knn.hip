@@ -17,6 +17,8 @@
 // first, ((b0 + b1) + b2) / 3, so the value is independent of visit order.
 #include <float.h>
 
+#include <mutex>
+
 #include "gs_common.hpp"
 #include "gs_internal.hpp"
 
@@ -27,8 +29,8 @@ struct KnnLayout {
   long long ncell_max;
 };
 
-static KnnLayout knn_layout(int P) {
-  KnnLayout L;
+constexpr KnnLayout knn_layout(int P) {
+  KnnLayout L{};
   const long long nb = (P + 255) / 256;
   L.ncell_max = (long long)P / 2 + 64;
   const long long nscan = (L.ncell_max + 1023) / 1024;
@@ -262,19 +264,8 @@ __global__ __launch_bounds__(256) void k_knn_query(int P, const KnnParams* prm, 
   out[myidx] = ((b0 + b1) + b2) / 3.0f;
 }
 
-}  // namespace gs
-
-using namespace gs;
-
-extern "C" size_t gs_knn_workspace_bytes(int P) { return knn_layout(P < 0 ? 0 : P).total; }
-
-extern "C" int gs_knn3_meandist(const float* points, int P, float* out, void* ws, size_t ws_bytes, void* stream) {
-  if (P < 0 || (P > 0 && (!points || !out || !ws))) return GS_E_ARG;
-  if (P == 0) return GS_OK;
-  const KnnLayout L = knn_layout(P);
-  if (ws_bytes < L.total) return GS_E_WORKSPACE;
-  hipStream_t s = (hipStream_t)stream;
-  char* b = (char*)ws;
+// The launch sequence over a workspace laid out by knn_layout(P).
+static int knn_launch(const float* points, int P, float* out, char* b, const KnnLayout& L, hipStream_t s) {
   float* partial = (float*)(b + L.partial);
   KnnParams* prm = (KnnParams*)(b + L.params);
   uint32_t* cell_of = (uint32_t*)(b + L.cell_of);
@@ -296,4 +287,54 @@ extern "C" int gs_knn3_meandist(const float* points, int P, float* out, void* ws
   hipLaunchKernelGGL(k_knn_scatter, dim3(nb), dim3(256), 0, s, points, P, cell_of, starts, counts, sorted);
   hipLaunchKernelGGL(k_knn_query, dim3(nb), dim3(256), 0, s, P, prm, starts, cell_of, sorted, out);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+// Scratch of gs_knn3_meandist_static: the whole workspace of
+// gs_knn3_meandist for GS_KNN_STATIC_MAX_POINTS points (about 24 MiB).  Every
+// GPU the library runs on holds its own copy until the process ends.  Host
+// code never passes this name to a kernel: it is a host-side stand-in, and
+// the device address comes from hipGetSymbolAddress (knn_static_scratch).
+static __device__ __align__(256) unsigned char scratch[knn_layout(GS_KNN_STATIC_MAX_POINTS).total];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on that device and kept.  nullptr when the lookup fails.
+void* knn_static_scratch() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" size_t gs_knn_workspace_bytes(int P) { return knn_layout(P < 0 ? 0 : P).total; }
+
+extern "C" int gs_knn3_meandist(const float* points, int P, float* out, void* ws, size_t ws_bytes, void* stream) {
+  if (P < 0 || (P > 0 && (!points || !out || !ws))) return GS_E_ARG;
+  if (P == 0) return GS_OK;
+  const KnnLayout L = knn_layout(P);
+  if (ws_bytes < L.total) return GS_E_WORKSPACE;
+  return knn_launch(points, P, out, (char*)ws, L, (hipStream_t)stream);
+}
+
+extern "C" int gs_knn3_meandist_static(const float* points, int P, float* out, void* stream) {
+  if (P < 0 || (P > 0 && (!points || !out))) return GS_E_ARG;
+  if (P > GS_KNN_STATIC_MAX_POINTS) return GS_E_STATIC_LIMIT;
+  if (P == 0) return GS_OK;
+  void* ws = knn_static_scratch();
+  if (!ws) return GS_E_SCRATCH;
+  return knn_launch(points, P, out, (char*)ws, knn_layout(P), (hipStream_t)stream);
 }

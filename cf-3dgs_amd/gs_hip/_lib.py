@@ -139,6 +139,7 @@ EXPORTS = {
                                   c_size_t, ctypes.POINTER(c_int), c_void_p]),
     "gs_knn_workspace_bytes": (c_size_t, [c_int]),
     "gs_knn3_meandist": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "gs_knn3_meandist_static": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "gs_loss_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
     "gs_ssim_l1_forward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_float), c_float,
                                    c_int, c_void_p, c_void_p, c_size_t, c_void_p]),

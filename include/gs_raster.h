@@ -32,6 +32,8 @@ extern "C" {
 #define GS_E_ARG 1       /* bad argument (shape, null pointer, size) */
 #define GS_E_WORKSPACE 2 /* workspace smaller than the layout requires */
 #define GS_E_LAUNCH 3    /* the runtime refused a launch */
+#define GS_E_STATIC_LIMIT 4 /* more items than an entry point's static scratch holds */
+#define GS_E_SCRATCH 5      /* the device address of a static scratch could not be obtained */
 
 /* counters: uint32[4] on the device.
  *   [0] num_rendered of the last preprocess
@@ -291,6 +293,19 @@ int gs_sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, uint
  * Exact (same value as a brute-force search).  ws: gs_knn_workspace_bytes(P). */
 size_t gs_knn_workspace_bytes(int P);
 int gs_knn3_meandist(const float* points, int P, float* out, void* ws, size_t ws_bytes, void* stream);
+
+/* gs_knn3_meandist without a workspace argument: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * under graph capture, serving loops).  Its workspace is a static device
+ * array of the library (about 24 MiB, one per GPU, held until the process
+ * ends), sized for at most GS_KNN_STATIC_MAX_POINTS points; more returns
+ * GS_E_STATIC_LIMIT before anything is enqueued.  It allocates nothing.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (two streams, two threads) race and must be serialised by the caller.
+ * The stream must belong to the current device.  Same result as
+ * gs_knn3_meandist, bit for bit. */
+#define GS_KNN_STATIC_MAX_POINTS (1 << 20)
+int gs_knn3_meandist_static(const float* points, int P, float* out, void* stream);
 
 /* Frame ingest: a depth map and its image become a frame's initial Gaussians
  * on the device.  The reference does this on the host: kornia depth_to_3d,

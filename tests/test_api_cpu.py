@@ -65,6 +65,12 @@ def test_c_abi_rejects_bad_arguments_without_device():
     g = _lib.GsGaussians()
     assert lib.gs_preprocess(ctypes.byref(f), ctypes.byref(g), None, None, 0, None, None) == 1
     assert lib.gs_knn3_meandist(None, -1, None, None, 0, None) == 1
+    # the no-workspace kNN refuses more points than its static scratch holds
+    # before it touches a device (the pointers are never read)
+    one = ctypes.c_void_p(16)
+    assert lib.gs_knn3_meandist_static(one, (1 << 20) + 1, one, None) == 4
+    assert b"static scratch" in lib.gs_error_string(4)
+    assert lib.gs_knn3_meandist_static(None, 5, None, None) == 1
     assert lib.gs_error_string(1) == b"invalid argument"
     # the optimiser-in-backward entry points refuse missing arguments before any launch
     ad = _lib.GsAdamAct(step=1)
