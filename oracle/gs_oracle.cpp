@@ -386,27 +386,64 @@ long long bin(int P, int W, int H, const int* radii, const T* xy, const T* depth
         ++o;
       }
   }
-  std::vector<long long> idx(n);
-  for (long long k = 0; k < n; ++k) idx[k] = k;
-  std::stable_sort(idx.begin(), idx.end(), [&](long long a, long long b) { return keys_dup[a] < keys_dup[b]; });
-  for (long long k = 0; k < n; ++k) {
-    keys[k] = keys_dup[idx[k]];
-    vals[k] = vals_dup[idx[k]];
+  // Stable sort of the 64-bit (tile, depth) keys, in parallel: a stable
+  // counting sort on the tile (each thread scatters its own contiguous chunk
+  // of the list, in order, from its own per-tile offsets), then a stable
+  // sort on the depth bits inside every tile.  The result is the order of a
+  // stable sort of the whole keys: equal keys keep their list order.
+  const int nthr = std::max(1, omp_get_max_threads());
+  std::vector<long long> cnt((size_t)nthr * Tn + 1, 0);
+#pragma omp parallel num_threads(nthr)
+  {
+    const int t = omp_get_thread_num();
+    const long long k0 = n * t / nthr, k1 = n * (t + 1) / nthr;
+    long long* c = cnt.data() + (size_t)t * Tn;
+    for (long long k = k0; k < k1; ++k) ++c[keys_dup[k] >> 32];
+  }
+  // exclusive prefix in (tile, thread) order
+  std::vector<long long> start((size_t)nthr * Tn, 0);
+  long long run = 0;
+  for (int tile = 0; tile < Tn; ++tile)
+    for (int t = 0; t < nthr; ++t) {
+      start[(size_t)t * Tn + tile] = run;
+      run += cnt[(size_t)t * Tn + tile];
+    }
+#pragma omp parallel num_threads(nthr)
+  {
+    const int t = omp_get_thread_num();
+    const long long k0 = n * t / nthr, k1 = n * (t + 1) / nthr;
+    long long* o = start.data() + (size_t)t * Tn;
+    for (long long k = k0; k < k1; ++k) {
+      const long long d = o[keys_dup[k] >> 32]++;
+      keys[d] = keys_dup[k];
+      vals[d] = vals_dup[k];
+    }
+  }
+  std::vector<long long> tile_start(Tn + 1, 0);
+  for (int tile = 0; tile < Tn; ++tile) tile_start[tile + 1] = tile_start[tile] + [&] {
+      long long c = 0;
+      for (int t = 0; t < nthr; ++t) c += cnt[(size_t)t * Tn + tile];
+      return c;
+    }();
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int tile = 0; tile < Tn; ++tile) {
+    const long long a = tile_start[tile], b = tile_start[tile + 1];
+    if (b - a < 2) continue;
+    std::vector<std::pair<uint32_t, uint32_t>> kv((size_t)(b - a));
+    for (long long k = a; k < b; ++k) kv[k - a] = {(uint32_t)keys[k], vals[k]};
+    std::stable_sort(kv.begin(), kv.end(), [](const std::pair<uint32_t, uint32_t>& x,
+                                               const std::pair<uint32_t, uint32_t>& y) { return x.first < y.first; });
+    for (long long k = a; k < b; ++k) {
+      keys[k] = ((uint64_t)(uint32_t)tile << 32) | kv[k - a].first;
+      vals[k] = kv[k - a].second;
+    }
   }
   std::memset(ranges, 0, sizeof(uint32_t) * 2 * (size_t)Tn);
-  for (long long k = 0; k < n; ++k) {
-    const uint32_t t = (uint32_t)(keys[k] >> 32);
-    if (k == 0) {
-      ranges[2 * t] = 0;
-    } else {
-      const uint32_t tp = (uint32_t)(keys[k - 1] >> 32);
-      if (t != tp) {
-        ranges[2 * tp + 1] = (uint32_t)k;
-        ranges[2 * t] = (uint32_t)k;
-      }
+  for (int tile = 0; tile < Tn; ++tile)
+    if (tile_start[tile + 1] > tile_start[tile]) {
+      ranges[2 * tile] = (uint32_t)tile_start[tile];
+      ranges[2 * tile + 1] = (uint32_t)tile_start[tile + 1];
     }
-    if (k == n - 1) ranges[2 * t + 1] = (uint32_t)n;
-  }
   return n;
 }
 

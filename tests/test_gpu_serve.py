@@ -77,3 +77,28 @@ def test_request_larger_than_budget_is_rejected():
     stats = Server(scenes, budget_bytes=budget).run([big, small], realtime=False)
     assert stats["rejected"] == 1 and big.error and big.result is None
     assert small.result is not None
+
+
+def test_mix_equals_alone_with_a_large_scene_at_1080p():
+    """The serving workload's large end: a 1.2M-Gaussian scene rendered at
+    1920x1080 and 960x540 beside a 200k scene, with a 480x270 pose solve of
+    200 iterations in the same ticks.  Every result equals its run alone bit
+    for bit."""
+    print("serve: 1.2M scene at 1920x1080 in a mix", flush=True)
+    scenes = [make_scene(0, 1_200_000, DEV), make_scene(1, 200_000, DEV)]
+    reqs = [Request(rid=0, kind="render", arrival=0.0, scene=0, W=1920, H=1080, seed=11),
+            Request(rid=1, kind="solve", arrival=0.0, W=480, H=270, seed=12),
+            Request(rid=2, kind="render", arrival=0.0, scene=1, W=1920, H=1080, seed=13),
+            Request(rid=3, kind="render", arrival=0.0, scene=0, W=960, H=540, seed=14),
+            Request(rid=4, kind="render", arrival=0.0, scene=1, W=480, H=270, seed=15)]
+    kw = dict(solve_iters=200, iters_per_tick=10, streams=4)
+    stats = Server(scenes, **kw).run(reqs, realtime=False)
+    print(stats, flush=True)
+    assert stats["requests"] == len(reqs) and stats["rejected"] == 0
+    for r in reqs:
+        alone = copy.copy(r)
+        alone.result = None
+        alone.done = None
+        alone.waited_for_memory = False
+        Server(scenes, **kw).run([alone], realtime=False)
+        assert torch.equal(alone.result, r.result), (r.kind, r.rid)
