@@ -23,6 +23,11 @@
 //                   order, then written out by consecutive threads: each
 //                   digit's run leaves as contiguous stores instead of one
 //                   scattered 4-byte store per lane.
+// Upsweep and scatter map workgroups to key blocks XCD-aware (xcd_remap):
+// dispatch deals workgroups round-robin over the 8 XCDs, and each XCD gets a
+// contiguous run of blocks.  A cache line of a digit's counts covers 32
+// consecutive blocks, so it is then written and read within one XCD's L2
+// instead of once by each of the eight.
 #include "gs_common.hpp"
 #include "gs_internal.hpp"
 
@@ -69,8 +74,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
                                                                int nb_max) {
   __shared__ uint32_t s_h[4][R];
   const uint32_t n = n_eff(n_dev, cap);
-  const uint32_t start = blockIdx.x * (uint32_t)(kSortThreads * ITEMS);
-  if (start >= n) return;
+  constexpr uint32_t TILE = kSortThreads * ITEMS;
+  const uint32_t nblk = (n + TILE - 1) / TILE;
+  if (blockIdx.x >= nblk) return;
+  const uint32_t blk = xcd_remap(blockIdx.x, nblk);  // consecutive blocks on one XCD (see the file comment)
+  const uint32_t start = blk * TILE;
   for (int k = threadIdx.x; k < 4 * R; k += kSortThreads) (&s_h[0][0])[k] = 0;
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -165,7 +173,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
   }
   __syncthreads();
   for (int d = threadIdx.x; d < R; d += kSortThreads)
-    hist[(size_t)d * nb_max + blockIdx.x] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
+    hist[(size_t)d * nb_max + blk] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
 }
 
 // One workgroup per digit: exclusive scan over the active blocks' counts.
@@ -220,14 +228,16 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   __shared__ K s_key[TILE];
   __shared__ uint32_t s_val[V ? TILE : 1];  // keys-only sorts carry no values
   const uint32_t n = n_eff(n_dev, cap);
-  const uint32_t start = blockIdx.x * (uint32_t)TILE;
-  if (start >= n) return;
+  const uint32_t nblk = (n + (uint32_t)TILE - 1) / (uint32_t)TILE;
+  if (blockIdx.x >= nblk) return;
+  const uint32_t blk = xcd_remap(blockIdx.x, nblk);  // the upsweep's mapping
+  const uint32_t start = blk * (uint32_t)TILE;
   const int t = threadIdx.x;
   // global base of each digit for this block = exclusive scan of totals + row offset
   {
     const uint32_t v = totals[t];
     uint32_t tot;
-    s_scan[t] = block_incl_scan256(v, s_w, tot) - v + hist[(size_t)t * nb_max + blockIdx.x];
+    s_scan[t] = block_incl_scan256(v, s_w, tot) - v + hist[(size_t)t * nb_max + blk];
   }
   for (int k = t; k < 4 * R; k += kSortThreads) (&s_cnt[0][0])[k] = 0;
   __syncthreads();
