@@ -171,8 +171,34 @@ __device__ __forceinline__ void wave_sum10(const float v[10], float w[3]) {
   }
 }
 
-template <bool DET>
+// DEP: the frame has a depth or an alpha cotangent.  Without one (a colour
+// loss, as in the training step and the pose solve) the depth and alpha
+// terms, their running sums and the depth gradient are identically zero and
+// are not computed.
+// wave_sum10 restricted to v[0..4] (the pose-only backward's terms): each of
+// these five sums is formed by the same operations, in the same lanes, as in
+// wave_sum10, so the two give the same bits.  w[0] rows hold values
+// {0, 2, 1, 3}, row 0 of w[1] value 4.
+__device__ __forceinline__ void wave_sum5(const float v[5], float w[2]) {
+  float u[3];
+  swap_add32(v[0], v[1], u[0]);
+  swap_add32(v[2], v[3], u[1]);
+  swap_add32(v[4], 0.0f, u[2]);
+  swap_add16(u[0], u[1], w[0]);
+  w[0] = row_sum16(w[0]);
+  swap_add16(u[2], 0.0f, w[1]);
+  w[1] = row_sum16(w[1]);
+}
+
+template <bool DET, bool DEP, bool PO>
 __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) {
+  // PO (pose-only, fixed-order, colour cotangent only): the caller wants only
+  // the pose gradient, which at active SH degree 0 depends on a splat's
+  // screen position and conic alone, so only those five terms are summed and
+  // stored per (Gaussian, tile) pair.  Their arithmetic is the full path's,
+  // term for term, so the pose gradient has the same bits either way.
+  static_assert(!PO || (DET && !DEP), "the pose-only backward is fixed-order and colour-only");
+  constexpr int kTerms = PO ? 5 : 10;
   constexpr int kParts = DET ? 4 : 1;  // LDS partial sets: one per wave when deterministic
   // The item's 256-entry chunk is staged 128 entries at a time (back half
   // first): half the LDS per workgroup, so twice as many workgroups fit on a
@@ -184,7 +210,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
   __shared__ float4 s_c[kSub];
   __shared__ uint32_t s_mask[kSub];    // quadrants each staged splat can reach (splat_quads)
   __shared__ uint8_t s_list[4][kSub];  // per wave: the staged splats that reach its pixels
-  __shared__ float s_acc[10 * kParts][kSub];
+  __shared__ float s_acc[kTerms * kParts][kSub];
   if (item >= a.items[0]) return;
   const int tile = (int)a.items[1 + 2 * item];
   const uint32_t chunk = a.items[2 + 2 * item];
@@ -211,8 +237,10 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
     dpix0 = a.dL_dcolor[pix];
     dpix1 = a.dL_dcolor[HW + pix];
     dpix2 = a.dL_dcolor[2 * HW + pix];
-    if (a.dL_ddepth) dD = a.dL_ddepth[pix];
-    if (a.dL_dalpha) dA = a.dL_dalpha[pix];
+    if constexpr (DEP) {
+      if (a.dL_ddepth) dD = a.dL_ddepth[pix];
+      if (a.dL_dalpha) dA = a.dL_dalpha[pix];
+    }
     if (chunk + 1 < nchunk) {
       // suffix sums from the forward's state at the boundary: sum_{k >= b} c_k a_k T_k = C_final - C_b
       const float* ck = a.ckpt + (size_t)(a.ckpt_base[tile] + chunk) * 5 * 256 + threadIdx.x;
@@ -222,8 +250,10 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       acc0 = (a.accum[pix] - ck[256]) * inv;
       acc1 = (a.accum[HW + pix] - ck[512]) * inv;
       acc2 = (a.accum[2 * HW + pix] - ck[768]) * inv;
-      acc_d = (a.accum[3 * HW + pix] - ck[1024]) * inv;
-      acc_a = 1.0f - T_fin * inv;
+      if constexpr (DEP) {
+        acc_d = (a.accum[3 * HW + pix] - ck[1024]) * inv;
+        acc_a = 1.0f - T_fin * inv;
+      }
     }
   }
   const float bg_dot = (a.bg[0] * dpix0 + a.bg[1] * dpix1) + a.bg[2] * dpix2;
@@ -256,7 +286,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       }
       s_mask[threadIdx.x] = m;
     }
-    for (int k = threadIdx.x; k < 10 * kParts * kSub; k += 256) (&s_acc[0][0])[k] = 0.0f;
+    for (int k = threadIdx.x; k < kTerms * kParts * kSub; k += 256) (&s_acc[0][0])[k] = 0.0f;
     __syncthreads();
     // the wave's own list: splats that miss its 8 x 8 pixels are ones no lane would use
     const int lim = wlast > sbase ? (int)min(wlast - sbase, (uint32_t)kSub) : 0;
@@ -300,8 +330,10 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       float dL_da = (gc.x - acc0) * dpix0;
       dL_da += (gc.y - acc1) * dpix1;
       dL_da += (gc.z - acc2) * dpix2;
-      dL_da += (gb.z - acc_d) * dD;
-      dL_da += (1.0f - acc_a) * dA;
+      if constexpr (DEP) {
+        dL_da += (gb.z - acc_d) * dD;
+        dL_da += (1.0f - acc_a) * dA;
+      }
       dL_da *= Tn;
       dL_da += (-T_fin * inv_1ma) * bg_dot;
       const float dL_dG = gb.y * dL_da;
@@ -314,19 +346,32 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       v[2] = -0.5f * gdx * dx * dL_dG;
       v[3] = -gdx * dy * dL_dG;
       v[4] = -0.5f * gdy * dy * dL_dG;
-      v[5] = G * dL_da;
-      v[6] = w * dpix0;
-      v[7] = w * dpix1;
-      v[8] = w * dpix2;
-      v[9] = w * dD;
+      if constexpr (!PO) {
+        v[5] = G * dL_da;
+        v[6] = w * dpix0;
+        v[7] = w * dpix1;
+        v[8] = w * dpix2;
+        v[9] = DEP ? w * dD : 0.0f;
+      }
       T = Tn;
       const float oma = 1.0f - alpha;
       acc0 = alpha * gc.x + oma * acc0;
       acc1 = alpha * gc.y + oma * acc1;
       acc2 = alpha * gc.z + oma * acc2;
-      acc_d = alpha * gb.z + oma * acc_d;
-      acc_a = alpha + oma * acc_a;
-      {
+      if constexpr (DEP) {
+        acc_d = alpha * gb.z + oma * acc_d;
+        acc_a = alpha + oma * acc_a;
+      }
+      if constexpr (PO) {
+        float w2[2];
+        wave_sum5(v, w2);
+        if ((lane & 15) == 0) {
+          const int q = lane >> 4;
+          const int o = q == 1 ? 2 : (q == 2 ? 1 : q);
+          s_acc[wv * kTerms + o][j] = w2[0];
+          if (q == 0) s_acc[wv * kTerms + 4][j] = w2[1];
+        }
+      } else {
         float w[3];
         wave_sum10(v, w);
         if ((lane & 15) == 0) {
@@ -335,7 +380,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
 #pragma unroll
           for (int m = 0; m < 3; ++m) {
             const int k = 4 * m + o;
-            if (k < 10) {
+            if (k < (DEP ? 10 : 9)) {
               if constexpr (DET) {
                 s_acc[wv * 10 + k][j] = w[m];
               } else {
@@ -360,10 +405,10 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
         const uint32_t ntg = w * (uint32_t)(y1 - y0);
         const uint32_t slot =
             a.block_excl[g >> 8] + a.local_incl[g] - ntg + (uint32_t)(ty - y0) * w + (uint32_t)(tx - x0);
-        float* dst = a.part + (size_t)slot * 10;
+        float* dst = a.part + (size_t)slot * kTerms;
 #pragma unroll
-        for (int k = 0; k < 10; ++k)
-          dst[k] = ((s_acc[k][t] + s_acc[10 + k][t]) + s_acc[20 + k][t]) + s_acc[30 + k][t];
+        for (int k = 0; k < kTerms; ++k)
+          dst[k] = ((s_acc[k][t] + s_acc[kTerms + k][t]) + s_acc[2 * kTerms + k][t]) + s_acc[3 * kTerms + k][t];
       } else {
         float* dst = a.g_render + (size_t)s_id[t] * kRG;
 #pragma unroll
@@ -379,14 +424,14 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
 // The grid is an upper bound on the item count, so items are taken in
 // dispatch order: live items then go round-robin over all 8 XCDs (a
 // contiguous-per-XCD remap would put them all on the first few).
-template <bool DET>
+template <bool DET, bool DEP, bool PO>
 __global__ __launch_bounds__(256) void k_render_bwd(RBArgs a) {
-  render_bwd_item<DET>(a, blockIdx.x);
+  render_bwd_item<DET, DEP, PO>(a, blockIdx.x);
 }
 
 // Many frames in one launch: frame f owns blocks [prefix[f], prefix[f + 1]),
 // an upper bound on its work items.
-template <bool DET>
+template <bool DET, bool DEP, bool PO>
 __global__ __launch_bounds__(256) void k_render_bwd_multi(const uint32_t* prefix, const RBArgs* args, int n) {
   const uint32_t b = blockIdx.x;
   int lo = 0, hi = n - 1;
@@ -396,12 +441,14 @@ __global__ __launch_bounds__(256) void k_render_bwd_multi(const uint32_t* prefix
     else hi = mid - 1;
   }
   const RBArgs a = args[lo];
-  render_bwd_item<DET>(a, b - prefix[lo]);
+  render_bwd_item<DET, DEP, PO>(a, b - prefix[lo]);
 }
 
 // Deterministic mode: per Gaussian, the partials of its (Gaussian, tile)
-// pairs summed in rect order into its render-gradient record.  Nothing is
-// read when the frame overflowed the binning capacity (no pairs were made).
+// pairs (NT terms each: 10, or the pose-only backward's 5) summed in rect
+// order into its render-gradient record.  Nothing is read when the frame
+// overflowed the binning capacity (no pairs were made).
+template <int NT>
 __global__ __launch_bounds__(256) void k_det_gather(int P, const uint32_t* tiles, const uint32_t* local_incl,
                                                     const uint32_t* block_excl, const uint32_t* counters,
                                                     long long capacity, const float* part, float* g_render) {
@@ -414,9 +461,9 @@ __global__ __launch_bounds__(256) void k_det_gather(int P, const uint32_t* tiles
   if ((long long)counters[0] <= capacity && nt > 0) {
     const uint32_t off = block_excl[g >> 8] + local_incl[g] - nt;
     for (uint32_t e = off; e < off + nt; ++e) {
-      const float* src = part + (size_t)e * 10;
+      const float* src = part + (size_t)e * NT;
 #pragma unroll
-      for (int k = 0; k < 10; ++k) acc[k] += src[k];
+      for (int k = 0; k < NT; ++k) acc[k] += src[k];
     }
   }
   float4* dst = reinterpret_cast<float4*>(g_render + (size_t)g * kRG);
@@ -1002,6 +1049,17 @@ static long long rb_max_items(const gs_frame_t* f, long long capacity) {
   return capacity / 256 + (long long)gx * gy + 1;
 }
 
+// The pose-only backward applies when the fixed-order reductions are on, only
+// the pose gradient is wanted, the loss has no depth or alpha term, and the
+// colours do not depend on the view direction (active SH degree 0, or
+// precomputed colours): the pose gradient then reads only the screen-position
+// and conic terms of each pair.
+static bool pose_only(const gs_frame_t* f, const gs_gaussians_t* g, const gs_grads_t* out, bool dep) {
+  return f->deterministic && f->pose && out->dL_dpose && !dep && !out->dL_dmeans2D && !out->dL_dcolors &&
+         !out->dL_dopacity && !out->dL_dmeans3D && !out->dL_dcov3D && !out->dL_dsh && !out->dL_dscales &&
+         !out->dL_drotations && (f->sh_degree == 0 || !g->shs);
+}
+
 static int bwd_check(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
                      void* binning, long long capacity, const void* image, const uint32_t* counters,
                      const float* dL_dcolor, const gs_grads_t* out, void* scratch, size_t scratch_bytes) {
@@ -1026,11 +1084,11 @@ static int bwd_check(const gs_frame_t* f, const gs_gaussians_t* g, const int* ra
 }
 
 // Phase 1: clear the accumulators and build the work-item table.
-static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s) {
+static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s, bool po) {
   // deterministic mode writes every record in k_det_gather; pairs past a
   // tile's last contributor are never visited, so their partials start at 0
   float* zp = f->deterministic ? r.part : r.g_render;
-  const long long zn = f->deterministic ? capacity * 10 : (long long)P * kRG;
+  const long long zn = f->deterministic ? capacity * (po ? 5 : 10) : (long long)P * kRG;
   // the work-item table, from the forward's item log (both live in the
   // binning workspace)
   uint32_t* items = capacity > 0 && P > 0 ? (uint32_t*)r.items : nullptr;
@@ -1040,14 +1098,21 @@ static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const RB
 }
 
 // Phase 2: the render backward of one frame.
-static void bwd_render(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s) {
+static void bwd_render(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s, bool po) {
   if (capacity <= 0 || P <= 0) return;
   const long long max_items = rb_max_items(f, capacity);
   timing::Scope ts(timing::kRenderBwd, s);
-  if (f->deterministic)
-    hipLaunchKernelGGL(k_render_bwd<true>, dim3((unsigned)max_items), dim3(256), 0, s, r);
-  else
-    hipLaunchKernelGGL(k_render_bwd<false>, dim3((unsigned)max_items), dim3(256), 0, s, r);
+  const dim3 grid((unsigned)max_items);
+  const bool dep = r.dL_ddepth || r.dL_dalpha;
+  if (po) {
+    hipLaunchKernelGGL((k_render_bwd<true, false, true>), grid, dim3(256), 0, s, r);
+  } else if (f->deterministic) {
+    if (dep) hipLaunchKernelGGL((k_render_bwd<true, true, false>), grid, dim3(256), 0, s, r);
+    else hipLaunchKernelGGL((k_render_bwd<true, false, false>), grid, dim3(256), 0, s, r);
+  } else {
+    if (dep) hipLaunchKernelGGL((k_render_bwd<false, true, false>), grid, dim3(256), 0, s, r);
+    else hipLaunchKernelGGL((k_render_bwd<false, false, false>), grid, dim3(256), 0, s, r);
+  }
 }
 
 // Phase 3: per-Gaussian gradients and the pose gradient.
@@ -1061,16 +1126,22 @@ struct AdamFusedHost {
 
 static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
                        long long capacity, const uint32_t* counters, const gs_grads_t* out, const RBArgs& r,
-                       hipStream_t s, const AdamFusedHost* ad = nullptr) {
+                       hipStream_t s, bool po, const AdamFusedHost* ad = nullptr) {
   const int P = g->P;
   gs_geom_layout_t LG;
   gs_geom_layout(P, &LG);
   const char* gb = (const char*)geom;
   float* pose_part = (float*)((char*)r.g_render + align_up((size_t)P * kRG * sizeof(float), 256));
-  if (f->deterministic && P > 0)
-    hipLaunchKernelGGL(k_det_gather, dim3((P + 255) / 256), dim3(256), 0, s, P,
-                       (const uint32_t*)(gb + LG.tiles), (const uint32_t*)(gb + LG.offsets),
-                       (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, (const float*)r.part, r.g_render);
+  if (f->deterministic && P > 0) {
+    if (po)
+      hipLaunchKernelGGL(k_det_gather<5>, dim3((P + 255) / 256), dim3(256), 0, s, P,
+                         (const uint32_t*)(gb + LG.tiles), (const uint32_t*)(gb + LG.offsets),
+                         (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, (const float*)r.part, r.g_render);
+    else
+      hipLaunchKernelGGL(k_det_gather<10>, dim3((P + 255) / 256), dim3(256), 0, s, P,
+                         (const uint32_t*)(gb + LG.tiles), (const uint32_t*)(gb + LG.offsets),
+                         (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, (const float*)r.part, r.g_render);
+  }
   if (P > 0) {
     PBArgs b;
     b.P = P;
@@ -1146,9 +1217,10 @@ extern "C" int gs_backward(GS_BWD_ARGS) {
   if (rc != GS_OK) return rc;
   hipStream_t s = (hipStream_t)stream;
   const RBArgs r = make_rb_args(f, g->P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
-  bwd_prepare(f, g->P, capacity, r, s);
-  bwd_render(f, g->P, capacity, r, s);
-  bwd_finish(f, g, radii, geom, capacity, counters, out, r, s);
+  const bool po = pose_only(f, g, out, dL_ddepth || dL_dalpha);
+  bwd_prepare(f, g->P, capacity, r, s, po);
+  bwd_render(f, g->P, capacity, r, s, po);
+  bwd_finish(f, g, radii, geom, capacity, counters, out, r, s, po);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
@@ -1183,9 +1255,9 @@ extern "C" int gs_backward_adam(const gs_frame_t* f, const gs_gaussians_t* g, co
   ad.inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
   hipStream_t s = (hipStream_t)stream;
   const RBArgs r = make_rb_args(f, P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
-  bwd_prepare(f, P, capacity, r, s);
-  bwd_render(f, P, capacity, r, s);
-  bwd_finish(f, g, radii, geom, capacity, counters, out, r, s, &ad);
+  bwd_prepare(f, P, capacity, r, s, false);
+  bwd_render(f, P, capacity, r, s, false);
+  bwd_finish(f, g, radii, geom, capacity, counters, out, r, s, false, &ad);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
@@ -1194,7 +1266,7 @@ extern "C" int gs_backward_prepare(GS_BWD_ARGS) {
                            scratch_bytes);
   if (rc != GS_OK) return rc;
   const RBArgs r = make_rb_args(f, g->P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
-  bwd_prepare(f, g->P, capacity, r, (hipStream_t)stream);
+  bwd_prepare(f, g->P, capacity, r, (hipStream_t)stream, pose_only(f, g, out, dL_ddepth || dL_dalpha));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
@@ -1203,7 +1275,8 @@ extern "C" int gs_backward_finish(GS_BWD_ARGS) {
                            scratch_bytes);
   if (rc != GS_OK) return rc;
   const RBArgs r = make_rb_args(f, g->P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
-  bwd_finish(f, g, radii, geom, capacity, counters, out, r, (hipStream_t)stream);
+  bwd_finish(f, g, radii, geom, capacity, counters, out, r, (hipStream_t)stream,
+             pose_only(f, g, out, dL_ddepth || dL_dalpha));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
@@ -1247,17 +1320,30 @@ extern "C" int gs_multi_render_bwd_build(int n, const gs_frame_t* frames, const 
   return GS_OK;
 }
 
-extern "C" int gs_multi_render_bwd(int n, const void* dev_set, unsigned total_blocks, int deterministic,
-                                   void* stream) {
-  if (n <= 0 || !dev_set) return GS_E_ARG;
+extern "C" int gs_multi_render_bwd(int n, const void* dev_set, unsigned total_blocks, int flags, void* stream) {
+  if (n <= 0 || !dev_set || (flags & ~7)) return GS_E_ARG;
+  const bool det = flags & 1, dep = !(flags & GS_RB_COLOR_ONLY), po = flags & GS_RB_POSE_ONLY;
+  if (po && (!det || dep)) return GS_E_ARG;
   if (total_blocks == 0) return GS_OK;
   hipStream_t s = (hipStream_t)stream;
   const uint32_t* prefix = (const uint32_t*)dev_set;
   const RBArgs* args = (const RBArgs*)((const char*)dev_set + multi_rb_args_offset(n));
   timing::Scope ts(timing::kRenderBwd, s);
-  if (deterministic)
-    hipLaunchKernelGGL(k_render_bwd_multi<true>, dim3(total_blocks), dim3(256), 0, s, prefix, args, n);
-  else
-    hipLaunchKernelGGL(k_render_bwd_multi<false>, dim3(total_blocks), dim3(256), 0, s, prefix, args, n);
+  const dim3 grid(total_blocks);
+  if (po) {
+    hipLaunchKernelGGL((k_render_bwd_multi<true, false, true>), grid, dim3(256), 0, s, prefix, args, n);
+  } else if (det) {
+    if (dep) hipLaunchKernelGGL((k_render_bwd_multi<true, true, false>), grid, dim3(256), 0, s, prefix, args, n);
+    else hipLaunchKernelGGL((k_render_bwd_multi<true, false, false>), grid, dim3(256), 0, s, prefix, args, n);
+  } else {
+    if (dep) hipLaunchKernelGGL((k_render_bwd_multi<false, true, false>), grid, dim3(256), 0, s, prefix, args, n);
+    else hipLaunchKernelGGL((k_render_bwd_multi<false, false, false>), grid, dim3(256), 0, s, prefix, args, n);
+  }
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_backward_is_pose_only(const gs_frame_t* f, const gs_gaussians_t* g, const gs_grads_t* out,
+                                        int has_depth_or_alpha) {
+  if (!f || !g || !out) return 0;
+  return pose_only(f, g, out, has_depth_or_alpha != 0) ? 1 : 0;
 }

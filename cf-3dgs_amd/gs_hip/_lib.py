@@ -22,6 +22,8 @@ c_double = ctypes.c_double
 GS_OK = 0
 GS_STATUS_CAPACITY = 1
 GS_STATUS_PREFILTERED = 2
+GS_RB_COLOR_ONLY = 2  # gs_multi_render_bwd flag: no frame of the set has a depth or alpha cotangent
+GS_RB_POSE_ONLY = 4   # gs_multi_render_bwd flag: the frames take the pose-only backward
 TILE = 16
 
 
@@ -131,6 +133,8 @@ EXPORTS = {
                                           ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_void_p, c_size_t,
                                           ctypes.POINTER(ctypes.c_uint)]),
     "gs_multi_render_bwd": (c_int, [c_int, c_void_p, ctypes.c_uint, c_int, c_void_p]),
+    "gs_backward_is_pose_only": (c_int, [ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians),
+                                         ctypes.POINTER(GsGrads), c_int]),
     "gs_mark_visible": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gs_sort_tmp_bytes": (c_size_t, [c_longlong]),
     "gs_sort_pairs_u64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_longlong, c_int, c_void_p,

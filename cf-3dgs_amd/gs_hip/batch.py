@@ -229,7 +229,15 @@ class FrameSets:
                                                  ctypes.byref(blocks)), "gs_multi_render_bwd_build")
         self.rbwd = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
         self.rbwd_blocks = blocks.value
-        self.deterministic = int(slots[0].frame.deterministic)
+        # the pose solve's loss has no depth or alpha term: colour-only kernels,
+        # and the pose-only backward when the library takes it for these frames
+        self.rbwd_flags = int(slots[0].frame.deterministic) | _lib.GS_RB_COLOR_ONLY
+        po = [lib.gs_backward_is_pose_only(ctypes.byref(s.frame), ctypes.byref(s.gauss), ctypes.byref(s.grads), 0)
+              for s in slots]
+        if all(po):
+            self.rbwd_flags |= _lib.GS_RB_POSE_ONLY
+        elif any(po):
+            raise RuntimeError("a render-backward set mixes pose-only and full frames")
 
     def blend_launch(self, stream: ctypes.c_void_p) -> None:
         _lib.check(_lib.load().gs_multi_blend(self.n, ctypes.c_void_p(self.blend.data_ptr()), self.blend_tiles,
@@ -237,7 +245,7 @@ class FrameSets:
 
     def render_bwd_launch(self, stream: ctypes.c_void_p) -> None:
         _lib.check(_lib.load().gs_multi_render_bwd(self.n, ctypes.c_void_p(self.rbwd.data_ptr()),
-                                                   self.rbwd_blocks, self.deterministic, stream),
+                                                   self.rbwd_blocks, self.rbwd_flags, stream),
                    "gs_multi_render_bwd")
 
 
@@ -394,7 +402,7 @@ def size_bucket(instances: Sequence[PoseInstance]) -> tuple:
     return (len(instances), max(P, 1).bit_length(), max(px, 1).bit_length())
 
 
-def autotune(instances: Sequence[PoseInstance], streams=(1, 2, 4),
+def autotune(instances: Sequence[PoseInstance], streams=(2, 4, 8),
              binnings=(_lib.BINNING_DEPTH_FIRST, _lib.BINNING_SORT64), multis=(True, False),
              iters_per_graph: int = 5, replays: int = 2) -> dict:
     """Times ``replays`` graph replays of every (streams, binning, one-launch

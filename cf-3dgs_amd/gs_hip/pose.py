@@ -238,18 +238,22 @@ def solve_plain(inst: PoseInstance, iters: int = ITERS, lr: float = LR, lambda_d
 
 
 def solve_plain_native(inst: PoseInstance, iters: int = ITERS, lr: float = LR, lambda_dssim: float = 0.2,
-                       X0: Optional[torch.Tensor] = None):
+                       X0: Optional[torch.Tensor] = None, pose_only: bool = False):
     """The plain one-solve-per-launch path on this library's kernels: per
     iteration an eager launch sequence (preprocess, num_rendered readback,
-    binning + blend, fused loss forward and backward, pose-only backward with
-    fixed-order reductions, SE(3) Adam) and the reference's NaN check on
-    the host.  Every reduction is in a fixed order, so it computes bit for
-    bit what the batched solver computes for the same instance.
-    Returns dict(pose [7], loss, seconds)."""
+    binning + blend, fused loss forward and backward, the rasteriser backward
+    with fixed-order reductions, SE(3) Adam) and the reference's NaN check on
+    the host.  The backward computes every gradient, as the reference's
+    rasteriser does for a pose solve (its Gaussians' parameters still require
+    gradients); ``pose_only`` asks for the pose gradient alone, which runs
+    the pose-only backward.  Every reduction is in a fixed order, so either
+    way it computes bit for bit what the batched solver computes for the same
+    instance.  Returns dict(pose [7], loss, seconds)."""
     from .loss import fused_photometric_value_and_grad
     device = inst.means.device
     opt = SE3Adam(1, device, lr=lr, eps=EPS, X0=X0)
     stats = None
+    want = {"pose"} if pose_only else None
     torch.cuda.synchronize(device)
     t0 = time.perf_counter()
     for _ in range(iters):
@@ -258,7 +262,7 @@ def solve_plain_native(inst: PoseInstance, iters: int = ITERS, lr: float = LR, l
                                              inst.rotations, None, pose=pose, want_alpha=False)
         stats, dcolor = fused_photometric_value_and_grad(color, inst.target, lambda_dssim, clamp01=True)
         g = _raster.backward(inst.settings, st, inst.means, inst.shs, None, inst.opacities, inst.scales,
-                             inst.rotations, None, pose, dcolor, want={"pose"}, deterministic=True)
+                             inst.rotations, None, pose, dcolor, want=want, deterministic=True)
         if torch.isnan(stats[0]):
             raise FloatingPointError("pose solve: loss is NaN")
         opt.grad[0].copy_(g["pose"])

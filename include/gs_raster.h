@@ -246,7 +246,21 @@ int gs_backward_adam(const gs_frame_t* frame, const gs_gaussians_t* g, const int
  * with the middle step of every frame replaced by one gs_multi_blend /
  * gs_multi_render_bwd launch over the set.  total_tiles / total_blocks are
  * returned by the builds.  All frames of a render-backward set share one
- * `deterministic` setting. */
+ * `deterministic` setting.  gs_multi_render_bwd's `flags`: bit 0 (1) the
+ * fixed-order reductions; GS_RB_COLOR_ONLY when no frame of the set has a
+ * depth or alpha cotangent (their terms are then skipped; a frame that has
+ * one must not be launched with it); GS_RB_POSE_ONLY (with 1 and
+ * GS_RB_COLOR_ONLY) when gs_backward_is_pose_only holds for the frames: the
+ * pose-only backward, whose prepare / finish steps gs_backward_prepare and
+ * gs_backward_finish then take by themselves.
+ * gs_backward_is_pose_only: 1 when gs_backward (and its prepare / finish
+ * steps) would run the pose-only backward for these arguments -- fixed-order
+ * reductions, only out->dL_dpose set, no depth or alpha cotangent, active SH
+ * degree 0 or precomputed colours.  It sums five terms per (Gaussian, tile)
+ * pair instead of ten; the pose gradient has the same bits as the full
+ * backward's. */
+#define GS_RB_COLOR_ONLY 2
+#define GS_RB_POSE_ONLY 4
 int gs_render_binning(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters, long long capacity,
                       void* binning, size_t binning_bytes, void* image, size_t image_bytes, void* stream);
 size_t gs_multi_blend_bytes(int n);
@@ -269,7 +283,9 @@ int gs_multi_render_bwd_build(int n, const gs_frame_t* frames, const int* P, con
                               const float* const* dL_dcolor, const float* const* dL_ddepth,
                               const float* const* dL_dalpha, void* const* scratch, void* host_set,
                               size_t set_bytes, unsigned* total_blocks);
-int gs_multi_render_bwd(int n, const void* dev_set, unsigned total_blocks, int deterministic, void* stream);
+int gs_multi_render_bwd(int n, const void* dev_set, unsigned total_blocks, int flags, void* stream);
+int gs_backward_is_pose_only(const gs_frame_t* f, const gs_gaussians_t* g, const gs_grads_t* out,
+                             int has_depth_or_alpha);
 
 /* Near-plane visibility (replaces diff_gaussian_rasterization._C.mark_visible,
  * GaussianRasterizer.markVisible). visible[P] = 1 if the view-space z > 0.2. */

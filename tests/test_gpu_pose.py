@@ -233,3 +233,32 @@ def test_capacity_overflow_regrows():
     poses, _ = small.run()
     assert small.regrows >= 1
     assert torch.equal(poses, ref_poses)
+
+
+@pytest.mark.parametrize("W,H,P", [(160, 90, 14400), (480, 270, 129_600)])
+def test_pose_only_backward_matches_the_full_backward_bitwise(W, H, P):
+    """The pose-only backward (only the pose gradient wanted, fixed-order,
+    colour loss, SH degree 0) sums five terms per pair instead of ten; its
+    pose gradient has the same bits as the full backward's, and the library
+    says it takes that path."""
+    from gs_hip import raster
+    from gs_hip.loss import fused_photometric_value_and_grad
+    print("pose-only vs full backward", W, H, P, flush=True)
+    inst = build_instance(71, W=W, H=H, P=P)
+    pose = se3_exp(torch.tensor([0.01, -0.02, 0.015, 0.02, 0.01, -0.03], dtype=torch.float64)).float().to(DEV)
+    color, _, _, _, st = raster.forward(inst.settings, inst.means, inst.shs, None, inst.opacities, inst.scales,
+                                        inst.rotations, None, pose=pose, want_alpha=False)
+    _, dcolor = fused_photometric_value_and_grad(color, inst.target, 0.2, clamp01=True)
+    args = (inst.settings, st, inst.means, inst.shs, None, inst.opacities, inst.scales, inst.rotations, None, pose,
+            dcolor)
+    full = raster.backward(*args, deterministic=True)
+    po = raster.backward(*args, want={"pose"}, deterministic=True)
+    torch.cuda.synchronize()
+    assert torch.equal(full["pose"], po["pose"]) and torch.count_nonzero(po["pose"]) == 7
+    fa = raster.make_frame(inst.settings, DEV, pose, None, True)
+    ga = raster.make_gaussians(inst.means, inst.shs, None, inst.opacities, inst.scales, inst.rotations, None, DEV)
+    g = _lib.GsGrads()
+    g.dL_dpose = po["pose"].data_ptr()
+    import ctypes
+    assert _lib.load().gs_backward_is_pose_only(ctypes.byref(fa.frame), ctypes.byref(ga.g), ctypes.byref(g), 0) == 1
+    assert _lib.load().gs_backward_is_pose_only(ctypes.byref(fa.frame), ctypes.byref(ga.g), ctypes.byref(g), 1) == 0
