@@ -175,6 +175,12 @@ EXPORTS = {
                                    c_void_p, c_void_p, c_void_p]),
     "gs_activate_backward_adam": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p, ctypes.POINTER(GsAdamAct), c_void_p]),
+    "gs_densify_stats": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gs_densify_workspace_bytes": (c_size_t, [c_int]),
+    "gs_densify_mark": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
+                                c_float, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "gs_densify_emit": (c_int, [c_int, c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "gs_timing_enable": (c_int, [c_int]),
     "gs_timing_reset": (c_int, []),
     "gs_timing_read": (c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int)]),

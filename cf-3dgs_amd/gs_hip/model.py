@@ -121,6 +121,8 @@ class GaussianParams:
         self.active_sh_degree = sh_degree
         self.optimizer = None
         self._screen_zeros = None
+        self.percent_dense = 0.0
+        self.xyz_gradient_accum = self.denom = self.max_radii2D = None
 
     @property
     def _features_dc(self):
@@ -170,8 +172,11 @@ class GaussianParams:
 
     def training_setup(self, spatial_lr_scale: float = 1.0, position_lr_init: float = 0.00016,
                        feature_lr: float = 0.0025, opacity_lr: float = 0.05, scaling_lr: float = 0.005,
-                       rotation_lr: float = 0.001):
+                       rotation_lr: float = 0.001, percent_dense: float = 0.01):
+        from . import densify
         from .optim import FusedAdam
+        self.percent_dense = percent_dense
+        densify.init_stats(self)
         row = self._features.shape[1] * self._features.shape[2]
         groups = [
             {"params": [self._xyz], "lr": position_lr_init * spatial_lr_scale, "name": "xyz"},
@@ -184,6 +189,44 @@ class GaussianParams:
         ]
         self.optimizer = FusedAdam(groups, lr=0.0, eps=1e-15)
         return self.optimizer
+
+
+    # -- adaptive density control (scene/gaussian_model_cf.py:452-457,523-719) --
+    def add_densification_stats(self, viewspace_points, radii):
+        from . import densify
+        densify.add_densification_stats(self, viewspace_points, radii)
+
+    def densify_and_prune(self, max_grad, min_opacity, extent, max_screen_size=None, generator=None):
+        from . import densify
+        return densify.densify_and_prune(self, max_grad, min_opacity, extent, max_screen_size, generator)
+
+    def densify(self, max_grad, min_opacity, extent, max_screen_size=None, generator=None):
+        from . import densify
+        return densify.densify(self, max_grad, min_opacity, extent, max_screen_size, generator)
+
+    def prune(self, max_grad, min_opacity, extent, max_screen_size=None):
+        from . import densify
+        return densify.prune(self, max_grad, min_opacity, extent, max_screen_size)
+
+    def reset_opacity(self):
+        from . import densify
+        densify.reset_opacity(self)
+
+    # -- PLY files (scene/gaussian_model_cf.py:398-432,459-515) --
+    def save_ply(self, path: str) -> None:
+        from . import ply
+        ply.save_ply(path, self._xyz, self._features_dc, self._features_rest, self._opacity, self._scaling,
+                     self._rotation)
+
+    @classmethod
+    def from_ply(cls, path: str, max_sh_degree: int = 3, num_gauss: int = -1, device="cuda") -> "GaussianParams":
+        """load_ply: the parameters of a Gaussian PLY file, active SH degree
+        = max_sh_degree."""
+        from . import ply
+        d = ply.load_ply(path, max_sh_degree, num_gauss)
+        t = {k: torch.from_numpy(v) for k, v in d.items()}
+        return cls(t["xyz"], t["features_dc"], t["features_rest"], t["scaling"], t["rotation"], t["opacity"],
+                   max_sh_degree, device=device)
 
 
 def render(gaussians: GaussianParams, cam: Camera, bg: torch.Tensor, pose=None, scaling_modifier: float = 1.0,

@@ -470,6 +470,43 @@ int gs_activate_backward_adam(int P, float* raw_scale, float* raw_rot, float* ra
  * reference counterpart; the reference's debug mode only synchronises).
  * Stages: preprocess, scan, duplicate, sort, ranges, blend, render_bwd,
  * preprocess_bwd, knn.  gs_timing_read waits for the recorded events. */
+/* Adaptive density control (replaces the torch-op bookkeeping of
+ * CFGaussianModel: add_densification_stats scene/gaussian_model_cf.py:716-719
+ * with the trainer's max_radii2D update trainer/cf3dgs_trainer.py:144-152;
+ * densify_and_clone / densify_and_split / prune_points / densify_and_prune /
+ * prune / densify :523-714, including _prune_optimizer and
+ * cat_tensors_to_optimizer's moment handling).
+ *
+ * gs_densify_stats: for every i with radii[i] > 0: grad_accum[i] +=
+ *   |grad_means2D[i].xy|, denom[i] += 1, max_radii2D[i] = max(., radii[i]).
+ * gs_densify_mark: per Gaussian (raw log-scales [P,3], raw opacities [P]):
+ *   g = grad_accum / denom (NaN -> 0); with `densify`: clone if
+ *   |g| >= grad_threshold and max exp(scale) <= dense_extent, split if
+ *   g >= grad_threshold and max exp(scale) > dense_extent; with `prune`:
+ *   sigmoid(opacity) < min_opacity, or (max_screen > 0) max_radii2D >
+ *   max_screen, or (big_extent > 0) max exp(scale) > big_extent -- tested for
+ *   the original, its clone, and its split copies (scale log(exp(s) * 0.625),
+ *   statistics zero).  Writes flags[P] (4-byte aligned) and, on the device,
+ *   totals[4] = (originals kept, clones kept, split copies kept per copy,
+ *   split-selected); ws: gs_densify_workspace_bytes(P).
+ * gs_densify_emit: src/dst[15] = xyz [P,3], features [P,M,3], opacity [P,1],
+ *   scaling [P,3], rotation [P,4], then their exp_avg, then their exp_avg_sq
+ *   (the ten moment pointers all NULL when there are none); dst arrays hold
+ *   totals[0] + totals[1] + 2 totals[2] rows, in the order
+ *   [originals kept][clones kept][split copy 1 kept][split copy 2 kept].
+ *   split_samples [2 totals[3], 3]: the N(0, exp(scale)) draws of the split
+ *   copies (rows of copy 1, then of copy 2, selected Gaussians in index
+ *   order); new points get zero moments. */
+int gs_densify_stats(int P, const float* grad_means2D, const int* radii, float* grad_accum, float* denom,
+                     float* max_radii2D, void* stream);
+size_t gs_densify_workspace_bytes(int P);
+int gs_densify_mark(int P, const float* grad_accum, const float* denom, const float* scaling, const float* opacity,
+                    const float* max_radii2D, float grad_threshold, float dense_extent, float min_opacity,
+                    float big_extent, float max_screen, int densify, int prune, uint8_t* flags, int* totals,
+                    void* ws, size_t ws_bytes, void* stream);
+int gs_densify_emit(int P, int M, const float* const* src, float* const* dst, const float* split_samples,
+                    const uint8_t* flags, const int* totals, const void* ws, size_t ws_bytes, void* stream);
+
 int gs_timing_enable(int on);
 int gs_timing_reset(void);
 int gs_timing_read(const char* stage, double* total_ms, int* launches);
