@@ -191,6 +191,45 @@ def build_instance(seed: int, W: int = 480, H: int = 270, P: Optional[int] = Non
     return inst
 
 
+def frame_payload(seed: int, W: int = 480, H: int = 270, fovx_deg: float = 60.0):
+    """What a pose-solve request brings: the new frame's depth map [H, W],
+    its RGB image [H, W, 3] in [0, 1] (the seeded draws of ``lift_points``,
+    pixel for pixel) and its intrinsics (fx, fy, cx, cy), plus the frame's
+    camera (identity view)."""
+    g = torch.Generator().manual_seed(seed)
+    fx = fov2focal(math.radians(fovx_deg), W)
+    d = synthetic_depth(W, H, g)
+    rgb = torch.rand((H * W, 3), generator=g).reshape(H, W, 3)
+    cam = make_camera(np.eye(3), np.zeros(3), math.radians(fovx_deg), focal2fov(fx, H), W, H)
+    return d, rgb, (fx, fx, W / 2.0, H / 2.0), cam
+
+
+def build_instance_device(seed: int, W: int = 480, H: int = 270, fovx_deg: float = 60.0, device="cuda",
+                          sigma_rot: float = 0.05, sigma_t: float = 0.05) -> PoseInstance:
+    """``build_instance`` with the frame ingested on the device: the request's
+    depth map and image are uploaded and go through the library's frame
+    ingest (lift, distCUDA2 scales, create_from_pcd with one Gaussian per
+    pixel and active SH degree 0, ``gs_hip.ingest``), then the activations
+    and the target render at the hidden pose.  Only the payload crosses from
+    the host."""
+    from . import ingest
+    device = torch.device(device)
+    d, rgb, K, cam = frame_payload(seed, W, H, fovx_deg)
+    r = ingest.ingest_frame(d.to(device), rgb.to(device), K, voxel_size=None, max_sh_degree=0,
+                            channels_first=False)
+    n = r["xyz"].shape[0]
+    rs = raster_settings(cam.to(device), torch.zeros(3, device=device), 0)
+    inst = PoseInstance(seed=seed, W=W, H=H, P=n, means=r["xyz"], shs=r["features"].contiguous(),
+                        opacities=torch.sigmoid(r["opacity"]), scales=torch.exp(r["scaling"]),
+                        rotations=torch.nn.functional.normalize(r["rotation"], dim=-1), settings=rs, target=None,
+                        gt_pose=random_pose(seed + 7919, sigma_rot, sigma_t).to(device))
+    with torch.no_grad():
+        color, _, _, _, _ = _raster.forward(rs, inst.means, inst.shs, None, inst.opacities, inst.scales,
+                                            inst.rotations, None, pose=inst.gt_pose, want_alpha=False)
+    inst.target = color.clamp(0.0, 1.0).contiguous()
+    return inst
+
+
 # ---------------------------------------------------------------------------
 # plain solver: the reference's loop shape, one iteration per Python step
 

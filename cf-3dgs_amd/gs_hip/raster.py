@@ -11,6 +11,7 @@ from __future__ import annotations
 import ctypes
 import functools
 import os
+import threading
 from dataclasses import dataclass
 from typing import Optional
 
@@ -160,20 +161,27 @@ def make_gaussians(means3D, shs, colors_precomp, opacities, scales, rotations, c
 
 
 _CAPACITY_QUANTUM = 1 << 16
-_PINNED = {}
-_EVENTS = {}
+_CAPACITY_MAX = 0x7FFFFFFF  # list entries are indexed with 32-bit offsets
+# per thread: a pinned counter buffer and an event per device for the
+# num_rendered readback (a forward waits for its own copy before returning,
+# so calls on one thread never overlap; calls on different threads get
+# different buffers)
+_TLS = threading.local()
 # capacity of the last forward per (P, W, H, schedule): the guess under which
 # the part of the binning that does not need num_rendered is queued before the
 # readback
 _LAST_CAPACITY = {}
 
 
-def _pinned_counters(device: torch.device) -> torch.Tensor:
-    t = _PINNED.get(device)
-    if t is None:
-        t = torch.empty(4, dtype=torch.int32, pin_memory=True)
-        _PINNED[device] = t
-    return t
+def _readback(device: torch.device):
+    """(pinned int32[4], event) of the calling thread for ``device``."""
+    d = getattr(_TLS, "readback", None)
+    if d is None:
+        d = _TLS.readback = {}
+    rb = d.get(device)
+    if rb is None:
+        rb = d[device] = (torch.empty(4, dtype=torch.int32, pin_memory=True), torch.cuda.Event())
+    return rb
 
 
 @functools.lru_cache(maxsize=256)
@@ -244,11 +252,8 @@ def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations
     # the one host synchronisation of the forward (the reference's num_rendered
     # readback): only the copy is waited for.  Meanwhile the binning work that
     # does not need the count runs under the last capacity used for this size.
-    host = _pinned_counters(device)
+    host, ev = _readback(device)
     host.copy_(counters, non_blocking=True)
-    ev = _EVENTS.get(device)
-    if ev is None:
-        ev = _EVENTS[device] = torch.cuda.Event()
     ev.record(torch.cuda.current_stream(device))
     key = (P, W, H, fa.frame.binning)
     guess = _LAST_CAPACITY.get(key, 0)
@@ -265,7 +270,9 @@ def forward(settings, means3D, shs, colors_precomp, opacities, scales, rotations
     if status & _lib.GS_STATUS_PREFILTERED:
         raise RuntimeError("a point was culled by the near plane although prefiltered=True")
     # capacity rounded up so the caching allocator can reuse the binning block
-    capacity = min(-(-num_rendered // _CAPACITY_QUANTUM) * _CAPACITY_QUANTUM, 0x7FFFFFFF)
+    if num_rendered > _CAPACITY_MAX:
+        raise RuntimeError(f"{num_rendered} list entries exceed the binning limit of {_CAPACITY_MAX}")
+    capacity = min(-(-num_rendered // _CAPACITY_QUANTUM) * _CAPACITY_QUANTUM, _CAPACITY_MAX)
     if guess and num_rendered <= guess and capacity * 2 > guess:
         capacity = guess
         _lib.check(lib.gs_render_resume(ctypes.byref(fa.frame), P, _ptr(geom), _ptr(counters), capacity,

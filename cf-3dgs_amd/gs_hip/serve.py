@@ -3,20 +3,26 @@
 A request is either
   * a render of a resident scene at a camera pose (forward only), or
   * a pose solve of a new frame: the frame-ingest of trainer/trainer.py:476-512
-    (depth lift, create_from_pcd) followed by the per-frame solve of
+    (depth lift, distCUDA2, create_from_pcd), done on the device from the
+    request's depth map and image, followed by the per-frame solve of
     trainer/cf3dgs_trainer.py:261-285 (``iters`` iterations of render -> loss
     -> backward -> Adam on the SE(3) tangent).
 
-The loop runs in ticks.  At the start of a tick every request that has
-arrived is queued; queued requests are admitted in arrival order while the
-memory budget and the slot count allow (a request at the head that does not
-fit blocks the ones behind it and is counted as having waited for memory).
-In the tick, every admitted render and ``iters_per_tick`` iterations of every
-active solve are launched, each request on its own stream (round-robin over
-``streams``), so their kernels overlap on the device; the tick ends with one
-device synchronisation, after which finished requests are retired and their
-memory released.  A solve admitted in tick k thus runs alongside solves that
-started earlier and renders that arrive later.
+The host loop never waits for the whole device.  Each request runs on its
+own stream (round-robin over ``streams``) and reports progress through
+events:
+  * arrived requests are queued and admitted in arrival order while the
+    memory budget and the slot count allow (a request at the head that does
+    not fit blocks the ones behind it and is counted as having waited for
+    memory; one that could never fit is rejected);
+  * a render queues its preprocess and an asynchronous copy of its list
+    length; once that copy has landed, its binning and blend; once those
+    are done it is retired and its memory released;
+  * a solve keeps up to two chunks of ``iters_per_tick`` iterations queued
+    on its stream and queues the next chunk as soon as the older one has
+    finished;
+so requests that arrive while others are on the device are admitted as
+memory frees up and their kernels overlap with the running ones.
 
 Solves use the fixed-order backward, and renders are deterministic, so a
 result does not depend on what else was in flight: a request served alone
@@ -39,7 +45,7 @@ from . import _lib
 from . import raster as _raster
 from .batch import _Slot, launch_slot, make_slot, slot_bytes, start_capacities
 from .camera import focal2fov, fov2focal, look_at, make_camera, raster_settings
-from .pose import EPS, LR, PoseInstance, SE3Adam, build_instance
+from .pose import EPS, LR, PoseInstance, SE3Adam, build_instance, build_instance_device
 
 GB = 1 << 30
 
@@ -133,7 +139,9 @@ class _Render:
     binning: torch.Tensor = None
     color: torch.Tensor = None
     depth: torch.Tensor = None
-    event: object = None
+    event: object = None       # the list-length copy, then the end of the blend
+    host: torch.Tensor = None  # pinned copy of the counters
+    launched: bool = False     # binning + blend queued
 
 
 @dataclass
@@ -145,6 +153,9 @@ class _Solve:
     adam: SE3Adam
     out3: torch.Tensor
     iters_done: int = 0
+    chunks: deque = field(default_factory=deque)  # events of the queued iteration chunks
+    host: torch.Tensor = None   # pinned copy of the sticky status word at the end
+    final: object = None        # its event
 
 
 class Server:
@@ -164,6 +175,7 @@ class Server:
             raise ValueError("the resident scenes alone exceed the memory budget")
         self.max_slots = int(max_slots)
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, int(streams)))]
+        self.ingest_stream = torch.cuda.Stream(device=self.device)
         self.solve_iters = int(solve_iters)
         self.iters_per_tick = max(1, int(iters_per_tick))
         self.lr = float(lr)
@@ -221,6 +233,8 @@ class Server:
             _lib.check(lib.gs_preprocess(ctypes.byref(r.frame), ctypes.byref(r.gauss), _raster._ptr(r.radii),
                                          _raster._ptr(r.geom), r.geom.numel(), _raster._ptr(r.counters),
                                          ctypes.c_void_p(r.stream.cuda_stream)), "gs_preprocess")
+            r.host = torch.empty(4, dtype=torch.int32, pin_memory=True)
+            r.host.copy_(r.counters, non_blocking=True)
             r.event = torch.cuda.Event()
             r.event.record(r.stream)
         return r
@@ -241,16 +255,32 @@ class Server:
                                      cap, _raster._ptr(r.binning), r.binning.numel(), _raster._ptr(r.image),
                                      r.image.numel(), _raster._ptr(r.color), _raster._ptr(r.depth), None,
                                      ctypes.c_void_p(r.stream.cuda_stream)), "gs_render")
+            r.event = torch.cuda.Event()
+            r.event.record(r.stream)
+        r.launched = True
 
     def _start_solve(self, req: Request, min_capacity: int = 0) -> _Solve:
-        st = self._stream()
-        st.wait_stream(torch.cuda.current_stream(self.device))
-        with torch.cuda.stream(st):
-            inst = build_instance(req.seed, W=req.W, H=req.H, P=req.P, device=self.device)
+        # the frame is built on a stream of its own: its target render and the
+        # start-capacity forward read list lengths back, and on a shared
+        # request stream they would wait behind other requests' work.  Its
+        # tensors are released only after the solve's last event, so the
+        # solve stream may use them without further bookkeeping.
+        ing = self.ingest_stream
+        ing.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(ing):
+            # one Gaussian per pixel: the frame is ingested on the device from
+            # its depth map and image; a subsampled or super-sampled frame
+            # (req.P given) takes the host lift
+            if req.P is None:
+                inst = build_instance_device(req.seed, W=req.W, H=req.H, device=self.device)
+            else:
+                inst = build_instance(req.seed, W=req.W, H=req.H, P=req.P, device=self.device)
             adam = SE3Adam(1, self.device, lr=self.lr, eps=EPS)
             cap = max(start_capacities([inst], adam.pose, self.binning, self.headroom)[0], int(min_capacity))
             slot = make_slot(inst, cap, self.binning, adam.pose.data_ptr(), adam.grad.data_ptr())
             out3 = torch.zeros(3, dtype=torch.float32, device=self.device)
+        st = self._stream()
+        st.wait_stream(ing)
         actual = int(slot_bytes(inst.P, inst.W, inst.H, cap, self.binning) + 4 * 14 * inst.P
                      + 4 * 3 * inst.W * inst.H)
         self.used += actual - req.reserved
@@ -264,6 +294,15 @@ class Server:
             for _ in range(n):
                 launch_slot(s.slot, s.out3.data_ptr(), self.one.data_ptr(), 0.2, sp)
                 s.adam.step()
+            ev = torch.cuda.Event()
+            ev.record(s.stream)
+            s.chunks.append(ev)
+            if s.iters_done + n >= self.solve_iters:
+                # the sticky status word, for the capacity check at the end
+                s.host = torch.empty(4, dtype=torch.int32, pin_memory=True)
+                s.host.copy_(s.slot.counters, non_blocking=True)
+                s.final = torch.cuda.Event()
+                s.final.record(s.stream)
         s.iters_done += n
 
     # -- the loop ----------------------------------------------------------------
@@ -273,6 +312,7 @@ class Server:
         treated as arrived at the start (a closed-loop capacity run)."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         queue: deque = deque()
+        renders: List[_Render] = []
         solves: List[_Solve] = []
         done: List[Request] = []
         rejected: List[Request] = []
@@ -282,16 +322,17 @@ class Server:
         def now():
             return time.perf_counter() - t0
 
-        while pending or queue or solves:
+        def retire(req: Request) -> None:
+            req.done = now()
+            self.used -= req.reserved
+            done.append(req)
+
+        while pending or queue or renders or solves:
+            progressed = False
             t = now()
             while pending and (not realtime or pending[0].arrival <= t):
                 queue.append(pending.popleft())
-            if not queue and not solves:
-                # idle until the next arrival
-                time.sleep(max(0.0, min(pending[0].arrival - t, 0.01)))
-                continue
             # admission, in arrival order
-            renders: List[_Render] = []
             while queue:
                 req = queue[0]
                 need = self._render_bytes(req) if req.kind == "render" else self._solve_bytes(req)
@@ -317,42 +358,49 @@ class Server:
                     renders.append(self._start_render(req))
                 else:
                     solves.append(self._start_solve(req))
-            # solves of this tick
-            for s in solves:
-                self._launch_solve_iters(s, min(self.iters_per_tick, self.solve_iters - s.iters_done))
-            # renders: one readback of all list lengths, then the binning + blend
-            if renders:
-                cur = torch.cuda.current_stream(self.device)
-                for r in renders:
-                    cur.wait_event(r.event)
-                counts = torch.stack([r.counters[0] for r in renders]).cpu()
-                for r, c in zip(renders, counts.tolist()):
-                    self._finish_render_launch(r, int(c) & 0xFFFFFFFF)
-            torch.cuda.synchronize(self.device)
-            self.ticks += 1
-            t_end = now()
+                progressed = True
+            # renders: binning + blend once the list length has landed, retire when drawn
+            still_r = []
             for r in renders:
-                r.req.result = r.color
-                r.req.done = t_end
-                self.used -= r.req.reserved
-                done.append(r.req)
-            still = []
+                if r.event.query():
+                    progressed = True
+                    if not r.launched:
+                        self._finish_render_launch(r, int(r.host[0]) & 0xFFFFFFFF)
+                    else:
+                        r.req.result = r.color
+                        retire(r.req)
+                        continue
+                still_r.append(r)
+            renders = still_r
+            # solves: keep two chunks queued; retire after the last one
+            still_s = []
             for s in solves:
-                if s.iters_done >= self.solve_iters:
-                    if int(s.slot.counters[3].item()) & _lib.GS_STATUS_CAPACITY:
+                while s.chunks and s.chunks[0].query():
+                    s.chunks.popleft()
+                    progressed = True
+                if s.iters_done < self.solve_iters and len(s.chunks) < 2:
+                    self._launch_solve_iters(s, min(self.iters_per_tick, self.solve_iters - s.iters_done))
+                    progressed = True
+                if s.iters_done >= self.solve_iters and s.final is not None and s.final.query():
+                    progressed = True
+                    if int(s.host[3]) & _lib.GS_STATUS_CAPACITY:
                         # binning capacity exceeded during the solve: grow and start over
                         self.used -= s.req.reserved
                         s.req.reserved = 0
-                        still.append(self._start_solve(s.req, min_capacity=int(s.slot.capacity * 1.5)))
+                        still_s.append(self._start_solve(s.req, min_capacity=int(s.slot.capacity * 1.5)))
                         continue
                     s.req.result = s.adam.pose[0].clone()
                     s.req.loss = float(s.out3[0])
-                    s.req.done = t_end
-                    self.used -= s.req.reserved
-                    done.append(s.req)
-                else:
-                    still.append(s)
-            solves = still
+                    retire(s.req)
+                    continue
+                still_s.append(s)
+            solves = still_s
+            self.ticks += 1
+            if not progressed:
+                # nothing landed and nothing new arrived: wait briefly
+                nxt = pending[0].arrival - now() if (realtime and pending and not queue) else 0.0
+                time.sleep(min(max(nxt, 0.0), 0.002) if nxt > 0 else 0.0001)
+        torch.cuda.synchronize(self.device)
         wall = now()
         out = summarize(done, wall)
         out["rejected"] = len(rejected)

@@ -179,3 +179,47 @@ def test_prepare_resume_equals_one_call(binning):
     for k in ("color", "depth", "radii"):
         assert torch.equal(first[k], grown[k]), k
     assert grown["st"].capacity >= grown["st"].num_rendered > 256
+
+
+def test_forwards_from_two_threads_do_not_share_the_readback():
+    """Each thread reads num_rendered back through a pinned buffer and event
+    of its own: two threads rendering different scenes at the same time get
+    exactly their single-threaded results."""
+    import threading
+    from gs_hip import camera as gcam
+    from gs_hip import raster
+    print("two-thread forwards", flush=True)
+    dev = torch.device("cuda:0")
+    jobs = []
+    for k, (P, W, H) in enumerate([(20_000, 320, 180), (60_000, 240, 160)]):
+        sc = scenes.plain_scene(seed=80 + k, P=P, W=W, H=H)
+        rs = gcam.raster_settings(sc["cam"].to(dev), sc["bg"].to(dev), 3)
+        args = [sc[n].to(dev) for n in ("means", "shs")] + [None, sc["opacities"].to(dev), sc["scales"].to(dev),
+                                                            sc["rotations"].to(dev), None]
+        jobs.append((rs, args))
+    ref = []
+    for rs, args in jobs:
+        c, r, _, _, st = raster.forward(rs, *args, want_alpha=False)
+        ref.append((c.clone(), st.num_rendered))
+    torch.cuda.synchronize()
+    errors = []
+
+    def work(k):
+        try:
+            rs, args = jobs[k]
+            with torch.cuda.stream(torch.cuda.Stream(device=dev)):
+                for _ in range(25):
+                    c, _, _, _, st = raster.forward(rs, *args, want_alpha=False)
+                    torch.cuda.current_stream(dev).synchronize()
+                    if st.num_rendered != ref[k][1] or not torch.equal(c, ref[k][0]):
+                        errors.append((k, st.num_rendered))
+        except Exception as e:  # noqa: BLE001 -- reported below
+            errors.append((k, repr(e)))
+
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in ts)
+    assert not errors, errors[:4]
