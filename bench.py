@@ -23,6 +23,20 @@ import os
 import sys
 import time
 
+# the CPU set this process may run on, read before an OpenMP runtime binds the
+# main thread to one core
+try:
+    _USABLE_CPUS = len(os.sched_getaffinity(0))
+except (AttributeError, OSError):
+    _USABLE_CPUS = os.cpu_count()
+# the CPU baseline's OpenMP threads stay on neighbouring cores; set before any
+# OpenMP runtime (torch's, the oracle's) is loaded
+os.environ.setdefault("OMP_PROC_BIND", "close")
+# hardware queues per process (HIP's default is 4): the batched pose solver
+# spreads its instances over up to 8 streams, which only run concurrently when
+# each has a queue of its own; read when HIP initialises
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (ROOT, os.path.join(ROOT, "cf-3dgs_amd")):
     if _p not in sys.path:
@@ -55,8 +69,8 @@ def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, pas
         return 8 * n_dup + 8 * ((W + 15) // 16) * ((H + 15) // 16)
     if stage == "blend":        # per list entry: id 4 + splat 48; per pixel out: rgb 12, depth 4, alpha 4, T 4, n 4
         return 52 * n_dup + 28 * hw
-    if stage == "render_bwd":   # per list entry: id 4 + splat 48 + 40 B of gradient; per pixel in: 5 cotangents + T + n
-        return 92 * n_dup + 28 * hw
+    if stage == "render_bwd":   # per list entry: 44 B of splat (xy, conic, opacity, rgb, depth) + 40 B of gradient
+        return 84 * n_dup + 40 * hw  # per pixel: 5 cotangents, final T, n_contrib, 3 colour sums
     if stage == "preprocess_bwd":  # in: render grads 48, splat 16, cov 24, radii 4, xyz 12, scale 12, rot 16, SH 12M
         if adam_in_backward:       # no xyz / SH gradients out; their moments in, parameters and moments out
             return P * (132 + 12 * M + 92 - 12) + P * (3 + 3 * M) * 4 * (2 + 3)
@@ -67,6 +81,26 @@ def stage_bytes(stage: str, P: int, n_dup: int, W: int, H: int, M: int = 16, pas
 TRAFFIC_PROFILE = os.path.join("profiles", "training_step_500k_960x540_hbm_traffic.md")
 STAGE_KERNEL = {"render_bwd": "k_render_bwd", "blend": "k_blend", "preprocess_bwd": "k_preprocess_bwd",
                 "preprocess": "k_preprocess"}
+
+
+SQ_PROFILE = os.path.join("profiles", "training_step_500k_960x540_sq_counters.md")
+
+
+def measured_issue_share(stage: str):
+    """VALU issue share of the stage's kernel from the committed SQ counter
+    summary, or None when it has no row for it."""
+    kernel = STAGE_KERNEL.get(stage)
+    path = os.path.join(ROOT, SQ_PROFILE)
+    if kernel is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0].strip("`").split("::")[-1].split("<")[0] == kernel:
+            try:
+                return float(cells[5])
+            except ValueError:
+                return None
+    return None
 
 
 def measured_traffic(stage: str):
@@ -98,8 +132,23 @@ def build_state(device, P, W, H):
     return g, sc["cam"].to(device), sc["bg"].to(device), sc["target"].to(device), sc
 
 
-def cpu_baseline(sc, budget_s: float = 12.0):
-    """Oracle fwd + bwd iterations/s on the same scene (rasteriser only)."""
+def host_cpu() -> dict:
+    """The host CPU as /proc/cpuinfo and the scheduler report it."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": _USABLE_CPUS,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")}
+
+
+def cpu_baseline(sc, runs: int = 5):
+    """Oracle fwd + bwd iterations/s on the same scene (rasteriser only):
+    median of ``runs`` timed iterations after one warm-up."""
     import numpy as np
     import torch
     from gs_hip.camera import oracle_camera
@@ -118,15 +167,81 @@ def cpu_baseline(sc, budget_s: float = 12.0):
         O.backward(scene, cam, f, dc, np.zeros_like(f["depth"]))
 
     one()  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while n < 2 or (time.perf_counter() - t0 < budget_s and n < 10):
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
         one()
-        n += 1
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "iters/s", "cores": O.get_threads(), "kind": "port",
-            "sample": f"{n} rasteriser fwd+bwd iterations (preprocess, stable sort, blend, analytic backward; "
-                      f"no SSIM/Adam) of the same 500k-Gaussian 960x540 scene, C++/OpenMP oracle, "
-                      f"{dt:.1f} s"}
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    return {"value": round(1.0 / med, 3), "unit": "iters/s", "cores": O.get_threads(), "kind": "port",
+            "host": host_cpu(), "seconds_per_iter": [round(t, 3) for t in ts],
+            "sample": f"median of {runs} rasteriser fwd+bwd iterations after 1 warm-up (preprocess, parallel "
+                      f"stable sort, blend, analytic backward; no SSIM/Adam) of the same 500k-Gaussian 960x540 "
+                      f"scene, C++/OpenMP oracle, {sum(ts):.1f} s"}
+
+
+def plain_gpu_bench(device, P: int, W: int, H: int, steps: int = 10, warmup: int = 3):
+    """The same training step in the reference's structure on this GPU: the
+    drop-in rasteriser with the reference's 64-bit (tile, depth) key sort,
+    the torch activations and ``torch.cat`` of the SH coefficients
+    (scene/gaussian_model_cf.py:48-56,115-169), the ``zeros_like + 0`` screen-space
+    leaf with ``retain_grad`` (:793-805), ``image.clamp(0, 1)``, the torch
+    loss with its ``torch.isnan`` host check (trainer/losses.py:98-136) and
+    torch.optim.Adam over the six parameter groups of ``training_setup``
+    (:254-268).  Median of 3 timed runs of ``steps`` steps."""
+    import numpy as np
+    import torch
+    import torch.nn.functional as F
+    from diff_gaussian_rasterization import GaussianRasterizer
+    from gs_hip import _lib, raster, scenes
+    from gs_hip.camera import raster_settings
+    from gs_hip.loss import photometric_loss
+    sc = scenes.training_scene(seed=0, P=P, W=W, H=H)
+    prm = {k: torch.nn.Parameter(sc[k].to(device).contiguous())
+           for k in ("xyz", "features_dc", "features_rest", "scaling", "rotation", "opacity")}
+    opt = torch.optim.Adam([
+        {"params": [prm["xyz"]], "lr": 0.00016, "name": "xyz"},
+        {"params": [prm["features_dc"]], "lr": 0.0025, "name": "f_dc"},
+        {"params": [prm["features_rest"]], "lr": 0.0025 / 20.0, "name": "f_rest"},
+        {"params": [prm["opacity"]], "lr": 0.05, "name": "opacity"},
+        {"params": [prm["scaling"]], "lr": 0.005, "name": "scaling"},
+        {"params": [prm["rotation"]], "lr": 0.001, "name": "rotation"}], lr=0.0, eps=1e-15)
+    cam, bg, target = sc["cam"].to(device), sc["bg"].to(device), sc["target"].to(device)
+    rast = GaussianRasterizer(raster_settings=raster_settings(cam, bg, sc["sh_degree"]))
+
+    def step():
+        means2D = torch.zeros_like(prm["xyz"], requires_grad=True) + 0
+        means2D.retain_grad()
+        color, radii, depth = rast(means3D=prm["xyz"], means2D=means2D,
+                                   shs=torch.cat((prm["features_dc"], prm["features_rest"]), dim=1),
+                                   colors_precomp=None, opacities=torch.sigmoid(prm["opacity"]),
+                                   scales=torch.exp(prm["scaling"]), rotations=F.normalize(prm["rotation"]),
+                                   cov3D_precomp=None)
+        loss = photometric_loss(color.clamp(0.0, 1.0), target)
+        if torch.isnan(loss):
+            raise FloatingPointError("loss is NaN")
+        loss.backward()
+        opt.step()
+        opt.zero_grad(set_to_none=True)
+
+    saved = raster.BINNING
+    raster.BINNING = _lib.BINNING_SORT64
+    try:
+        for _ in range(warmup):
+            step()
+        runs = []
+        for _ in range(3):
+            torch.cuda.synchronize(device)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                step()
+            torch.cuda.synchronize(device)
+            runs.append(steps / (time.perf_counter() - t0))
+    finally:
+        raster.BINNING = saved
+    return {"value": round(float(np.median(runs)), 3), "unit": "iters/s", "runs": [round(r, 2) for r in runs],
+            "workload": "same step, reference structure: 64-bit key sort, torch activations + cat, torch "
+                        "L1 + SSIM (conv2d) with the isnan check, torch.optim.Adam (6 groups)"}
 
 
 def c1_bench(device, steps: int = 50, cpu: bool = True):
@@ -283,12 +398,15 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
         return {"workload": f"{n} pose solves x {iters} Adam iterations (batched path only)",
                 "batched_solves_per_s": round(n / batched_s, 3), "batched_seconds": round(batched_s, 3),
                 "autotune": {"streams": cfg["streams"], "binning": cfg["binning"], "multi": cfg["multi"]}}
-    plain_t, diffs, same = [], [], 0
+    plain_t, po_t, diffs, same, same_po = [], [], [], 0, 0
     for k, inst in enumerate(insts):
         o = solve_plain_native(inst, iters=iters)
         plain_t.append(o["seconds"])
         diffs.append(pose_errors(o["pose"], r["poses"][k]))
         same += int(torch.equal(o["pose"], r["poses"][k]))
+        q = solve_plain_native(inst, iters=iters, pose_only=True)
+        po_t.append(q["seconds"])
+        same_po += int(torch.equal(q["pose"], r["poses"][k]))
     seq_lat = []
     acc = 0.0
     for t in plain_t:
@@ -318,6 +436,9 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
         "plain_native_solves_per_s": round(n / sum(plain_t), 3),
         "plain_native_p50_s": round(pct(seq_lat, 50), 3), "plain_native_p99_s": round(pct(seq_lat, 99), 3),
         "speedup_vs_plain_native": round(sum(plain_t) / batched_s, 3),
+        "plain_native_pose_only_solves_per_s": round(n / sum(po_t), 3),
+        "speedup_vs_plain_native_pose_only": round(sum(po_t) / batched_s, 3),
+        "bitwise_equal_plain_pose_only": f"{same_po}/{n}",
         "plain_torch_s_per_solve_sample": round(sum(torch_t) / max(1, len(torch_t)), 3),
         "speedup_vs_plain_torch": round((sum(torch_t) / max(1, len(torch_t))) * n / batched_s, 3),
         "max_pose_diff_vs_plain_native": [max(d[0] for d in diffs), max(d[1] for d in diffs)],
@@ -363,6 +484,18 @@ def serve_bench(device, n_scenes: int = 16, n_requests: int = 60, calib_requests
            "capacity_rps": round(capacity, 3), "arrival_rate_rps": round(lam, 3),
            "scene_P": [s.P for s in scenes], "peak_reserved_GB": round(srv.peak_used / 2 ** 30, 2)}
     out.update({k: (round(v, 4) if isinstance(v, float) else v) for k, v in stats.items()})
+    # the same arrivals under a budget that binds: the resident scenes plus
+    # room for about two and a half of the largest requests
+    reqs = make_requests(n_requests, rate=lam, n_scenes=n_scenes, seed=12)
+    probe = Server(scenes)
+    need = max(probe._render_bytes(r) if r.kind == "render" else probe._solve_bytes(r) for r in reqs)
+    budget = probe.resident + 2.5 * need
+    _log(f"serve: Poisson run under a {budget / 2 ** 30:.2f} GB budget")
+    tight = Server(scenes, budget_bytes=budget)
+    st2 = tight.run(reqs, realtime=True)
+    out["budget_bound"] = {"budget_GB": round(budget / 2 ** 30, 3), "resident_GB": round(probe.resident / 2 ** 30, 3),
+                           "peak_reserved_GB": round(tight.peak_used / 2 ** 30, 3),
+                           **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st2.items()}}
     return out
 
 
@@ -378,6 +511,8 @@ def main():
     ap.add_argument("--no-pose", action="store_true", help="skip the batched pose-solve measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-plain", action="store_true", help="skip the 10k-Gaussian 256x256 measurement")
+    ap.add_argument("--no-plain-gpu", action="store_true",
+                    help="skip the training step in the reference's structure (64-bit sort, torch ops around)")
     ap.add_argument("--pose-instances", type=int, default=64)
     ap.add_argument("--pose-quick", action="store_true", help="time only the batched pose solves (profiling runs)")
     ap.add_argument("--torch-loss", action="store_true", help="loss with torch conv2d windows instead of the fused kernel")
@@ -466,12 +601,23 @@ def main():
     if dom:
         d = stages[dom]
         traffic = measured_traffic(dom)
+        # the two roofs the record offers are HBM and MFMA; the path has no
+        # matrix work, so the kernel is placed against HBM on its algorithmic
+        # bytes, and the measured VALU issue share names what limits it
+        issue = measured_issue_share(dom)
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(d["GB_s"], 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(d["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "traffic_unit": "bytes per launch", "traffic_source": TRAFFIC_PROFILE if traffic else None,
-                    "alg_bytes": round(d["alg_bytes"]), "ms_per_launch": round(d["ms_per_step"], 4)}
+                    "alg_bytes": round(d["alg_bytes"]), "ms_per_launch": round(d["ms_per_step"], 4),
+                    "limiter": "valu_issue" if issue is not None and issue >= 0.6 else "hbm",
+                    "valu_issue_share": issue, "issue_source": SQ_PROFILE}
     _log(f"training step: {ms:.3f} ms")
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
+    plain_gpu = None
+    if not args.no_plain_gpu:
+        _log("training step in the reference's structure")
+        plain_gpu = plain_gpu_bench(device, args.P, args.width, args.height)
+        plain_gpu["speedup"] = round((1000.0 / ms) / plain_gpu["value"], 3)
     c1 = None if args.no_plain else c1_bench(device, cpu=not args.no_cpu_baseline)
     knn = None if args.no_plain else knn_bench(device)
     ing = None if args.no_plain else ingest_bench(device, cpu=not args.no_cpu_baseline)
@@ -487,6 +633,7 @@ def main():
                    "adam_in_backward": not args.no_adam_in_backward, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "plain_gpu_baseline": plain_gpu,
         "plain_10k_256": c1,
         "distCUDA2": knn,
         "ingest": ing,
