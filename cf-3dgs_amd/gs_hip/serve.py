@@ -187,13 +187,26 @@ class Server:
         self._next_stream = 0
         self.used = float(self.resident)
         self.peak_used = float(self.used)
+        self.seen_counts: Dict[tuple, int] = {}  # (scene, W, H) -> last list length
         self.ticks = 0
 
     # -- memory estimates ------------------------------------------------------
+    @staticmethod
+    def _round_cap(n: int) -> int:
+        return min(-(-max(int(n), 1) // 4096) * 4096, 0x7FFFFFFF)
+
     def _render_bytes(self, req: Request, capacity: Optional[int] = None) -> int:
+        """Device bytes of a render: its workspaces at ``capacity`` list
+        entries, or, before the count is known, at the last count seen for
+        the same scene and resolution (+5%), else ``render_dup_estimate``
+        entries per Gaussian."""
         sc = self.scenes[req.scene]
         P = sc.P
-        cap = capacity if capacity is not None else int(P * self.dup_est)
+        if capacity is not None:
+            cap = capacity
+        else:
+            seen = self.seen_counts.get((req.scene, req.W, req.H))
+            cap = self._round_cap(seen * 1.05) if seen is not None else int(P * self.dup_est)
         return int(_raster._geom_layout(P).total + _raster._image_layout(req.W, req.H).total
                    + _raster._binning_layout(min(cap, 0x7FFFFFFF), P, req.W, req.H, self.binning).total
                    + 4 * P + 16 + 4 * 4 * req.W * req.H)
@@ -242,7 +255,8 @@ class Server:
     def _finish_render_launch(self, r: _Render, num_rendered: int) -> None:
         lib = _lib.load()
         sc = self.scenes[r.req.scene]
-        cap = min(-(-max(num_rendered, 1) // 4096) * 4096, 0x7FFFFFFF)
+        cap = self._round_cap(num_rendered)
+        self.seen_counts[(r.req.scene, r.req.W, r.req.H)] = num_rendered
         LB = _raster._binning_layout(cap, sc.P, r.req.W, r.req.H, self.binning)
         # the reservation now uses the real list length
         new = self._render_bytes(r.req, cap)
@@ -359,14 +373,28 @@ class Server:
                 else:
                     solves.append(self._start_solve(req))
                 progressed = True
-            # renders: binning + blend once the list length has landed, retire when drawn
+            # renders: binning + blend once the list length has landed and its
+            # workspace fits the budget, retire when drawn.  A render whose
+            # real workspace does not fit waits (it counts as having waited
+            # for memory) until others release theirs; the oldest waiting one
+            # goes ahead when nothing else could release any.
             still_r = []
+            waiting = [r for r in renders if not r.launched and r.event.query()]
+            others_busy = bool(solves) or any(r.launched for r in renders)
             for r in renders:
                 if r.event.query():
-                    progressed = True
                     if not r.launched:
-                        self._finish_render_launch(r, int(r.host[0]) & 0xFFFFFFFF)
+                        n = int(r.host[0]) & 0xFFFFFFFF
+                        extra = self._render_bytes(r.req, self._round_cap(n)) - r.req.reserved
+                        if self.used + extra > self.budget and (others_busy or r is not waiting[0]):
+                            r.req.waited_for_memory = True
+                            still_r.append(r)
+                            continue
+                        progressed = True
+                        self._finish_render_launch(r, n)
+                        others_busy = True
                     else:
+                        progressed = True
                         r.req.result = r.color
                         retire(r.req)
                         continue
