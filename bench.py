@@ -22,6 +22,7 @@ import json
 import os
 import sys
 import time
+from typing import Optional
 
 # the CPU set this process may run on, read before an OpenMP runtime binds the
 # main thread to one core
@@ -368,7 +369,8 @@ def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, quick: bool = False):
+def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, quick: bool = False,
+               fixed: Optional[dict] = None):
     """C4: ``n`` pose solves (P ~ U[20k, 200k], resolution ~ U{320x180,
     480x270, 640x360, 960x540}, seeded) batched in captured HIP graphs, against
     the same solves run one at a time on the plain one-solve-per-launch path
@@ -387,7 +389,9 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
     torch.cuda.synchronize(device)
     _log(f"pose: {n} instances built; autotuning")
     t0 = time.perf_counter()
-    cfg = autotune(insts)
+    # a fixed schedule (profiling runs: the profiler's per-launch cost would
+    # distort the autotuner's timings) or the autotuner's choice
+    cfg = dict(fixed, candidates=[], ms_per_iter=0.0) if fixed else autotune(insts)
     t_tune = time.perf_counter() - t0
     solver = BatchedPoseSolver(insts, iters=iters, streams=cfg["streams"], binning=cfg["binning"],
                                multi=cfg["multi"])
@@ -515,6 +519,8 @@ def main():
                     help="skip the training step in the reference's structure (64-bit sort, torch ops around)")
     ap.add_argument("--pose-instances", type=int, default=64)
     ap.add_argument("--pose-quick", action="store_true", help="time only the batched pose solves (profiling runs)")
+    ap.add_argument("--pose-schedule", default=None,
+                    help="streams,binning,multi for the batched solves instead of autotuning (e.g. 8,0,0)")
     ap.add_argument("--torch-loss", action="store_true", help="loss with torch conv2d windows instead of the fused kernel")
     ap.add_argument("--no-adam-in-backward", action="store_true",
                     help="write the means' and SH gradients and step them in the optimiser launch")
@@ -621,7 +627,11 @@ def main():
     c1 = None if args.no_plain else c1_bench(device, cpu=not args.no_cpu_baseline)
     knn = None if args.no_plain else knn_bench(device)
     ing = None if args.no_plain else ingest_bench(device, cpu=not args.no_cpu_baseline)
-    pose = None if args.no_pose else pose_bench(device, n=args.pose_instances, quick=args.pose_quick)
+    fixed = None
+    if args.pose_schedule:
+        st_, bi_, mu_ = (int(x) for x in args.pose_schedule.split(","))
+        fixed = dict(streams=st_, binning=bi_, multi=bool(mu_))
+    pose = None if args.no_pose else pose_bench(device, n=args.pose_instances, quick=args.pose_quick, fixed=fixed)
     serving = None if args.no_serve else serve_bench(device)
     res = {
         "metric": METRIC, "value": round(1000.0 / ms, 3), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
