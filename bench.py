@@ -489,15 +489,15 @@ def serve_bench(device, n_scenes: int = 16, n_requests: int = 60, calib_requests
            "scene_P": [s.P for s in scenes], "peak_reserved_GB": round(srv.peak_used / 2 ** 30, 2)}
     out.update({k: (round(v, 4) if isinstance(v, float) else v) for k, v in stats.items()})
     # the same arrivals under a budget that binds: the resident scenes plus
-    # room for about two and a half of the largest requests
+    # room for two of the largest requests of the first run; the server starts
+    # from the list lengths the first run has seen
     reqs = make_requests(n_requests, rate=lam, n_scenes=n_scenes, seed=12)
-    probe = Server(scenes)
-    need = max(probe._render_bytes(r) if r.kind == "render" else probe._solve_bytes(r) for r in reqs)
-    budget = probe.resident + 2.5 * need
+    budget = srv.resident + 2.0 * srv.max_request_bytes
     _log(f"serve: Poisson run under a {budget / 2 ** 30:.2f} GB budget")
-    tight = Server(scenes, budget_bytes=budget)
+    tight = Server(scenes, budget_bytes=budget, seen_counts=srv.seen_counts)
     st2 = tight.run(reqs, realtime=True)
-    out["budget_bound"] = {"budget_GB": round(budget / 2 ** 30, 3), "resident_GB": round(probe.resident / 2 ** 30, 3),
+    out["budget_bound"] = {"budget_GB": round(budget / 2 ** 30, 3), "resident_GB": round(srv.resident / 2 ** 30, 3),
+                           "largest_request_GB": round(srv.max_request_bytes / 2 ** 30, 3),
                            "peak_reserved_GB": round(tight.peak_used / 2 ** 30, 3),
                            **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st2.items()}}
     return out

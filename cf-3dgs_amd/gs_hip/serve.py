@@ -163,7 +163,8 @@ class Server:
 
     def __init__(self, scenes: Sequence[Scene], budget_bytes: float = 200 * GB, max_slots: int = 32,
                  streams: int = 4, solve_iters: int = 200, iters_per_tick: int = 10, lr: float = LR,
-                 binning: int = _lib.BINNING_DEPTH_FIRST, headroom: float = 1.3, render_dup_estimate: float = 8.0):
+                 binning: int = _lib.BINNING_DEPTH_FIRST, headroom: float = 1.3, render_dup_estimate: float = 8.0,
+                 seen_counts: Optional[Dict[tuple, int]] = None):
         if not scenes:
             raise ValueError("no scenes")
         self.scenes = {s.sid: s for s in scenes}
@@ -187,7 +188,9 @@ class Server:
         self._next_stream = 0
         self.used = float(self.resident)
         self.peak_used = float(self.used)
-        self.seen_counts: Dict[tuple, int] = {}  # (scene, W, H) -> last list length
+        # (scene, W, H) -> last list length; a server may start from another's
+        self.seen_counts: Dict[tuple, int] = dict(seen_counts or {})
+        self.max_request_bytes = 0  # largest final reservation of a served request
         self.ticks = 0
 
     # -- memory estimates ------------------------------------------------------
@@ -339,6 +342,7 @@ class Server:
         def retire(req: Request) -> None:
             req.done = now()
             self.used -= req.reserved
+            self.max_request_bytes = max(self.max_request_bytes, int(req.reserved))
             done.append(req)
 
         while pending or queue or renders or solves:
