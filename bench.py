@@ -489,10 +489,10 @@ def serve_bench(device, n_scenes: int = 16, n_requests: int = 60, calib_requests
            "scene_P": [s.P for s in scenes], "peak_reserved_GB": round(srv.peak_used / 2 ** 30, 2)}
     out.update({k: (round(v, 4) if isinstance(v, float) else v) for k, v in stats.items()})
     # the same arrivals under a budget that binds: the resident scenes plus
-    # room for two of the largest requests of the first run; the server starts
-    # from the list lengths the first run has seen
+    # room for 1.25 of the largest requests of the first run; the server
+    # starts from the list lengths the first run has seen
     reqs = make_requests(n_requests, rate=lam, n_scenes=n_scenes, seed=12)
-    budget = srv.resident + 2.0 * srv.max_request_bytes
+    budget = srv.resident + 1.25 * srv.max_request_bytes
     _log(f"serve: Poisson run under a {budget / 2 ** 30:.2f} GB budget")
     tight = Server(scenes, budget_bytes=budget, seen_counts=srv.seen_counts)
     st2 = tight.run(reqs, realtime=True)

@@ -281,6 +281,22 @@ class BatchedPoseSolver:
         self.slots = [self._make_slot(inst, cap, i) for i, (inst, cap) in enumerate(zip(self.instances, caps))]
         self.multi = bool(multi)
         self.sets = FrameSets(self.slots, self.device) if self.multi else None
+        # per-frame launches: every iteration ends when its slowest stream
+        # does, so the instances are dealt to the streams longest first onto
+        # the least-loaded one (cost ~ list length + pixels)
+        cost = [cap + inst.W * inst.H for inst, cap in zip(self.instances, caps)]
+        load = [0] * self.nstreams
+        self.groups: List[List[int]] = [[] for _ in range(self.nstreams)]
+        for i in sorted(range(self.n), key=lambda k: -cost[k]):
+            j = min(range(self.nstreams), key=lambda q: load[q])
+            self.groups[j].append(i)
+            load[j] += cost[i]
+        for grp in self.groups:
+            grp.sort()
+        self.stream_of = [0] * self.n
+        for j, grp in enumerate(self.groups):
+            for i in grp:
+                self.stream_of[i] = j
 
     def _make_slot(self, inst: PoseInstance, capacity: int, i: int) -> _Slot:
         return make_slot(inst, capacity, self.binning, _vp(self.adam.pose, 28 * i), _vp(self.adam.grad, 28 * i))
@@ -290,38 +306,43 @@ class BatchedPoseSolver:
         launch_slot(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, stream)
 
     def _per_slot(self, fn):
-        """fn(i, slot, stream) for every slot, slot i on stream i mod S, between
-        a fork from and a join into the current stream."""
+        """fn(i, slot, stream) for every slot, on its stream, between a fork
+        from and a join into the current stream."""
         cur = torch.cuda.current_stream(self.device)
         for st in self.streams:
             st.wait_stream(cur)
         for i, s in enumerate(self.slots):
-            st = self.streams[i % self.nstreams]
+            st = self.streams[self.stream_of[i]]
             with torch.cuda.stream(st):
                 fn(i, s, ctypes.c_void_p(st.cuda_stream))
         for st in self.streams:
             cur.wait_stream(st)
 
+    def _launch_iterations(self, k: int) -> None:
+        for _ in range(k):
+            self._launch_iteration()
+
     def _launch_iteration(self):
         if not self.multi:
             self._per_slot(lambda i, s, st: self._launch_slot(s, i, st))
-        else:
-            # per frame: preprocess + binning; all frames: one blend launch;
-            # per frame: loss and backward set-up; all frames: one render
-            # backward launch; per frame: per-Gaussian / pose gradients
-            lib = _lib.load()
-            cur = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-            self._per_slot(lambda i, s, st: launch_slot_binning(s, st))
-            self.sets.blend_launch(cur)
+            self.adam.step()
+            return
+        # per frame: preprocess + binning; all frames: one blend launch;
+        # per frame: loss and backward set-up; all frames: one render
+        # backward launch; per frame: per-Gaussian / pose gradients
+        lib = _lib.load()
+        cur = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        self._per_slot(lambda i, s, st: launch_slot_binning(s, st))
+        self.sets.blend_launch(cur)
 
-            def loss_and_prepare(i, s, st):
-                launch_slot_loss(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, st)
-                _lib.check(lib.gs_backward_prepare(*_bwd_args(s), st), "gs_backward_prepare")
+        def loss_and_prepare(i, s, st):
+            launch_slot_loss(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, st)
+            _lib.check(lib.gs_backward_prepare(*_bwd_args(s), st), "gs_backward_prepare")
 
-            self._per_slot(loss_and_prepare)
-            self.sets.render_bwd_launch(cur)
-            self._per_slot(lambda i, s, st: _lib.check(lib.gs_backward_finish(*_bwd_args(s), st),
-                                                       "gs_backward_finish"))
+        self._per_slot(loss_and_prepare)
+        self.sets.render_bwd_launch(cur)
+        self._per_slot(lambda i, s, st: _lib.check(lib.gs_backward_finish(*_bwd_args(s), st),
+                                                   "gs_backward_finish"))
         self.adam.step()
 
     def _reset(self):
@@ -336,8 +357,7 @@ class BatchedPoseSolver:
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            for _ in range(self.iters_per_graph):
-                self._launch_iteration()
+            self._launch_iterations(self.iters_per_graph)
         torch.cuda.synchronize(self.device)
         self.graph = g
 
@@ -351,11 +371,10 @@ class BatchedPoseSolver:
             full, rest = divmod(self.iters, self.iters_per_graph)
             for _ in range(full):
                 self.graph.replay()
-            for _ in range(rest):
-                self._launch_iteration()
+            if rest:
+                self._launch_iterations(rest)
         else:
-            for _ in range(self.iters):
-                self._launch_iteration()
+            self._launch_iterations(self.iters)
 
     def overflowed(self) -> List[int]:
         flags = torch.stack([s.counters[3] for s in self.slots]).cpu()

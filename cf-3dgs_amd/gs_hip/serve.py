@@ -390,7 +390,10 @@ class Server:
                     if not r.launched:
                         n = int(r.host[0]) & 0xFFFFFFFF
                         extra = self._render_bytes(r.req, self._round_cap(n)) - r.req.reserved
-                        if self.used + extra > self.budget and (others_busy or r is not waiting[0]):
+                        # (an event may complete between the two queries: such a
+                        # render counts as the oldest waiting one when none was)
+                        oldest = waiting[0] if waiting else r
+                        if self.used + extra > self.budget and (others_busy or r is not oldest):
                             r.req.waited_for_memory = True
                             still_r.append(r)
                             continue

@@ -202,20 +202,23 @@ def test_native_and_torch_plain_reach_the_same_pose():
     assert max(ra, ta, rb, tb) < 2e-3 and max(rab, tab) < 2e-3
 
 
-def test_graph_replays_match_eager():
-    print("graph replays vs eager", flush=True)
-    insts = [build_instance(41 + k, W=128, H=72) for k in range(3)]
+@pytest.mark.parametrize("multi", [True, False])
+def test_graph_replays_match_eager(multi):
+    """Every graph replay agrees bit for bit with eager launches of the same
+    iterations: one-launch stages, or per-frame launches on streams (the
+    instances dealt to the streams by cost)."""
+    print("graph replays vs eager, multi", multi, flush=True)
+    insts = [build_instance(41 + k, W=128, H=72) for k in range(3)] + [build_instance(44, W=160, H=90)]
     ipg = 4
-    eager = BatchedPoseSolver(insts, iters=ipg, streams=3, use_graph=False)
-    graph = BatchedPoseSolver(insts, iters=ipg, streams=3, iters_per_graph=ipg)
+    eager = BatchedPoseSolver(insts, iters=ipg, streams=3, use_graph=False, multi=multi)
+    graph = BatchedPoseSolver(insts, iters=ipg, streams=3, iters_per_graph=ipg, multi=multi)
     graph._reset()
     graph._capture()
     graph._reset()
     eager._reset()
     for rep in range(4):
         graph.graph.replay()
-        for _ in range(ipg):
-            eager._launch_iteration()
+        eager._launch_iterations(ipg)
         torch.cuda.synchronize()
         # host work and eager launches between replays
         assert int(graph.adam.step_count.item()) == ipg * (rep + 1)
