@@ -394,14 +394,15 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
     cfg = dict(fixed, candidates=[], ms_per_iter=0.0) if fixed else autotune(insts)
     t_tune = time.perf_counter() - t0
     solver = BatchedPoseSolver(insts, iters=iters, streams=cfg["streams"], binning=cfg["binning"],
-                               multi=cfg["multi"])
+                               multi=cfg["multi"], groups=cfg["groups"])
     r = solver.timed_run()
     batched_s = r["seconds"]
     _log(f"pose: batched {batched_s:.2f} s; plain one-at-a-time solves")
     if quick:
         return {"workload": f"{n} pose solves x {iters} Adam iterations (batched path only)",
                 "batched_solves_per_s": round(n / batched_s, 3), "batched_seconds": round(batched_s, 3),
-                "autotune": {"streams": cfg["streams"], "binning": cfg["binning"], "multi": cfg["multi"]}}
+                "autotune": {"streams": cfg["streams"], "binning": cfg["binning"],
+                             "blend_groups": cfg["groups"] if cfg["multi"] else 0}}
     plain_t, po_t, diffs, same, same_po = [], [], [], 0, 0
     for k, inst in enumerate(insts):
         o = solve_plain_native(inst, iters=iters)
@@ -450,10 +451,10 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
         "pose_err_vs_truth_median": [rot_gt[len(rot_gt) // 2], sorted(tr_gt)[len(tr_gt) // 2]],
         "pose_err_vs_truth_max": [max(rot_gt), max(tr_gt)],
         "autotune": {"streams": cfg["streams"], "binning": "depth_first" if cfg["binning"] == 0 else "sort64",
-                     "one_launch_blend_and_backward": cfg["multi"], "seconds": round(t_tune, 2),
+                     "blend_groups": cfg["groups"] if cfg["multi"] else 0, "seconds": round(t_tune, 2),
                      "candidates_ms_per_iter": {f"s{c['streams']}-{'df' if c['binning'] == 0 else 's64'}-"
-                                                f"{'multi' if c['multi'] else 'per-frame'}": round(c["ms_per_iter"], 2)
-                                                for c in cfg["candidates"]}},
+                                                f"{'g%d' % c['groups'] if c['multi'] else 'per-frame'}":
+                                                round(c["ms_per_iter"], 2) for c in cfg["candidates"]}},
         "regrows": solver.regrows,
         "c3_single_frame_s": {"plain_torch": round(c3_torch["seconds"], 3),
                               "plain_native": round(c3_native["seconds"], 3),
@@ -489,10 +490,12 @@ def serve_bench(device, n_scenes: int = 16, n_requests: int = 60, calib_requests
            "scene_P": [s.P for s in scenes], "peak_reserved_GB": round(srv.peak_used / 2 ** 30, 2)}
     out.update({k: (round(v, 4) if isinstance(v, float) else v) for k, v in stats.items()})
     # the same arrivals under a budget that binds: the resident scenes plus
-    # room for 1.25 of the largest requests of the first run; the server
-    # starts from the list lengths the first run has seen
+    # 1.1 times the largest request of the first run (its admission estimate,
+    # the last list length seen + 5%, must fit), so that request runs only
+    # when little else is in flight; the server starts from the list lengths
+    # the first run has seen
     reqs = make_requests(n_requests, rate=lam, n_scenes=n_scenes, seed=12)
-    budget = srv.resident + 1.25 * srv.max_request_bytes
+    budget = srv.resident + 1.1 * srv.max_request_bytes
     _log(f"serve: Poisson run under a {budget / 2 ** 30:.2f} GB budget")
     tight = Server(scenes, budget_bytes=budget, seen_counts=srv.seen_counts)
     st2 = tight.run(reqs, realtime=True)
@@ -520,7 +523,8 @@ def main():
     ap.add_argument("--pose-instances", type=int, default=64)
     ap.add_argument("--pose-quick", action="store_true", help="time only the batched pose solves (profiling runs)")
     ap.add_argument("--pose-schedule", default=None,
-                    help="streams,binning,multi for the batched solves instead of autotuning (e.g. 8,0,0)")
+                    help="streams,binning,groups for the batched solves instead of autotuning (e.g. 8,0,0; "
+                         "groups 0 = per-frame blend and backward launches)")
     ap.add_argument("--torch-loss", action="store_true", help="loss with torch conv2d windows instead of the fused kernel")
     ap.add_argument("--no-adam-in-backward", action="store_true",
                     help="write the means' and SH gradients and step them in the optimiser launch")
@@ -629,8 +633,8 @@ def main():
     ing = None if args.no_plain else ingest_bench(device, cpu=not args.no_cpu_baseline)
     fixed = None
     if args.pose_schedule:
-        st_, bi_, mu_ = (int(x) for x in args.pose_schedule.split(","))
-        fixed = dict(streams=st_, binning=bi_, multi=bool(mu_))
+        st_, bi_, gr_ = (int(x) for x in args.pose_schedule.split(","))
+        fixed = dict(streams=st_, binning=bi_, multi=gr_ > 0, groups=max(gr_, 1))
     pose = None if args.no_pose else pose_bench(device, n=args.pose_instances, quick=args.pose_quick, fixed=fixed)
     serving = None if args.no_serve else serve_bench(device)
     res = {

@@ -20,8 +20,10 @@ solves (one per frame instance) runs in lock-step:
     then draws nothing for that iteration), read once after the solve, and
     the affected instances are re-solved with a larger capacity.
 
-``autotune`` picks the stream count and binning schedule per size bucket by
-timing graph replays.
+``autotune`` picks the stream count, the binning schedule and the blend /
+render-backward schedule (per frame, or one launch over all instances, or
+over each of 2, 4 or 8 instance groups, one stream per group) per size
+bucket by timing graph replays.
 """
 from __future__ import annotations
 
@@ -197,6 +199,50 @@ def launch_slot(s: _Slot, out3_ptr: int, one_ptr: int, lam: float, stream: ctype
     _lib.check(lib.gs_backward(*_bwd_args(s), stream), "gs_backward")
 
 
+def _lpt(items: Sequence[int], cost: Sequence[float], bins: int) -> List[List[int]]:
+    """``items`` dealt longest first onto the least-loaded of ``bins`` bins
+    (each bin's items in ascending order)."""
+    load = [0.0] * bins
+    out: List[List[int]] = [[] for _ in range(bins)]
+    for i in sorted(items, key=lambda k: -cost[k]):
+        j = min(range(bins), key=lambda q: load[q])
+        out[j].append(i)
+        load[j] += cost[i]
+    return [sorted(b) for b in out]
+
+
+def group_layout(cost: Sequence[float], nstreams: int, groups: int):
+    """(group count, instances of each group, streams of each group, stream of
+    each instance) for instances of the given costs.
+
+    Every iteration ends when its slowest stream does, so work is dealt
+    longest first onto the least-loaded bin: instances onto groups, then each
+    group's instances onto its streams.  One group spreads its per-frame work
+    over all ``nstreams`` streams.  Several groups take one stream each:
+    every captured dependency must fork from or join into the capturing
+    stream (HIP's end of capture fails when one side stream waits on
+    another), so a group cannot fork streams of its own."""
+    n = len(cost)
+    G = max(1, min(int(groups), n))
+    if G > 1 and G != nstreams:
+        raise ValueError("with several instance groups each group runs on one stream: streams must equal groups")
+    members = _lpt(range(n), cost, G)
+    gstreams = [list(range(nstreams))] if G == 1 else [[j] for j in range(G)]
+    stream_of = [0] * n
+    for mem, sidx in zip(members, gstreams):
+        for q, part in zip(sidx, _lpt(mem, cost, len(sidx))):
+            for i in part:
+                stream_of[i] = q
+    return G, members, gstreams, stream_of
+
+
+def schedule_candidates(binnings: Sequence[int], streams: Sequence[int], groups: Sequence[int]):
+    """The autotuner's (binning, streams, groups) candidates: per-frame
+    (groups 0) and one group at every stream count, several groups only with
+    as many streams."""
+    return [(b, s, g) for b in binnings for s in streams for g in groups if g <= 1 or g == s]
+
+
 class FrameSets:
     """Device descriptions of many solves' workspaces for the one-launch blend
     and render backward (gs_multi_blend / gs_multi_render_bwd)."""
@@ -256,7 +302,7 @@ class BatchedPoseSolver:
     def __init__(self, instances: Sequence[PoseInstance], iters: int = ITERS, lr: float = LR,
                  streams: int = 4, binning: int = _lib.BINNING_DEPTH_FIRST, iters_per_graph: int = 10,
                  headroom: float = 1.3, use_graph: bool = True, lambda_dssim: float = 0.2,
-                 X0: Optional[torch.Tensor] = None, multi: bool = True):
+                 X0: Optional[torch.Tensor] = None, multi: bool = True, groups: int = 1):
         if not instances:
             raise ValueError("no instances")
         self.instances = list(instances)
@@ -280,23 +326,18 @@ class BatchedPoseSolver:
         caps = start_capacities(self.instances, self.adam.pose, self.binning, self.headroom)
         self.slots = [self._make_slot(inst, cap, i) for i, (inst, cap) in enumerate(zip(self.instances, caps))]
         self.multi = bool(multi)
-        self.sets = FrameSets(self.slots, self.device) if self.multi else None
-        # per-frame launches: every iteration ends when its slowest stream
-        # does, so the instances are dealt to the streams longest first onto
-        # the least-loaded one (cost ~ list length + pixels)
+        # every iteration ends when its slowest stream does, so work is dealt
+        # longest first onto the least-loaded bin (cost ~ list length + pixels):
+        # instances onto groups (one-launch blend and backward per group; the
+        # groups share no stream, so one group's launches overlap the others'
+        # per-frame work), then each group's instances onto its streams
         cost = [cap + inst.W * inst.H for inst, cap in zip(self.instances, caps)]
-        load = [0] * self.nstreams
-        self.groups: List[List[int]] = [[] for _ in range(self.nstreams)]
-        for i in sorted(range(self.n), key=lambda k: -cost[k]):
-            j = min(range(self.nstreams), key=lambda q: load[q])
-            self.groups[j].append(i)
-            load[j] += cost[i]
-        for grp in self.groups:
-            grp.sort()
-        self.stream_of = [0] * self.n
-        for j, grp in enumerate(self.groups):
-            for i in grp:
-                self.stream_of[i] = j
+        self.ngroups, self.members, self.group_streams, self.stream_of = group_layout(
+            cost, self.nstreams, int(groups) if self.multi else 1)
+        self.sets = self._frame_sets() if self.multi else None
+
+    def _frame_sets(self) -> List[FrameSets]:
+        return [FrameSets([self.slots[i] for i in members], self.device) for members in self.members]
 
     def _make_slot(self, inst: PoseInstance, capacity: int, i: int) -> _Slot:
         return make_slot(inst, capacity, self.binning, _vp(self.adam.pose, 28 * i), _vp(self.adam.grad, 28 * i))
@@ -305,18 +346,23 @@ class BatchedPoseSolver:
     def _launch_slot(self, s: _Slot, i: int, stream: ctypes.c_void_p):
         launch_slot(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, stream)
 
-    def _per_slot(self, fn):
-        """fn(i, slot, stream) for every slot, on its stream, between a fork
-        from and a join into the current stream."""
-        cur = torch.cuda.current_stream(self.device)
-        for st in self.streams:
-            st.wait_stream(cur)
-        for i, s in enumerate(self.slots):
+    def _per_slot(self, fn, members: Optional[Sequence[int]] = None, lead=None):
+        """fn(i, slot, stream) for every slot (of ``members``), on its stream,
+        between a fork from and a join into ``lead`` (default: the current
+        stream)."""
+        lead = torch.cuda.current_stream(self.device) if lead is None else lead
+        members = range(self.n) if members is None else members
+        used = sorted({self.stream_of[i] for i in members})
+        for q in used:
+            if self.streams[q] is not lead:
+                self.streams[q].wait_stream(lead)
+        for i in members:
             st = self.streams[self.stream_of[i]]
             with torch.cuda.stream(st):
-                fn(i, s, ctypes.c_void_p(st.cuda_stream))
-        for st in self.streams:
-            cur.wait_stream(st)
+                fn(i, self.slots[i], ctypes.c_void_p(st.cuda_stream))
+        for q in used:
+            if self.streams[q] is not lead:
+                lead.wait_stream(self.streams[q])
 
     def _launch_iterations(self, k: int) -> None:
         for _ in range(k):
@@ -327,22 +373,41 @@ class BatchedPoseSolver:
             self._per_slot(lambda i, s, st: self._launch_slot(s, i, st))
             self.adam.step()
             return
-        # per frame: preprocess + binning; all frames: one blend launch;
-        # per frame: loss and backward set-up; all frames: one render
-        # backward launch; per frame: per-Gaussian / pose gradients
+        # per group: per frame preprocess + binning; one blend launch over the
+        # group; per frame loss and backward set-up; one render backward
+        # launch; per frame the per-Gaussian / pose gradients.  One group forks
+        # its per-frame work from the current stream onto all streams and runs
+        # its one-launch stages on the current stream; with more groups each
+        # group runs everything on a stream of its own, forked from the
+        # current stream, and the groups join before the step.  (Under graph
+        # capture every dependency is a fork from or a join into the current
+        # stream: HIP's end of capture fails when one side stream waits on
+        # another.)
         lib = _lib.load()
-        cur = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        self._per_slot(lambda i, s, st: launch_slot_binning(s, st))
-        self.sets.blend_launch(cur)
+        cur = torch.cuda.current_stream(self.device)
 
         def loss_and_prepare(i, s, st):
             launch_slot_loss(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, st)
             _lib.check(lib.gs_backward_prepare(*_bwd_args(s), st), "gs_backward_prepare")
 
-        self._per_slot(loss_and_prepare)
-        self.sets.render_bwd_launch(cur)
-        self._per_slot(lambda i, s, st: _lib.check(lib.gs_backward_finish(*_bwd_args(s), st),
-                                                   "gs_backward_finish"))
+        def finish(i, s, st):
+            _lib.check(lib.gs_backward_finish(*_bwd_args(s), st), "gs_backward_finish")
+
+        used = sorted(set(self.stream_of))
+        if self.ngroups > 1:
+            for q in used:
+                self.streams[q].wait_stream(cur)
+        for members, sidx, sets in zip(self.members, self.group_streams, self.sets):
+            lead = cur if self.ngroups == 1 else self.streams[sidx[0]]
+            lp = ctypes.c_void_p(lead.cuda_stream)
+            self._per_slot(lambda i, s, st: launch_slot_binning(s, st), members, lead)
+            sets.blend_launch(lp)
+            self._per_slot(loss_and_prepare, members, lead)
+            sets.render_bwd_launch(lp)
+            self._per_slot(finish, members, lead)
+        if self.ngroups > 1:
+            for q in used:
+                cur.wait_stream(self.streams[q])
         self.adam.step()
 
     def _reset(self):
@@ -393,7 +458,7 @@ class BatchedPoseSolver:
                 s = self.slots[i]
                 self.slots[i] = self._make_slot(s.inst, _round_capacity(int(s.capacity * 1.5), 1.0), i)
             if self.multi:
-                self.sets = FrameSets(self.slots, self.device)
+                self.sets = self._frame_sets()
         raise RuntimeError(f"binning capacity still exceeded after {max_regrow} regrowths")
 
     def timed_run(self) -> Dict[str, float]:
@@ -422,19 +487,24 @@ def size_bucket(instances: Sequence[PoseInstance]) -> tuple:
 
 
 def autotune(instances: Sequence[PoseInstance], streams=(2, 4, 8),
-             binnings=(_lib.BINNING_DEPTH_FIRST, _lib.BINNING_SORT64), multis=(True, False),
+             binnings=(_lib.BINNING_DEPTH_FIRST, _lib.BINNING_SORT64), groups=(0, 1, 2, 4, 8),
              iters_per_graph: int = 5, replays: int = 2) -> dict:
-    """Times ``replays`` graph replays of every (streams, binning, one-launch
-    blend/backward or not) candidate and returns (and caches per size bucket)
-    the fastest configuration."""
+    """Times ``replays`` graph replays of every (streams, binning schedule,
+    blend / render-backward schedule) candidate and returns (and caches per
+    size bucket) the fastest configuration.  The blend schedule is ``groups``:
+    0 launches the blend and the render backward per frame; g > 0 splits the
+    instances into g groups with one launch of each over a group (one group
+    spreads its per-frame work over all streams; several groups take one
+    stream each, so g > 1 is tried with g streams)."""
     key = size_bucket(instances)
     if key in _TUNED:
         return _TUNED[key]
     best = None
     results = []
-    for b, s, m in [(b, s, m) for b in binnings for s in streams for m in multis]:
+    cands = schedule_candidates(binnings, streams, groups)
+    for b, s, g in cands:
         solver = BatchedPoseSolver(instances, iters=iters_per_graph * replays, streams=s, binning=b,
-                                   iters_per_graph=iters_per_graph, multi=m)
+                                   iters_per_graph=iters_per_graph, multi=g > 0, groups=max(g, 1))
         solver._reset()
         solver._capture()
         solver._reset()
@@ -445,10 +515,11 @@ def autotune(instances: Sequence[PoseInstance], streams=(2, 4, 8),
             solver.graph.replay()
         torch.cuda.synchronize(solver.device)
         dt = (time.perf_counter() - t0) / (replays * iters_per_graph)
-        results.append(dict(streams=s, binning=b, multi=m, ms_per_iter=dt * 1e3))
+        results.append(dict(streams=s, binning=b, multi=g > 0, groups=g, ms_per_iter=dt * 1e3))
         if best is None or dt < best[0]:
-            best = (dt, s, b, m)
+            best = (dt, s, b, g)
         del solver
-    cfg = dict(streams=best[1], binning=best[2], multi=best[3], ms_per_iter=best[0] * 1e3, candidates=results)
+    cfg = dict(streams=best[1], binning=best[2], multi=best[3] > 0, groups=max(best[3], 1),
+               ms_per_iter=best[0] * 1e3, candidates=results)
     _TUNED[key] = cfg
     return cfg

@@ -214,3 +214,29 @@ def test_multi_frame_set_builders_on_the_host():
     frames[1].deterministic = 0  # one kernel variant per set
     assert lib.gs_multi_render_bwd_build(n, frames, P, ptr, cap, ptr, ptr, ptr, None, None, ptr, host2, nr,
                                          ctypes.byref(blocks)) == 1
+
+
+def test_batched_solver_group_layout():
+    """Instance groups: every instance in exactly one group, costs dealt
+    longest first (no group's load above the lightest one's plus its largest
+    item), one group over all streams, several groups one stream each, and
+    any other stream count refused."""
+    from gs_hip import batch
+    cost = [9, 1, 4, 3, 5, 2, 2, 7, 6, 8]
+    G, members, gstreams, stream_of = batch.group_layout(cost, 3, 1)
+    assert G == 1 and sorted(members[0]) == list(range(10)) and gstreams == [[0, 1, 2]]
+    loads = [sum(cost[i] for i in range(10) if stream_of[i] == q) for q in range(3)]
+    assert sum(loads) == sum(cost) and max(loads) - min(loads) <= max(cost)
+    G, members, gstreams, stream_of = batch.group_layout(cost, 4, 4)
+    assert G == 4 and gstreams == [[0], [1], [2], [3]]
+    assert sorted(i for m in members for i in m) == list(range(10))
+    for j, m in enumerate(members):
+        assert all(stream_of[i] == j for i in m)
+    loads = [sum(cost[i] for i in m) for m in members]
+    assert max(loads) - min(loads) <= max(cost)
+    assert batch.group_layout(cost[:2], 2, 4)[0] == 2  # no more groups than instances
+    with pytest.raises(ValueError):
+        batch.group_layout(cost, 3, 2)
+    cands = batch.schedule_candidates((0,), (2, 4, 8), (0, 1, 2, 4, 8))
+    assert cands == [(0, 2, 0), (0, 2, 1), (0, 2, 2), (0, 4, 0), (0, 4, 1), (0, 4, 4), (0, 8, 0), (0, 8, 1),
+                     (0, 8, 8)]

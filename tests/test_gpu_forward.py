@@ -74,6 +74,17 @@ def test_training_scene_binning_bit_exact(binning):
     assert n > 5_000_000
 
 
+@pytest.mark.parametrize("W,H", [(320, 180), (600, 300), (1920, 1080)], ids=["240-tiles", "722-tiles", "8160-tiles"])
+def test_list_sort_widths_bit_exact(W, H):
+    """Depth-first tile lists of 240, 722 and 8160 tiles: one 8-bit pass on
+    the tile bits, then two 8-bit passes with a 2- and a 5-bit top digit.
+    Sorted ids, ranges and images equal the oracle's in each case."""
+    print("list sort widths", W, H, flush=True)
+    sc = scenes.plain_scene(seed=21, P=20000, W=W, H=H)
+    g, o, n = _compare(sc, 3, binning=0)
+    assert n > 20000
+
+
 @pytest.mark.parametrize("deg", [0, 1, 2])
 def test_sh_degrees(deg):
     sc = scenes.plain_scene(seed=3, P=3000)
