@@ -469,6 +469,7 @@ struct BlendArgs {
   const uint32_t* ckpt_base;
   float* ckpt;
   uint32_t* tile_maxc;
+  const uint32_t* tile_order;  // the tiles, longest list first (k_tile_ckpt_scan)
   float* accum;
   // per (tile, batch) at ckpt_base[tile] + tile + batch: the longest
   // per-wave list of the batch, the backward's cost estimate for that chunk
@@ -622,7 +623,7 @@ __device__ __forceinline__ void blend_tile(const BlendArgs& a, uint32_t tile) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_blend(BlendArgs a) { blend_tile(a, xcd_remap(blockIdx.x, gridDim.x)); }
+__global__ __launch_bounds__(256) void k_blend(BlendArgs a) { blend_tile(a, a.tile_order[blockIdx.x]); }
 
 // Frame f of a set whose per-frame block counts have the exclusive prefix
 // `prefix` (n + 1 entries): the last f with prefix[f] <= b.
@@ -636,23 +637,30 @@ __device__ __forceinline__ int frame_of(const uint32_t* prefix, int n, uint32_t 
   return lo;
 }
 
-// Blend of many frames in one launch: one workgroup per tile of every frame.
+// Blend of many frames in one launch: one workgroup per tile of every frame
+// (each frame's tiles in its longest-first order).
 __global__ __launch_bounds__(256) void k_blend_multi(const uint32_t* prefix, const BlendArgs* args, int n) {
-  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t b = blockIdx.x;
   const int f = frame_of(prefix, n, b);
   const BlendArgs a = args[f];
-  blend_tile(a, b - prefix[f]);
+  blend_tile(a, a.tile_order[b - prefix[f]]);
 }
 
 // One workgroup: checkpoint base of every tile = exclusive prefix over tiles
 // of (batches in its list - 1); ckpt_base[n_tiles] = total checkpoints.
 // It also clears the blend's work-item log.
 // Also turns the ranges of empty tiles, left at [UINT_MAX, 0) by the
-// depth-first list sort, into [0, 0).
+// depth-first list sort, into [0, 0), and orders the tiles for the blend,
+// longest list first (a counting sort on the list length in 64-entry steps,
+// capped at 255 steps; the order within a step is whatever the LDS atomics
+// give, which no result depends on).  A launch lasts as long as its last
+// tiles: the longest lists start first, and consecutive workgroups go to
+// different XCDs, so the long lists of one image region spread over the GPU.
 __global__ __launch_bounds__(1024) void k_tile_ckpt_scan(uint32_t* ranges, int nt, uint32_t* ckpt_base,
-                                                         uint32_t* item_log) {
+                                                         uint32_t* item_log, uint32_t* tile_order) {
   __shared__ uint32_t s[1024];
   __shared__ uint32_t s_carry;
+  __shared__ uint32_t s_cls[256];
   const int t = threadIdx.x;
   if (t == 0) {
     s_carry = 0;
@@ -682,6 +690,21 @@ __global__ __launch_bounds__(1024) void k_tile_ckpt_scan(uint32_t* ranges, int n
     __syncthreads();
   }
   if (t == 0) ckpt_base[nt] = s_carry;
+  if (t < 256) s_cls[t] = 0;
+  __syncthreads();
+  auto cls_of = [&](int b) { return min((ranges[2 * b + 1] - ranges[2 * b] + 63u) / 64u, 255u); };
+  for (int b = t; b < nt; b += 1024) atomicAdd(&s_cls[cls_of(b)], 1u);
+  __syncthreads();
+  {
+    // first position of each class, longest class first
+    const uint32_t v = t < 256 ? s_cls[255 - t] : 0u;
+    uint32_t tot;
+    const uint32_t incl = block_scan_incl<1024>(v, s, tot);
+    __syncthreads();
+    if (t < 256) s_cls[255 - t] = incl - v;
+  }
+  __syncthreads();
+  for (int b = t; b < nt; b += 1024) tile_order[atomicAdd(&s_cls[cls_of(b)], 1u)] = (uint32_t)b;
 }
 
 __global__ __launch_bounds__(256) void k_mark_visible(const float* means, int P, const float* view,
@@ -805,6 +828,8 @@ extern "C" int gs_image_layout(int width, int height, gs_image_layout_t* out) {
   out->ckpt_base = o;
   o = align_up(o + 4 * ((size_t)nt + 1), 256);
   out->tile_maxc = o;
+  o = align_up(o + 4 * (size_t)nt, 256);
+  out->tile_order = o;
   o = align_up(o + 4 * (size_t)nt, 256);
   out->total = o;
   return GS_OK;
@@ -1001,7 +1026,7 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
   }
   if (post)
     hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, ranges, nt, (uint32_t*)(ib + LI.ckpt_base),
-                       (uint32_t*)(bb + LB.item_log));
+                       (uint32_t*)(bb + LB.item_log), (uint32_t*)(ib + LI.tile_order));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
@@ -1037,6 +1062,7 @@ static BlendArgs make_blend_args(const gs_frame_t* f, int P, const void* geom, l
   b.ckpt_base = (const uint32_t*)(ib + LI.ckpt_base);
   b.ckpt = (float*)(bb + LB.ckpt);
   b.tile_maxc = (uint32_t*)(ib + LI.tile_maxc);
+  b.tile_order = (const uint32_t*)(ib + LI.tile_order);
   b.bweight = (uint32_t*)(bb + LB.bweight);
   b.item_log = (uint32_t*)(bb + LB.item_log);
   b.accum = (float*)(ib + LI.accum);
@@ -1157,4 +1183,4 @@ extern "C" const char* gs_error_string(int code) {
   }
 }
 
-extern "C" int gs_abi_version(void) { return 2; }
+extern "C" int gs_abi_version(void) { return 3; }

@@ -88,7 +88,8 @@ class GsBinningLayout(ctypes.Structure):
 
 class GsImageLayout(ctypes.Structure):
     _fields_ = [("ranges", c_size_t), ("final_T", c_size_t), ("n_contrib", c_size_t), ("accum", c_size_t),
-                ("ckpt_base", c_size_t), ("tile_maxc", c_size_t), ("total", c_size_t)]
+                ("ckpt_base", c_size_t), ("tile_maxc", c_size_t), ("tile_order", c_size_t),
+                ("total", c_size_t)]
 
 
 EXPORTS = {

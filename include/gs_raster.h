@@ -148,6 +148,7 @@ typedef struct gs_image_layout {
   size_t accum;     /* float[4][H*W] colour sums before the background term, depth */
   size_t ckpt_base; /* uint32[n_tiles+1] first checkpoint of each tile */
   size_t tile_maxc; /* uint32[n_tiles] largest contributor index in the tile */
+  size_t tile_order; /* uint32[n_tiles] the blend's tile order, longest list first */
   size_t total;
 } gs_image_layout_t;
 
@@ -513,7 +514,8 @@ int gs_timing_read(const char* stage, double* total_ms, int* launches);
 
 const char* gs_error_string(int code);
 /* 2: binning layout with the backward work items, one-word list entries and
- * packed tile rects; split forward (gs_render_prepare / gs_render_resume). */
+ * packed tile rects; split forward (gs_render_prepare / gs_render_resume).
+ * 3: image layout with the blend's tile order (tile_order). */
 int gs_abi_version(void);
 
 #ifdef __cplusplus

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 15
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.gs_abi_version() == 2
+    assert lib.gs_abi_version() == 3
 
 
 def test_layouts():
@@ -50,6 +50,7 @@ def test_layouts():
     assert G.total >= 1000 * (48 + 24 + 8) and G.splat % 256 == 0 and G.cov3D % 256 == 0
     I = _lib.image_layout(250, 131)
     assert I.total >= 8 * 16 * 9 + 8 * 250 * 131
+    assert I.tile_order >= I.tile_maxc + 4 * 16 * 9 and I.total >= I.tile_order + 4 * 16 * 9
     with pytest.raises(_lib.GsError):
         _lib.binning_layout(-1, 10, 10, 10)
     with pytest.raises(_lib.GsError):
