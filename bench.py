@@ -33,10 +33,11 @@ except (AttributeError, OSError):
 # the CPU baseline's OpenMP threads stay on neighbouring cores; set before any
 # OpenMP runtime (torch's, the oracle's) is loaded
 os.environ.setdefault("OMP_PROC_BIND", "close")
-# hardware queues per process (HIP's default is 4): the batched pose solver
-# spreads its instances over up to 8 streams, which only run concurrently when
-# each has a queue of its own; read when HIP initialises
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# hardware queues per process, read when HIP initialises.  HIP's default of 4
+# is the fastest measured for the batched pose solver (8 streams share them):
+# with 8 or 16 queues its iteration took twice as long (22 vs 11 ms), as if
+# the queues were time-sliced
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "4")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (ROOT, os.path.join(ROOT, "cf-3dgs_amd")):
@@ -397,10 +398,12 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
                                multi=cfg["multi"], groups=cfg["groups"])
     r = solver.timed_run()
     batched_s = r["seconds"]
+    submit_s = r["submit_seconds"]
     _log(f"pose: batched {batched_s:.2f} s; plain one-at-a-time solves")
     if quick:
         return {"workload": f"{n} pose solves x {iters} Adam iterations (batched path only)",
                 "batched_solves_per_s": round(n / batched_s, 3), "batched_seconds": round(batched_s, 3),
+                "batched_host_seconds_in_replays": round(submit_s, 3) if submit_s is not None else None,
                 "autotune": {"streams": cfg["streams"], "binning": cfg["binning"],
                              "blend_groups": cfg["groups"] if cfg["multi"] else 0}}
     plain_t, po_t, diffs, same, same_po = [], [], [], 0, 0
@@ -437,6 +440,7 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
         "workload": f"{n} pose solves x {iters} Adam iterations, P ~ U[20k,200k], res ~ U{{320x180..960x540}}",
         "batched_solves_per_s": round(n / batched_s, 3),
         "batched_seconds": round(batched_s, 3),
+        "batched_host_seconds_in_replays": round(submit_s, 3) if submit_s is not None else None,
         "batched_p50_s": round(batched_s, 3), "batched_p99_s": round(batched_s, 3),
         "plain_native_solves_per_s": round(n / sum(plain_t), 3),
         "plain_native_p50_s": round(pct(seq_lat, 50), 3), "plain_native_p99_s": round(pct(seq_lat, 99), 3),

@@ -23,7 +23,8 @@
 //                    of its own (the Gaussian's offset in the Gaussian-order
 //                    tile prefix + the tile's rank in its rect) -- no atomics;
 //   k_det_gather     deterministic mode: per Gaussian, its partials summed in
-//                    rect order;
+//                    rect order (the pose-only backward sums its 5-term
+//                    partials the same way inside k_preprocess_bwd);
 //   k_preprocess_bwd per Gaussian: conic -> 2-D cov -> 3-D cov / mean,
 //                    projection, depth, SH direction; scale/rotation; the
 //                    rigid pose gradient is reduced per wave (DPP) and each
@@ -445,8 +446,8 @@ __global__ __launch_bounds__(256) void k_render_bwd_multi(const uint32_t* prefix
 }
 
 // Deterministic mode: per Gaussian, the partials of its (Gaussian, tile)
-// pairs (NT terms each: 10, or the pose-only backward's 5) summed in rect
-// order into its render-gradient record.  Nothing is read when the frame
+// pairs (NT terms each) summed in rect order into its render-gradient
+// record.  Nothing is read when the frame
 // overflowed the binning capacity (no pairs were made).
 template <int NT>
 __global__ __launch_bounds__(256) void k_det_gather(int P, const uint32_t* tiles, const uint32_t* local_incl,
@@ -497,6 +498,15 @@ struct PBArgs {
   float* dL_dscales;
   float* dL_drots;
   float* pose_part;  // [4 * gridDim.x][12] per-wave sums: dL/dt (3), dL/dR row-major (9)
+  // pose-only backward: the render gradient comes straight from the fixed-order
+  // (Gaussian, tile) partials (5 terms each), summed here in rect order
+  // instead of by a separate gather launch; g_render is then not read
+  const float* part5;
+  const uint32_t* g_tiles;
+  const uint32_t* local_incl;
+  const uint32_t* block_excl;
+  const uint32_t* counters;
+  long long capacity;
   // gs_backward_adam: one Adam step of the means and SH coefficients instead
   // of writing their gradients (adam_on == 0 otherwise)
   int adam_on;
@@ -641,10 +651,28 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
   if (a.pose) make_pose(a.pose, ps);
   if (i < a.P) {
     const bool vis = a.radii[i] > 0;
-    const float* rg = a.g_render + (size_t)i * kRG;
-    const float4 rA = reinterpret_cast<const float4*>(rg)[0];  // m2x m2y ca cb
-    const float4 rB = reinterpret_cast<const float4*>(rg)[1];  // cc opac r g
-    const float4 rC = reinterpret_cast<const float4*>(rg)[2];  // b depth - -
+    float4 rA, rB, rC;  // m2x m2y ca cb | cc opac r g | b depth - -
+    if (a.part5) {
+      // the sums k_det_gather<5> forms, in its order (terms 5..9 are zero)
+      float acc[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      const uint32_t nt = a.g_tiles[i];
+      if ((long long)a.counters[0] <= a.capacity && nt > 0) {
+        const uint32_t off = a.block_excl[i >> 8] + a.local_incl[i] - nt;
+        for (uint32_t e = off; e < off + nt; ++e) {
+          const float* src = a.part5 + (size_t)e * 5;
+#pragma unroll
+          for (int k = 0; k < 5; ++k) acc[k] += src[k];
+        }
+      }
+      rA = make_float4(acc[0], acc[1], acc[2], acc[3]);
+      rB = make_float4(acc[4], 0.f, 0.f, 0.f);
+      rC = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      const float* rg = a.g_render + (size_t)i * kRG;
+      rA = reinterpret_cast<const float4*>(rg)[0];
+      rB = reinterpret_cast<const float4*>(rg)[1];
+      rC = reinterpret_cast<const float4*>(rg)[2];
+    }
     // (every per-Gaussian output is optional: a pose-only solve asks for none)
     if (a.dL_dmeans2D) {
       a.dL_dmeans2D[3 * i] = vis ? rA.x : 0.f;
@@ -1132,16 +1160,11 @@ static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* 
   gs_geom_layout(P, &LG);
   const char* gb = (const char*)geom;
   float* pose_part = (float*)((char*)r.g_render + align_up((size_t)P * kRG * sizeof(float), 256));
-  if (f->deterministic && P > 0) {
-    if (po)
-      hipLaunchKernelGGL(k_det_gather<5>, dim3((P + 255) / 256), dim3(256), 0, s, P,
-                         (const uint32_t*)(gb + LG.tiles), (const uint32_t*)(gb + LG.offsets),
-                         (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, (const float*)r.part, r.g_render);
-    else
-      hipLaunchKernelGGL(k_det_gather<10>, dim3((P + 255) / 256), dim3(256), 0, s, P,
-                         (const uint32_t*)(gb + LG.tiles), (const uint32_t*)(gb + LG.offsets),
-                         (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, (const float*)r.part, r.g_render);
-  }
+  // (the pose-only backward gathers its 5-term partials inside k_preprocess_bwd)
+  if (f->deterministic && P > 0 && !po)
+    hipLaunchKernelGGL(k_det_gather<10>, dim3((P + 255) / 256), dim3(256), 0, s, P,
+                       (const uint32_t*)(gb + LG.tiles), (const uint32_t*)(gb + LG.offsets),
+                       (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, (const float*)r.part, r.g_render);
   if (P > 0) {
     PBArgs b;
     b.P = P;
@@ -1176,6 +1199,12 @@ static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* 
     b.dL_dscales = out->dL_dscales;
     b.dL_drots = out->dL_drotations;
     b.pose_part = pose_part;
+    b.part5 = po ? (const float*)r.part : nullptr;
+    b.g_tiles = (const uint32_t*)(gb + LG.tiles);
+    b.local_incl = (const uint32_t*)(gb + LG.offsets);
+    b.block_excl = (const uint32_t*)(gb + LG.scan_tmp);
+    b.counters = counters;
+    b.capacity = capacity;
     b.adam_on = ad ? 1 : 0;
     b.means_p = ad ? ad->means_p : nullptr;
     b.means_m = ad ? ad->in->means_m : nullptr;

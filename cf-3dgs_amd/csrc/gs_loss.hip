@@ -9,6 +9,8 @@
 //                and the three partials dS/dmu_x, dS/dE[x^2], dS/dE[xy]
 //                (written as maps for the backward); block sums of S, |x-y|
 //   k_loss_final one workgroup: sums the block partials (fixed order) -> loss
+//                (gs_ssim_l1_forward_backward has the backward's first
+//                workgroup do it instead)
 //   k_ssim_bwd   window-correlates the three partial maps (the adjoint of the
 //                zero-padded forward window) and combines them with x, y and
 //                the L1 sign into dloss/dx, scaled by the upstream gradient.
@@ -187,10 +189,10 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(LossArgs a) {
   }
 }
 
-// out[0] = loss, out[1] = mean SSIM, out[2] = mean |x - y|
-__global__ __launch_bounds__(256) void k_loss_final(const float* partial, int nb, float inv_n, float lambda,
-                                                    float* out) {
-  __shared__ double red[2][256];
+// out[0] = loss, out[1] = mean SSIM, out[2] = mean |x - y|, by one
+// 256-thread workgroup (red: its [2][256] doubles of LDS)
+__device__ __forceinline__ void loss_final_block(const float* partial, int nb, float inv_n, float lambda, float* out,
+                                                 double (*red)[256]) {
   double s = 0.0, l = 0.0;
   for (int b = threadIdx.x; b < nb; b += 256) {
     s += (double)partial[2 * b];
@@ -215,6 +217,12 @@ __global__ __launch_bounds__(256) void k_loss_final(const float* partial, int nb
   }
 }
 
+__global__ __launch_bounds__(256) void k_loss_final(const float* partial, int nb, float inv_n, float lambda,
+                                                    float* out) {
+  __shared__ double red[2][256];
+  loss_final_block(partial, nb, inv_n, lambda, out, red);
+}
+
 struct LossBwdArgs {
   int C, H, W;
   float lambda, inv_n;
@@ -225,11 +233,21 @@ struct LossBwdArgs {
   const float* maps;
   const float* grad_out;  // [1] upstream gradient of the loss (device)
   float* dx;
+  // gs_ssim_l1_forward_backward: the first workgroup also reduces the
+  // forward's block partials into out3 (the gradient does not need the loss
+  // value), which spares the separate k_loss_final launch
+  const float* partial;
+  int nb;
+  float* out3;
 };
 
 __global__ __launch_bounds__(256) void k_ssim_bwd(LossBwdArgs a) {
   __shared__ float sm[3][kLIH][kLIW];
   __shared__ float sh[3][kLIH][kLTX];
+  if (a.out3 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) {
+    __shared__ double red[2][256];
+    loss_final_block(a.partial, a.nb, a.inv_n, a.lambda, a.out3, red);
+  }
   const int ch = blockIdx.z;
   const size_t HW = (size_t)a.H * a.W;
   const int r0 = blockIdx.y * kLTY - kLR, c0 = blockIdx.x * kLTX - kLR;
@@ -362,6 +380,50 @@ extern "C" int gs_ssim_l1_backward(const float* pred, const float* target, int C
   a.maps = (const float*)ws;
   a.grad_out = grad_out;
   a.dx = dpred;
+  a.partial = nullptr;
+  a.nb = 0;
+  a.out3 = nullptr;
   hipLaunchKernelGGL(k_ssim_bwd, loss_grid(C, H, W), dim3(256), 0, (hipStream_t)stream, a);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_ssim_l1_forward_backward(const float* pred, const float* target, int C, int H, int W,
+                                           const float* window11, float lambda_dssim, int clamp01, float* out3,
+                                           const float* grad_out, void* ws, size_t ws_bytes, float* dpred,
+                                           void* stream) {
+  if (!pred || !target || !window11 || !out3 || !grad_out || !ws || !dpred || C <= 0 || H <= 0 || W <= 0)
+    return GS_E_ARG;
+  if (ws_bytes < gs_loss_workspace_bytes(C, H, W)) return GS_E_WORKSPACE;
+  LossArgs f;
+  f.C = C;
+  f.H = H;
+  f.W = W;
+  f.lambda = lambda_dssim;
+  f.clamp01 = clamp01;
+  for (int k = 0; k < 11; ++k) f.win.g[k] = window11[k];
+  f.x = pred;
+  f.y = target;
+  f.maps = (float*)ws;
+  f.partial = (float*)((char*)ws + align_up(3 * sizeof(float) * (size_t)C * H * W, 256));
+  const dim3 g = loss_grid(C, H, W);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_ssim_fwd, g, dim3(256), 0, s, f);
+  LossBwdArgs b;
+  b.C = C;
+  b.H = H;
+  b.W = W;
+  b.lambda = lambda_dssim;
+  b.clamp01 = clamp01;
+  b.inv_n = 1.0f / ((float)C * H * W);
+  b.win = f.win;
+  b.x = pred;
+  b.y = target;
+  b.maps = f.maps;
+  b.grad_out = grad_out;
+  b.dx = dpred;
+  b.partial = f.partial;
+  b.nb = (int)(g.x * g.y * g.z);
+  b.out3 = out3;
+  hipLaunchKernelGGL(k_ssim_bwd, g, dim3(256), 0, s, b);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }

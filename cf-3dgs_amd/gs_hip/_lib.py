@@ -150,6 +150,9 @@ EXPORTS = {
                                    c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "gs_ssim_l1_backward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_float), c_float,
                                     c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
+    "gs_ssim_l1_forward_backward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_float),
+                                            c_float, c_int, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p,
+                                            c_void_p]),
     "gs_se3_retract": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gs_se3_grad": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gs_se3_adam": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float,

@@ -385,6 +385,13 @@ int gs_ssim_l1_backward(const float* pred, const float* target, int C, int H, in
                         float* dpred, void* stream);
 /* (clamp01 != 0: the loss of clamp(pred, 0, 1), the caller's image.clamp(0, 1)
  * at scene/gaussian_model_cf.py:885, with clamp's gradient mask.) */
+/* gs_ssim_l1_forward then gs_ssim_l1_backward in two launches instead of
+ * three, for callers that do not look at the loss before the backward (the
+ * batched pose solve; the reference's loop checks torch.isnan(loss) in
+ * between, trainer/losses.py:126): out3 and dpred are the same bits. */
+int gs_ssim_l1_forward_backward(const float* pred, const float* target, int C, int H, int W, const float* window11,
+                                float lambda_dssim, int clamp01, float* out3, const float* grad_out, void* ws,
+                                size_t ws_bytes, float* dpred, void* stream);
 
 /* SE(3) pose parameters as lietorch's LieGroupParameter holds them
  * (scene/gaussian_model_cf.py:328-340 init_RT, :123-133 P[0].retr().act(xyz)):

@@ -50,3 +50,34 @@ def test_fused_loss_upstream_gradient_scales():
     l2, _ = fused_photometric_loss(x2, gt.to(DEV))
     l2.backward()
     torch.testing.assert_close(x1.grad, 3.5 * x2.grad, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("C,H,W,clamp", [(3, 540, 960, 1), (3, 37, 23, 0)])
+def test_forward_backward_in_one_call_equals_the_two_calls(C, H, W, clamp):
+    """gs_ssim_l1_forward_backward (the backward's first workgroup reduces the
+    forward's partials) gives the loss statistics and the gradient of
+    gs_ssim_l1_forward + gs_ssim_l1_backward bit for bit."""
+    import ctypes
+    from gs_hip import _lib
+    from gs_hip.loss import _window_taps
+    lib = _lib.load()
+    g = torch.Generator().manual_seed(7)
+    y = torch.rand((C, H, W), generator=g).to(DEV)
+    x = (torch.rand((C, H, W), generator=g) * 1.4 - 0.2).to(DEV)
+    one = torch.full((1,), 0.75, device=DEV)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    nb = lib.gs_loss_workspace_bytes(C, H, W)
+    taps = _window_taps()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(DEV).cuda_stream)
+    ws1, ws2 = (torch.empty(nb, dtype=torch.uint8, device=DEV) for _ in range(2))
+    out1, out2 = torch.zeros(3, device=DEV), torch.zeros(3, device=DEV)
+    dx1, dx2 = torch.zeros_like(x), torch.zeros_like(x)
+    assert lib.gs_ssim_l1_forward(p(x), p(y), C, H, W, taps, 0.2, clamp, p(out1), p(ws1), nb, stream) == 0
+    assert lib.gs_ssim_l1_backward(p(x), p(y), C, H, W, taps, 0.2, clamp, p(one), p(ws1), nb, p(dx1), stream) == 0
+    assert lib.gs_ssim_l1_forward_backward(p(x), p(y), C, H, W, taps, 0.2, clamp, p(out2), p(one), p(ws2), nb,
+                                           p(dx2), stream) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(out1, out2) and torch.equal(dx1, dx2)
+    assert float(dx1.abs().max()) > 0
+    assert lib.gs_ssim_l1_forward_backward(p(x), p(y), C, H, W, taps, 0.2, clamp, p(out2), p(one), p(ws2), nb - 1,
+                                           p(dx2), stream) == 2  # GS_E_WORKSPACE
