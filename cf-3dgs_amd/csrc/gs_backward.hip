@@ -170,6 +170,10 @@ __device__ __forceinline__ void wave_sum10(const float v[10], float w[3]) {
     swap_add16(u[2 * m], u[2 * m + 1], w[m]);
     w[m] = row_sum16(w[m]);
   }
+  // keep the last row step with the reduction: the sums are consumed only
+  // by the storing lanes, and an add sunk into that branch would leave its
+  // DPP permutation as a separate move
+  asm volatile("" : : "v"(w[0]), "v"(w[1]), "v"(w[2]));
 }
 
 // DEP: the frame has a depth or an alpha cotangent.  Without one (a colour
@@ -189,6 +193,7 @@ __device__ __forceinline__ void wave_sum5(const float v[5], float w[2]) {
   w[0] = row_sum16(w[0]);
   swap_add16(u[2], 0.0f, w[1]);
   w[1] = row_sum16(w[1]);
+  asm volatile("" : : "v"(w[0]), "v"(w[1]));  // as in wave_sum10
 }
 
 template <bool DET, bool DEP, bool PO>
@@ -291,7 +296,9 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
     __syncthreads();
     // the wave's own list: splats that miss its 8 x 8 pixels are ones no lane would use
     const int lim = wlast > sbase ? (int)min(wlast - sbase, (uint32_t)kSub) : 0;
-    const int nw = compact_wave_list<kSub / 64>(s_mask, s_list[wv], wv, lane, lim);
+    // (a sum of ballot counts, so wave-uniform; read from the first lane so
+    // that the pair loop below counts in a scalar register)
+    const int nw = __builtin_amdgcn_readfirstlane(compact_wave_list<kSub / 64>(s_mask, s_list[wv], wv, lane, lim));
     // Branch-free per pair: masked lanes cost a SIMD as much as active ones,
     // so every lane evaluates the pair; the only branches are wave-uniform
     // skips when no lane is active.  An inactive lane runs with alpha = G = 0,
@@ -314,13 +321,20 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       const float4 gb = s_b[j];
       const float dx = ga.x - pfx, dy = ga.y - pfy;
       const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
-      // gb.w: alpha < 1/255 for sure below it
-      bool active = inside && q < last && !(power > 0.0f || power < gb.w);
-      if (__builtin_amdgcn_ballot_w64(active) == 0ull) continue;
+      // gb.w: alpha < 1/255 for sure below it.  (q < last implies inside:
+      // last is 0 outside the image.)  The wave-uniform skips test the
+      // product of one ballot per compare, which is the ballot of the
+      // conjunction: a ballot of a single compare is that compare's lane mask.
+      const bool in_list = q < last, p_le0 = !(power > 0.0f), p_ge = !(power < gb.w);
+      uint64_t live = __builtin_amdgcn_ballot_w64(in_list) & __builtin_amdgcn_ballot_w64(p_le0) &
+                      __builtin_amdgcn_ballot_w64(p_ge);
+      if (live == 0ull) continue;
       const float Graw = spec_expf(power);
       const float alpha_raw = fminf(0.99f, gb.y * Graw);
-      active = active && !(alpha_raw < a_min);
-      if (__builtin_amdgcn_ballot_w64(active) == 0ull) continue;
+      const bool a_ok = !(alpha_raw < a_min);
+      live &= __builtin_amdgcn_ballot_w64(a_ok);
+      if (live == 0ull) continue;
+      const bool active = in_list && p_le0 && p_ge && a_ok;
       const float G = active ? Graw : 0.0f;
       const float alpha = active ? alpha_raw : 0.0f;
       const float4 gc = s_c[j];

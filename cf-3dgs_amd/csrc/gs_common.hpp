@@ -216,9 +216,12 @@ __device__ __forceinline__ void sh_basis_grad(int deg, float x, float y, float z
 
 // ---------------------------------------------------------------------------
 // Wave64 helpers.
+// A DPP lane permutation within 16-lane rows (quad_perm, row_mirror,
+// row_half_mirror: every lane has a source lane, so no lane keeps an old
+// value and none needs to be zero-filled first).
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
 }
 
 // Sum over all 64 lanes; every lane must be active.  Result is wave-uniform.
