@@ -395,7 +395,7 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
     cfg = dict(fixed, candidates=[], ms_per_iter=0.0) if fixed else autotune(insts)
     t_tune = time.perf_counter() - t0
     solver = BatchedPoseSolver(insts, iters=iters, streams=cfg["streams"], binning=cfg["binning"],
-                               multi=cfg["multi"], groups=cfg["groups"])
+                               multi=cfg["multi"], groups=cfg["groups"], shared_sort=cfg["shared_sort"])
     r = solver.timed_run()
     batched_s = r["seconds"]
     submit_s = r["submit_seconds"]
@@ -405,7 +405,8 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
                 "batched_solves_per_s": round(n / batched_s, 3), "batched_seconds": round(batched_s, 3),
                 "batched_host_seconds_in_replays": round(submit_s, 3) if submit_s is not None else None,
                 "autotune": {"streams": cfg["streams"], "binning": cfg["binning"],
-                             "blend_groups": cfg["groups"] if cfg["multi"] else 0}}
+                             "blend_groups": cfg["groups"] if cfg["multi"] else 0,
+                             "shared_depth_sort": cfg["shared_sort"]}}
     plain_t, po_t, diffs, same, same_po = [], [], [], 0, 0
     for k, inst in enumerate(insts):
         o = solve_plain_native(inst, iters=iters)
@@ -455,9 +456,11 @@ def pose_bench(device, n: int = 64, iters: int = 300, torch_sample: int = 2, qui
         "pose_err_vs_truth_median": [rot_gt[len(rot_gt) // 2], sorted(tr_gt)[len(tr_gt) // 2]],
         "pose_err_vs_truth_max": [max(rot_gt), max(tr_gt)],
         "autotune": {"streams": cfg["streams"], "binning": "depth_first" if cfg["binning"] == 0 else "sort64",
-                     "blend_groups": cfg["groups"] if cfg["multi"] else 0, "seconds": round(t_tune, 2),
+                     "blend_groups": cfg["groups"] if cfg["multi"] else 0,
+                     "shared_depth_sort": cfg["shared_sort"], "seconds": round(t_tune, 2),
                      "candidates_ms_per_iter": {f"s{c['streams']}-{'df' if c['binning'] == 0 else 's64'}-"
-                                                f"{'g%d' % c['groups'] if c['multi'] else 'per-frame'}":
+                                                f"{'g%d' % c['groups'] if c['multi'] else 'per-frame'}"
+                                                f"{'-shared-sort' if c['shared_sort'] else ''}":
                                                 round(c["ms_per_iter"], 2) for c in cfg["candidates"]}},
         "regrows": solver.regrows,
         "c3_single_frame_s": {"plain_torch": round(c3_torch["seconds"], 3),
@@ -527,8 +530,9 @@ def main():
     ap.add_argument("--pose-instances", type=int, default=64)
     ap.add_argument("--pose-quick", action="store_true", help="time only the batched pose solves (profiling runs)")
     ap.add_argument("--pose-schedule", default=None,
-                    help="streams,binning,groups for the batched solves instead of autotuning (e.g. 8,0,0; "
-                         "groups 0 = per-frame blend and backward launches)")
+                    help="streams,binning,groups[,shared] for the batched solves instead of autotuning (e.g. "
+                         "8,0,0,1; groups 0 = per-frame blend and backward launches; shared 1 = one segmented "
+                         "depth sort per stream or group)")
     ap.add_argument("--torch-loss", action="store_true", help="loss with torch conv2d windows instead of the fused kernel")
     ap.add_argument("--no-adam-in-backward", action="store_true",
                     help="write the means' and SH gradients and step them in the optimiser launch")
@@ -637,8 +641,10 @@ def main():
     ing = None if args.no_plain else ingest_bench(device, cpu=not args.no_cpu_baseline)
     fixed = None
     if args.pose_schedule:
-        st_, bi_, gr_ = (int(x) for x in args.pose_schedule.split(","))
-        fixed = dict(streams=st_, binning=bi_, multi=gr_ > 0, groups=max(gr_, 1))
+        parts = [int(x) for x in args.pose_schedule.split(",")]
+        st_, bi_, gr_ = parts[:3]
+        fixed = dict(streams=st_, binning=bi_, multi=gr_ > 0, groups=max(gr_, 1),
+                     shared_sort=bool(parts[3]) if len(parts) > 3 else False)
     pose = None if args.no_pose else pose_bench(device, n=args.pose_instances, quick=args.pose_quick, fixed=fixed)
     serving = None if args.no_serve else serve_bench(device)
     res = {

@@ -22,6 +22,8 @@
 #include "gs_common.hpp"
 #include "gs_internal.hpp"
 
+#include <vector>
+
 namespace gs {
 
 struct PreArgs {
@@ -907,13 +909,17 @@ extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* 
 // num_rendered (clearing the tile ranges; in the depth-first schedule also
 // the depth keys, the 32-bit depth sort and the prefix of the tile counts in
 // depth order), so a caller can queue it before it waits for the readback;
-// kResume is the rest, after kPrepare with the same arguments.
-enum BinPhase { kBinAll = 0, kPrepare = 1, kResume = 2 };
+// kResume is the rest, after kPrepare with the same arguments.  kKeys and
+// kAfterSort split the depth-first schedule around its depth sort instead, so
+// that one segmented sort can serve many frames: kKeys is the depth keys
+// alone, kAfterSort everything after the depth sort.
+enum BinPhase { kBinAll = 0, kPrepare = 1, kResume = 2, kKeys = 3, kAfterSort = 4 };
 
 static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
                           void* binning, size_t binning_bytes, void* image, size_t image_bytes, hipStream_t s,
                           int phase = kBinAll) {
-  const bool pre = phase != kResume, post = phase != kPrepare;
+  const bool pre = phase == kBinAll || phase == kPrepare, post = phase == kBinAll || phase == kResume;
+  const bool split = phase == kKeys || phase == kAfterSort;
   if (check_frame(f) != GS_OK || P < 0 || !counters || !image) return GS_E_ARG;
   if (P > 0 && !geom) return GS_E_ARG;
   if (capacity < 0 || capacity > 0x7fffffffLL) return GS_E_ARG;
@@ -932,6 +938,10 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
   char* ib = (char*)image;
   uint32_t* ranges = (uint32_t*)(ib + LI.ranges);
   const bool df = capacity > 0 && P > 0 && f->binning == GS_BINNING_DEPTH_FIRST;
+  if (split && !df) return GS_E_ARG;  // only the depth-first schedule has a depth sort to split around
+  // depth-first stages: keys | depth sort | tile-count prefix | duplicate, list sort, checkpoint scan
+  const bool do_keys = pre || phase == kKeys, do_dsort = pre, do_dscan = pre || phase == kAfterSort;
+  const bool do_rest = post || phase == kAfterSort;
   if (pre && !df)  // the depth-first schedule clears them in k_df_keys
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nblocks256(2LL * nt), 1024)), dim3(256), 0, s, ranges, 2LL * nt,
                        0u);
@@ -943,25 +953,27 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
     uint32_t* dva = (uint32_t*)(bb + LB.dvals_a);
     const int nbp = nblocks256(P);
     hipError_t e = hipSuccess;
-    int in_b = sort_passes(32) & 1;  // where the depth sort leaves its result
-    if (pre) {
+    int in_b = sort_passes(32) & 1;  // where the depth sort (or a segmented sort of it) leaves its result
+    if (do_keys || do_dsort) {
       {
         timing::Scope ts(timing::kSort, s);
-        hipLaunchKernelGGL(k_df_keys, dim3(std::max(nbp, nblocks256(2LL * nt))), dim3(256), 0, s, P, tiles, splat,
-                           gx, gy, dka, dva, (uint2*)(bb + LB.drect), counters, ranges, 2 * nt);
-        e = sort_pairs_u32(dka, dva, (uint32_t*)(bb + LB.dkeys_b), (uint32_t*)(bb + LB.dvals_b), counters + 2, P,
-                           32, bb + LB.hist, s, &in_b);
+        if (do_keys)
+          hipLaunchKernelGGL(k_df_keys, dim3(std::max(nbp, nblocks256(2LL * nt))), dim3(256), 0, s, P, tiles,
+                             splat, gx, gy, dka, dva, (uint2*)(bb + LB.drect), counters, ranges, 2 * nt);
+        if (do_dsort)
+          e = sort_pairs_u32(dka, dva, (uint32_t*)(bb + LB.dkeys_b), (uint32_t*)(bb + LB.dvals_b), counters + 2, P,
+                             32, bb + LB.hist, s, &in_b);
       }
       if (e != hipSuccess) return GS_E_LAUNCH;
     }
     const uint32_t* dvals = (const uint32_t*)(bb + (in_b ? LB.dvals_b : LB.dvals_a));
-    if (pre) {
+    if (do_dscan) {
       timing::Scope ts(timing::kDuplicate, s);
       hipLaunchKernelGGL(k_df_scan, dim3(nbp), dim3(256), 0, s, P, dvals, tiles, (uint32_t*)(bb + LB.dscan),
                          (uint32_t*)(bb + LB.dblocks));
       hipLaunchKernelGGL(k_excl_scan_1wg, dim3(1), dim3(1024), 0, s, (uint32_t*)(bb + LB.dblocks), nbp);
     }
-    if (!post) return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+    if (!do_rest) return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
     {
       timing::Scope ts(timing::kDuplicate, s);
       DfDupArgs d;
@@ -1024,7 +1036,7 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
     hipLaunchKernelGGL(k_ranges, dim3(nblocks256(capacity)), dim3(256), 0, s,
                        (const uint64_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, ranges);
   }
-  if (post)
+  if (post || phase == kAfterSort)
     hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, ranges, nt, (uint32_t*)(ib + LI.ckpt_base),
                        (uint32_t*)(bb + LB.item_log), (uint32_t*)(ib + LI.tile_order));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
@@ -1112,6 +1124,61 @@ extern "C" int gs_render_resume(const gs_frame_t* f, int P, const void* geom, ui
                            out_color, out_depth, out_alpha, (hipStream_t)stream);
 }
 
+// ---- the depth-first binning around a depth sort shared by many frames -----
+extern "C" int gs_render_depth_keys(const gs_frame_t* f, int P, const void* geom, uint32_t* counters,
+                                    long long capacity, void* binning, size_t binning_bytes, void* image,
+                                    size_t image_bytes, void* stream) {
+  return render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                        (hipStream_t)stream, kKeys);
+}
+
+extern "C" int gs_render_binning_after_depth_sort(const gs_frame_t* f, int P, const void* geom, uint32_t* counters,
+                                                  long long capacity, void* binning, size_t binning_bytes,
+                                                  void* image, size_t image_bytes, void* stream) {
+  return render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                        (hipStream_t)stream, kAfterSort);
+}
+
+extern "C" int gs_render_after_depth_sort(const gs_frame_t* f, int P, const void* geom, uint32_t* counters,
+                                          long long capacity, void* binning, size_t binning_bytes, void* image,
+                                          size_t image_bytes, float* out_color, float* out_depth, float* out_alpha,
+                                          void* stream) {
+  return render_blend_from(kAfterSort, f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                           out_color, out_depth, out_alpha, (hipStream_t)stream);
+}
+
+extern "C" size_t gs_multi_depth_sort_bytes(int n) { return seg_sort_desc_bytes(n, 32); }
+
+extern "C" int gs_multi_depth_sort_build(int n, const gs_frame_t* frames, const int* P, const long long* capacity,
+                                         void* const* binning, const size_t* binning_bytes,
+                                         uint32_t* const* counters, void* host_set, size_t set_bytes,
+                                         unsigned* blocks) {
+  if (n <= 0 || !frames || !P || !capacity || !binning || !binning_bytes || !counters || !host_set || !blocks)
+    return GS_E_ARG;
+  if (set_bytes < gs_multi_depth_sort_bytes(n)) return GS_E_WORKSPACE;
+  std::vector<SortSeg> segs((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const gs_frame_t* f = &frames[i];
+    if (check_frame(f) != GS_OK || f->binning != GS_BINNING_DEPTH_FIRST || P[i] <= 0 || capacity[i] <= 0 ||
+        capacity[i] > 0x7fffffffLL || P[i] > kSegMaxKeys || !binning[i] || !counters[i])
+      return GS_E_ARG;
+    gs_binning_layout_t LB;
+    if (gs_binning_layout(capacity[i], P[i], f->width, f->height, f->binning, &LB) != GS_OK) return GS_E_ARG;
+    if (binning_bytes[i] < LB.total) return GS_E_WORKSPACE;
+    char* bb = (char*)binning[i];
+    segs[i] = SortSeg{(uint32_t*)(bb + LB.dkeys_a), (uint32_t*)(bb + LB.dvals_a), (uint32_t*)(bb + LB.dkeys_b),
+                      (uint32_t*)(bb + LB.dvals_b), counters[i] + 2, (long long)P[i], bb + LB.hist};
+  }
+  return seg_sort_build(n, segs.data(), 32, host_set, set_bytes, blocks) ? GS_OK : GS_E_ARG;
+}
+
+extern "C" int gs_multi_depth_sort(int n, const void* dev_set, unsigned blocks, void* stream) {
+  if (n <= 0 || !dev_set || blocks == 0) return GS_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  timing::Scope ts(timing::kSort, s);
+  return seg_sort_launch(n, dev_set, blocks, 32, s) == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
 // ---- many frames in one blend launch --------------------------------------
 // Set layout: uint32 prefix[n + 1] of tiles per frame, then BlendArgs[n] at a
 // 256-byte boundary.
@@ -1183,4 +1250,4 @@ extern "C" const char* gs_error_string(int code) {
   }
 }
 
-extern "C" int gs_abi_version(void) { return 3; }
+extern "C" int gs_abi_version(void) { return 4; }

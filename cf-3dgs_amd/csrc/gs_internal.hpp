@@ -58,4 +58,25 @@ hipError_t sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, 
                           const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
                           int* result_in_b, uint32_t* ranges = nullptr);
 
+// A segmented sort: many independent 32-bit (key, value) sorts, each on its
+// own buffers exactly as sort_pairs_u32 would run it (same passes, same
+// workgroup tiles, the same result buffer), but each kernel of a pass is one
+// launch over every segment.  Segments of at most kSegMaxKeys keys.
+struct SortSeg {
+  uint32_t* keys_a;
+  uint32_t* vals_a;
+  uint32_t* keys_b;
+  uint32_t* vals_b;
+  const uint32_t* n_dev;
+  long long capacity;
+  void* tmp;  // sort_tmp_bytes(capacity)
+};
+constexpr long long kSegMaxKeys = 1LL << 21;
+size_t seg_sort_desc_bytes(int n, int end_bit);
+// Host-side descriptor of the segments' passes (its pointers are the
+// segments' device buffers); the caller copies it to the device.  Returns
+// false on a segment it cannot take.  *blocks is the upsweep / scatter grid.
+bool seg_sort_build(int n, const SortSeg* segs, int end_bit, void* host, size_t bytes, unsigned* blocks);
+hipError_t seg_sort_launch(int n, const void* dev_desc, unsigned blocks, int end_bit, hipStream_t s);
+
 }  // namespace gs

@@ -68,16 +68,18 @@ __device__ __forceinline__ uint32_t n_eff(const uint32_t* n_dev, long long cap) 
   return ((long long)n > cap) ? 0u : n;
 }
 
+// Workgroup `bid` of an upsweep over one key array (the kernels below pass
+// their block index, or, in a segmented sort, the block index within the
+// segment).
 template <typename K, int ITEMS, int R>
-__global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restrict__ keys, const uint32_t* n_dev,
-                                                               long long cap, int shift, int bits, uint32_t* hist,
-                                                               int nb_max) {
+__device__ __forceinline__ void upsweep_block(const K* __restrict__ keys, const uint32_t* n_dev, long long cap,
+                                              int shift, int bits, uint32_t* hist, int nb_max, uint32_t bid) {
   __shared__ uint32_t s_h[4][R];
   const uint32_t n = n_eff(n_dev, cap);
   constexpr uint32_t TILE = kSortThreads * ITEMS;
   const uint32_t nblk = (n + TILE - 1) / TILE;
-  if (blockIdx.x >= nblk) return;
-  const uint32_t blk = xcd_remap(blockIdx.x, nblk);  // consecutive blocks on one XCD (see the file comment)
+  if (bid >= nblk) return;
+  const uint32_t blk = xcd_remap(bid, nblk);  // consecutive blocks on one XCD (see the file comment)
   const uint32_t start = blk * TILE;
   for (int k = threadIdx.x; k < 4 * R; k += kSortThreads) (&s_h[0][0])[k] = 0;
   __syncthreads();
@@ -176,17 +178,23 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restri
     hist[(size_t)d * nb_max + blk] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
 }
 
+template <typename K, int ITEMS, int R>
+__global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const K* __restrict__ keys, const uint32_t* n_dev,
+                                                               long long cap, int shift, int bits, uint32_t* hist,
+                                                               int nb_max) {
+  upsweep_block<K, ITEMS, R>(keys, n_dev, cap, shift, bits, hist, nb_max, blockIdx.x);
+}
+
 // One workgroup per digit: exclusive scan over the active blocks' counts.
 // Each thread takes kScanRun consecutive counts per round (all loaded before
 // any is used) and scans them in registers around one workgroup scan of the
 // run sums, so a row of up to 256 * kScanRun blocks costs one workgroup scan.
 constexpr int kScanRun = 8;
-__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* totals, const uint32_t* n_dev,
-                                                   long long cap, int nb_max, int tile) {
+__device__ __forceinline__ void scan_row(uint32_t* hist, uint32_t* totals, const uint32_t* n_dev, long long cap,
+                                         int nb_max, int tile, int d) {
   __shared__ uint32_t s_w[4];
   const uint32_t n = n_eff(n_dev, cap);
   const int nb = (int)((n + tile - 1) / tile);
-  const int d = blockIdx.x;
   uint32_t* row = hist + (size_t)d * nb_max;
   const int t = threadIdx.x;
   uint32_t carry = 0;
@@ -211,15 +219,17 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* tot
   if (t == 0) totals[d] = carry;
 }
 
+__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* hist, uint32_t* totals, const uint32_t* n_dev,
+                                                   long long cap, int nb_max, int tile) {
+  scan_row(hist, totals, n_dev, cap, nb_max, tile, blockIdx.x);
+}
+
 template <typename K, int ITEMS, int R, bool V>
-__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restrict__ keys_in,
-                                                               const uint32_t* __restrict__ vals_in,
-                                                               K* __restrict__ keys_out,
-                                                               uint32_t* __restrict__ vals_out,
-                                                               const uint32_t* n_dev, long long cap, int shift,
-                                                               int bits, const uint32_t* hist,
-                                                               const uint32_t* totals, int nb_max, uint32_t* ranges,
-                                                               int tshift) {
+__device__ __forceinline__ void scatter_block(const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+                                              K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+                                              const uint32_t* n_dev, long long cap, int shift, int bits,
+                                              const uint32_t* hist, const uint32_t* totals, int nb_max,
+                                              uint32_t* ranges, int tshift, uint32_t bid) {
   static_assert(R == kSortThreads, "one digit per thread");
   constexpr int TILE = kSortThreads * ITEMS;
   __shared__ uint32_t s_cnt[4][R];
@@ -229,8 +239,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   __shared__ uint32_t s_val[V ? TILE : 1];  // keys-only sorts carry no values
   const uint32_t n = n_eff(n_dev, cap);
   const uint32_t nblk = (n + (uint32_t)TILE - 1) / (uint32_t)TILE;
-  if (blockIdx.x >= nblk) return;
-  const uint32_t blk = xcd_remap(blockIdx.x, nblk);  // the upsweep's mapping
+  if (bid >= nblk) return;
+  const uint32_t blk = xcd_remap(bid, nblk);  // the upsweep's mapping
   const uint32_t start = blk * (uint32_t)TILE;
   const int t = threadIdx.x;
   // global base of each digit for this block = exclusive scan of totals + row offset
@@ -303,6 +313,19 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
   }
 }
 
+template <typename K, int ITEMS, int R, bool V>
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restrict__ keys_in,
+                                                               const uint32_t* __restrict__ vals_in,
+                                                               K* __restrict__ keys_out,
+                                                               uint32_t* __restrict__ vals_out,
+                                                               const uint32_t* n_dev, long long cap, int shift,
+                                                               int bits, const uint32_t* hist,
+                                                               const uint32_t* totals, int nb_max, uint32_t* ranges,
+                                                               int tshift) {
+  scatter_block<K, ITEMS, R, V>(keys_in, vals_in, keys_out, vals_out, n_dev, cap, shift, bits, hist, totals, nb_max,
+                                ranges, tshift, blockIdx.x);
+}
+
 // Keys per workgroup: short sorts use 1024 (4 per thread) so that they still
 // fill the 256 CUs (at 500k keys, 2048 per workgroup sorts slower); long
 // ones 2048 (8 per thread).  Measured at 5.7M keys, 8 per thread sorts faster
@@ -361,6 +384,123 @@ hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
     uint32_t* tv = vin;
     vin = vout;
     vout = tv;
+  }
+  return hipGetLastError();
+}
+
+// ---- segmented sort ---------------------------------------------------------
+// Descriptor: uint32 prefix[n + 1] (each segment's upsweep / scatter
+// workgroups, rounded up to a multiple of 8 so that a segment's first block
+// starts a round of the XCD round-robin, as a launch of its own does), then,
+// pass by pass, one SegPass per segment.  Every segment sorts with 1024-key
+// workgroups (sort_tile of a capacity up to kSegMaxKeys).
+struct SegPass {
+  const uint32_t* kin;
+  const uint32_t* vin;
+  uint32_t* kout;
+  uint32_t* vout;
+  const uint32_t* n_dev;
+  uint32_t* hist;
+  uint32_t* totals;
+  long long cap;
+  int nb_max;
+  int shift;
+  int bits;
+  int pad;
+};
+constexpr int kSegItems = 4;
+
+static size_t seg_pass_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
+
+__device__ __forceinline__ int seg_of(const uint32_t* prefix, int n, uint32_t b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_seg_upsweep(const uint32_t* prefix, const SegPass* ps, int n) {
+  const uint32_t b = blockIdx.x;
+  const int f = seg_of(prefix, n, b);
+  const SegPass p = ps[f];
+  upsweep_block<uint32_t, kSegItems, kRadix>(p.kin, p.n_dev, p.cap, p.shift, p.bits, p.hist, p.nb_max,
+                                             b - prefix[f]);
+}
+
+// 256 workgroups per segment, one per digit.
+__global__ __launch_bounds__(256) void k_seg_scan(const SegPass* ps) {
+  const SegPass p = ps[blockIdx.x >> kRadixBits];
+  scan_row(p.hist, p.totals, p.n_dev, p.cap, p.nb_max, kSortThreads * kSegItems, (int)(blockIdx.x & (kRadix - 1)));
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const uint32_t* prefix, const SegPass* ps, int n) {
+  const uint32_t b = blockIdx.x;
+  const int f = seg_of(prefix, n, b);
+  const SegPass p = ps[f];
+  scatter_block<uint32_t, kSegItems, kRadix, true>(p.kin, p.vin, p.kout, p.vout, p.n_dev, p.cap, p.shift, p.bits,
+                                                   p.hist, p.totals, p.nb_max, nullptr, 0, b - prefix[f]);
+}
+
+size_t seg_sort_desc_bytes(int n, int end_bit) {
+  if (n <= 0 || end_bit < 1 || end_bit > 32) return 0;
+  return seg_pass_offset(n) + sizeof(SegPass) * (size_t)n * (size_t)sort_passes(end_bit);
+}
+
+bool seg_sort_build(int n, const SortSeg* segs, int end_bit, void* host, size_t bytes, unsigned* blocks) {
+  if (n <= 0 || !segs || !host || !blocks || end_bit < 1 || end_bit > 32) return false;
+  if (bytes < seg_sort_desc_bytes(n, end_bit)) return false;
+  const int passes = sort_passes(end_bit);
+  uint32_t* prefix = (uint32_t*)host;
+  SegPass* ps = (SegPass*)((char*)host + seg_pass_offset(n));
+  unsigned long long total = 0;
+  for (int f = 0; f < n; ++f) {
+    const SortSeg& g = segs[f];
+    if (g.capacity <= 0 || g.capacity > kSegMaxKeys || !g.keys_a || !g.vals_a || !g.keys_b || !g.vals_b ||
+        !g.n_dev || !g.tmp)
+      return false;
+    if (((uintptr_t)g.keys_a | (uintptr_t)g.keys_b) & 15) return false;  // 16-byte key loads
+    const int tile = sort_tile(g.capacity);
+    if (tile != kSortThreads * kSegItems) return false;
+    const int nb_max = (int)((g.capacity + tile - 1) / tile);
+    uint32_t* hist = (uint32_t*)g.tmp;
+    uint32_t* totals = hist + align_up((size_t)kRadix * (size_t)nb_max, 64);  // sort_pairs' layout of tmp
+    prefix[f] = (uint32_t)total;
+    total += align_up((size_t)nb_max, 8);
+    if (total > 0x7fffffffULL) return false;
+    for (int p = 0; p < passes; ++p) {
+      SegPass& q = ps[(size_t)p * n + f];
+      const bool odd = p & 1;
+      q.kin = odd ? g.keys_b : g.keys_a;
+      q.vin = odd ? g.vals_b : g.vals_a;
+      q.kout = odd ? g.keys_a : g.keys_b;
+      q.vout = odd ? g.vals_a : g.vals_b;
+      q.n_dev = g.n_dev;
+      q.hist = hist;
+      q.totals = totals;
+      q.cap = g.capacity;
+      q.nb_max = nb_max;
+      q.shift = kRadixBits * p;
+      q.bits = end_bit - q.shift < kRadixBits ? end_bit - q.shift : kRadixBits;
+      q.pad = 0;
+    }
+  }
+  prefix[n] = (uint32_t)total;
+  *blocks = (unsigned)total;
+  return true;
+}
+
+hipError_t seg_sort_launch(int n, const void* dev_desc, unsigned blocks, int end_bit, hipStream_t s) {
+  const uint32_t* prefix = (const uint32_t*)dev_desc;
+  const SegPass* ps = (const SegPass*)((const char*)dev_desc + seg_pass_offset(n));
+  const int passes = sort_passes(end_bit);
+  for (int p = 0; p < passes; ++p) {
+    const SegPass* q = ps + (size_t)p * n;
+    hipLaunchKernelGGL(k_seg_upsweep, dim3(blocks), dim3(kSortThreads), 0, s, prefix, q, n);
+    hipLaunchKernelGGL(k_seg_scan, dim3((unsigned)n * kRadix), dim3(256), 0, s, q);
+    hipLaunchKernelGGL(k_seg_scatter, dim3(blocks), dim3(kSortThreads), 0, s, prefix, q, n);
   }
   return hipGetLastError();
 }

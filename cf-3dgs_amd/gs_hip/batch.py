@@ -20,10 +20,11 @@ solves (one per frame instance) runs in lock-step:
     then draws nothing for that iteration), read once after the solve, and
     the affected instances are re-solved with a larger capacity.
 
-``autotune`` picks the stream count, the binning schedule and the blend /
+``autotune`` picks the stream count, the binning schedule, the blend /
 render-backward schedule (per frame, or one launch over all instances, or
-over each of 2, 4 or 8 instance groups, one stream per group) per size
-bucket by timing graph replays.
+over each of 2, 4 or 8 instance groups, one stream per group) and whether the
+depth sorts of a stream's or a group's instances run as one segmented sort,
+per size bucket, by timing graph replays.
 """
 from __future__ import annotations
 
@@ -167,6 +168,61 @@ def launch_slot_binning(s: _Slot, stream: ctypes.c_void_p) -> None:
                "gs_render_binning")
 
 
+def launch_slot_keys(s: _Slot, stream: ctypes.c_void_p) -> None:
+    """Preprocess (pose fused) and the depth keys of one solve: the binning up
+    to its depth sort, which a DepthSortSet then runs for many solves."""
+    lib = _lib.load()
+    c = ctypes.c_void_p
+    _lib.check(lib.gs_preprocess(ctypes.byref(s.frame), ctypes.byref(s.gauss), c(s.radii.data_ptr()),
+                                 c(s.geom.data_ptr()), s.geom.numel(), c(s.counters.data_ptr()), stream),
+               "gs_preprocess")
+    _lib.check(lib.gs_render_depth_keys(ctypes.byref(s.frame), s.inst.P, c(s.geom.data_ptr()),
+                                        c(s.counters.data_ptr()), s.capacity, c(s.binning.data_ptr()),
+                                        s.binning.numel(), c(s.image.data_ptr()), s.image.numel(), stream),
+               "gs_render_depth_keys")
+
+
+def launch_slot_after_sort(s: _Slot, stream: ctypes.c_void_p, blend: bool) -> None:
+    """The binning after the depth sort (and the blend when ``blend``)."""
+    lib = _lib.load()
+    c = ctypes.c_void_p
+    args = (ctypes.byref(s.frame), s.inst.P, c(s.geom.data_ptr()), c(s.counters.data_ptr()), s.capacity,
+            c(s.binning.data_ptr()), s.binning.numel(), c(s.image.data_ptr()), s.image.numel())
+    if blend:
+        _lib.check(lib.gs_render_after_depth_sort(*args, c(s.color.data_ptr()), c(s.depth.data_ptr()), None, stream),
+                   "gs_render_after_depth_sort")
+    else:
+        _lib.check(lib.gs_render_binning_after_depth_sort(*args, stream), "gs_render_binning_after_depth_sort")
+
+
+class DepthSortSet:
+    """The depth sorts of many solves as one segmented sort
+    (gs_multi_depth_sort): 12 launches for the set instead of 12 per solve."""
+
+    def __init__(self, slots: Sequence[_Slot], device):
+        lib = _lib.load()
+        n = len(slots)
+        self.n = n
+        vp = ctypes.c_void_p
+        frames = (_lib.GsFrame * n)(*[s.frame for s in slots])
+        P = (ctypes.c_int * n)(*[s.inst.P for s in slots])
+        cap = (ctypes.c_longlong * n)(*[s.capacity for s in slots])
+        bins = (vp * n)(*[s.binning.data_ptr() for s in slots])
+        nbytes = (ctypes.c_size_t * n)(*[s.binning.numel() for s in slots])
+        counters = (vp * n)(*[s.counters.data_ptr() for s in slots])
+        nb = int(lib.gs_multi_depth_sort_bytes(n))
+        host = (ctypes.c_uint8 * nb)()
+        blocks = ctypes.c_uint(0)
+        _lib.check(lib.gs_multi_depth_sort_build(n, frames, P, cap, bins, nbytes, counters, host, nb,
+                                                 ctypes.byref(blocks)), "gs_multi_depth_sort_build")
+        self.dev = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+        self.blocks = blocks.value
+
+    def launch(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_depth_sort(self.n, ctypes.c_void_p(self.dev.data_ptr()), self.blocks, stream),
+                   "gs_multi_depth_sort")
+
+
 def launch_slot_loss(s: _Slot, out3_ptr: int, one_ptr: int, lam: float, stream: ctypes.c_void_p) -> None:
     """Fused loss forward (stats to ``out3_ptr``) and backward (d loss scaled
     by the float at ``one_ptr``) of one solve's rendered image."""
@@ -234,11 +290,14 @@ def group_layout(cost: Sequence[float], nstreams: int, groups: int):
     return G, members, gstreams, stream_of
 
 
-def schedule_candidates(binnings: Sequence[int], streams: Sequence[int], groups: Sequence[int]):
-    """The autotuner's (binning, streams, groups) candidates: per-frame
-    (groups 0) and one group at every stream count, several groups only with
-    as many streams."""
-    return [(b, s, g) for b in binnings for s in streams for g in groups if g <= 1 or g == s]
+def schedule_candidates(binnings: Sequence[int], streams: Sequence[int], groups: Sequence[int],
+                        shared: Sequence[bool] = (False, True)):
+    """The autotuner's (binning, streams, groups, shared depth sort)
+    candidates: per-frame (groups 0) and one group at every stream count,
+    several groups only with as many streams; the shared depth sort only with
+    the depth-first binning."""
+    return [(b, s, g, ss) for b in binnings for s in streams for g in groups for ss in shared
+            if (g <= 1 or g == s) and (not ss or b == _lib.BINNING_DEPTH_FIRST)]
 
 
 class FrameSets:
@@ -300,7 +359,8 @@ class BatchedPoseSolver:
     def __init__(self, instances: Sequence[PoseInstance], iters: int = ITERS, lr: float = LR,
                  streams: int = 4, binning: int = _lib.BINNING_DEPTH_FIRST, iters_per_graph: int = 10,
                  headroom: float = 1.3, use_graph: bool = True, lambda_dssim: float = 0.2,
-                 X0: Optional[torch.Tensor] = None, multi: bool = True, groups: int = 1):
+                 X0: Optional[torch.Tensor] = None, multi: bool = True, groups: int = 1,
+                 shared_sort: bool = False):
         if not instances:
             raise ValueError("no instances")
         self.instances = list(instances)
@@ -333,9 +393,22 @@ class BatchedPoseSolver:
         self.ngroups, self.members, self.group_streams, self.stream_of = group_layout(
             cost, self.nstreams, int(groups) if self.multi else 1)
         self.sets = self._frame_sets() if self.multi else None
+        # shared depth sort: one segmented sort per group (one-launch blend
+        # schedules) or per stream (per-frame schedules)
+        self.shared_sort = bool(shared_sort)
+        if self.shared_sort and self.binning != _lib.BINNING_DEPTH_FIRST:
+            raise ValueError("the shared depth sort needs the depth-first binning")
+        self.stream_members = {q: [i for i in range(self.n) if self.stream_of[i] == q]
+                               for q in sorted(set(self.stream_of))}
+        self.dsets = self._depth_sort_sets() if self.shared_sort else None
 
     def _frame_sets(self) -> List[FrameSets]:
         return [FrameSets([self.slots[i] for i in members], self.device) for members in self.members]
+
+    def _depth_sort_sets(self):
+        if self.multi:
+            return [DepthSortSet([self.slots[i] for i in members], self.device) for members in self.members]
+        return {q: DepthSortSet([self.slots[i] for i in mem], self.device) for q, mem in self.stream_members.items()}
 
     def _make_slot(self, inst: PoseInstance, capacity: int, i: int) -> _Slot:
         return make_slot(inst, capacity, self.binning, _vp(self.adam.pose, 28 * i), _vp(self.adam.grad, 28 * i))
@@ -367,6 +440,9 @@ class BatchedPoseSolver:
             self._launch_iteration()
 
     def _launch_iteration(self):
+        if not self.multi and self.shared_sort:
+            self._launch_iteration_shared_sort()
+            return
         if not self.multi:
             self._per_slot(lambda i, s, st: self._launch_slot(s, i, st))
             self.adam.step()
@@ -395,10 +471,15 @@ class BatchedPoseSolver:
         if self.ngroups > 1:
             for q in used:
                 self.streams[q].wait_stream(cur)
-        for members, sidx, sets in zip(self.members, self.group_streams, self.sets):
+        for g, (members, sidx, sets) in enumerate(zip(self.members, self.group_streams, self.sets)):
             lead = cur if self.ngroups == 1 else self.streams[sidx[0]]
             lp = ctypes.c_void_p(lead.cuda_stream)
-            self._per_slot(lambda i, s, st: launch_slot_binning(s, st), members, lead)
+            if self.shared_sort:
+                self._per_slot(lambda i, s, st: launch_slot_keys(s, st), members, lead)
+                self.dsets[g].launch(lp)
+                self._per_slot(lambda i, s, st: launch_slot_after_sort(s, st, blend=False), members, lead)
+            else:
+                self._per_slot(lambda i, s, st: launch_slot_binning(s, st), members, lead)
             sets.blend_launch(lp)
             self._per_slot(loss_and_prepare, members, lead)
             sets.render_bwd_launch(lp)
@@ -406,6 +487,30 @@ class BatchedPoseSolver:
         if self.ngroups > 1:
             for q in used:
                 cur.wait_stream(self.streams[q])
+        self.adam.step()
+
+    def _launch_iteration_shared_sort(self):
+        """Per-frame launches with each stream's depth sorts as one segmented
+        sort: a stream runs its solves' preprocess and depth keys, their
+        shared depth sort, then each solve's rest of the iteration."""
+        lib = _lib.load()
+        cur = torch.cuda.current_stream(self.device)
+        for q in self.stream_members:
+            self.streams[q].wait_stream(cur)
+        for q, mem in self.stream_members.items():
+            st = self.streams[q]
+            sp = ctypes.c_void_p(st.cuda_stream)
+            with torch.cuda.stream(st):
+                for i in mem:
+                    launch_slot_keys(self.slots[i], sp)
+                self.dsets[q].launch(sp)
+                for i in mem:
+                    s = self.slots[i]
+                    launch_slot_after_sort(s, sp, blend=True)
+                    launch_slot_loss(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, sp)
+                    _lib.check(lib.gs_backward(*_bwd_args(s), sp), "gs_backward")
+        for q in self.stream_members:
+            cur.wait_stream(self.streams[q])
         self.adam.step()
 
     def _reset(self):
@@ -462,6 +567,8 @@ class BatchedPoseSolver:
                 self.slots[i] = self._make_slot(s.inst, _round_capacity(int(s.capacity * 1.5), 1.0), i)
             if self.multi:
                 self.sets = self._frame_sets()
+            if self.shared_sort:
+                self.dsets = self._depth_sort_sets()
         raise RuntimeError(f"binning capacity still exceeded after {max_regrow} regrowths")
 
     def timed_run(self) -> Dict[str, float]:
@@ -492,23 +599,25 @@ def size_bucket(instances: Sequence[PoseInstance]) -> tuple:
 
 def autotune(instances: Sequence[PoseInstance], streams=(2, 4, 8),
              binnings=(_lib.BINNING_DEPTH_FIRST, _lib.BINNING_SORT64), groups=(0, 1, 2, 4, 8),
-             iters_per_graph: int = 5, replays: int = 2) -> dict:
+             shared=(False, True), iters_per_graph: int = 5, replays: int = 2) -> dict:
     """Times ``replays`` graph replays of every (streams, binning schedule,
-    blend / render-backward schedule) candidate and returns (and caches per
-    size bucket) the fastest configuration.  The blend schedule is ``groups``:
-    0 launches the blend and the render backward per frame; g > 0 splits the
-    instances into g groups with one launch of each over a group (one group
-    spreads its per-frame work over all streams; several groups take one
-    stream each, so g > 1 is tried with g streams)."""
+    blend / render-backward schedule, shared depth sort) candidate and
+    returns (and caches per size bucket) the fastest configuration.  The
+    blend schedule is ``groups``: 0 launches the blend and the render
+    backward per frame; g > 0 splits the instances into g groups with one
+    launch of each over a group (one group spreads its per-frame work over
+    all streams; several groups take one stream each, so g > 1 is tried with
+    g streams).  The shared depth sort runs the depth sorts of a stream's
+    (per frame) or a group's instances as one segmented sort."""
     key = size_bucket(instances)
     if key in _TUNED:
         return _TUNED[key]
     best = None
     results = []
-    cands = schedule_candidates(binnings, streams, groups)
-    for b, s, g in cands:
+    cands = schedule_candidates(binnings, streams, groups, shared)
+    for b, s, g, ss in cands:
         solver = BatchedPoseSolver(instances, iters=iters_per_graph * replays, streams=s, binning=b,
-                                   iters_per_graph=iters_per_graph, multi=g > 0, groups=max(g, 1))
+                                   iters_per_graph=iters_per_graph, multi=g > 0, groups=max(g, 1), shared_sort=ss)
         solver._reset()
         solver._capture()
         solver._reset()
@@ -519,11 +628,11 @@ def autotune(instances: Sequence[PoseInstance], streams=(2, 4, 8),
             solver.graph.replay()
         torch.cuda.synchronize(solver.device)
         dt = (time.perf_counter() - t0) / (replays * iters_per_graph)
-        results.append(dict(streams=s, binning=b, multi=g > 0, groups=g, ms_per_iter=dt * 1e3))
+        results.append(dict(streams=s, binning=b, multi=g > 0, groups=g, shared_sort=ss, ms_per_iter=dt * 1e3))
         if best is None or dt < best[0]:
-            best = (dt, s, b, g)
+            best = (dt, s, b, g, ss)
         del solver
-    cfg = dict(streams=best[1], binning=best[2], multi=best[3] > 0, groups=max(best[3], 1),
+    cfg = dict(streams=best[1], binning=best[2], multi=best[3] > 0, groups=max(best[3], 1), shared_sort=best[4],
                ms_per_iter=best[0] * 1e3, candidates=results)
     _TUNED[key] = cfg
     return cfg

@@ -270,6 +270,33 @@ int gs_multi_blend_build(int n, const gs_frame_t* frames, const int* P, const vo
                          float* const* out_color, float* const* out_depth, float* const* out_alpha,
                          void* host_set, size_t set_bytes, unsigned* total_tiles);
 int gs_multi_blend(int n, const void* dev_set, unsigned total_tiles, void* stream);
+/* The depth-first binning of many frames around ONE depth sort: per frame
+ * gs_render_depth_keys (the depth keys), then gs_multi_depth_sort over the
+ * set (each frame's 32-bit depth sort on its own buffers, with the same
+ * passes and result as gs_render_binning's, but each of the 4 x 3 pass
+ * launches covers every frame), then per frame
+ * gs_render_binning_after_depth_sort (tile-count prefix, duplicate, list
+ * sort, checkpoint scan) or gs_render_after_depth_sort (the same and the
+ * blend).  Together they leave exactly what gs_render_binning / gs_render
+ * leave.  Frames must use GS_BINNING_DEPTH_FIRST with P in (0, 2^21] and a
+ * positive capacity; binning_bytes are the frames' binning workspace sizes.
+ * Replaces, for many frames at once, the key sort of forward phase 2 above
+ * (the upstream rasteriser's SortPairs of the point list, reached from
+ * scene/gaussian_model_cf.py:873-882 once per frame). */
+int gs_render_depth_keys(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters, long long capacity,
+                         void* binning, size_t binning_bytes, void* image, size_t image_bytes, void* stream);
+size_t gs_multi_depth_sort_bytes(int n);
+int gs_multi_depth_sort_build(int n, const gs_frame_t* frames, const int* P, const long long* capacity,
+                              void* const* binning, const size_t* binning_bytes, uint32_t* const* counters,
+                              void* host_set, size_t set_bytes, unsigned* blocks);
+int gs_multi_depth_sort(int n, const void* dev_set, unsigned blocks, void* stream);
+int gs_render_binning_after_depth_sort(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters,
+                                       long long capacity, void* binning, size_t binning_bytes, void* image,
+                                       size_t image_bytes, void* stream);
+int gs_render_after_depth_sort(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters,
+                               long long capacity, void* binning, size_t binning_bytes, void* image,
+                               size_t image_bytes, float* out_color, float* out_depth, float* out_alpha,
+                               void* stream);
 int gs_backward_prepare(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
                         void* binning, long long capacity, const void* image, const uint32_t* counters,
                         const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
@@ -522,7 +549,9 @@ int gs_timing_read(const char* stage, double* total_ms, int* launches);
 const char* gs_error_string(int code);
 /* 2: binning layout with the backward work items, one-word list entries and
  * packed tile rects; split forward (gs_render_prepare / gs_render_resume).
- * 3: image layout with the blend's tile order (tile_order). */
+ * 3: image layout with the blend's tile order (tile_order).
+ * 4: the depth-first binning split around a depth sort shared by many frames
+ * (gs_render_depth_keys, gs_multi_depth_sort*, gs_render_*after_depth_sort). */
 int gs_abi_version(void);
 
 #ifdef __cplusplus

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 15
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.gs_abi_version() == 3
+    assert lib.gs_abi_version() == 4
 
 
 def test_layouts():
@@ -238,6 +238,61 @@ def test_batched_solver_group_layout():
     assert batch.group_layout(cost[:2], 2, 4)[0] == 2  # no more groups than instances
     with pytest.raises(ValueError):
         batch.group_layout(cost, 3, 2)
-    cands = batch.schedule_candidates((0,), (2, 4, 8), (0, 1, 2, 4, 8))
-    assert cands == [(0, 2, 0), (0, 2, 1), (0, 2, 2), (0, 4, 0), (0, 4, 1), (0, 4, 4), (0, 8, 0), (0, 8, 1),
-                     (0, 8, 8)]
+    cands = batch.schedule_candidates((0,), (2, 4, 8), (0, 1, 2, 4, 8), (False,))
+    assert cands == [(0, 2, 0, False), (0, 2, 1, False), (0, 2, 2, False), (0, 4, 0, False), (0, 4, 1, False),
+                     (0, 4, 4, False), (0, 8, 0, False), (0, 8, 1, False), (0, 8, 8, False)]
+    # the shared depth sort is a candidate with the depth-first binning only
+    df, s64 = batch._lib.BINNING_DEPTH_FIRST, batch._lib.BINNING_SORT64
+    cands = batch.schedule_candidates((df, s64), (4,), (0, 4))
+    assert cands == [(df, 4, 0, False), (df, 4, 0, True), (df, 4, 4, False), (df, 4, 4, True),
+                     (s64, 4, 0, False), (s64, 4, 4, False)]
+
+
+def test_multi_depth_sort_set_on_the_host():
+    """gs_multi_depth_sort_build only lays out a set (the device addresses it
+    is given are never read): one range of upsweep / scatter workgroups per
+    frame, ceil(P / 1024) rounded up to a multiple of 8, and refusals of what
+    the segmented sort cannot take."""
+    import ctypes
+    from gs_hip import _lib
+    lib = _lib.load()
+    sizes = [(320, 180, 20000), (960, 540, 200000), (480, 270, 1000)]
+    n = len(sizes)
+    frames = (_lib.GsFrame * n)()
+    for k, (W, H, P) in enumerate(sizes):
+        f = frames[k]
+        f.width, f.height, f.sh_degree = W, H, 0
+        f.viewmatrix, f.projmatrix, f.bg = 0x1000, 0x2000, 0x3000
+        f.binning = _lib.BINNING_DEPTH_FIRST
+    P = (ctypes.c_int * n)(*[p for _, _, p in sizes])
+    cap = (ctypes.c_longlong * n)(*[3 * p for _, _, p in sizes])
+    bytes_ = [_lib.binning_layout(3 * p, p, W, H, _lib.BINNING_DEPTH_FIRST).total for W, H, p in sizes]
+    bins = (ctypes.c_void_p * n)(*[0x10000000 * (k + 1) for k in range(n)])
+    nbytes = (ctypes.c_size_t * n)(*bytes_)
+    counters = (ctypes.c_void_p * n)(*[0x7000000 + 256 * k for k in range(n)])
+    nb = int(lib.gs_multi_depth_sort_bytes(n))
+    assert nb > 0 and lib.gs_multi_depth_sort_bytes(0) == 0
+    host = (ctypes.c_uint8 * nb)()
+    blocks = ctypes.c_uint(0)
+    assert lib.gs_multi_depth_sort_build(n, frames, P, cap, bins, nbytes, counters, host, nb,
+                                         ctypes.byref(blocks)) == 0
+    prefix = np.frombuffer(bytes(host), dtype=np.uint32, count=n + 1)
+    per = [-(-(-(-p // 1024)) // 8) * 8 for _, _, p in sizes]
+    assert prefix.tolist() == [0] + np.cumsum(per).tolist() and blocks.value == sum(per)
+    # refusals: a set buffer too small, a binning workspace too small, the
+    # 64-bit key binning (no depth sort), more keys than a segment takes
+    assert lib.gs_multi_depth_sort_build(n, frames, P, cap, bins, nbytes, counters, host, nb - 1,
+                                         ctypes.byref(blocks)) == 2
+    small = (ctypes.c_size_t * n)(*[b - 1 for b in bytes_])
+    assert lib.gs_multi_depth_sort_build(n, frames, P, cap, bins, small, counters, host, nb,
+                                         ctypes.byref(blocks)) == 2
+    frames[1].binning = _lib.BINNING_SORT64
+    assert lib.gs_multi_depth_sort_build(n, frames, P, cap, bins, nbytes, counters, host, nb,
+                                         ctypes.byref(blocks)) == 1
+    frames[1].binning = _lib.BINNING_DEPTH_FIRST
+    big = (ctypes.c_int * n)(20000, (1 << 21) + 1, 1000)
+    bigcap = (ctypes.c_longlong * n)(60000, 3 << 21, 3000)
+    bigbytes = (ctypes.c_size_t * n)(*[_lib.binning_layout(int(c), int(p), W, H, 0).total
+                                       for (W, H, _), p, c in zip(sizes, big, bigcap)])
+    assert lib.gs_multi_depth_sort_build(n, frames, big, bigcap, bins, bigbytes, counters, host, nb,
+                                         ctypes.byref(blocks)) == 1

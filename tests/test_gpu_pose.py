@@ -168,20 +168,24 @@ def test_plain_solve_converges():
     assert rot < 0.1 * rot0 and trans < 0.1 * trans0
 
 
-@pytest.mark.parametrize("binning,groups", [(_lib.BINNING_DEPTH_FIRST, 1), (_lib.BINNING_DEPTH_FIRST, 2),
-                                            (_lib.BINNING_SORT64, 0)])
-def test_batched_equals_plain_native(binning, groups):
+@pytest.mark.parametrize("binning,groups,shared", [(_lib.BINNING_DEPTH_FIRST, 1, False),
+                                                   (_lib.BINNING_DEPTH_FIRST, 2, False),
+                                                   (_lib.BINNING_SORT64, 0, False),
+                                                   (_lib.BINNING_DEPTH_FIRST, 0, True),
+                                                   (_lib.BINNING_DEPTH_FIRST, 2, True)])
+def test_batched_equals_plain_native(binning, groups, shared):
     """Batched (graph, 2 streams; blend and render backward in one launch
-    over all frames or over each of two instance groups, or per frame) vs the
-    plain one-solve-per-launch path on the same kernels: every reduction is in
-    a fixed order, so the final poses agree bit for bit (well inside 1e-4 rad
-    / 1e-4)."""
-    print("batched vs plain native, binning", binning, "groups", groups, flush=True)
+    over all frames or over each of two instance groups, or per frame; the
+    depth sorts per frame or as one segmented sort per stream or group) vs
+    the plain one-solve-per-launch path on the same kernels: every reduction
+    is in a fixed order, so the final poses agree bit for bit (well inside
+    1e-4 rad / 1e-4)."""
+    print("batched vs plain native, binning", binning, "groups", groups, "shared sort", shared, flush=True)
     insts = [build_instance(31 + k, W=W, H=H, P=P) for k, (W, H, P) in
              enumerate([(160, 90, 12000), (192, 108, 30000), (128, 72, 9216)])]
     plain = [solve_plain_native(i, iters=300)["pose"] for i in insts]
     solver = BatchedPoseSolver(insts, iters=300, streams=2, binning=binning, iters_per_graph=10, multi=groups > 0,
-                               groups=max(groups, 1))
+                               groups=max(groups, 1), shared_sort=shared)
     poses, losses = solver.run()
     assert torch.isfinite(losses).all()
     for k, inst in enumerate(insts):
@@ -211,16 +215,19 @@ _SIDE_TO_SIDE = pytest.mark.skip(
            "topology (several groups take one stream each)")
 
 
-@pytest.mark.parametrize("groups,streams", [(0, 3), (1, 3), (2, 2), pytest.param(2, 3, marks=_SIDE_TO_SIDE)])
-def test_graph_replays_match_eager(groups, streams):
+@pytest.mark.parametrize("groups,streams,shared", [(0, 3, False), (1, 3, False), (2, 2, False), (0, 3, True),
+                                                   (1, 3, True),
+                                                   pytest.param(2, 3, False, marks=_SIDE_TO_SIDE)])
+def test_graph_replays_match_eager(groups, streams, shared):
     """Every graph replay agrees bit for bit with eager launches of the same
     iterations: per-frame launches on streams (the instances dealt to the
     streams by cost), or one-launch blend and backward over all instances or
-    over each of two instance groups, each group on a stream of its own."""
-    print("graph replays vs eager, groups", groups, "streams", streams, flush=True)
+    over each of two instance groups, each group on a stream of its own; the
+    depth sorts per frame or as one segmented sort per stream or group."""
+    print("graph replays vs eager, groups", groups, "streams", streams, "shared sort", shared, flush=True)
     insts = [build_instance(41 + k, W=128, H=72) for k in range(3)] + [build_instance(44, W=160, H=90)]
     ipg = 4
-    kw = dict(multi=groups > 0, groups=max(groups, 1))
+    kw = dict(multi=groups > 0, groups=max(groups, 1), shared_sort=shared)
     eager = BatchedPoseSolver(insts, iters=ipg, streams=streams, use_graph=False, **kw)
     graph = BatchedPoseSolver(insts, iters=ipg, streams=streams, iters_per_graph=ipg, **kw)
     graph._reset()
