@@ -909,17 +909,19 @@ extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* 
 // num_rendered (clearing the tile ranges; in the depth-first schedule also
 // the depth keys, the 32-bit depth sort and the prefix of the tile counts in
 // depth order), so a caller can queue it before it waits for the readback;
-// kResume is the rest, after kPrepare with the same arguments.  kKeys and
-// kAfterSort split the depth-first schedule around its depth sort instead, so
+// kResume is the rest, after kPrepare with the same arguments.  The other
+// phases split the depth-first schedule around its two sorts instead, so
 // that one segmented sort can serve many frames: kKeys is the depth keys
-// alone, kAfterSort everything after the depth sort.
-enum BinPhase { kBinAll = 0, kPrepare = 1, kResume = 2, kKeys = 3, kAfterSort = 4 };
+// alone; kAfterSort everything after the depth sort; kDup the tile-count
+// prefix and the duplicate (after the depth sort, before the list sort);
+// kTail what follows the list sort (the checkpoint scan).
+enum BinPhase { kBinAll = 0, kPrepare = 1, kResume = 2, kKeys = 3, kAfterSort = 4, kDup = 5, kTail = 6 };
 
 static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t* counters, long long capacity,
                           void* binning, size_t binning_bytes, void* image, size_t image_bytes, hipStream_t s,
                           int phase = kBinAll) {
   const bool pre = phase == kBinAll || phase == kPrepare, post = phase == kBinAll || phase == kResume;
-  const bool split = phase == kKeys || phase == kAfterSort;
+  const bool split = phase == kKeys || phase == kAfterSort || phase == kDup || phase == kTail;
   if (check_frame(f) != GS_OK || P < 0 || !counters || !image) return GS_E_ARG;
   if (P > 0 && !geom) return GS_E_ARG;
   if (capacity < 0 || capacity > 0x7fffffffLL) return GS_E_ARG;
@@ -939,9 +941,11 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
   uint32_t* ranges = (uint32_t*)(ib + LI.ranges);
   const bool df = capacity > 0 && P > 0 && f->binning == GS_BINNING_DEPTH_FIRST;
   if (split && !df) return GS_E_ARG;  // only the depth-first schedule has a depth sort to split around
-  // depth-first stages: keys | depth sort | tile-count prefix | duplicate, list sort, checkpoint scan
-  const bool do_keys = pre || phase == kKeys, do_dsort = pre, do_dscan = pre || phase == kAfterSort;
-  const bool do_rest = post || phase == kAfterSort;
+  // depth-first stages: keys | depth sort | tile-count prefix | duplicate | list sort | checkpoint scan
+  const bool do_keys = pre || phase == kKeys, do_dsort = pre;
+  const bool do_dscan = pre || phase == kAfterSort || phase == kDup;
+  const bool do_dup = post || phase == kAfterSort || phase == kDup, do_lsort = post || phase == kAfterSort;
+  const bool do_ckpt = post || phase == kAfterSort || phase == kTail;
   if (pre && !df)  // the depth-first schedule clears them in k_df_keys
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min(nblocks256(2LL * nt), 1024)), dim3(256), 0, s, ranges, 2LL * nt,
                        0u);
@@ -973,8 +977,7 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
                          (uint32_t*)(bb + LB.dblocks));
       hipLaunchKernelGGL(k_excl_scan_1wg, dim3(1), dim3(1024), 0, s, (uint32_t*)(bb + LB.dblocks), nbp);
     }
-    if (!do_rest) return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
-    {
+    if (do_dup) {
       timing::Scope ts(timing::kDuplicate, s);
       DfDupArgs d;
       d.P = P;
@@ -992,7 +995,7 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
       d.id_bits = LB.id_bits;
       hipLaunchKernelGGL(k_df_duplicate, dim3(nbp), dim3(256), 0, s, d);
     }
-    {
+    if (do_lsort) {
       timing::Scope ts(timing::kSort, s);
       const int tb = bits_for((uint32_t)nt) > 0 ? bits_for((uint32_t)nt) : 1;
       // the last pass also records the tile ranges
@@ -1036,7 +1039,7 @@ static int render_binning(const gs_frame_t* f, int P, const void* geom, uint32_t
     hipLaunchKernelGGL(k_ranges, dim3(nblocks256(capacity)), dim3(256), 0, s,
                        (const uint64_t*)(bb + LB.keys_sorted), (const uint32_t*)counters, capacity, ranges);
   }
-  if (post || phase == kAfterSort)
+  if (post || (df && do_ckpt))
     hipLaunchKernelGGL(k_tile_ckpt_scan, dim3(1), dim3(1024), 0, s, ranges, nt, (uint32_t*)(ib + LI.ckpt_base),
                        (uint32_t*)(bb + LB.item_log), (uint32_t*)(ib + LI.tile_order));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
@@ -1147,7 +1150,7 @@ extern "C" int gs_render_after_depth_sort(const gs_frame_t* f, int P, const void
                            out_color, out_depth, out_alpha, (hipStream_t)stream);
 }
 
-extern "C" size_t gs_multi_depth_sort_bytes(int n) { return seg_sort_desc_bytes(n, 32); }
+extern "C" size_t gs_multi_depth_sort_bytes(int n) { return seg_sort_desc_bytes(n, sort_passes(32)); }
 
 extern "C" int gs_multi_depth_sort_build(int n, const gs_frame_t* frames, const int* P, const long long* capacity,
                                          void* const* binning, const size_t* binning_bytes,
@@ -1167,16 +1170,86 @@ extern "C" int gs_multi_depth_sort_build(int n, const gs_frame_t* frames, const 
     if (binning_bytes[i] < LB.total) return GS_E_WORKSPACE;
     char* bb = (char*)binning[i];
     segs[i] = SortSeg{(uint32_t*)(bb + LB.dkeys_a), (uint32_t*)(bb + LB.dvals_a), (uint32_t*)(bb + LB.dkeys_b),
-                      (uint32_t*)(bb + LB.dvals_b), counters[i] + 2, (long long)P[i], bb + LB.hist};
+                      (uint32_t*)(bb + LB.dvals_b), counters[i] + 2, (long long)P[i], bb + LB.hist, 0, 32,
+                      nullptr};
   }
-  return seg_sort_build(n, segs.data(), 32, host_set, set_bytes, blocks) ? GS_OK : GS_E_ARG;
+  unsigned per[kSegMaxPasses];
+  int passes = 0;
+  if (!seg_sort_build(n, segs.data(), host_set, set_bytes, per, &passes)) return GS_E_ARG;
+  *blocks = per[0];  // every frame runs all four passes
+  return GS_OK;
 }
 
 extern "C" int gs_multi_depth_sort(int n, const void* dev_set, unsigned blocks, void* stream) {
   if (n <= 0 || !dev_set || blocks == 0) return GS_E_ARG;
   hipStream_t s = (hipStream_t)stream;
   timing::Scope ts(timing::kSort, s);
-  return seg_sort_launch(n, dev_set, blocks, 32, s) == hipSuccess ? GS_OK : GS_E_LAUNCH;
+  const unsigned per[kSegMaxPasses] = {blocks, blocks, blocks, blocks};
+  static_assert(kSegMaxPasses == 4, "the depth sort runs four 8-bit passes");
+  return seg_sort_launch(n, dev_set, per, true, s) == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_render_binning_duplicate(const gs_frame_t* f, int P, const void* geom, uint32_t* counters,
+                                           long long capacity, void* binning, size_t binning_bytes, void* image,
+                                           size_t image_bytes, void* stream) {
+  return render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                        (hipStream_t)stream, kDup);
+}
+
+extern "C" size_t gs_multi_list_sort_bytes(int n) { return seg_sort_desc_bytes(n, kSegMaxPasses); }
+
+extern "C" int gs_multi_list_sort_build(int n, const gs_frame_t* frames, const int* P, const long long* capacity,
+                                        void* const* binning, const size_t* binning_bytes,
+                                        uint32_t* const* counters, void* const* image, void* host_set,
+                                        size_t set_bytes, unsigned* blocks) {
+  if (n <= 0 || !frames || !P || !capacity || !binning || !binning_bytes || !counters || !image || !host_set ||
+      !blocks)
+    return GS_E_ARG;
+  if (set_bytes < gs_multi_list_sort_bytes(n)) return GS_E_WORKSPACE;
+  std::vector<SortSeg> segs((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const gs_frame_t* f = &frames[i];
+    if (check_frame(f) != GS_OK || f->binning != GS_BINNING_DEPTH_FIRST || P[i] <= 0 || capacity[i] <= 0 ||
+        capacity[i] > kSegMaxKeys || !binning[i] || !counters[i] || !image[i])
+      return GS_E_ARG;
+    gs_binning_layout_t LB;
+    gs_image_layout_t LI;
+    if (gs_binning_layout(capacity[i], P[i], f->width, f->height, f->binning, &LB) != GS_OK) return GS_E_ARG;
+    if (!LB.id_bits) return GS_E_ARG;  // the shared list sort takes one-word entries only
+    if (binning_bytes[i] < LB.total) return GS_E_WORKSPACE;
+    gs_image_layout(f->width, f->height, &LI);
+    int gx, gy;
+    const int nt = grid_tiles(f->width, f->height, gx, gy);
+    const int tb = bits_for((uint32_t)nt) > 0 ? bits_for((uint32_t)nt) : 1;
+    char* bb = (char*)binning[i];
+    segs[i] = SortSeg{(uint32_t*)(bb + LB.keys_dup), nullptr, (uint32_t*)(bb + LB.keys_alt), nullptr, counters[i],
+                      capacity[i], bb + LB.hist, LB.id_bits, LB.id_bits + tb,
+                      (uint32_t*)((char*)image[i] + LI.ranges)};
+  }
+  int passes = 0;
+  return seg_sort_build(n, segs.data(), host_set, set_bytes, blocks, &passes) ? GS_OK : GS_E_ARG;
+}
+
+extern "C" int gs_multi_list_sort(int n, const void* dev_set, const unsigned* blocks, void* stream) {
+  if (n <= 0 || !dev_set || !blocks || blocks[0] == 0) return GS_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  timing::Scope ts(timing::kSort, s);
+  return seg_sort_launch(n, dev_set, blocks, false, s) == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_render_binning_after_list_sort(const gs_frame_t* f, int P, const void* geom, uint32_t* counters,
+                                                 long long capacity, void* binning, size_t binning_bytes,
+                                                 void* image, size_t image_bytes, void* stream) {
+  return render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                        (hipStream_t)stream, kTail);
+}
+
+extern "C" int gs_render_after_list_sort(const gs_frame_t* f, int P, const void* geom, uint32_t* counters,
+                                         long long capacity, void* binning, size_t binning_bytes, void* image,
+                                         size_t image_bytes, float* out_color, float* out_depth, float* out_alpha,
+                                         void* stream) {
+  return render_blend_from(kTail, f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
+                           out_color, out_depth, out_alpha, (hipStream_t)stream);
 }
 
 // ---- many frames in one blend launch --------------------------------------

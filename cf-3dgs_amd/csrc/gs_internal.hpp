@@ -58,25 +58,30 @@ hipError_t sort_pairs_u32(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b, 
                           const uint32_t* n_dev, long long capacity, int end_bit, void* tmp, hipStream_t s,
                           int* result_in_b, uint32_t* ranges = nullptr);
 
-// A segmented sort: many independent 32-bit (key, value) sorts, each on its
-// own buffers exactly as sort_pairs_u32 would run it (same passes, same
-// workgroup tiles, the same result buffer), but each kernel of a pass is one
-// launch over every segment.  Segments of at most kSegMaxKeys keys.
+// A segmented sort: many independent 32-bit sorts, each on its own buffers
+// exactly as sort_pairs_u32 / sort_keys_u32 would run it (same passes, same
+// workgroup tiles, the result in the same buffer, the same tile ranges), but
+// each kernel of a pass is one launch over every segment.  All segments carry
+// values, or none does.  Segments of at most kSegMaxKeys keys.
 struct SortSeg {
   uint32_t* keys_a;
-  uint32_t* vals_a;
+  uint32_t* vals_a;  // nullptr: keys only
   uint32_t* keys_b;
   uint32_t* vals_b;
   const uint32_t* n_dev;
   long long capacity;
   void* tmp;  // sort_tmp_bytes(capacity)
+  int start_bit, end_bit;  // key bits sorted on
+  uint32_t* ranges;        // the last pass records the tile ranges (key >> start_bit), or nullptr
 };
 constexpr long long kSegMaxKeys = 1LL << 21;
-size_t seg_sort_desc_bytes(int n, int end_bit);
+constexpr int kSegMaxPasses = 4;
+size_t seg_sort_desc_bytes(int n, int passes);
 // Host-side descriptor of the segments' passes (its pointers are the
 // segments' device buffers); the caller copies it to the device.  Returns
-// false on a segment it cannot take.  *blocks is the upsweep / scatter grid.
-bool seg_sort_build(int n, const SortSeg* segs, int end_bit, void* host, size_t bytes, unsigned* blocks);
-hipError_t seg_sort_launch(int n, const void* dev_desc, unsigned blocks, int end_bit, hipStream_t s);
+// false on a segment it cannot take.  blocks[p] is pass p's upsweep /
+// scatter grid (0 past the longest segment's passes); *passes the pass count.
+bool seg_sort_build(int n, const SortSeg* segs, void* host, size_t bytes, unsigned* blocks, int* passes);
+hipError_t seg_sort_launch(int n, const void* dev_desc, const unsigned* blocks, bool values, hipStream_t s);
 
 }  // namespace gs

@@ -31,6 +31,8 @@
 #include "gs_common.hpp"
 #include "gs_internal.hpp"
 
+#include <algorithm>
+
 namespace gs {
 
 // Lanes whose digit equals this lane's digit (only lanes with valid set).
@@ -389,11 +391,12 @@ hipError_t sort_pairs(K* keys_a, uint32_t* vals_a, K* keys_b, uint32_t* vals_b, 
 }
 
 // ---- segmented sort ---------------------------------------------------------
-// Descriptor: uint32 prefix[n + 1] (each segment's upsweep / scatter
-// workgroups, rounded up to a multiple of 8 so that a segment's first block
-// starts a round of the XCD round-robin, as a launch of its own does), then,
-// pass by pass, one SegPass per segment.  Every segment sorts with 1024-key
-// workgroups (sort_tile of a capacity up to kSegMaxKeys).
+// Descriptor, pass by pass: uint32 prefix[n + 1] (each segment's upsweep /
+// scatter workgroups in that pass, rounded up to a multiple of 8 so that a
+// segment's first block starts a round of the XCD round-robin, as a launch
+// of its own does; 0 once the segment has no pass left), then one SegPass
+// per segment.  Every segment sorts with 1024-key workgroups (sort_tile of a
+// capacity up to kSegMaxKeys).
 struct SegPass {
   const uint32_t* kin;
   const uint32_t* vin;
@@ -402,17 +405,21 @@ struct SegPass {
   const uint32_t* n_dev;
   uint32_t* hist;
   uint32_t* totals;
+  uint32_t* ranges;
   long long cap;
-  int nb_max;
+  int nb_max;  // 0: the segment has no such pass
   int shift;
   int bits;
-  int pad;
+  int tshift;
 };
 constexpr int kSegItems = 4;
 
-static size_t seg_pass_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
+static size_t seg_pass_bytes(int n) {
+  return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256) + align_up(sizeof(SegPass) * (size_t)n, 256);
+}
 
 __device__ __forceinline__ int seg_of(const uint32_t* prefix, int n, uint32_t b) {
+  // the last segment whose range starts at or before b (empty ranges are skipped)
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -433,45 +440,51 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_upsweep(const uint32_t* pr
 // 256 workgroups per segment, one per digit.
 __global__ __launch_bounds__(256) void k_seg_scan(const SegPass* ps) {
   const SegPass p = ps[blockIdx.x >> kRadixBits];
+  if (p.nb_max == 0) return;
   scan_row(p.hist, p.totals, p.n_dev, p.cap, p.nb_max, kSortThreads * kSegItems, (int)(blockIdx.x & (kRadix - 1)));
 }
 
+template <bool V>
 __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const uint32_t* prefix, const SegPass* ps, int n) {
   const uint32_t b = blockIdx.x;
   const int f = seg_of(prefix, n, b);
   const SegPass p = ps[f];
-  scatter_block<uint32_t, kSegItems, kRadix, true>(p.kin, p.vin, p.kout, p.vout, p.n_dev, p.cap, p.shift, p.bits,
-                                                   p.hist, p.totals, p.nb_max, nullptr, 0, b - prefix[f]);
+  scatter_block<uint32_t, kSegItems, kRadix, V>(p.kin, p.vin, p.kout, p.vout, p.n_dev, p.cap, p.shift, p.bits,
+                                                p.hist, p.totals, p.nb_max, p.ranges, p.tshift, b - prefix[f]);
 }
 
-size_t seg_sort_desc_bytes(int n, int end_bit) {
-  if (n <= 0 || end_bit < 1 || end_bit > 32) return 0;
-  return seg_pass_offset(n) + sizeof(SegPass) * (size_t)n * (size_t)sort_passes(end_bit);
+size_t seg_sort_desc_bytes(int n, int passes) {
+  if (n <= 0 || passes < 1 || passes > kSegMaxPasses) return 0;
+  return seg_pass_bytes(n) * (size_t)passes;
 }
 
-bool seg_sort_build(int n, const SortSeg* segs, int end_bit, void* host, size_t bytes, unsigned* blocks) {
-  if (n <= 0 || !segs || !host || !blocks || end_bit < 1 || end_bit > 32) return false;
-  if (bytes < seg_sort_desc_bytes(n, end_bit)) return false;
-  const int passes = sort_passes(end_bit);
-  uint32_t* prefix = (uint32_t*)host;
-  SegPass* ps = (SegPass*)((char*)host + seg_pass_offset(n));
-  unsigned long long total = 0;
+bool seg_sort_build(int n, const SortSeg* segs, void* host, size_t bytes, unsigned* blocks, int* passes) {
+  if (n <= 0 || !segs || !host || !blocks || !passes) return false;
+  int np = 0;
+  const bool values = segs[0].vals_a != nullptr;
   for (int f = 0; f < n; ++f) {
     const SortSeg& g = segs[f];
-    if (g.capacity <= 0 || g.capacity > kSegMaxKeys || !g.keys_a || !g.vals_a || !g.keys_b || !g.vals_b ||
-        !g.n_dev || !g.tmp)
-      return false;
+    if (g.capacity <= 0 || g.capacity > kSegMaxKeys || !g.keys_a || !g.keys_b || !g.n_dev || !g.tmp) return false;
+    if ((g.vals_a != nullptr) != values || (values && !g.vals_b)) return false;
+    if (g.start_bit < 0 || g.end_bit <= g.start_bit || g.end_bit > 32) return false;
     if (((uintptr_t)g.keys_a | (uintptr_t)g.keys_b) & 15) return false;  // 16-byte key loads
-    const int tile = sort_tile(g.capacity);
-    if (tile != kSortThreads * kSegItems) return false;
-    const int nb_max = (int)((g.capacity + tile - 1) / tile);
-    uint32_t* hist = (uint32_t*)g.tmp;
-    uint32_t* totals = hist + align_up((size_t)kRadix * (size_t)nb_max, 64);  // sort_pairs' layout of tmp
-    prefix[f] = (uint32_t)total;
-    total += align_up((size_t)nb_max, 8);
-    if (total > 0x7fffffffULL) return false;
-    for (int p = 0; p < passes; ++p) {
-      SegPass& q = ps[(size_t)p * n + f];
+    if (sort_tile(g.capacity) != kSortThreads * kSegItems) return false;
+    np = std::max(np, sort_passes(g.end_bit - g.start_bit));
+  }
+  if (np > kSegMaxPasses || bytes < seg_sort_desc_bytes(n, np)) return false;
+  for (int p = 0; p < kSegMaxPasses; ++p) blocks[p] = 0;
+  for (int p = 0; p < np; ++p) {
+    char* base = (char*)host + seg_pass_bytes(n) * (size_t)p;
+    uint32_t* prefix = (uint32_t*)base;
+    SegPass* ps = (SegPass*)(base + align_up(sizeof(uint32_t) * (size_t)(n + 1), 256));
+    unsigned long long total = 0;
+    for (int f = 0; f < n; ++f) {
+      const SortSeg& g = segs[f];
+      const int tile = kSortThreads * kSegItems;
+      const int nb_max = (int)((g.capacity + tile - 1) / tile);
+      const int gp = sort_passes(g.end_bit - g.start_bit);
+      uint32_t* hist = (uint32_t*)g.tmp;
+      SegPass& q = ps[f];
       const bool odd = p & 1;
       q.kin = odd ? g.keys_b : g.keys_a;
       q.vin = odd ? g.vals_b : g.vals_a;
@@ -479,28 +492,35 @@ bool seg_sort_build(int n, const SortSeg* segs, int end_bit, void* host, size_t 
       q.vout = odd ? g.vals_a : g.vals_b;
       q.n_dev = g.n_dev;
       q.hist = hist;
-      q.totals = totals;
+      q.totals = hist + align_up((size_t)kRadix * (size_t)nb_max, 64);  // sort_pairs' layout of tmp
+      q.ranges = p + 1 == gp ? g.ranges : nullptr;
       q.cap = g.capacity;
-      q.nb_max = nb_max;
-      q.shift = kRadixBits * p;
-      q.bits = end_bit - q.shift < kRadixBits ? end_bit - q.shift : kRadixBits;
-      q.pad = 0;
+      q.nb_max = p < gp ? nb_max : 0;
+      q.shift = g.start_bit + kRadixBits * p;
+      q.bits = g.end_bit - q.shift < kRadixBits ? g.end_bit - q.shift : kRadixBits;
+      q.tshift = g.start_bit;
+      prefix[f] = (uint32_t)total;
+      total += p < gp ? align_up((size_t)nb_max, 8) : 0;
+      if (total > 0x7fffffffULL) return false;
     }
+    prefix[n] = (uint32_t)total;
+    blocks[p] = (unsigned)total;
   }
-  prefix[n] = (uint32_t)total;
-  *blocks = (unsigned)total;
+  *passes = np;
   return true;
 }
 
-hipError_t seg_sort_launch(int n, const void* dev_desc, unsigned blocks, int end_bit, hipStream_t s) {
-  const uint32_t* prefix = (const uint32_t*)dev_desc;
-  const SegPass* ps = (const SegPass*)((const char*)dev_desc + seg_pass_offset(n));
-  const int passes = sort_passes(end_bit);
-  for (int p = 0; p < passes; ++p) {
-    const SegPass* q = ps + (size_t)p * n;
-    hipLaunchKernelGGL(k_seg_upsweep, dim3(blocks), dim3(kSortThreads), 0, s, prefix, q, n);
+hipError_t seg_sort_launch(int n, const void* dev_desc, const unsigned* blocks, bool values, hipStream_t s) {
+  for (int p = 0; p < kSegMaxPasses && blocks[p] > 0; ++p) {
+    const char* base = (const char*)dev_desc + seg_pass_bytes(n) * (size_t)p;
+    const uint32_t* prefix = (const uint32_t*)base;
+    const SegPass* q = (const SegPass*)(base + align_up(sizeof(uint32_t) * (size_t)(n + 1), 256));
+    hipLaunchKernelGGL(k_seg_upsweep, dim3(blocks[p]), dim3(kSortThreads), 0, s, prefix, q, n);
     hipLaunchKernelGGL(k_seg_scan, dim3((unsigned)n * kRadix), dim3(256), 0, s, q);
-    hipLaunchKernelGGL(k_seg_scatter, dim3(blocks), dim3(kSortThreads), 0, s, prefix, q, n);
+    if (values)
+      hipLaunchKernelGGL(k_seg_scatter<true>, dim3(blocks[p]), dim3(kSortThreads), 0, s, prefix, q, n);
+    else
+      hipLaunchKernelGGL(k_seg_scatter<false>, dim3(blocks[p]), dim3(kSortThreads), 0, s, prefix, q, n);
   }
   return hipGetLastError();
 }

@@ -130,6 +130,20 @@ EXPORTS = {
     "gs_multi_depth_sort": (c_int, [c_int, c_void_p, ctypes.c_uint, c_void_p]),
     "gs_render_binning_after_depth_sort": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong,
                                                    c_void_p, c_size_t, c_void_p, c_size_t, c_void_p]),
+    "gs_render_binning_duplicate": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong,
+                                            c_void_p, c_size_t, c_void_p, c_size_t, c_void_p]),
+    "gs_multi_list_sort_bytes": (c_size_t, [c_int]),
+    "gs_multi_list_sort_build": (c_int, [c_int, ctypes.POINTER(GsFrame), ctypes.POINTER(c_int),
+                                         ctypes.POINTER(c_longlong), ctypes.POINTER(c_void_p),
+                                         ctypes.POINTER(c_size_t), ctypes.POINTER(c_void_p),
+                                         ctypes.POINTER(c_void_p), c_void_p, c_size_t,
+                                         ctypes.POINTER(ctypes.c_uint)]),
+    "gs_multi_list_sort": (c_int, [c_int, c_void_p, ctypes.POINTER(ctypes.c_uint), c_void_p]),
+    "gs_render_binning_after_list_sort": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong,
+                                                  c_void_p, c_size_t, c_void_p, c_size_t, c_void_p]),
+    "gs_render_after_list_sort": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong,
+                                          c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p,
+                                          c_void_p]),
     "gs_render_after_depth_sort": (c_int, [ctypes.POINTER(GsFrame), c_int, c_void_p, c_void_p, c_longlong,
                                            c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p,
                                            c_void_p]),

@@ -195,6 +195,57 @@ def launch_slot_after_sort(s: _Slot, stream: ctypes.c_void_p, blend: bool) -> No
         _lib.check(lib.gs_render_binning_after_depth_sort(*args, stream), "gs_render_binning_after_depth_sort")
 
 
+def launch_slot_dup(s: _Slot, stream: ctypes.c_void_p) -> None:
+    """After the depth sort: the tile-count prefix and the duplicate."""
+    c = ctypes.c_void_p
+    _lib.check(_lib.load().gs_render_binning_duplicate(ctypes.byref(s.frame), s.inst.P, c(s.geom.data_ptr()),
+                                                       c(s.counters.data_ptr()), s.capacity, c(s.binning.data_ptr()),
+                                                       s.binning.numel(), c(s.image.data_ptr()), s.image.numel(),
+                                                       stream), "gs_render_binning_duplicate")
+
+
+def launch_slot_tail(s: _Slot, stream: ctypes.c_void_p, blend: bool) -> None:
+    """After the list sort: the checkpoint scan (and the blend when ``blend``)."""
+    lib = _lib.load()
+    c = ctypes.c_void_p
+    args = (ctypes.byref(s.frame), s.inst.P, c(s.geom.data_ptr()), c(s.counters.data_ptr()), s.capacity,
+            c(s.binning.data_ptr()), s.binning.numel(), c(s.image.data_ptr()), s.image.numel())
+    if blend:
+        _lib.check(lib.gs_render_after_list_sort(*args, c(s.color.data_ptr()), c(s.depth.data_ptr()), None, stream),
+                   "gs_render_after_list_sort")
+    else:
+        _lib.check(lib.gs_render_binning_after_list_sort(*args, stream), "gs_render_binning_after_list_sort")
+
+
+class ListSortSet:
+    """The tile-list sorts of many solves as one segmented sort
+    (gs_multi_list_sort).  Raises GsError for solves it cannot take
+    (two-word list entries, a capacity above 2^21)."""
+
+    def __init__(self, slots: Sequence[_Slot], device):
+        lib = _lib.load()
+        n = len(slots)
+        self.n = n
+        vp = ctypes.c_void_p
+        frames = (_lib.GsFrame * n)(*[s.frame for s in slots])
+        P = (ctypes.c_int * n)(*[s.inst.P for s in slots])
+        cap = (ctypes.c_longlong * n)(*[s.capacity for s in slots])
+        bins = (vp * n)(*[s.binning.data_ptr() for s in slots])
+        nbytes = (ctypes.c_size_t * n)(*[s.binning.numel() for s in slots])
+        counters = (vp * n)(*[s.counters.data_ptr() for s in slots])
+        images = (vp * n)(*[s.image.data_ptr() for s in slots])
+        nb = int(lib.gs_multi_list_sort_bytes(n))
+        host = (ctypes.c_uint8 * nb)()
+        self.blocks = (ctypes.c_uint * 4)()
+        _lib.check(lib.gs_multi_list_sort_build(n, frames, P, cap, bins, nbytes, counters, images, host, nb,
+                                                self.blocks), "gs_multi_list_sort_build")
+        self.dev = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+
+    def launch(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_list_sort(self.n, ctypes.c_void_p(self.dev.data_ptr()), self.blocks, stream),
+                   "gs_multi_list_sort")
+
+
 class DepthSortSet:
     """The depth sorts of many solves as one segmented sort
     (gs_multi_depth_sort): 12 launches for the set instead of 12 per solve."""
@@ -400,15 +451,25 @@ class BatchedPoseSolver:
             raise ValueError("the shared depth sort needs the depth-first binning")
         self.stream_members = {q: [i for i in range(self.n) if self.stream_of[i] == q]
                                for q in sorted(set(self.stream_of))}
-        self.dsets = self._depth_sort_sets() if self.shared_sort else None
+        self.dsets = self.lsets = None
+        if self.shared_sort:
+            self.dsets, self.lsets = self._sort_sets()
 
     def _frame_sets(self) -> List[FrameSets]:
         return [FrameSets([self.slots[i] for i in members], self.device) for members in self.members]
 
-    def _depth_sort_sets(self):
-        if self.multi:
-            return [DepthSortSet([self.slots[i] for i in members], self.device) for members in self.members]
-        return {q: DepthSortSet([self.slots[i] for i in mem], self.device) for q, mem in self.stream_members.items()}
+    def _sort_sets(self):
+        """Shared depth and list sorts, one per group (one-launch blend
+        schedules) or per stream (per-frame schedules); the list sorts stay
+        per frame (None) when a solve's list cannot join a shared one."""
+        parts = self.members if self.multi else list(self.stream_members.values())
+        keys = range(len(parts)) if self.multi else list(self.stream_members)
+        dsets = {k: DepthSortSet([self.slots[i] for i in mem], self.device) for k, mem in zip(keys, parts)}
+        try:
+            lsets = {k: ListSortSet([self.slots[i] for i in mem], self.device) for k, mem in zip(keys, parts)}
+        except _lib.GsError:
+            lsets = None
+        return dsets, lsets
 
     def _make_slot(self, inst: PoseInstance, capacity: int, i: int) -> _Slot:
         return make_slot(inst, capacity, self.binning, _vp(self.adam.pose, 28 * i), _vp(self.adam.grad, 28 * i))
@@ -477,7 +538,12 @@ class BatchedPoseSolver:
             if self.shared_sort:
                 self._per_slot(lambda i, s, st: launch_slot_keys(s, st), members, lead)
                 self.dsets[g].launch(lp)
-                self._per_slot(lambda i, s, st: launch_slot_after_sort(s, st, blend=False), members, lead)
+                if self.lsets is not None:
+                    self._per_slot(lambda i, s, st: launch_slot_dup(s, st), members, lead)
+                    self.lsets[g].launch(lp)
+                    self._per_slot(lambda i, s, st: launch_slot_tail(s, st, blend=False), members, lead)
+                else:
+                    self._per_slot(lambda i, s, st: launch_slot_after_sort(s, st, blend=False), members, lead)
             else:
                 self._per_slot(lambda i, s, st: launch_slot_binning(s, st), members, lead)
             sets.blend_launch(lp)
@@ -490,9 +556,10 @@ class BatchedPoseSolver:
         self.adam.step()
 
     def _launch_iteration_shared_sort(self):
-        """Per-frame launches with each stream's depth sorts as one segmented
-        sort: a stream runs its solves' preprocess and depth keys, their
-        shared depth sort, then each solve's rest of the iteration."""
+        """Per-frame launches with each stream's depth sorts and list sorts as
+        segmented sorts: a stream runs its solves' preprocess and depth keys,
+        their shared depth sort, their duplicates, their shared list sort,
+        then each solve's rest of the iteration."""
         lib = _lib.load()
         cur = torch.cuda.current_stream(self.device)
         for q in self.stream_members:
@@ -504,9 +571,16 @@ class BatchedPoseSolver:
                 for i in mem:
                     launch_slot_keys(self.slots[i], sp)
                 self.dsets[q].launch(sp)
+                if self.lsets is not None:
+                    for i in mem:
+                        launch_slot_dup(self.slots[i], sp)
+                    self.lsets[q].launch(sp)
                 for i in mem:
                     s = self.slots[i]
-                    launch_slot_after_sort(s, sp, blend=True)
+                    if self.lsets is not None:
+                        launch_slot_tail(s, sp, blend=True)
+                    else:
+                        launch_slot_after_sort(s, sp, blend=True)
                     launch_slot_loss(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, sp)
                     _lib.check(lib.gs_backward(*_bwd_args(s), sp), "gs_backward")
         for q in self.stream_members:
@@ -568,7 +642,7 @@ class BatchedPoseSolver:
             if self.multi:
                 self.sets = self._frame_sets()
             if self.shared_sort:
-                self.dsets = self._depth_sort_sets()
+                self.dsets, self.lsets = self._sort_sets()
         raise RuntimeError(f"binning capacity still exceeded after {max_regrow} regrowths")
 
     def timed_run(self) -> Dict[str, float]:

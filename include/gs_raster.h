@@ -297,6 +297,29 @@ int gs_render_after_depth_sort(const gs_frame_t* frame, int P, const void* geom,
                                long long capacity, void* binning, size_t binning_bytes, void* image,
                                size_t image_bytes, float* out_color, float* out_depth, float* out_alpha,
                                void* stream);
+/* The tile-list sort shared the same way: after the shared depth sort, per
+ * frame gs_render_binning_duplicate (tile-count prefix and duplicate), then
+ * gs_multi_list_sort over the set (each frame's list sort on its tile bits,
+ * with its own pass count, leaving the list and the tile ranges where
+ * gs_render_binning does; frames with one-word list entries only, capacity
+ * up to 2^21), then per frame gs_render_binning_after_list_sort (checkpoint
+ * scan) or gs_render_after_list_sort (the same and the blend).  blocks[4]
+ * holds the grid of each pass (0 past the last). */
+int gs_render_binning_duplicate(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters,
+                                long long capacity, void* binning, size_t binning_bytes, void* image,
+                                size_t image_bytes, void* stream);
+size_t gs_multi_list_sort_bytes(int n);
+int gs_multi_list_sort_build(int n, const gs_frame_t* frames, const int* P, const long long* capacity,
+                             void* const* binning, const size_t* binning_bytes, uint32_t* const* counters,
+                             void* const* image, void* host_set, size_t set_bytes, unsigned* blocks);
+int gs_multi_list_sort(int n, const void* dev_set, const unsigned* blocks, void* stream);
+int gs_render_binning_after_list_sort(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters,
+                                      long long capacity, void* binning, size_t binning_bytes, void* image,
+                                      size_t image_bytes, void* stream);
+int gs_render_after_list_sort(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters,
+                              long long capacity, void* binning, size_t binning_bytes, void* image,
+                              size_t image_bytes, float* out_color, float* out_depth, float* out_alpha,
+                              void* stream);
 int gs_backward_prepare(const gs_frame_t* frame, const gs_gaussians_t* g, const int* radii, const void* geom,
                         void* binning, long long capacity, const void* image, const uint32_t* counters,
                         const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
@@ -550,8 +573,9 @@ const char* gs_error_string(int code);
 /* 2: binning layout with the backward work items, one-word list entries and
  * packed tile rects; split forward (gs_render_prepare / gs_render_resume).
  * 3: image layout with the blend's tile order (tile_order).
- * 4: the depth-first binning split around a depth sort shared by many frames
- * (gs_render_depth_keys, gs_multi_depth_sort*, gs_render_*after_depth_sort). */
+ * 4: the depth-first binning split around a depth sort and a list sort
+ * shared by many frames (gs_render_depth_keys, gs_multi_depth_sort*,
+ * gs_render_binning_duplicate, gs_multi_list_sort*, gs_render_*after_*_sort). */
 int gs_abi_version(void);
 
 #ifdef __cplusplus

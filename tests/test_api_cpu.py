@@ -296,3 +296,42 @@ def test_multi_depth_sort_set_on_the_host():
                                        for (W, H, _), p, c in zip(sizes, big, bigcap)])
     assert lib.gs_multi_depth_sort_build(n, frames, big, bigcap, bins, bigbytes, counters, host, nb,
                                          ctypes.byref(blocks)) == 1
+
+
+def test_multi_list_sort_set_on_the_host():
+    """gs_multi_list_sort_build: a frame whose tile ids take 8 bits (240
+    tiles) sorts its list in one pass, one with 2040 tiles in two; the second
+    pass has workgroups only for the latter.  Two-word list entries are
+    refused."""
+    import ctypes
+    from gs_hip import _lib
+    lib = _lib.load()
+    sizes = [(320, 180, 20000, 60000), (960, 540, 200000, 1500000)]
+    n = len(sizes)
+    frames = (_lib.GsFrame * n)()
+    for k, (W, H, _, _) in enumerate(sizes):
+        f = frames[k]
+        f.width, f.height, f.sh_degree = W, H, 0
+        f.viewmatrix, f.projmatrix, f.bg = 0x1000, 0x2000, 0x3000
+        f.binning = _lib.BINNING_DEPTH_FIRST
+    P = (ctypes.c_int * n)(*[p for _, _, p, _ in sizes])
+    cap = (ctypes.c_longlong * n)(*[c for _, _, _, c in sizes])
+    layouts = [_lib.binning_layout(c, p, W, H, _lib.BINNING_DEPTH_FIRST) for W, H, p, c in sizes]
+    assert all(L.id_bits > 0 for L in layouts)
+    bins = (ctypes.c_void_p * n)(*[0x10000000 * (k + 1) for k in range(n)])
+    nbytes = (ctypes.c_size_t * n)(*[L.total for L in layouts])
+    counters = (ctypes.c_void_p * n)(*[0x7000000 + 256 * k for k in range(n)])
+    images = (ctypes.c_void_p * n)(*[0x9000000 + 0x100000 * k for k in range(n)])
+    nb = int(lib.gs_multi_list_sort_bytes(n))
+    host = (ctypes.c_uint8 * nb)()
+    blocks = (ctypes.c_uint * 4)()
+    assert lib.gs_multi_list_sort_build(n, frames, P, cap, bins, nbytes, counters, images, host, nb, blocks) == 0
+    per = [-(-(-(-c // 1024)) // 8) * 8 for _, _, _, c in sizes]
+    assert list(blocks) == [per[0] + per[1], per[1], 0, 0]
+    # two-word entries (ids and tile ids do not fit one word) are refused
+    big = (ctypes.c_int * n)(20000, (1 << 21) + 1)
+    bigbytes = (ctypes.c_size_t * n)(*[_lib.binning_layout(int(c), int(p), W, H, 0).total
+                                       for (W, H, _, c), p in zip(sizes, big)])
+    assert _lib.binning_layout(1500000, (1 << 21) + 1, 960, 540, 0).id_bits == 0
+    assert lib.gs_multi_list_sort_build(n, frames, big, cap, bins, bigbytes, counters, images, host, nb,
+                                        blocks) == 1

@@ -1,10 +1,13 @@
 """The depth-first binning split around one depth sort shared by many frames
-(gs_render_depth_keys, gs_multi_depth_sort, gs_render_after_depth_sort)
-leaves exactly what gs_render leaves, frame by frame: the same depth order,
-the same sorted tile lists and ranges, the same image and per-pixel state.
+(gs_render_depth_keys, gs_multi_depth_sort, gs_render_after_depth_sort),
+and also around one shared list sort (gs_render_binning_duplicate,
+gs_multi_list_sort, gs_render_after_list_sort), leaves exactly what
+gs_render leaves, frame by frame: the same depth order, the same sorted tile
+lists and ranges, the same image and per-pixel state.
 
 Frames of the batched pose workload's kinds: different resolutions and
-Gaussian counts (each with its own workspaces), sorted together."""
+Gaussian counts (each with its own workspaces), sorted together; their list
+sorts take one pass (up to 256 tiles) or two."""
 import ctypes
 
 import numpy as np
@@ -13,7 +16,8 @@ import torch
 
 from gs_hip import _lib
 from gs_hip import raster as _raster
-from gs_hip.batch import DepthSortSet, launch_slot_after_sort, launch_slot_keys, make_slot, start_capacities
+from gs_hip.batch import (DepthSortSet, ListSortSet, launch_slot_after_sort, launch_slot_dup, launch_slot_keys,
+                          launch_slot_tail, make_slot, start_capacities)
 from gs_hip.pose import build_instance
 
 pytestmark = pytest.mark.gpu
@@ -28,10 +32,11 @@ def _f32(t: torch.Tensor, off: int, count: int) -> np.ndarray:
     return t[off:off + 4 * count].cpu().numpy().view(np.float32)
 
 
+@pytest.mark.parametrize("shared_list", [False, True])
 @pytest.mark.parametrize("sizes", [[(160, 90, 12000), (320, 180, 40000), (128, 72, 5000)],
-                                   [(960, 540, 200000), (480, 270, 60000)]])
-def test_shared_depth_sort_equals_per_frame_binning(sizes):
-    print("shared depth sort, frames", sizes, flush=True)
+                                   [(960, 540, 200000), (480, 270, 60000), (320, 180, 20000)]])
+def test_shared_sorts_equal_per_frame_binning(sizes, shared_list):
+    print("shared depth sort, shared list sort", shared_list, "frames", sizes, flush=True)
     insts = [build_instance(300 + k, W=W, H=H, P=P) for k, (W, H, P) in enumerate(sizes)]
     pose = torch.tensor([0.0, 0, 0, 0, 0, 0, 1], dtype=torch.float32, device=DEV).repeat(len(insts), 1)
     caps = start_capacities(insts, pose, _lib.BINNING_DEPTH_FIRST, 1.3)
@@ -56,8 +61,15 @@ def test_shared_depth_sort_equals_per_frame_binning(sizes):
     for s in b:
         launch_slot_keys(s, stream)
     DepthSortSet(b, DEV).launch(stream)
-    for s in b:
-        launch_slot_after_sort(s, stream, blend=True)
+    if shared_list:
+        for s in b:
+            launch_slot_dup(s, stream)
+        ListSortSet(b, DEV).launch(stream)
+        for s in b:
+            launch_slot_tail(s, stream, blend=True)
+    else:
+        for s in b:
+            launch_slot_after_sort(s, stream, blend=True)
     torch.cuda.synchronize()
     for k, (sa, sb) in enumerate(zip(a, b)):
         W, H, P = sa.inst.W, sa.inst.H, sa.inst.P
