@@ -75,13 +75,16 @@ __device__ __forceinline__ void stage_halo(float (*dst[NQ])[kLIW], const float* 
   }
 }
 
-__global__ __launch_bounds__(256) void k_ssim_fwd(LossArgs a) {
+// Tile (bx, by) of channel bz of a (gx x gy x C) grid of one image (the
+// kernels pass their block index; a launch over many images, each image's
+// block coordinates).
+__device__ __forceinline__ void ssim_fwd_block(const LossArgs& a, int bx, int by, int bz, int gx, int gy) {
   __shared__ float sx[kLIH][kLIW];
   __shared__ float sy[kLIH][kLIW];
   __shared__ float sh[5][kLIH][kLTX];
   __shared__ float red[2][4];
-  const int ch = blockIdx.z;
-  const int r0 = blockIdx.y * kLTY - kLR, c0 = blockIdx.x * kLTX - kLR;
+  const int ch = bz;
+  const int r0 = by * kLTY - kLR, c0 = bx * kLTX - kLR;
   const float* X = a.x + (size_t)ch * a.H * a.W;
   const float* Y = a.y + (size_t)ch * a.H * a.W;
   const int t = threadIdx.x;
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(LossArgs a) {
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      const int gr = blockIdx.y * kLTY + r + u, gc = blockIdx.x * kLTX + c;
+      const int gr = by * kLTY + r + u, gc = bx * kLTX + c;
       if (gr >= a.H || gc >= a.W) continue;
       const float s1 = e11[u] - m1[u] * m1[u], s2 = e22[u] - m2[u] * m2[u], s12 = e12[u] - m1[u] * m2[u];
       const float A1 = 2.f * m1[u] * m2[u] + kC1, A2 = 2.f * s12 + kC2;
@@ -183,10 +186,14 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(LossArgs a) {
   }
   __syncthreads();
   if (t == 0) {
-    const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const int b = (bz * gy + by) * gx + bx;
     a.partial[2 * b] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
     a.partial[2 * b + 1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
   }
+}
+
+__global__ __launch_bounds__(256) void k_ssim_fwd(LossArgs a) {
+  ssim_fwd_block(a, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, gridDim.y);
 }
 
 // out[0] = loss, out[1] = mean SSIM, out[2] = mean |x - y|, by one
@@ -241,16 +248,16 @@ struct LossBwdArgs {
   float* out3;
 };
 
-__global__ __launch_bounds__(256) void k_ssim_bwd(LossBwdArgs a) {
+__device__ __forceinline__ void ssim_bwd_block(const LossBwdArgs& a, int bx, int by, int bz) {
   __shared__ float sm[3][kLIH][kLIW];
   __shared__ float sh[3][kLIH][kLTX];
-  if (a.out3 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) {
+  if (a.out3 && bx == 0 && by == 0 && bz == 0) {
     __shared__ double red[2][256];
     loss_final_block(a.partial, a.nb, a.inv_n, a.lambda, a.out3, red);
   }
-  const int ch = blockIdx.z;
+  const int ch = bz;
   const size_t HW = (size_t)a.H * a.W;
-  const int r0 = blockIdx.y * kLTY - kLR, c0 = blockIdx.x * kLTX - kLR;
+  const int r0 = by * kLTY - kLR, c0 = bx * kLTX - kLR;
   const int t = threadIdx.x;
   {
     float (*dst[3])[kLIW] = {sm[0], sm[1], sm[2]};
@@ -308,7 +315,7 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(LossBwdArgs a) {
   }
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int gr = blockIdx.y * kLTY + r + u, gcol = blockIdx.x * kLTX + c;
+    const int gr = by * kLTY + r + u, gcol = bx * kLTX + c;
     if (gr >= a.H || gcol >= a.W) continue;
     const size_t o = (size_t)ch * HW + (size_t)gr * a.W + gcol;
     const float xraw = a.x[o], yv = a.y[o];
@@ -320,6 +327,39 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(LossBwdArgs a) {
     const float sgn = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
     a.dx[o] = pass * (go * ((1.0f - a.lambda) * sgn * a.inv_n - a.lambda * dssim));
   }
+}
+
+__global__ __launch_bounds__(256) void k_ssim_bwd(LossBwdArgs a) { ssim_bwd_block(a, blockIdx.x, blockIdx.y, blockIdx.z); }
+
+// ---- the loss of many images in one launch of each kernel ------------------
+// Image f owns blocks [prefix[f], prefix[f + 1]) of a 1-D grid, in the order
+// of its own launch (x fastest, then y, then the channel).
+__device__ __forceinline__ int image_of(const uint32_t* prefix, int n, uint32_t b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_ssim_fwd_multi(const uint32_t* prefix, const LossArgs* args, int n) {
+  const uint32_t b = blockIdx.x;
+  const int f = image_of(prefix, n, b);
+  const LossArgs a = args[f];  // a copy: loaded once, not re-read after the stores to the maps
+  const int gx = (a.W + kLTX - 1) / kLTX, gy = (a.H + kLTY - 1) / kLTY;
+  const int l = (int)(b - prefix[f]);
+  ssim_fwd_block(a, l % gx, (l / gx) % gy, l / (gx * gy), gx, gy);
+}
+
+__global__ __launch_bounds__(256) void k_ssim_bwd_multi(const uint32_t* prefix, const LossBwdArgs* args, int n) {
+  const uint32_t b = blockIdx.x;
+  const int f = image_of(prefix, n, b);
+  const LossBwdArgs a = args[f];
+  const int gx = (a.W + kLTX - 1) / kLTX, gy = (a.H + kLTY - 1) / kLTY;
+  const int l = (int)(b - prefix[f]);
+  ssim_bwd_block(a, l % gx, (l / gx) % gy, l / (gx * gy));
 }
 
 }  // namespace gs
@@ -387,14 +427,13 @@ extern "C" int gs_ssim_l1_backward(const float* pred, const float* target, int C
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
-extern "C" int gs_ssim_l1_forward_backward(const float* pred, const float* target, int C, int H, int W,
-                                           const float* window11, float lambda_dssim, int clamp01, float* out3,
-                                           const float* grad_out, void* ws, size_t ws_bytes, float* dpred,
-                                           void* stream) {
+// The two launches' arguments of gs_ssim_l1_forward_backward.
+static int fwd_bwd_args(const float* pred, const float* target, int C, int H, int W, const float* window11,
+                        float lambda_dssim, int clamp01, float* out3, const float* grad_out, void* ws,
+                        size_t ws_bytes, float* dpred, LossArgs& f, LossBwdArgs& b) {
   if (!pred || !target || !window11 || !out3 || !grad_out || !ws || !dpred || C <= 0 || H <= 0 || W <= 0)
     return GS_E_ARG;
   if (ws_bytes < gs_loss_workspace_bytes(C, H, W)) return GS_E_WORKSPACE;
-  LossArgs f;
   f.C = C;
   f.H = H;
   f.W = W;
@@ -406,9 +445,6 @@ extern "C" int gs_ssim_l1_forward_backward(const float* pred, const float* targe
   f.maps = (float*)ws;
   f.partial = (float*)((char*)ws + align_up(3 * sizeof(float) * (size_t)C * H * W, 256));
   const dim3 g = loss_grid(C, H, W);
-  hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_ssim_fwd, g, dim3(256), 0, s, f);
-  LossBwdArgs b;
   b.C = C;
   b.H = H;
   b.W = W;
@@ -424,6 +460,69 @@ extern "C" int gs_ssim_l1_forward_backward(const float* pred, const float* targe
   b.partial = f.partial;
   b.nb = (int)(g.x * g.y * g.z);
   b.out3 = out3;
+  return GS_OK;
+}
+
+extern "C" int gs_ssim_l1_forward_backward(const float* pred, const float* target, int C, int H, int W,
+                                           const float* window11, float lambda_dssim, int clamp01, float* out3,
+                                           const float* grad_out, void* ws, size_t ws_bytes, float* dpred,
+                                           void* stream) {
+  LossArgs f;
+  LossBwdArgs b;
+  const int rc = fwd_bwd_args(pred, target, C, H, W, window11, lambda_dssim, clamp01, out3, grad_out, ws, ws_bytes,
+                              dpred, f, b);
+  if (rc != GS_OK) return rc;
+  const dim3 g = loss_grid(C, H, W);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_ssim_fwd, g, dim3(256), 0, s, f);
   hipLaunchKernelGGL(k_ssim_bwd, g, dim3(256), 0, s, b);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+// Set layout: uint32 prefix[n + 1], then LossArgs[n], then LossBwdArgs[n],
+// each at a 256-byte boundary.
+static size_t multi_loss_fwd_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
+static size_t multi_loss_bwd_offset(int n) {
+  return multi_loss_fwd_offset(n) + align_up(sizeof(LossArgs) * (size_t)n, 256);
+}
+
+extern "C" size_t gs_multi_loss_bytes(int n) {
+  return n <= 0 ? 0 : multi_loss_bwd_offset(n) + sizeof(LossBwdArgs) * (size_t)n;
+}
+
+extern "C" int gs_multi_loss_build(int n, const float* const* pred, const float* const* target, const int* C,
+                                   const int* H, const int* W, const float* window11, float lambda_dssim,
+                                   int clamp01, float* const* out3, const float* grad_out, void* const* ws,
+                                   const size_t* ws_bytes, float* const* dpred, void* host_set, size_t set_bytes,
+                                   unsigned* blocks) {
+  if (n <= 0 || !pred || !target || !C || !H || !W || !out3 || !ws || !ws_bytes || !dpred || !host_set || !blocks)
+    return GS_E_ARG;
+  if (set_bytes < gs_multi_loss_bytes(n)) return GS_E_WORKSPACE;
+  uint32_t* prefix = (uint32_t*)host_set;
+  LossArgs* fa = (LossArgs*)((char*)host_set + multi_loss_fwd_offset(n));
+  LossBwdArgs* ba = (LossBwdArgs*)((char*)host_set + multi_loss_bwd_offset(n));
+  unsigned long long total = 0;
+  for (int i = 0; i < n; ++i) {
+    const int rc = fwd_bwd_args(pred[i], target[i], C[i], H[i], W[i], window11, lambda_dssim, clamp01, out3[i],
+                                grad_out, ws[i], ws_bytes[i], dpred[i], fa[i], ba[i]);
+    if (rc != GS_OK) return rc;
+    const dim3 g = loss_grid(C[i], H[i], W[i]);
+    prefix[i] = (uint32_t)total;
+    total += (unsigned long long)g.x * g.y * g.z;
+    if (total > 0x7fffffffULL) return GS_E_ARG;
+  }
+  prefix[n] = (uint32_t)total;
+  *blocks = (unsigned)total;
+  return GS_OK;
+}
+
+extern "C" int gs_multi_loss(int n, const void* dev_set, unsigned blocks, void* stream) {
+  if (n <= 0 || !dev_set || blocks == 0) return GS_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t* prefix = (const uint32_t*)dev_set;
+  const LossArgs* fa = (const LossArgs*)((const char*)dev_set + multi_loss_fwd_offset(n));
+  const LossBwdArgs* ba = (const LossBwdArgs*)((const char*)dev_set + multi_loss_bwd_offset(n));
+  hipLaunchKernelGGL(k_ssim_fwd_multi, dim3(blocks), dim3(256), 0, s, prefix, fa, n);
+  hipLaunchKernelGGL(k_ssim_bwd_multi, dim3(blocks), dim3(256), 0, s, prefix, ba, n);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }

@@ -180,6 +180,13 @@ EXPORTS = {
     "gs_ssim_l1_forward_backward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, ctypes.POINTER(c_float),
                                             c_float, c_int, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p,
                                             c_void_p]),
+    "gs_multi_loss_bytes": (c_size_t, [c_int]),
+    "gs_multi_loss_build": (c_int, [c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), ctypes.POINTER(c_int),
+                                    ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_float), c_float,
+                                    c_int, ctypes.POINTER(c_void_p), c_void_p, ctypes.POINTER(c_void_p),
+                                    ctypes.POINTER(c_size_t), ctypes.POINTER(c_void_p), c_void_p, c_size_t,
+                                    ctypes.POINTER(ctypes.c_uint)]),
+    "gs_multi_loss": (c_int, [c_int, c_void_p, ctypes.c_uint, c_void_p]),
     "gs_se3_retract": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gs_se3_grad": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "gs_se3_adam": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float,

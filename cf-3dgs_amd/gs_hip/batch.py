@@ -246,6 +246,34 @@ class ListSortSet:
                    "gs_multi_list_sort")
 
 
+class LossSet:
+    """The fused loss forward and backward of many solves in one launch of
+    each kernel (gs_multi_loss); solve k writes its stats to out3 row k."""
+
+    def __init__(self, slots: Sequence[_Slot], out3_ptrs: Sequence[int], one_ptr: int, lam: float, device):
+        lib = _lib.load()
+        n = len(slots)
+        self.n = n
+        vp = ctypes.c_void_p
+        arr = lambda vals: (vp * n)(*vals)  # noqa: E731
+        ints = lambda vals: (ctypes.c_int * n)(*vals)  # noqa: E731
+        nb = int(lib.gs_multi_loss_bytes(n))
+        host = (ctypes.c_uint8 * nb)()
+        blocks = ctypes.c_uint(0)
+        _lib.check(lib.gs_multi_loss_build(
+            n, arr([s.color.data_ptr() for s in slots]), arr([s.inst.target.data_ptr() for s in slots]),
+            ints([3] * n), ints([s.inst.H for s in slots]), ints([s.inst.W for s in slots]), _taps(), lam, 1,
+            arr(list(out3_ptrs)), vp(one_ptr), arr([s.loss_ws.data_ptr() for s in slots]),
+            (ctypes.c_size_t * n)(*[s.loss_ws.numel() for s in slots]), arr([s.dcolor.data_ptr() for s in slots]),
+            host, nb, ctypes.byref(blocks)), "gs_multi_loss_build")
+        self.dev = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+        self.blocks = blocks.value
+
+    def launch(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_loss(self.n, ctypes.c_void_p(self.dev.data_ptr()), self.blocks, stream),
+                   "gs_multi_loss")
+
+
 class DepthSortSet:
     """The depth sorts of many solves as one segmented sort
     (gs_multi_depth_sort): 12 launches for the set instead of 12 per solve."""
@@ -444,19 +472,26 @@ class BatchedPoseSolver:
         self.ngroups, self.members, self.group_streams, self.stream_of = group_layout(
             cost, self.nstreams, int(groups) if self.multi else 1)
         self.sets = self._frame_sets() if self.multi else None
-        # shared depth sort: one segmented sort per group (one-launch blend
-        # schedules) or per stream (per-frame schedules)
+        # shared stages: the depth and list sorts as one segmented sort each
+        # per group (one-launch blend schedules) or per stream (per-frame
+        # schedules), and with one-launch blends the group's loss as one
+        # launch of each loss kernel
         self.shared_sort = bool(shared_sort)
         if self.shared_sort and self.binning != _lib.BINNING_DEPTH_FIRST:
             raise ValueError("the shared depth sort needs the depth-first binning")
         self.stream_members = {q: [i for i in range(self.n) if self.stream_of[i] == q]
                                for q in sorted(set(self.stream_of))}
-        self.dsets = self.lsets = None
+        self.dsets = self.lsets = self.losssets = None
         if self.shared_sort:
             self.dsets, self.lsets = self._sort_sets()
+            self.losssets = self._loss_sets() if self.multi else None
 
     def _frame_sets(self) -> List[FrameSets]:
         return [FrameSets([self.slots[i] for i in members], self.device) for members in self.members]
+
+    def _loss_sets(self):
+        return [LossSet([self.slots[i] for i in mem], [self.out3.data_ptr() + 12 * i for i in mem],
+                        self.one.data_ptr(), self.lam, self.device) for mem in self.members]
 
     def _sort_sets(self):
         """Shared depth and list sorts, one per group (one-launch blend
@@ -525,6 +560,9 @@ class BatchedPoseSolver:
             launch_slot_loss(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, st)
             _lib.check(lib.gs_backward_prepare(*_bwd_args(s), st), "gs_backward_prepare")
 
+        def prepare(i, s, st):
+            _lib.check(lib.gs_backward_prepare(*_bwd_args(s), st), "gs_backward_prepare")
+
         def finish(i, s, st):
             _lib.check(lib.gs_backward_finish(*_bwd_args(s), st), "gs_backward_finish")
 
@@ -547,7 +585,11 @@ class BatchedPoseSolver:
             else:
                 self._per_slot(lambda i, s, st: launch_slot_binning(s, st), members, lead)
             sets.blend_launch(lp)
-            self._per_slot(loss_and_prepare, members, lead)
+            if self.losssets is not None:
+                self.losssets[g].launch(lp)
+                self._per_slot(prepare, members, lead)
+            else:
+                self._per_slot(loss_and_prepare, members, lead)
             sets.render_bwd_launch(lp)
             self._per_slot(finish, members, lead)
         if self.ngroups > 1:
@@ -643,6 +685,7 @@ class BatchedPoseSolver:
                 self.sets = self._frame_sets()
             if self.shared_sort:
                 self.dsets, self.lsets = self._sort_sets()
+                self.losssets = self._loss_sets() if self.multi else None
         raise RuntimeError(f"binning capacity still exceeded after {max_regrow} regrowths")
 
     def timed_run(self) -> Dict[str, float]:

@@ -442,6 +442,17 @@ int gs_ssim_l1_backward(const float* pred, const float* target, int C, int H, in
 int gs_ssim_l1_forward_backward(const float* pred, const float* target, int C, int H, int W, const float* window11,
                                 float lambda_dssim, int clamp01, float* out3, const float* grad_out, void* ws,
                                 size_t ws_bytes, float* dpred, void* stream);
+/* gs_ssim_l1_forward_backward of n images (each with its own size, buffers,
+ * workspace and out3; one window, lambda, clamp and grad_out for all) in one
+ * launch of each of its two kernels.  Every image's results have the bits of
+ * its own call.  No reference counterpart (the reference computes one
+ * frame's loss at a time); for batched solves. */
+size_t gs_multi_loss_bytes(int n);
+int gs_multi_loss_build(int n, const float* const* pred, const float* const* target, const int* C, const int* H,
+                        const int* W, const float* window11, float lambda_dssim, int clamp01, float* const* out3,
+                        const float* grad_out, void* const* ws, const size_t* ws_bytes, float* const* dpred,
+                        void* host_set, size_t set_bytes, unsigned* blocks);
+int gs_multi_loss(int n, const void* dev_set, unsigned blocks, void* stream);
 
 /* SE(3) pose parameters as lietorch's LieGroupParameter holds them
  * (scene/gaussian_model_cf.py:328-340 init_RT, :123-133 P[0].retr().act(xyz)):
@@ -575,7 +586,8 @@ const char* gs_error_string(int code);
  * 3: image layout with the blend's tile order (tile_order).
  * 4: the depth-first binning split around a depth sort and a list sort
  * shared by many frames (gs_render_depth_keys, gs_multi_depth_sort*,
- * gs_render_binning_duplicate, gs_multi_list_sort*, gs_render_*after_*_sort). */
+ * gs_render_binning_duplicate, gs_multi_list_sort*, gs_render_*after_*_sort)
+ * and the loss of many images in one launch per kernel (gs_multi_loss*). */
 int gs_abi_version(void);
 
 #ifdef __cplusplus

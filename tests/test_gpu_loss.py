@@ -81,3 +81,50 @@ def test_forward_backward_in_one_call_equals_the_two_calls(C, H, W, clamp):
     assert float(dx1.abs().max()) > 0
     assert lib.gs_ssim_l1_forward_backward(p(x), p(y), C, H, W, taps, 0.2, clamp, p(out2), p(one), p(ws2), nb - 1,
                                            p(dx2), stream) == 2  # GS_E_WORKSPACE
+
+
+@pytest.mark.parametrize("clamp", [0, 1])
+def test_multi_image_loss_equals_each_images_own_call(clamp):
+    """gs_multi_loss (every image's forward and backward blocks in one launch
+    of each kernel) gives each image the statistics and the gradient of its
+    own gs_ssim_l1_forward_backward call, bit for bit; the images differ in
+    size (tiles cut at the right and bottom edges) and channel count."""
+    import ctypes
+    from gs_hip import _lib
+    from gs_hip.loss import _window_taps
+    lib = _lib.load()
+    shapes = [(3, 90, 160), (3, 270, 480), (1, 37, 53), (3, 540, 960)]
+    n = len(shapes)
+    g = torch.Generator().manual_seed(11)
+    xs = [(torch.rand(s, generator=g) * 1.4 - 0.2).to(DEV) for s in shapes]
+    ys = [torch.rand(s, generator=g).to(DEV) for s in shapes]
+    one = torch.full((1,), 0.75, device=DEV)
+    taps = _window_taps()
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    stream = ctypes.c_void_p(torch.cuda.current_stream(DEV).cuda_stream)
+    nbs = [lib.gs_loss_workspace_bytes(*s) for s in shapes]
+    ws1 = [torch.empty(b, dtype=torch.uint8, device=DEV) for b in nbs]
+    ws2 = [torch.empty(b, dtype=torch.uint8, device=DEV) for b in nbs]
+    out1, out2 = torch.zeros((n, 3), device=DEV), torch.zeros((n, 3), device=DEV)
+    dx1 = [torch.zeros_like(x) for x in xs]
+    dx2 = [torch.zeros_like(x) for x in xs]
+    for k, (C, H, W) in enumerate(shapes):
+        assert lib.gs_ssim_l1_forward_backward(p(xs[k]), p(ys[k]), C, H, W, taps, 0.2, clamp, p(out1[k]), p(one),
+                                               p(ws1[k]), nbs[k], p(dx1[k]), stream) == 0
+    vp = ctypes.c_void_p
+    arr = lambda ts: (vp * n)(*[t.data_ptr() for t in ts])  # noqa: E731
+    ints = lambda v: (ctypes.c_int * n)(*v)  # noqa: E731
+    sb = lib.gs_multi_loss_bytes(n)
+    host = (ctypes.c_uint8 * sb)()
+    blocks = ctypes.c_uint(0)
+    assert lib.gs_multi_loss_build(n, arr(xs), arr(ys), ints([s[0] for s in shapes]), ints([s[1] for s in shapes]),
+                                   ints([s[2] for s in shapes]), taps, 0.2, clamp, arr([out2[k] for k in range(n)]),
+                                   p(one), arr(ws2), (ctypes.c_size_t * n)(*nbs), arr(dx2), host, sb,
+                                   ctypes.byref(blocks)) == 0
+    dev = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(DEV)
+    assert lib.gs_multi_loss(n, p(dev), blocks.value, stream) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(out1, out2)
+    for k in range(n):
+        assert torch.equal(dx1[k], dx2[k]), k
+    assert float(out1[:, 0].abs().min()) > 0
