@@ -39,7 +39,11 @@
 
 namespace gs {
 
-constexpr int kRG = 12;  // render-gradient record: m2x m2y ca cb cc opac r g b depth - -
+// render-gradient record: m2x m2y ca cb cc opac r g b depth - -, where ca and
+// cc are the conic gradients times -2: their factor -0.5 is applied to the
+// Gaussian's sum instead of to every pixel's term (a power of two, so the
+// sums and their rounding are exactly the same)
+constexpr int kRG = 12;
 
 struct RBArgs {
   int W, H, gx, gy, nt;
@@ -358,9 +362,9 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
       float v[10];
       v[0] = dL_dG * dG_dx * ddelx;
       v[1] = dL_dG * dG_dy * ddely;
-      v[2] = -0.5f * gdx * dx * dL_dG;
+      v[2] = gdx * dx * dL_dG;  // times -0.5 in k_preprocess_bwd
       v[3] = -gdx * dy * dL_dG;
-      v[4] = -0.5f * gdy * dy * dL_dG;
+      v[4] = gdy * dy * dL_dG;  // times -0.5 in k_preprocess_bwd
       if constexpr (!PO) {
         v[5] = G * dL_da;
         v[6] = w * dpix0;
@@ -734,7 +738,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
       const float cb = T0[0] * ST1[0] + T0[1] * ST1[1] + T0[2] * ST1[2];
       const float cc = T1[0] * ST1[0] + T1[1] * ST1[1] + T1[2] * ST1[2] + 0.3f;
       const float den = ca * cc - cb * cb;
-      const float gA = rA.z, gB = rA.w, gC = rB.x;
+      const float gA = -0.5f * rA.z, gB = rA.w, gC = -0.5f * rB.x;  // (the record's ca and cc are -2x)
       const float d2i = 1.0f / (den * den + 0.0000001f);
       float dLa = 0.f, dLb = 0.f, dLc = 0.f;
       if (d2i != 0.0f) {
