@@ -200,6 +200,11 @@ __device__ __forceinline__ void wave_sum5(const float v[5], float w[2]) {
   asm volatile("" : : "v"(w[0]), "v"(w[1]));  // as in wave_sum10
 }
 
+template <bool B>
+struct BoolTag {
+  static constexpr bool value = B;
+};
+
 template <bool DET, bool DEP, bool PO>
 __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) {
   // PO (pose-only, fixed-order, colour cotangent only): the caller wants only
@@ -267,6 +272,7 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
     }
   }
   const float bg_dot = (a.bg[0] * dpix0 + a.bg[1] * dpix1) + a.bg[2] * dpix2;
+  const bool bg_black = a.bg[0] == 0.0f && a.bg[1] == 0.0f && a.bg[2] == 0.0f;
   const float ddelx = 0.5f * (float)a.W, ddely = 0.5f * (float)a.H;
   const float a_min = 1.0f / 255.0f;
   const int lane = threadIdx.x & 63;
@@ -312,104 +318,111 @@ __device__ __forceinline__ void render_bwd_item(const RBArgs& a, uint32_t item) 
     // are advanced right after each splat (the reference advances them
     // lazily at the next one, from last_alpha / last_color; same expression,
     // same rounding).
-    for (int i = nw - 1; i >= 0; --i) {
-      // Gradient arithmetic may fuse multiply-adds (the library is built
-      // without contraction for the forward's bit-exactness).  What decides
-      // which pairs contribute -- blend_power, spec_expf and the single
-      // product alpha = o G -- is outside this scope or has no add, so it
-      // stays exactly the forward's.
-#pragma clang fp contract(fast)
-      const int j = s_list[wv][i];
-      const uint32_t q = sbase + j;  // 0-based list position; contributor index q + 1
-      const float4 ga = s_a[j];
-      const float4 gb = s_b[j];
-      const float dx = ga.x - pfx, dy = ga.y - pfy;
-      const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
-      // gb.w: alpha < 1/255 for sure below it.  (q < last implies inside:
-      // last is 0 outside the image.)  The wave-uniform skips test the
-      // product of one ballot per compare, which is the ballot of the
-      // conjunction: a ballot of a single compare is that compare's lane mask.
-      const bool in_list = q < last, p_le0 = !(power > 0.0f), p_ge = !(power < gb.w);
-      uint64_t live = __builtin_amdgcn_ballot_w64(in_list) & __builtin_amdgcn_ballot_w64(p_le0) &
-                      __builtin_amdgcn_ballot_w64(p_ge);
-      if (live == 0ull) continue;
-      const float Graw = spec_expf(power);
-      const float alpha_raw = fminf(0.99f, gb.y * Graw);
-      const bool a_ok = !(alpha_raw < a_min);
-      live &= __builtin_amdgcn_ballot_w64(a_ok);
-      if (live == 0ull) continue;
-      const bool active = in_list && p_le0 && p_ge && a_ok;
-      const float G = active ? Graw : 0.0f;
-      const float alpha = active ? alpha_raw : 0.0f;
-      const float4 gc = s_c[j];
-      // T before this splat; the gradient tolerates the 1-ulp reciprocal
-      const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
-      const float Tn = T * inv_1ma;
-      const float w = alpha * Tn;
-      float dL_da = (gc.x - acc0) * dpix0;
-      dL_da += (gc.y - acc1) * dpix1;
-      dL_da += (gc.z - acc2) * dpix2;
-      if constexpr (DEP) {
-        dL_da += (gb.z - acc_d) * dD;
-        dL_da += (1.0f - acc_a) * dA;
-      }
-      dL_da *= Tn;
-      dL_da += (-T_fin * inv_1ma) * bg_dot;
-      const float dL_dG = gb.y * dL_da;
-      const float gdx = G * dx, gdy = G * dy;
-      const float dG_dx = -gdx * ga.z - gdy * ga.w;
-      const float dG_dy = -gdy * gb.x - gdx * ga.w;
-      float v[10];
-      v[0] = dL_dG * dG_dx * ddelx;
-      v[1] = dL_dG * dG_dy * ddely;
-      v[2] = gdx * dx * dL_dG;  // times -0.5 in k_preprocess_bwd
-      v[3] = -gdx * dy * dL_dG;
-      v[4] = gdy * dy * dL_dG;  // times -0.5 in k_preprocess_bwd
-      if constexpr (!PO) {
-        v[5] = G * dL_da;
-        v[6] = w * dpix0;
-        v[7] = w * dpix1;
-        v[8] = w * dpix2;
-        v[9] = DEP ? w * dD : 0.0f;
-      }
-      T = Tn;
-      const float oma = 1.0f - alpha;
-      acc0 = alpha * gc.x + oma * acc0;
-      acc1 = alpha * gc.y + oma * acc1;
-      acc2 = alpha * gc.z + oma * acc2;
-      if constexpr (DEP) {
-        acc_d = alpha * gb.z + oma * acc_d;
-        acc_a = alpha + oma * acc_a;
-      }
-      if constexpr (PO) {
-        float w2[2];
-        wave_sum5(v, w2);
-        if ((lane & 15) == 0) {
-          const int q = lane >> 4;
-          const int o = q == 1 ? 2 : (q == 2 ? 1 : q);
-          s_acc[wv * kTerms + o][j] = w2[0];
-          if (q == 0) s_acc[wv * kTerms + 4][j] = w2[1];
+    // The pair loop in two forms: a black background (the reference's
+    // default) adds nothing to dL/dalpha -- its term is a signed zero -- so
+    // that form leaves the term out (the choice is wave-uniform).
+    auto pair_loop = [&](auto bg_on) {
+      for (int i = nw - 1; i >= 0; --i) {
+        // Gradient arithmetic may fuse multiply-adds (the library is built
+        // without contraction for the forward's bit-exactness).  What decides
+        // which pairs contribute -- blend_power, spec_expf and the single
+        // product alpha = o G -- is outside this scope or has no add, so it
+        // stays exactly the forward's.
+  #pragma clang fp contract(fast)
+        const int j = s_list[wv][i];
+        const uint32_t q = sbase + j;  // 0-based list position; contributor index q + 1
+        const float4 ga = s_a[j];
+        const float4 gb = s_b[j];
+        const float dx = ga.x - pfx, dy = ga.y - pfy;
+        const float power = blend_power(ga.z, ga.w, gb.x, dx, dy);
+        // gb.w: alpha < 1/255 for sure below it.  (q < last implies inside:
+        // last is 0 outside the image.)  The wave-uniform skips test the
+        // product of one ballot per compare, which is the ballot of the
+        // conjunction: a ballot of a single compare is that compare's lane mask.
+        const bool in_list = q < last, p_le0 = !(power > 0.0f), p_ge = !(power < gb.w);
+        uint64_t live = __builtin_amdgcn_ballot_w64(in_list) & __builtin_amdgcn_ballot_w64(p_le0) &
+                        __builtin_amdgcn_ballot_w64(p_ge);
+        if (live == 0ull) continue;
+        const float Graw = spec_expf(power);
+        const float alpha_raw = fminf(0.99f, gb.y * Graw);
+        const bool a_ok = !(alpha_raw < a_min);
+        live &= __builtin_amdgcn_ballot_w64(a_ok);
+        if (live == 0ull) continue;
+        const bool active = in_list && p_le0 && p_ge && a_ok;
+        const float G = active ? Graw : 0.0f;
+        const float alpha = active ? alpha_raw : 0.0f;
+        const float4 gc = s_c[j];
+        // T before this splat; the gradient tolerates the 1-ulp reciprocal
+        const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
+        const float Tn = T * inv_1ma;
+        const float w = alpha * Tn;
+        float dL_da = (gc.x - acc0) * dpix0;
+        dL_da += (gc.y - acc1) * dpix1;
+        dL_da += (gc.z - acc2) * dpix2;
+        if constexpr (DEP) {
+          dL_da += (gb.z - acc_d) * dD;
+          dL_da += (1.0f - acc_a) * dA;
         }
-      } else {
-        float w[3];
-        wave_sum10(v, w);
-        if ((lane & 15) == 0) {
-          const int q = lane >> 4;
-          const int o = q == 1 ? 2 : (q == 2 ? 1 : q);  // value of row q: 4m + {0, 2, 1, 3}[q]
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            const int k = 4 * m + o;
-            if (k < (DEP ? 10 : 9)) {
-              if constexpr (DET) {
-                s_acc[wv * 10 + k][j] = w[m];
-              } else {
-                atomicAdd(&s_acc[k][j], w[m]);  // (a zero adds nothing; testing for it costs more)
+        dL_da *= Tn;
+        if constexpr (decltype(bg_on)::value) dL_da += (-T_fin * inv_1ma) * bg_dot;
+        const float dL_dG = gb.y * dL_da;
+        const float gdx = G * dx, gdy = G * dy;
+        const float dG_dx = -gdx * ga.z - gdy * ga.w;
+        const float dG_dy = -gdy * gb.x - gdx * ga.w;
+        float v[10];
+        v[0] = dL_dG * dG_dx * ddelx;
+        v[1] = dL_dG * dG_dy * ddely;
+        v[2] = gdx * dx * dL_dG;  // times -0.5 in k_preprocess_bwd
+        v[3] = -gdx * dy * dL_dG;
+        v[4] = gdy * dy * dL_dG;  // times -0.5 in k_preprocess_bwd
+        if constexpr (!PO) {
+          v[5] = G * dL_da;
+          v[6] = w * dpix0;
+          v[7] = w * dpix1;
+          v[8] = w * dpix2;
+          v[9] = DEP ? w * dD : 0.0f;
+        }
+        T = Tn;
+        const float oma = 1.0f - alpha;
+        acc0 = alpha * gc.x + oma * acc0;
+        acc1 = alpha * gc.y + oma * acc1;
+        acc2 = alpha * gc.z + oma * acc2;
+        if constexpr (DEP) {
+          acc_d = alpha * gb.z + oma * acc_d;
+          acc_a = alpha + oma * acc_a;
+        }
+        if constexpr (PO) {
+          float w2[2];
+          wave_sum5(v, w2);
+          if ((lane & 15) == 0) {
+            const int q = lane >> 4;
+            const int o = q == 1 ? 2 : (q == 2 ? 1 : q);
+            s_acc[wv * kTerms + o][j] = w2[0];
+            if (q == 0) s_acc[wv * kTerms + 4][j] = w2[1];
+          }
+        } else {
+          float w[3];
+          wave_sum10(v, w);
+          if ((lane & 15) == 0) {
+            const int q = lane >> 4;
+            const int o = q == 1 ? 2 : (q == 2 ? 1 : q);  // value of row q: 4m + {0, 2, 1, 3}[q]
+  #pragma unroll
+            for (int m = 0; m < 3; ++m) {
+              const int k = 4 * m + o;
+              if (k < (DEP ? 10 : 9)) {
+                if constexpr (DET) {
+                  s_acc[wv * 10 + k][j] = w[m];
+                } else {
+                  atomicAdd(&s_acc[k][j], w[m]);  // (a zero adds nothing; testing for it costs more)
+                }
               }
             }
           }
         }
       }
-    }
+    };
+    if (bg_black) pair_loop(BoolTag<false>{});
+    else pair_loop(BoolTag<true>{});
     __syncthreads();
     const uint32_t t = threadIdx.x;
     if (t < send - sbase) {
