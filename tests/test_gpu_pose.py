@@ -244,13 +244,19 @@ def test_graph_replays_match_eager(groups, streams, shared):
         assert torch.equal(graph.out3, eager.out3)
 
 
-def test_capacity_overflow_regrows():
-    print("capacity overflow regrow", flush=True)
+@pytest.mark.parametrize("groups,shared", [(1, False), (2, True)])
+def test_capacity_overflow_regrows(groups, shared):
+    """A binning capacity too small for some instance raises the sticky flag;
+    the solver grows those instances and solves again, with the same result.
+    With shared stages the regrowth also rebuilds the shared sorts' and the
+    loss's sets over the new buffers."""
+    print("capacity overflow regrow, groups", groups, "shared", shared, flush=True)
     insts = [build_instance(51 + k, W=128, H=72) for k in range(2)]
-    ref_solver = BatchedPoseSolver(insts, iters=20, streams=1, iters_per_graph=5)
+    kw = dict(streams=groups, iters_per_graph=5, groups=groups, shared_sort=shared)
+    ref_solver = BatchedPoseSolver(insts, iters=20, **kw)
     ref_poses, _ = ref_solver.run()
     assert ref_solver.regrows == 0
-    small = BatchedPoseSolver(insts, iters=20, streams=1, iters_per_graph=5, headroom=0.3)
+    small = BatchedPoseSolver(insts, iters=20, headroom=0.3, **kw)
     poses, _ = small.run()
     assert small.regrows >= 1
     assert torch.equal(poses, ref_poses)
