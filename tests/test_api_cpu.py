@@ -335,3 +335,50 @@ def test_multi_list_sort_set_on_the_host():
     assert _lib.binning_layout(1500000, (1 << 21) + 1, 960, 540, 0).id_bits == 0
     assert lib.gs_multi_list_sort_build(n, frames, big, cap, bins, bigbytes, counters, images, host, nb,
                                         blocks) == 1
+
+
+def test_shared_stage_entry_points_refuse_bad_arguments_without_device():
+    """The split binning, the shared sorts and the multi-image loss check
+    their arguments on the host before anything is enqueued (the device
+    addresses below are never read)."""
+    import ctypes
+    from gs_hip import _lib
+    lib = _lib.load()
+    f = _lib.GsFrame()
+    f.width, f.height, f.sh_degree = 320, 180, 0
+    f.viewmatrix, f.projmatrix, f.bg = 0x1000, 0x2000, 0x3000
+    P, cap = 20000, 60000
+    f.binning = _lib.BINNING_SORT64
+    LB = _lib.binning_layout(cap, P, 320, 180, _lib.BINNING_SORT64)
+    LI = _lib.image_layout(320, 180)
+    dev = ctypes.c_void_p(0x10000000)
+    args = (ctypes.byref(f), P, dev, dev, cap, dev, LB.total, dev, LI.total, None)
+    # the phases split around the sorts exist only in the depth-first schedule
+    for fn in (lib.gs_render_depth_keys, lib.gs_render_binning_duplicate, lib.gs_render_binning_after_depth_sort,
+               lib.gs_render_binning_after_list_sort):
+        assert fn(*args) == 1
+    f.binning = _lib.BINNING_DEPTH_FIRST
+    LB = _lib.binning_layout(cap, P, 320, 180, _lib.BINNING_DEPTH_FIRST)
+    short = (ctypes.byref(f), P, dev, dev, cap, dev, LB.total - 1, dev, LI.total, None)
+    assert lib.gs_render_depth_keys(*short) == 2
+    # launches of empty or missing sets
+    assert lib.gs_multi_depth_sort(0, dev, 8, None) == 1
+    assert lib.gs_multi_depth_sort(1, None, 8, None) == 1
+    assert lib.gs_multi_list_sort(1, dev, None, None) == 1
+    zero = (ctypes.c_uint * 4)(0, 0, 0, 0)
+    assert lib.gs_multi_list_sort(1, dev, zero, None) == 1
+    assert lib.gs_multi_loss(0, dev, 8, None) == 1
+    assert lib.gs_multi_loss_bytes(0) == 0 and lib.gs_multi_loss_bytes(2) > 0
+    # a loss workspace smaller than gs_loss_workspace_bytes
+    n = 1
+    vp = (ctypes.c_void_p * n)(0x20000000)
+    ints = lambda v: (ctypes.c_int * n)(v)  # noqa: E731
+    taps = (ctypes.c_float * 11)(*([1.0 / 11] * 11))
+    nb = lib.gs_loss_workspace_bytes(3, 90, 160)
+    host = (ctypes.c_uint8 * int(lib.gs_multi_loss_bytes(n)))()
+    blocks = ctypes.c_uint(0)
+    assert lib.gs_multi_loss_build(n, vp, vp, ints(3), ints(90), ints(160), taps, 0.2, 1, vp, dev, vp,
+                                   (ctypes.c_size_t * n)(nb - 1), vp, host, len(host), ctypes.byref(blocks)) == 2
+    assert lib.gs_multi_loss_build(n, vp, vp, ints(3), ints(90), ints(160), taps, 0.2, 1, vp, dev, vp,
+                                   (ctypes.c_size_t * n)(nb), vp, host, len(host), ctypes.byref(blocks)) == 0
+    assert blocks.value == 3 * ((160 + 31) // 32) * ((90 + 15) // 16)  # 32 x 16-pixel tiles per channel
