@@ -499,7 +499,10 @@ class BatchedPoseSolver:
         per frame (None) when a solve's list cannot join a shared one."""
         parts = self.members if self.multi else list(self.stream_members.values())
         keys = range(len(parts)) if self.multi else list(self.stream_members)
-        dsets = {k: DepthSortSet([self.slots[i] for i in mem], self.device) for k, mem in zip(keys, parts)}
+        try:
+            dsets = {k: DepthSortSet([self.slots[i] for i in mem], self.device) for k, mem in zip(keys, parts)}
+        except _lib.GsError as e:
+            raise ValueError(f"the shared depth sort takes instances of at most 2^21 Gaussians ({e})") from e
         try:
             lsets = {k: ListSortSet([self.slots[i] for i in mem], self.device) for k, mem in zip(keys, parts)}
         except _lib.GsError:
@@ -733,8 +736,12 @@ def autotune(instances: Sequence[PoseInstance], streams=(2, 4, 8),
     results = []
     cands = schedule_candidates(binnings, streams, groups, shared)
     for b, s, g, ss in cands:
-        solver = BatchedPoseSolver(instances, iters=iters_per_graph * replays, streams=s, binning=b,
-                                   iters_per_graph=iters_per_graph, multi=g > 0, groups=max(g, 1), shared_sort=ss)
+        try:
+            solver = BatchedPoseSolver(instances, iters=iters_per_graph * replays, streams=s, binning=b,
+                                       iters_per_graph=iters_per_graph, multi=g > 0, groups=max(g, 1),
+                                       shared_sort=ss)
+        except ValueError:  # a schedule these instances cannot take (the shared depth sort's size limit)
+            continue
         solver._reset()
         solver._capture()
         solver._reset()
