@@ -469,6 +469,7 @@ class BatchedPoseSolver:
         # groups share no stream, so one group's launches overlap the others'
         # per-frame work), then each group's instances onto its streams
         cost = [cap + inst.W * inst.H for inst, cap in zip(self.instances, caps)]
+        self.cost = cost
         self.ngroups, self.members, self.group_streams, self.stream_of = group_layout(
             cost, self.nstreams, int(groups) if self.multi else 1)
         self.sets = self._frame_sets() if self.multi else None
@@ -487,7 +488,12 @@ class BatchedPoseSolver:
             self.losssets = self._loss_sets() if self.multi else None
 
     def _frame_sets(self) -> List[FrameSets]:
-        return [FrameSets([self.slots[i] for i in members], self.device) for members in self.members]
+        # a one-launch stage dispatches its frames' workgroups frame after
+        # frame: costliest frame first, so that its heavy tiles and work items
+        # do not start last and run on alone (each frame's results do not
+        # depend on the order)
+        return [FrameSets([self.slots[i] for i in sorted(members, key=lambda i: -self.cost[i])], self.device)
+                for members in self.members]
 
     def _loss_sets(self):
         return [LossSet([self.slots[i] for i in mem], [self.out3.data_ptr() + 12 * i for i in mem],
