@@ -641,13 +641,14 @@ __device__ __forceinline__ void block_adam_moments(const PBArgs& a, float* s_g, 
   }
 }
 
-__global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
-  // [256][3M] SH rows of this workgroup: coefficients are read from and their
-  // gradients written back to the thread's own row, then the whole block is
-  // stored with coalesced 16-byte writes (M <= 16; larger M goes direct)
-  extern __shared__ float s_sh[];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int base = blockIdx.x * 256;
+// Workgroup bid of one frame's per-Gaussian backward: Gaussians [256 bid,
+// 256 bid + 256).  s_sh holds the [256][3M] SH rows of this workgroup:
+// coefficients are read from and their gradients written back to the
+// thread's own row, then the whole block is stored with coalesced 16-byte
+// writes (M <= 16; larger M goes direct)
+__device__ __forceinline__ void preprocess_bwd_block(const PBArgs& a, int bid, float* s_sh) {
+  const int i = bid * 256 + threadIdx.x;
+  const int base = bid * 256;
   const int nloc = min(256, a.P - base);
   const bool sh_grad = a.shs && (a.dL_dsh || a.adam_on);
   const bool staged = sh_grad && a.M <= 16;
@@ -983,7 +984,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
   if (a.pose) {
     // all lanes reach here (no early return above); every wave of the grid
     // stores its 12 sums, so the finalize reads a fully written array
-    float* dst = a.pose_part + ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 12;
+    float* dst = a.pose_part + ((size_t)bid * 4 + (threadIdx.x >> 6)) * 12;
     float w[3];
     wave_sum12(pacc, w);
     const int lane = threadIdx.x & 63;
@@ -996,8 +997,31 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_pose_finalize(const float* pose7, const float* part, int nparts,
-                                                        float* dL_dpose) {
+__global__ __launch_bounds__(256) void k_preprocess_bwd(PBArgs a) {
+  extern __shared__ float s_sh[];
+  preprocess_bwd_block(a, blockIdx.x, s_sh);
+}
+
+// Many frames in one launch: frame f owns workgroups [prefix[f], prefix[f +
+// 1]) = ceil(P_f / 256) of them.  The dynamic LDS covers the largest frame's
+// SH rows.
+__global__ __launch_bounds__(256) void k_preprocess_bwd_multi(const uint32_t* prefix, const PBArgs* args, int n) {
+  extern __shared__ float s_sh[];
+  const uint32_t b = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const PBArgs a = args[lo];
+  preprocess_bwd_block(a, (int)(b - prefix[lo]), s_sh);
+}
+
+// One frame's pose gradient from its nparts wave partials (12 terms each), by
+// one workgroup of 256 threads.
+__device__ __forceinline__ void pose_finalize_block(const float* pose7, const float* part, int nparts,
+                                                    float* dL_dpose) {
   __shared__ float s[12][256];
   const int t = threadIdx.x;
   float sum[12];
@@ -1037,6 +1061,25 @@ __global__ __launch_bounds__(256) void k_pose_finalize(const float* pose7, const
   dL_dpose[4] = dq[2];
   dL_dpose[5] = dq[3];
   dL_dpose[6] = dq[0];
+}
+
+__global__ __launch_bounds__(256) void k_pose_finalize(const float* pose7, const float* part, int nparts,
+                                                        float* dL_dpose) {
+  pose_finalize_block(pose7, part, nparts, dL_dpose);
+}
+
+// The pose reduction of one frame of a multi-frame finish.
+struct PoseFin {
+  const float* pose7;
+  const float* part;
+  float* dL_dpose;
+  int nparts;
+};
+
+// Workgroup f reduces frame f.
+__global__ __launch_bounds__(256) void k_pose_finalize_multi(const PoseFin* fin) {
+  const PoseFin p = fin[blockIdx.x];
+  pose_finalize_block(p.pose7, p.part, p.nparts, p.dL_dpose);
 }
 
 }  // namespace gs
@@ -1183,6 +1226,72 @@ struct AdamFusedHost {
   float means_step, shs_step, shs_lead_step, inv_sqrt_bc2;
 };
 
+// The per-Gaussian backward's arguments for one frame.
+static PBArgs make_pb_args(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
+                           long long capacity, const uint32_t* counters, const gs_grads_t* out, const RBArgs& r,
+                           bool po, const AdamFusedHost* ad) {
+  const int P = g->P;
+  gs_geom_layout_t LG;
+  gs_geom_layout(P, &LG);
+  const char* gb = (const char*)geom;
+  float* pose_part = (float*)((char*)r.g_render + align_up((size_t)P * kRG * sizeof(float), 256));
+  PBArgs b;
+  b.P = P;
+  b.M = g->M;
+  b.D = f->sh_degree;
+  b.W = f->width;
+  b.H = f->height;
+  b.tanfovx = f->tanfovx;
+  b.tanfovy = f->tanfovy;
+  b.fx = (float)f->width / (2.0f * f->tanfovx);
+  b.fy = (float)f->height / (2.0f * f->tanfovy);
+  b.scale_mod = f->scale_modifier;
+  b.view = f->viewmatrix;
+  b.proj = f->projmatrix;
+  b.campos = f->campos;
+  b.pose = f->pose;
+  b.means = g->means3D;
+  b.shs = g->shs;
+  b.scales = g->scales;
+  b.rots = g->rotations;
+  b.cov_pre = g->cov3D_precomp;
+  b.radii = radii;
+  b.splat = (const float4*)(gb + LG.splat);
+  b.cov3D = (const float*)(gb + LG.cov3D);
+  b.g_render = r.g_render;
+  b.dL_dmeans2D = out->dL_dmeans2D;
+  b.dL_dcolors = out->dL_dcolors;
+  b.dL_dopacity = out->dL_dopacity;
+  b.dL_dmeans3D = out->dL_dmeans3D;
+  b.dL_dcov3D = out->dL_dcov3D;
+  b.dL_dsh = out->dL_dsh;
+  b.dL_dscales = out->dL_dscales;
+  b.dL_drots = out->dL_drotations;
+  b.pose_part = pose_part;
+  b.part5 = po ? (const float*)r.part : nullptr;
+  b.g_tiles = (const uint32_t*)(gb + LG.tiles);
+  b.local_incl = (const uint32_t*)(gb + LG.offsets);
+  b.block_excl = (const uint32_t*)(gb + LG.scan_tmp);
+  b.counters = counters;
+  b.capacity = capacity;
+  b.adam_on = ad ? 1 : 0;
+  b.means_p = ad ? ad->means_p : nullptr;
+  b.means_m = ad ? ad->in->means_m : nullptr;
+  b.means_v = ad ? ad->in->means_v : nullptr;
+  b.shs_p = ad ? ad->shs_p : nullptr;
+  b.shs_m = ad ? ad->in->shs_m : nullptr;
+  b.shs_v = ad ? ad->in->shs_v : nullptr;
+  b.means_step = ad ? ad->means_step : 0.f;
+  b.shs_step = ad ? ad->shs_step : 0.f;
+  b.shs_lead_step = ad ? ad->shs_lead_step : 0.f;
+  b.shs_lead = ad ? ad->in->shs_lead : 0;
+  b.b1 = ad ? ad->in->beta1 : 0.f;
+  b.b2 = ad ? ad->in->beta2 : 0.f;
+  b.eps = ad ? ad->in->eps : 0.f;
+  b.inv_sqrt_bc2 = ad ? ad->inv_sqrt_bc2 : 0.f;
+  return b;
+}
+
 static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* radii, const void* geom,
                        long long capacity, const uint32_t* counters, const gs_grads_t* out, const RBArgs& r,
                        hipStream_t s, bool po, const AdamFusedHost* ad = nullptr) {
@@ -1197,60 +1306,7 @@ static void bwd_finish(const gs_frame_t* f, const gs_gaussians_t* g, const int* 
                        (const uint32_t*)(gb + LG.tiles), (const uint32_t*)(gb + LG.offsets),
                        (const uint32_t*)(gb + LG.scan_tmp), counters, capacity, (const float*)r.part, r.g_render);
   if (P > 0) {
-    PBArgs b;
-    b.P = P;
-    b.M = g->M;
-    b.D = f->sh_degree;
-    b.W = f->width;
-    b.H = f->height;
-    b.tanfovx = f->tanfovx;
-    b.tanfovy = f->tanfovy;
-    b.fx = (float)f->width / (2.0f * f->tanfovx);
-    b.fy = (float)f->height / (2.0f * f->tanfovy);
-    b.scale_mod = f->scale_modifier;
-    b.view = f->viewmatrix;
-    b.proj = f->projmatrix;
-    b.campos = f->campos;
-    b.pose = f->pose;
-    b.means = g->means3D;
-    b.shs = g->shs;
-    b.scales = g->scales;
-    b.rots = g->rotations;
-    b.cov_pre = g->cov3D_precomp;
-    b.radii = radii;
-    b.splat = (const float4*)(gb + LG.splat);
-    b.cov3D = (const float*)(gb + LG.cov3D);
-    b.g_render = r.g_render;
-    b.dL_dmeans2D = out->dL_dmeans2D;
-    b.dL_dcolors = out->dL_dcolors;
-    b.dL_dopacity = out->dL_dopacity;
-    b.dL_dmeans3D = out->dL_dmeans3D;
-    b.dL_dcov3D = out->dL_dcov3D;
-    b.dL_dsh = out->dL_dsh;
-    b.dL_dscales = out->dL_dscales;
-    b.dL_drots = out->dL_drotations;
-    b.pose_part = pose_part;
-    b.part5 = po ? (const float*)r.part : nullptr;
-    b.g_tiles = (const uint32_t*)(gb + LG.tiles);
-    b.local_incl = (const uint32_t*)(gb + LG.offsets);
-    b.block_excl = (const uint32_t*)(gb + LG.scan_tmp);
-    b.counters = counters;
-    b.capacity = capacity;
-    b.adam_on = ad ? 1 : 0;
-    b.means_p = ad ? ad->means_p : nullptr;
-    b.means_m = ad ? ad->in->means_m : nullptr;
-    b.means_v = ad ? ad->in->means_v : nullptr;
-    b.shs_p = ad ? ad->shs_p : nullptr;
-    b.shs_m = ad ? ad->in->shs_m : nullptr;
-    b.shs_v = ad ? ad->in->shs_v : nullptr;
-    b.means_step = ad ? ad->means_step : 0.f;
-    b.shs_step = ad ? ad->shs_step : 0.f;
-    b.shs_lead_step = ad ? ad->shs_lead_step : 0.f;
-    b.shs_lead = ad ? ad->in->shs_lead : 0;
-    b.b1 = ad ? ad->in->beta1 : 0.f;
-    b.b2 = ad ? ad->in->beta2 : 0.f;
-    b.eps = ad ? ad->in->eps : 0.f;
-    b.inv_sqrt_bc2 = ad ? ad->inv_sqrt_bc2 : 0.f;
+    const PBArgs b = make_pb_args(f, g, radii, geom, capacity, counters, out, r, po, ad);
     timing::Scope ts(timing::kPreprocessBwd, s);
     const bool sh_grad = g->shs && (out->dL_dsh || ad);
     // LDS rows as the kernel lays them out: 3M floats, plus 4 when 3M is a
@@ -1399,6 +1455,72 @@ extern "C" int gs_multi_render_bwd(int n, const void* dev_set, unsigned total_bl
     if (dep) hipLaunchKernelGGL((k_render_bwd_multi<false, true, false>), grid, dim3(256), 0, s, prefix, args, n);
     else hipLaunchKernelGGL((k_render_bwd_multi<false, false, false>), grid, dim3(256), 0, s, prefix, args, n);
   }
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+// ---- many frames' pose-only finish in two launches --------------------------
+// Set layout: uint32 prefix[n + 1] of per-Gaussian workgroup bounds per frame,
+// then PBArgs[n] and PoseFin[n], each at a 256-byte boundary.
+static size_t multi_fin_args_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
+static size_t multi_fin_pose_offset(int n) {
+  return multi_fin_args_offset(n) + align_up(sizeof(PBArgs) * (size_t)n, 256);
+}
+
+extern "C" size_t gs_multi_finish_bytes(int n) {
+  return n <= 0 ? 0 : multi_fin_pose_offset(n) + sizeof(PoseFin) * (size_t)n;
+}
+
+extern "C" int gs_multi_finish_build(int n, const gs_frame_t* frames, const gs_gaussians_t* gauss,
+                                     const int* const* radii, const void* const* geom, void* const* binning,
+                                     const long long* capacity, const void* const* image,
+                                     const uint32_t* const* counters, const float* const* dL_dcolor,
+                                     const gs_grads_t* out, void* const* scratch, const size_t* scratch_bytes,
+                                     void* host_set, size_t set_bytes, unsigned* total_blocks) {
+  if (n <= 0 || !frames || !gauss || !radii || !geom || !binning || !capacity || !image || !counters ||
+      !dL_dcolor || !out || !scratch || !scratch_bytes || !host_set || !total_blocks)
+    return GS_E_ARG;
+  if (set_bytes < gs_multi_finish_bytes(n)) return GS_E_WORKSPACE;
+  uint32_t* prefix = (uint32_t*)host_set;
+  PBArgs* args = (PBArgs*)((char*)host_set + multi_fin_args_offset(n));
+  PoseFin* fin = (PoseFin*)((char*)host_set + multi_fin_pose_offset(n));
+  unsigned long long total = 0;
+  for (int i = 0; i < n; ++i) {
+    const gs_frame_t* f = &frames[i];
+    const gs_gaussians_t* g = &gauss[i];
+    const int rc = bwd_check(f, g, radii[i], geom[i], binning[i], capacity[i], image[i], counters[i],
+                             dL_dcolor[i], &out[i], scratch[i], scratch_bytes[i]);
+    if (rc != GS_OK) return rc;
+    // only the pose-only backward (no per-Gaussian outputs, no SH rows in LDS)
+    if (!pose_only(f, g, &out[i], false)) return GS_E_ARG;
+    const int P = g->P;
+    prefix[i] = (uint32_t)total;
+    total += (unsigned long long)((P + 255) / 256);
+    if (total > 0x7fffffffULL) return GS_E_ARG;
+    const RBArgs r = make_rb_args(f, P, geom[i], capacity[i], binning[i], image[i], dL_dcolor[i], nullptr, nullptr,
+                                  scratch[i]);
+    args[i] = make_pb_args(f, g, radii[i], geom[i], capacity[i], counters[i], &out[i], r, true, nullptr);
+    fin[i].pose7 = f->pose;
+    fin[i].part = args[i].pose_part;
+    fin[i].dL_dpose = out[i].dL_dpose;
+    fin[i].nparts = (int)pose_parts(P);
+  }
+  prefix[n] = (uint32_t)total;
+  *total_blocks = (unsigned)total;
+  return GS_OK;
+}
+
+extern "C" int gs_multi_finish(int n, const void* dev_set, unsigned total_blocks, void* stream) {
+  if (n <= 0 || !dev_set) return GS_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t* prefix = (const uint32_t*)dev_set;
+  const PBArgs* args = (const PBArgs*)((const char*)dev_set + multi_fin_args_offset(n));
+  const PoseFin* fin = (const PoseFin*)((const char*)dev_set + multi_fin_pose_offset(n));
+  {
+    timing::Scope ts(timing::kPreprocessBwd, s);
+    if (total_blocks > 0)
+      hipLaunchKernelGGL(k_preprocess_bwd_multi, dim3(total_blocks), dim3(256), 0, s, prefix, args, n);
+  }
+  hipLaunchKernelGGL(k_pose_finalize_multi, dim3(n), dim3(256), 0, s, fin);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 

@@ -381,7 +381,8 @@ def schedule_candidates(binnings: Sequence[int], streams: Sequence[int], groups:
 
 class FrameSets:
     """Device descriptions of many solves' workspaces for the one-launch blend
-    and render backward (gs_multi_blend / gs_multi_render_bwd)."""
+    and render backward (gs_multi_blend / gs_multi_render_bwd) and, for
+    pose-only frames, the two-launch finish (gs_multi_finish)."""
 
     def __init__(self, slots: Sequence[_Slot], device):
         lib = _lib.load()
@@ -416,8 +417,23 @@ class FrameSets:
         self.rbwd_flags = int(slots[0].frame.deterministic) | _lib.GS_RB_COLOR_ONLY
         po = [lib.gs_backward_is_pose_only(ctypes.byref(s.frame), ctypes.byref(s.gauss), ctypes.byref(s.grads), 0)
               for s in slots]
+        self.finish = None
         if all(po):
             self.rbwd_flags |= _lib.GS_RB_POSE_ONLY
+            # and the pose-only finish (per-Gaussian pose partials, pose
+            # reduction) of all frames in two launches
+            nf = int(lib.gs_multi_finish_bytes(n))
+            host = (ctypes.c_uint8 * nf)()
+            fb = ctypes.c_uint(0)
+            gauss = (_lib.GsGaussians * n)(*[s.gauss for s in slots])
+            grads = (_lib.GsGrads * n)(*[s.grads for s in slots])
+            sbytes = (ctypes.c_size_t * n)(*[s.scratch.numel() for s in slots])
+            _lib.check(lib.gs_multi_finish_build(n, frames, gauss, ptrs("radii"), ptrs("geom"), ptrs("binning"), cap,
+                                                 ptrs("image"), ptrs("counters"), ptrs("dcolor"), grads,
+                                                 ptrs("scratch"), sbytes, host, nf, ctypes.byref(fb)),
+                       "gs_multi_finish_build")
+            self.finish = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+            self.finish_blocks = fb.value
         elif any(po):
             raise RuntimeError("a render-backward set mixes pose-only and full frames")
 
@@ -429,6 +445,10 @@ class FrameSets:
         _lib.check(_lib.load().gs_multi_render_bwd(self.n, ctypes.c_void_p(self.rbwd.data_ptr()),
                                                    self.rbwd_blocks, self.rbwd_flags, stream),
                    "gs_multi_render_bwd")
+
+    def finish_launch(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_finish(self.n, ctypes.c_void_p(self.finish.data_ptr()), self.finish_blocks,
+                                               stream), "gs_multi_finish")
 
 
 class BatchedPoseSolver:
@@ -554,9 +574,10 @@ class BatchedPoseSolver:
             return
         # per group: per frame preprocess + binning; one blend launch over the
         # group; per frame loss and backward set-up; one render backward
-        # launch; per frame the per-Gaussian / pose gradients.  One group forks
-        # its per-frame work from the current stream onto all streams and runs
-        # its one-launch stages on the current stream; with more groups each
+        # launch; the per-Gaussian / pose gradients (one launch each for
+        # pose-only frames, else per frame).  One group forks its per-frame
+        # work from the current stream onto all streams and runs its
+        # one-launch stages on the current stream; with more groups each
         # group runs everything on a stream of its own, forked from the
         # current stream, and the groups join before the step.  (Under graph
         # capture every dependency is a fork from or a join into the current
@@ -600,7 +621,10 @@ class BatchedPoseSolver:
             else:
                 self._per_slot(loss_and_prepare, members, lead)
             sets.render_bwd_launch(lp)
-            self._per_slot(finish, members, lead)
+            if sets.finish is not None:
+                sets.finish_launch(lp)
+            else:
+                self._per_slot(finish, members, lead)
         if self.ngroups > 1:
             for q in used:
                 cur.wait_stream(self.streams[q])

@@ -382,3 +382,62 @@ def test_shared_stage_entry_points_refuse_bad_arguments_without_device():
     assert lib.gs_multi_loss_build(n, vp, vp, ints(3), ints(90), ints(160), taps, 0.2, 1, vp, dev, vp,
                                    (ctypes.c_size_t * n)(nb), vp, host, len(host), ctypes.byref(blocks)) == 0
     assert blocks.value == 3 * ((160 + 31) // 32) * ((90 + 15) // 16)  # 32 x 16-pixel tiles per channel
+
+
+def test_multi_finish_set_on_the_host():
+    """gs_multi_finish_build lays out one workgroup range per frame (a
+    256-Gaussian block each) and refuses what gs_backward_finish refuses, plus
+    any frame that does not take the pose-only backward.  Built on the host
+    only: the device addresses below are never read."""
+    import ctypes
+    from gs_hip import _lib
+    lib = _lib.load()
+    n = 3
+    Ps = [1000, 256, 0]
+    cap = 4000
+    frames = (_lib.GsFrame * n)()
+    gauss = (_lib.GsGaussians * n)()
+    grads = (_lib.GsGrads * n)()
+    for i in range(n):
+        f = frames[i]
+        f.width, f.height, f.sh_degree, f.deterministic = 160, 90, 0, 1
+        f.viewmatrix, f.projmatrix, f.bg, f.pose = 0x1000, 0x2000, 0x3000, 0x4000
+        f.binning = _lib.BINNING_DEPTH_FIRST
+        g = gauss[i]
+        g.P, g.M = Ps[i], 1
+        g.means3D, g.colors_precomp, g.opacities = 0x5000, 0x6000, 0x7000
+        g.scales, g.rotations = 0x8000, 0x9000
+        grads[i].dL_dpose = 0xA000
+    vp = lambda: (ctypes.c_void_p * n)(*([0x10000000] * n))  # noqa: E731
+    caps = (ctypes.c_longlong * n)(*([cap] * n))
+    sb = [int(lib.gs_backward_scratch_bytes(P, cap, 1)) for P in Ps]
+    sbytes = (ctypes.c_size_t * n)(*sb)
+    nb = int(lib.gs_multi_finish_bytes(n))
+    assert lib.gs_multi_finish_bytes(0) == 0 and nb > 0
+    host = (ctypes.c_uint8 * nb)()
+    blocks = ctypes.c_uint(0)
+
+    def build(nbytes=nb):
+        return lib.gs_multi_finish_build(n, frames, gauss, vp(), vp(), vp(), caps, vp(), vp(), vp(), grads, vp(),
+                                         sbytes, host, nbytes, ctypes.byref(blocks))
+
+    assert build() == 0
+    assert blocks.value == 4 + 1 + 0
+    prefix = list((ctypes.c_uint32 * (n + 1)).from_buffer(host))
+    assert prefix == [0, 4, 5, 5]
+    assert build(nb - 1) == 2
+    # a per-Gaussian output: the full backward, not this set's
+    grads[1].dL_dmeans3D = 0xB000
+    assert build() == 1
+    grads[1].dL_dmeans3D = None
+    # atomics-order reductions are never the pose-only backward
+    frames[0].deterministic = 0
+    assert build() == 1
+    frames[0].deterministic = 1
+    # scratch smaller than gs_backward_scratch_bytes
+    sbytes[2] = sb[2] - 1
+    assert build() == 2
+    sbytes[2] = sb[2]
+    assert build() == 0
+    assert lib.gs_multi_finish(0, ctypes.c_void_p(0x10000000), 1, None) == 1
+    assert lib.gs_multi_finish(1, None, 1, None) == 1
