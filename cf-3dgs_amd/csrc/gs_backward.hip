@@ -113,15 +113,43 @@ __device__ __forceinline__ void bwd_items(const uint32_t* item_log, uint32_t* it
 // The backward's set-up in one launch: workgroup 0 builds the work-item table
 // (bwd_items; none when items is null), the others clear the zn accumulator
 // floats at zp.
-__global__ __launch_bounds__(1024) void k_bwd_prepare(float* zp, long long zn, const uint32_t* item_log,
-                                                      uint32_t* items) {
+// Workgroup bid of a set-up over nblocks workgroups.
+__device__ __forceinline__ void bwd_prepare_block(float* zp, long long zn, const uint32_t* item_log, uint32_t* items,
+                                                  int bid, int nblocks) {
   const int first = items ? 1 : 0;
-  if (first && blockIdx.x == 0) {
+  if (first && bid == 0) {
     bwd_items(item_log, items);
     return;
   }
-  const long long nb = (long long)gridDim.x - first;
-  for (long long k = ((long long)blockIdx.x - first) * 1024 + threadIdx.x; k < zn; k += nb * 1024) zp[k] = 0.0f;
+  const long long nb = (long long)nblocks - first;
+  for (long long k = ((long long)bid - first) * 1024 + threadIdx.x; k < zn; k += nb * 1024) zp[k] = 0.0f;
+}
+
+__global__ __launch_bounds__(1024) void k_bwd_prepare(float* zp, long long zn, const uint32_t* item_log,
+                                                      uint32_t* items) {
+  bwd_prepare_block(zp, zn, item_log, items, blockIdx.x, gridDim.x);
+}
+
+// The set-up of one frame of a multi-frame launch.
+struct BwdPrep {
+  float* zp;
+  long long zn;
+  const uint32_t* item_log;
+  uint32_t* items;
+};
+
+// Many frames in one launch: frame f owns workgroups [prefix[f], prefix[f +
+// 1]), as many as its own launch would have.
+__global__ __launch_bounds__(1024) void k_bwd_prepare_multi(const uint32_t* prefix, const BwdPrep* prep, int n) {
+  const uint32_t b = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const BwdPrep p = prep[lo];
+  bwd_prepare_block(p.zp, p.zn, p.item_log, p.items, (int)(b - prefix[lo]), (int)(prefix[lo + 1] - prefix[lo]));
 }
 
 // Sum over the 16 lanes of each DPP row; every lane of the row gets the sum.
@@ -1186,17 +1214,27 @@ static int bwd_check(const gs_frame_t* f, const gs_gaussians_t* g, const int* ra
 }
 
 // Phase 1: clear the accumulators and build the work-item table.
-static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s, bool po) {
+// One frame's set-up and the workgroups it takes (0: nothing to do).
+static BwdPrep bwd_prep_of(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, bool po,
+                           unsigned* grid) {
+  BwdPrep p;
   // deterministic mode writes every record in k_det_gather; pairs past a
   // tile's last contributor are never visited, so their partials start at 0
-  float* zp = f->deterministic ? r.part : r.g_render;
-  const long long zn = f->deterministic ? capacity * (po ? 5 : 10) : (long long)P * kRG;
+  p.zp = f->deterministic ? r.part : r.g_render;
+  p.zn = f->deterministic ? capacity * (po ? 5 : 10) : (long long)P * kRG;
   // the work-item table, from the forward's item log (both live in the
   // binning workspace)
-  uint32_t* items = capacity > 0 && P > 0 ? (uint32_t*)r.items : nullptr;
-  const long long nzb = (zn + 1023) / 1024;
-  const unsigned grid = (unsigned)((nzb < 2048 ? nzb : 2048) + (items ? 1 : 0));
-  if (grid > 0) hipLaunchKernelGGL(k_bwd_prepare, dim3(grid), dim3(1024), 0, s, zp, zn, r.item_log, items);
+  p.item_log = r.item_log;
+  p.items = capacity > 0 && P > 0 ? (uint32_t*)r.items : nullptr;
+  const long long nzb = (p.zn + 1023) / 1024;
+  *grid = (unsigned)((nzb < 2048 ? nzb : 2048) + (p.items ? 1 : 0));
+  return p;
+}
+
+static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s, bool po) {
+  unsigned grid = 0;
+  const BwdPrep p = bwd_prep_of(f, P, capacity, r, po, &grid);
+  if (grid > 0) hipLaunchKernelGGL(k_bwd_prepare, dim3(grid), dim3(1024), 0, s, p.zp, p.zn, p.item_log, p.items);
 }
 
 // Phase 2: the render backward of one frame.
@@ -1458,32 +1496,35 @@ extern "C" int gs_multi_render_bwd(int n, const void* dev_set, unsigned total_bl
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
-// ---- many frames' pose-only finish in two launches --------------------------
+// ---- the pose-only backward set-up and finish of many frames ---------------
 // Set layout: uint32 prefix[n + 1] of per-Gaussian workgroup bounds per frame,
-// then PBArgs[n] and PoseFin[n], each at a 256-byte boundary.
-static size_t multi_fin_args_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
-static size_t multi_fin_pose_offset(int n) {
-  return multi_fin_args_offset(n) + align_up(sizeof(PBArgs) * (size_t)n, 256);
+// uint32 prefix[n + 1] of set-up workgroup bounds, then BwdPrep[n], PBArgs[n]
+// and PoseFin[n], each array at a 256-byte boundary.
+static size_t mpb_prep_prefix_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
+static size_t mpb_prep_offset(int n) { return 2 * mpb_prep_prefix_offset(n); }
+static size_t mpb_args_offset(int n) { return mpb_prep_offset(n) + align_up(sizeof(BwdPrep) * (size_t)n, 256); }
+static size_t mpb_fin_offset(int n) { return mpb_args_offset(n) + align_up(sizeof(PBArgs) * (size_t)n, 256); }
+
+extern "C" size_t gs_multi_pose_bwd_bytes(int n) {
+  return n <= 0 ? 0 : mpb_fin_offset(n) + sizeof(PoseFin) * (size_t)n;
 }
 
-extern "C" size_t gs_multi_finish_bytes(int n) {
-  return n <= 0 ? 0 : multi_fin_pose_offset(n) + sizeof(PoseFin) * (size_t)n;
-}
-
-extern "C" int gs_multi_finish_build(int n, const gs_frame_t* frames, const gs_gaussians_t* gauss,
-                                     const int* const* radii, const void* const* geom, void* const* binning,
-                                     const long long* capacity, const void* const* image,
-                                     const uint32_t* const* counters, const float* const* dL_dcolor,
-                                     const gs_grads_t* out, void* const* scratch, const size_t* scratch_bytes,
-                                     void* host_set, size_t set_bytes, unsigned* total_blocks) {
+extern "C" int gs_multi_pose_bwd_build(int n, const gs_frame_t* frames, const gs_gaussians_t* gauss,
+                                       const int* const* radii, const void* const* geom, void* const* binning,
+                                       const long long* capacity, const void* const* image,
+                                       const uint32_t* const* counters, const float* const* dL_dcolor,
+                                       const gs_grads_t* out, void* const* scratch, const size_t* scratch_bytes,
+                                       void* host_set, size_t set_bytes, unsigned* blocks) {
   if (n <= 0 || !frames || !gauss || !radii || !geom || !binning || !capacity || !image || !counters ||
-      !dL_dcolor || !out || !scratch || !scratch_bytes || !host_set || !total_blocks)
+      !dL_dcolor || !out || !scratch || !scratch_bytes || !host_set || !blocks)
     return GS_E_ARG;
-  if (set_bytes < gs_multi_finish_bytes(n)) return GS_E_WORKSPACE;
+  if (set_bytes < gs_multi_pose_bwd_bytes(n)) return GS_E_WORKSPACE;
   uint32_t* prefix = (uint32_t*)host_set;
-  PBArgs* args = (PBArgs*)((char*)host_set + multi_fin_args_offset(n));
-  PoseFin* fin = (PoseFin*)((char*)host_set + multi_fin_pose_offset(n));
-  unsigned long long total = 0;
+  uint32_t* pprefix = (uint32_t*)((char*)host_set + mpb_prep_prefix_offset(n));
+  BwdPrep* prep = (BwdPrep*)((char*)host_set + mpb_prep_offset(n));
+  PBArgs* args = (PBArgs*)((char*)host_set + mpb_args_offset(n));
+  PoseFin* fin = (PoseFin*)((char*)host_set + mpb_fin_offset(n));
+  unsigned long long total = 0, ptotal = 0;
   for (int i = 0; i < n; ++i) {
     const gs_frame_t* f = &frames[i];
     const gs_gaussians_t* g = &gauss[i];
@@ -1493,11 +1534,15 @@ extern "C" int gs_multi_finish_build(int n, const gs_frame_t* frames, const gs_g
     // only the pose-only backward (no per-Gaussian outputs, no SH rows in LDS)
     if (!pose_only(f, g, &out[i], false)) return GS_E_ARG;
     const int P = g->P;
-    prefix[i] = (uint32_t)total;
-    total += (unsigned long long)((P + 255) / 256);
-    if (total > 0x7fffffffULL) return GS_E_ARG;
     const RBArgs r = make_rb_args(f, P, geom[i], capacity[i], binning[i], image[i], dL_dcolor[i], nullptr, nullptr,
                                   scratch[i]);
+    unsigned pgrid = 0;
+    prep[i] = bwd_prep_of(f, P, capacity[i], r, true, &pgrid);
+    pprefix[i] = (uint32_t)ptotal;
+    ptotal += pgrid;
+    prefix[i] = (uint32_t)total;
+    total += (unsigned long long)((P + 255) / 256);
+    if (total > 0x7fffffffULL || ptotal > 0x7fffffffULL) return GS_E_ARG;
     args[i] = make_pb_args(f, g, radii[i], geom[i], capacity[i], counters[i], &out[i], r, true, nullptr);
     fin[i].pose7 = f->pose;
     fin[i].part = args[i].pose_part;
@@ -1505,22 +1550,32 @@ extern "C" int gs_multi_finish_build(int n, const gs_frame_t* frames, const gs_g
     fin[i].nparts = (int)pose_parts(P);
   }
   prefix[n] = (uint32_t)total;
-  *total_blocks = (unsigned)total;
+  pprefix[n] = (uint32_t)ptotal;
+  blocks[0] = (unsigned)ptotal;
+  blocks[1] = (unsigned)total;
   return GS_OK;
 }
 
-extern "C" int gs_multi_finish(int n, const void* dev_set, unsigned total_blocks, void* stream) {
-  if (n <= 0 || !dev_set) return GS_E_ARG;
+extern "C" int gs_multi_pose_bwd_prepare(int n, const void* dev_set, const unsigned* blocks, void* stream) {
+  if (n <= 0 || !dev_set || !blocks) return GS_E_ARG;
+  if (blocks[0] > 0)
+    hipLaunchKernelGGL(k_bwd_prepare_multi, dim3(blocks[0]), dim3(1024), 0, (hipStream_t)stream,
+                       (const uint32_t*)((const char*)dev_set + mpb_prep_prefix_offset(n)),
+                       (const BwdPrep*)((const char*)dev_set + mpb_prep_offset(n)), n);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_multi_pose_bwd_finish(int n, const void* dev_set, const unsigned* blocks, void* stream) {
+  if (n <= 0 || !dev_set || !blocks) return GS_E_ARG;
   hipStream_t s = (hipStream_t)stream;
-  const uint32_t* prefix = (const uint32_t*)dev_set;
-  const PBArgs* args = (const PBArgs*)((const char*)dev_set + multi_fin_args_offset(n));
-  const PoseFin* fin = (const PoseFin*)((const char*)dev_set + multi_fin_pose_offset(n));
   {
     timing::Scope ts(timing::kPreprocessBwd, s);
-    if (total_blocks > 0)
-      hipLaunchKernelGGL(k_preprocess_bwd_multi, dim3(total_blocks), dim3(256), 0, s, prefix, args, n);
+    if (blocks[1] > 0)
+      hipLaunchKernelGGL(k_preprocess_bwd_multi, dim3(blocks[1]), dim3(256), 0, s, (const uint32_t*)dev_set,
+                         (const PBArgs*)((const char*)dev_set + mpb_args_offset(n)), n);
   }
-  hipLaunchKernelGGL(k_pose_finalize_multi, dim3(n), dim3(256), 0, s, fin);
+  hipLaunchKernelGGL(k_pose_finalize_multi, dim3(n), dim3(256), 0, s,
+                     (const PoseFin*)((const char*)dev_set + mpb_fin_offset(n)));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 

@@ -382,7 +382,8 @@ def schedule_candidates(binnings: Sequence[int], streams: Sequence[int], groups:
 class FrameSets:
     """Device descriptions of many solves' workspaces for the one-launch blend
     and render backward (gs_multi_blend / gs_multi_render_bwd) and, for
-    pose-only frames, the two-launch finish (gs_multi_finish)."""
+    pose-only frames, the backward's one-launch set-up and two-launch finish
+    (gs_multi_pose_bwd_*)."""
 
     def __init__(self, slots: Sequence[_Slot], device):
         lib = _lib.load()
@@ -417,23 +418,22 @@ class FrameSets:
         self.rbwd_flags = int(slots[0].frame.deterministic) | _lib.GS_RB_COLOR_ONLY
         po = [lib.gs_backward_is_pose_only(ctypes.byref(s.frame), ctypes.byref(s.gauss), ctypes.byref(s.grads), 0)
               for s in slots]
-        self.finish = None
+        self.pose_bwd = None
         if all(po):
             self.rbwd_flags |= _lib.GS_RB_POSE_ONLY
-            # and the pose-only finish (per-Gaussian pose partials, pose
-            # reduction) of all frames in two launches
-            nf = int(lib.gs_multi_finish_bytes(n))
+            # and the pose-only set-up (one launch) and finish (per-Gaussian
+            # pose partials, pose reduction: two launches) of all frames
+            nf = int(lib.gs_multi_pose_bwd_bytes(n))
             host = (ctypes.c_uint8 * nf)()
-            fb = ctypes.c_uint(0)
+            self.pose_bwd_blocks = (ctypes.c_uint * 2)()
             gauss = (_lib.GsGaussians * n)(*[s.gauss for s in slots])
             grads = (_lib.GsGrads * n)(*[s.grads for s in slots])
             sbytes = (ctypes.c_size_t * n)(*[s.scratch.numel() for s in slots])
-            _lib.check(lib.gs_multi_finish_build(n, frames, gauss, ptrs("radii"), ptrs("geom"), ptrs("binning"), cap,
-                                                 ptrs("image"), ptrs("counters"), ptrs("dcolor"), grads,
-                                                 ptrs("scratch"), sbytes, host, nf, ctypes.byref(fb)),
-                       "gs_multi_finish_build")
-            self.finish = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
-            self.finish_blocks = fb.value
+            _lib.check(lib.gs_multi_pose_bwd_build(n, frames, gauss, ptrs("radii"), ptrs("geom"), ptrs("binning"),
+                                                   cap, ptrs("image"), ptrs("counters"), ptrs("dcolor"), grads,
+                                                   ptrs("scratch"), sbytes, host, nf, self.pose_bwd_blocks),
+                       "gs_multi_pose_bwd_build")
+            self.pose_bwd = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
         elif any(po):
             raise RuntimeError("a render-backward set mixes pose-only and full frames")
 
@@ -446,9 +446,13 @@ class FrameSets:
                                                    self.rbwd_blocks, self.rbwd_flags, stream),
                    "gs_multi_render_bwd")
 
-    def finish_launch(self, stream: ctypes.c_void_p) -> None:
-        _lib.check(_lib.load().gs_multi_finish(self.n, ctypes.c_void_p(self.finish.data_ptr()), self.finish_blocks,
-                                               stream), "gs_multi_finish")
+    def pose_bwd_prepare_launch(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_pose_bwd_prepare(self.n, ctypes.c_void_p(self.pose_bwd.data_ptr()),
+                                                         self.pose_bwd_blocks, stream), "gs_multi_pose_bwd_prepare")
+
+    def pose_bwd_finish_launch(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_pose_bwd_finish(self.n, ctypes.c_void_p(self.pose_bwd.data_ptr()),
+                                                        self.pose_bwd_blocks, stream), "gs_multi_pose_bwd_finish")
 
 
 class BatchedPoseSolver:
@@ -574,8 +578,9 @@ class BatchedPoseSolver:
             return
         # per group: per frame preprocess + binning; one blend launch over the
         # group; per frame loss and backward set-up; one render backward
-        # launch; the per-Gaussian / pose gradients (one launch each for
-        # pose-only frames, else per frame).  One group forks its per-frame
+        # launch; the per-Gaussian / pose gradients (for pose-only frames the
+        # backward's set-up and these are one launch per kernel, else per
+        # frame).  One group forks its per-frame
         # work from the current stream onto all streams and runs its
         # one-launch stages on the current stream; with more groups each
         # group runs everything on a stream of its own, forked from the
@@ -586,8 +591,11 @@ class BatchedPoseSolver:
         lib = _lib.load()
         cur = torch.cuda.current_stream(self.device)
 
-        def loss_and_prepare(i, s, st):
+        def loss(i, s, st):
             launch_slot_loss(s, self.out3.data_ptr() + 12 * i, self.one.data_ptr(), self.lam, st)
+
+        def loss_and_prepare(i, s, st):
+            loss(i, s, st)
             _lib.check(lib.gs_backward_prepare(*_bwd_args(s), st), "gs_backward_prepare")
 
         def prepare(i, s, st):
@@ -615,14 +623,18 @@ class BatchedPoseSolver:
             else:
                 self._per_slot(lambda i, s, st: launch_slot_binning(s, st), members, lead)
             sets.blend_launch(lp)
+            pb = sets.pose_bwd is not None
             if self.losssets is not None:
                 self.losssets[g].launch(lp)
-                self._per_slot(prepare, members, lead)
+                if not pb:
+                    self._per_slot(prepare, members, lead)
             else:
-                self._per_slot(loss_and_prepare, members, lead)
+                self._per_slot(loss if pb else loss_and_prepare, members, lead)
+            if pb:
+                sets.pose_bwd_prepare_launch(lp)
             sets.render_bwd_launch(lp)
-            if sets.finish is not None:
-                sets.finish_launch(lp)
+            if pb:
+                sets.pose_bwd_finish_launch(lp)
             else:
                 self._per_slot(finish, members, lead)
         if self.ngroups > 1:

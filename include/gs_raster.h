@@ -337,22 +337,26 @@ int gs_multi_render_bwd_build(int n, const gs_frame_t* frames, const int* P, con
 int gs_multi_render_bwd(int n, const void* dev_set, unsigned total_blocks, int flags, void* stream);
 int gs_backward_is_pose_only(const gs_frame_t* f, const gs_gaussians_t* g, const gs_grads_t* out,
                              int has_depth_or_alpha);
-/* The finish step of many pose-only frames in two launches: each frame's
- * per-Gaussian pose partials (one launch over all frames' Gaussians), then
- * each frame's pose reduction into out[i].dL_dpose (one workgroup per frame),
- * with the same bits as gs_backward_finish of each frame.  The build takes
- * the per-frame arguments of gs_backward_finish as arrays (no depth or alpha
- * cotangent), checks them as it does, and refuses (GS_E_ARG) a frame for which
- * gs_backward_is_pose_only(frame, g, out, 0) does not hold; total_blocks is
- * returned by the build.  The backward of the pose-fitting loop,
+/* The set-up and finish steps of many pose-only frames, one launch for the
+ * set-up and two for the finish: gs_multi_pose_bwd_prepare is
+ * gs_backward_prepare of every frame; gs_multi_pose_bwd_finish computes every
+ * frame's per-Gaussian pose partials (one launch over all frames' Gaussians),
+ * then each frame's pose reduction into out[i].dL_dpose (one workgroup per
+ * frame).  The results are those of the per-frame calls, bit for bit.  The
+ * build takes the per-frame arguments of gs_backward_prepare / _finish as
+ * arrays (no depth or alpha cotangent), checks them as those do, and refuses
+ * (GS_E_ARG) a frame for which gs_backward_is_pose_only(frame, g, out, 0) does
+ * not hold; it returns blocks[2] (set-up and per-Gaussian workgroups), which
+ * the two launches take back.  The backward of the pose-fitting loop,
  * trainer/cf3dgs_trainer.py:261-285, for many frames at once. */
-size_t gs_multi_finish_bytes(int n);
-int gs_multi_finish_build(int n, const gs_frame_t* frames, const gs_gaussians_t* g, const int* const* radii,
-                          const void* const* geom, void* const* binning, const long long* capacity,
-                          const void* const* image, const uint32_t* const* counters,
-                          const float* const* dL_dcolor, const gs_grads_t* out, void* const* scratch,
-                          const size_t* scratch_bytes, void* host_set, size_t set_bytes, unsigned* total_blocks);
-int gs_multi_finish(int n, const void* dev_set, unsigned total_blocks, void* stream);
+size_t gs_multi_pose_bwd_bytes(int n);
+int gs_multi_pose_bwd_build(int n, const gs_frame_t* frames, const gs_gaussians_t* g, const int* const* radii,
+                            const void* const* geom, void* const* binning, const long long* capacity,
+                            const void* const* image, const uint32_t* const* counters,
+                            const float* const* dL_dcolor, const gs_grads_t* out, void* const* scratch,
+                            const size_t* scratch_bytes, void* host_set, size_t set_bytes, unsigned* blocks);
+int gs_multi_pose_bwd_prepare(int n, const void* dev_set, const unsigned* blocks, void* stream);
+int gs_multi_pose_bwd_finish(int n, const void* dev_set, const unsigned* blocks, void* stream);
 
 /* Near-plane visibility (replaces diff_gaussian_rasterization._C.mark_visible,
  * GaussianRasterizer.markVisible). visible[P] = 1 if the view-space z > 0.2. */

@@ -384,17 +384,18 @@ def test_shared_stage_entry_points_refuse_bad_arguments_without_device():
     assert blocks.value == 3 * ((160 + 31) // 32) * ((90 + 15) // 16)  # 32 x 16-pixel tiles per channel
 
 
-def test_multi_finish_set_on_the_host():
-    """gs_multi_finish_build lays out one workgroup range per frame (a
-    256-Gaussian block each) and refuses what gs_backward_finish refuses, plus
-    any frame that does not take the pose-only backward.  Built on the host
-    only: the device addresses below are never read."""
+def test_multi_pose_bwd_set_on_the_host():
+    """gs_multi_pose_bwd_build lays out, per frame, the workgroups of its own
+    backward set-up and one per 256 Gaussians for the finish, and refuses what
+    gs_backward_prepare / _finish refuse, plus any frame that does not take
+    the pose-only backward.  Built on the host only: the device addresses
+    below are never read."""
     import ctypes
     from gs_hip import _lib
     lib = _lib.load()
     n = 3
     Ps = [1000, 256, 0]
-    cap = 4000
+    caps_l = [4000, 3_000_000, 0]
     frames = (_lib.GsFrame * n)()
     gauss = (_lib.GsGaussians * n)()
     grads = (_lib.GsGrads * n)()
@@ -409,22 +410,28 @@ def test_multi_finish_set_on_the_host():
         g.scales, g.rotations = 0x8000, 0x9000
         grads[i].dL_dpose = 0xA000
     vp = lambda: (ctypes.c_void_p * n)(*([0x10000000] * n))  # noqa: E731
-    caps = (ctypes.c_longlong * n)(*([cap] * n))
-    sb = [int(lib.gs_backward_scratch_bytes(P, cap, 1)) for P in Ps]
+    caps = (ctypes.c_longlong * n)(*caps_l)
+    sb = [int(lib.gs_backward_scratch_bytes(P, c, 1)) for P, c in zip(Ps, caps_l)]
     sbytes = (ctypes.c_size_t * n)(*sb)
-    nb = int(lib.gs_multi_finish_bytes(n))
-    assert lib.gs_multi_finish_bytes(0) == 0 and nb > 0
+    nb = int(lib.gs_multi_pose_bwd_bytes(n))
+    assert lib.gs_multi_pose_bwd_bytes(0) == 0 and nb > 0
     host = (ctypes.c_uint8 * nb)()
-    blocks = ctypes.c_uint(0)
+    blocks = (ctypes.c_uint * 2)()
 
     def build(nbytes=nb):
-        return lib.gs_multi_finish_build(n, frames, gauss, vp(), vp(), vp(), caps, vp(), vp(), vp(), grads, vp(),
-                                         sbytes, host, nbytes, ctypes.byref(blocks))
+        return lib.gs_multi_pose_bwd_build(n, frames, gauss, vp(), vp(), vp(), caps, vp(), vp(), vp(), grads, vp(),
+                                           sbytes, host, nbytes, blocks)
 
     assert build() == 0
-    assert blocks.value == 4 + 1 + 0
+    # set-up: the item-table workgroup plus ceil(5 capacity / 1024) clearing
+    # ones, at most 2048; none for a frame with no pairs
+    prep = [1 + 20, 1 + 2048, 0]
+    assert list(blocks) == [sum(prep), 4 + 1 + 0]
     prefix = list((ctypes.c_uint32 * (n + 1)).from_buffer(host))
     assert prefix == [0, 4, 5, 5]
+    off = -(-4 * (n + 1) // 256) * 256
+    pprefix = list((ctypes.c_uint32 * (n + 1)).from_buffer(host, off))
+    assert pprefix == [0, prep[0], prep[0] + prep[1], sum(prep)]
     assert build(nb - 1) == 2
     # a per-Gaussian output: the full backward, not this set's
     grads[1].dL_dmeans3D = 0xB000
@@ -435,9 +442,12 @@ def test_multi_finish_set_on_the_host():
     assert build() == 1
     frames[0].deterministic = 1
     # scratch smaller than gs_backward_scratch_bytes
-    sbytes[2] = sb[2] - 1
+    sbytes[1] = sb[1] - 1
     assert build() == 2
-    sbytes[2] = sb[2]
+    sbytes[1] = sb[1]
     assert build() == 0
-    assert lib.gs_multi_finish(0, ctypes.c_void_p(0x10000000), 1, None) == 1
-    assert lib.gs_multi_finish(1, None, 1, None) == 1
+    dev = ctypes.c_void_p(0x10000000)
+    for fn in (lib.gs_multi_pose_bwd_prepare, lib.gs_multi_pose_bwd_finish):
+        assert fn(0, dev, blocks, None) == 1
+        assert fn(1, None, blocks, None) == 1
+        assert fn(1, dev, None, None) == 1
