@@ -53,11 +53,12 @@ __device__ __forceinline__ uint32_t block_incl_scan_256(uint32_t v, uint32_t* s_
   return block_scan_incl<256>(v, s_tmp, total);
 }
 
-__global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
-  extern __shared__ float s_sh[];  // [256][prow] staged SH coefficients (3 nc used per row)
+// Workgroup bid of the nblocks of one frame's preprocess.  s_sh: [256][prow]
+// staged SH coefficients (3 nc used per row).
+__device__ __forceinline__ void preprocess_block(const PreArgs& a, int bid, int nblocks, float* s_sh) {
   __shared__ uint32_t s_scan[256];
   __shared__ uint32_t s_flag;
-  const int base = blockIdx.x * 256;
+  const int base = bid * 256;
   const int i = base + threadIdx.x;
   const int nc = (a.D + 1) * (a.D + 1);
   const int stride = 3 * nc;
@@ -210,14 +211,41 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
   const uint32_t incl = block_incl_scan_256(my_tiles, s_scan, total);
   if (i < a.P) a.local_incl[i] = incl;
   if (threadIdx.x == 0) {
-    a.block_sum[blockIdx.x] = total;
-    a.block_sum[gridDim.x + blockIdx.x] = s_flag;
+    a.block_sum[bid] = total;
+    a.block_sum[nblocks + bid] = s_flag;
   }
+}
+
+__global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
+  extern __shared__ float s_sh[];
+  preprocess_block(a, blockIdx.x, gridDim.x, s_sh);
+}
+
+// The block of many frames' launch that frame f owns: the last f with
+// prefix[f] <= b (frames owning no block share their start with the next).
+__device__ __forceinline__ int frame_of_block(const uint32_t* prefix, int n, uint32_t b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Many frames' preprocess in one launch: frame f owns workgroups [prefix[f],
+// prefix[f + 1]); the dynamic LDS covers the largest frame's SH rows.
+__global__ __launch_bounds__(256) void k_preprocess_multi(const uint32_t* prefix, const PreArgs* args, int n) {
+  extern __shared__ float s_sh[];
+  const uint32_t b = blockIdx.x;
+  const int f = frame_of_block(prefix, n, b);
+  const PreArgs a = args[f];
+  preprocess_block(a, (int)(b - prefix[f]), (int)(prefix[f + 1] - prefix[f]), s_sh);
 }
 
 // Single workgroup: exclusive prefix of the per-workgroup tile sums (in place),
 // num_rendered and the OR of the status flags.
-__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* block_sum, int nblocks, uint32_t* counters) {
+__device__ __forceinline__ void scan_top_block(uint32_t* block_sum, int nblocks, uint32_t* counters) {
   __shared__ uint32_t s[1024];
   __shared__ uint32_t s_carry, s_flags;
   const int t = threadIdx.x;
@@ -245,6 +273,22 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* block_sum, int nblo
     counters[0] = s_carry;
     counters[1] = s_flags;
   }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* block_sum, int nblocks, uint32_t* counters) {
+  scan_top_block(block_sum, nblocks, counters);
+}
+
+struct ScanTop {
+  uint32_t* block_sum;
+  uint32_t* counters;
+  int nblocks;
+};
+
+// Workgroup f: the scan of frame f.
+__global__ __launch_bounds__(1024) void k_scan_top_multi(const ScanTop* st) {
+  const ScanTop t = st[blockIdx.x];
+  scan_top_block(t.block_sum, t.nblocks, t.counters);
 }
 
 struct DupArgs {
@@ -321,10 +365,10 @@ __global__ __launch_bounds__(256) void k_ranges(const uint64_t* keys, const uint
 // Also sets the n_ranges words of the tile ranges to [UINT_MAX, 0) per tile,
 // the start the list sort's atomic min / max needs (the grid covers
 // max(P, n_ranges) threads).
-__global__ __launch_bounds__(256) void k_df_keys(int P, const uint32_t* tiles, const float4* splat, int gx, int gy,
-                                                 uint32_t* keys, uint32_t* vals, uint2* rect, uint32_t* counters,
-                                                 uint32_t* ranges, int n_ranges) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void df_keys_block(int P, const uint32_t* tiles, const float4* splat, int gx, int gy,
+                                              uint32_t* keys, uint32_t* vals, uint2* rect, uint32_t* counters,
+                                              uint32_t* ranges, int n_ranges, int bid) {
+  const int i = bid * 256 + threadIdx.x;
   if (i == 0) counters[2] = (uint32_t)P;
   if (i < n_ranges) ranges[i] = (i & 1) ? 0u : 0xffffffffu;
   if (i >= P) return;
@@ -338,6 +382,33 @@ __global__ __launch_bounds__(256) void k_df_keys(int P, const uint32_t* tiles, c
     tile_rect(s0.x, s0.y, __float_as_int(s2.w), gx, gy, x0, y0, x1, y1);
     rect[i] = make_uint2((uint32_t)x0 | ((uint32_t)y0 << 16), (uint32_t)x1 | ((uint32_t)y1 << 16));
   }
+}
+
+__global__ __launch_bounds__(256) void k_df_keys(int P, const uint32_t* tiles, const float4* splat, int gx, int gy,
+                                                 uint32_t* keys, uint32_t* vals, uint2* rect, uint32_t* counters,
+                                                 uint32_t* ranges, int n_ranges) {
+  df_keys_block(P, tiles, splat, gx, gy, keys, vals, rect, counters, ranges, n_ranges, blockIdx.x);
+}
+
+struct DfKeysArgs {
+  int P, gx, gy, n_ranges;
+  const uint32_t* tiles;
+  const float4* splat;
+  uint32_t* keys;
+  uint32_t* vals;
+  uint2* rect;
+  uint32_t* counters;
+  uint32_t* ranges;
+};
+
+// Many frames' depth keys in one launch: frame f owns workgroups [prefix[f],
+// prefix[f + 1]).
+__global__ __launch_bounds__(256) void k_df_keys_multi(const uint32_t* prefix, const DfKeysArgs* args, int n) {
+  const uint32_t b = blockIdx.x;
+  const int f = frame_of_block(prefix, n, b);
+  const DfKeysArgs a = args[f];
+  df_keys_block(a.P, a.tiles, a.splat, a.gx, a.gy, a.keys, a.vals, a.rect, a.counters, a.ranges, a.n_ranges,
+                (int)(b - prefix[f]));
 }
 
 // Workgroup-local inclusive prefix of the tile counts in depth order.
@@ -843,8 +914,9 @@ static int check_frame(const gs_frame_t* f) {
   return GS_OK;
 }
 
-extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* radii, void* geom,
-                             size_t geom_bytes, uint32_t* counters, void* stream) {
+// The checked preprocess arguments of one frame.
+static int make_pre_args(const gs_frame_t* f, const gs_gaussians_t* g, int* radii, void* geom, size_t geom_bytes,
+                         const uint32_t* counters, PreArgs& a) {
   if (check_frame(f) != GS_OK || !g || g->P < 0 || !counters) return GS_E_ARG;
   const int P = g->P;
   if (P > 0 && (!g->means3D || !g->opacities || !radii || !geom)) return GS_E_ARG;
@@ -857,9 +929,7 @@ extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* 
   gs_geom_layout_t L;
   gs_geom_layout(P, &L);
   if (geom_bytes < L.total) return GS_E_WORKSPACE;
-  hipStream_t s = (hipStream_t)stream;
   char* base = (char*)geom;
-  PreArgs a;
   a.P = P;
   a.M = g->M;
   a.D = f->sh_degree;
@@ -889,10 +959,25 @@ extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* 
   a.tiles = (uint32_t*)(base + L.tiles);
   a.local_incl = (uint32_t*)(base + L.offsets);
   a.block_sum = (uint32_t*)(base + L.scan_tmp);
-  const int nb = nblocks256(P);
+  return GS_OK;
+}
+
+// Dynamic LDS of a preprocess launch: the staged SH rows of 256 Gaussians
+// (an odd row stride).
+static size_t preprocess_lds(const PreArgs& a) {
+  const int nc = (a.D + 1) * (a.D + 1);
+  return a.shs ? (size_t)256 * ((3 * nc) | 1) * sizeof(float) : 0;
+}
+
+extern "C" int gs_preprocess(const gs_frame_t* f, const gs_gaussians_t* g, int* radii, void* geom,
+                             size_t geom_bytes, uint32_t* counters, void* stream) {
+  PreArgs a;
+  const int rc = make_pre_args(f, g, radii, geom, geom_bytes, counters, a);
+  if (rc != GS_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = nblocks256(a.P);
   if (nb > 0) {
-    const int nc = (a.D + 1) * (a.D + 1);
-    const size_t shm = g->shs ? (size_t)256 * ((3 * nc) | 1) * sizeof(float) : 0;
+    const size_t shm = preprocess_lds(a);
     timing::Scope ts(timing::kPreprocess, s);
     hipLaunchKernelGGL(k_preprocess, dim3(nb), dim3(256), shm, s, a);
   }
@@ -1133,6 +1218,110 @@ extern "C" int gs_render_depth_keys(const gs_frame_t* f, int P, const void* geom
                                     size_t image_bytes, void* stream) {
   return render_binning(f, P, geom, counters, capacity, binning, binning_bytes, image, image_bytes,
                         (hipStream_t)stream, kKeys);
+}
+
+// ---- many frames' preprocess and depth keys in three launches --------------
+// Set layout: uint32 prefix[n + 1] of preprocess workgroup bounds per frame,
+// uint32 prefix[n + 1] of depth-key workgroup bounds, then PreArgs[n],
+// ScanTop[n] and DfKeysArgs[n], each array at a 256-byte boundary.
+static size_t mpk_prefix2_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
+static size_t mpk_pre_offset(int n) { return 2 * mpk_prefix2_offset(n); }
+static size_t mpk_top_offset(int n) { return mpk_pre_offset(n) + align_up(sizeof(PreArgs) * (size_t)n, 256); }
+static size_t mpk_keys_offset(int n) { return mpk_top_offset(n) + align_up(sizeof(ScanTop) * (size_t)n, 256); }
+
+extern "C" size_t gs_multi_preprocess_keys_bytes(int n) {
+  return n <= 0 ? 0 : mpk_keys_offset(n) + sizeof(DfKeysArgs) * (size_t)n;
+}
+
+extern "C" int gs_multi_preprocess_keys_build(int n, const gs_frame_t* frames, const gs_gaussians_t* gauss,
+                                              int* const* radii, void* const* geom, const size_t* geom_bytes,
+                                              uint32_t* const* counters, const long long* capacity,
+                                              void* const* binning, const size_t* binning_bytes,
+                                              void* const* image, const size_t* image_bytes, void* host_set,
+                                              size_t set_bytes, unsigned* launch) {
+  if (n <= 0 || !frames || !gauss || !radii || !geom || !geom_bytes || !counters || !capacity || !binning ||
+      !binning_bytes || !image || !image_bytes || !host_set || !launch)
+    return GS_E_ARG;
+  if (set_bytes < gs_multi_preprocess_keys_bytes(n)) return GS_E_WORKSPACE;
+  uint32_t* pprefix = (uint32_t*)host_set;
+  uint32_t* kprefix = (uint32_t*)((char*)host_set + mpk_prefix2_offset(n));
+  PreArgs* pre = (PreArgs*)((char*)host_set + mpk_pre_offset(n));
+  ScanTop* top = (ScanTop*)((char*)host_set + mpk_top_offset(n));
+  DfKeysArgs* keys = (DfKeysArgs*)((char*)host_set + mpk_keys_offset(n));
+  unsigned long long ptotal = 0, ktotal = 0;
+  size_t shm = 0;
+  for (int i = 0; i < n; ++i) {
+    const gs_frame_t* f = &frames[i];
+    const int rc = make_pre_args(f, &gauss[i], radii[i], geom[i], geom_bytes[i], counters[i], pre[i]);
+    if (rc != GS_OK) return rc;
+    const int P = gauss[i].P;
+    // the depth keys of the depth-first schedule, as gs_render_depth_keys
+    // takes them
+    if (P <= 0 || f->binning != GS_BINNING_DEPTH_FIRST || capacity[i] <= 0 || capacity[i] > 0x7fffffffLL ||
+        !binning[i] || !image[i])
+      return GS_E_ARG;
+    gs_geom_layout_t LG;
+    gs_binning_layout_t LB;
+    gs_image_layout_t LI;
+    gs_geom_layout(P, &LG);
+    if (gs_binning_layout(capacity[i], P, f->width, f->height, f->binning, &LB) != GS_OK) return GS_E_ARG;
+    gs_image_layout(f->width, f->height, &LI);
+    if (binning_bytes[i] < LB.total || image_bytes[i] < LI.total) return GS_E_WORKSPACE;
+    int gx, gy;
+    const int nt = grid_tiles(f->width, f->height, gx, gy);
+    if (gx > 0xffff || gy > 0xffff) return GS_E_ARG;  // tile rects are packed in 16-bit fields
+    const int nbp = nblocks256(P);
+    pprefix[i] = (uint32_t)ptotal;
+    ptotal += (unsigned long long)nbp;
+    kprefix[i] = (uint32_t)ktotal;
+    ktotal += (unsigned long long)std::max(nbp, nblocks256(2LL * nt));
+    if (ptotal > 0x7fffffffULL || ktotal > 0x7fffffffULL) return GS_E_ARG;
+    shm = std::max(shm, preprocess_lds(pre[i]));
+    top[i].block_sum = pre[i].block_sum;
+    top[i].counters = counters[i];
+    top[i].nblocks = nbp;
+    const char* gb = (const char*)geom[i];
+    char* bb = (char*)binning[i];
+    DfKeysArgs& k = keys[i];
+    k.P = P;
+    k.gx = gx;
+    k.gy = gy;
+    k.n_ranges = 2 * nt;
+    k.tiles = (const uint32_t*)(gb + LG.tiles);
+    k.splat = (const float4*)(gb + LG.splat);
+    k.keys = (uint32_t*)(bb + LB.dkeys_a);
+    k.vals = (uint32_t*)(bb + LB.dvals_a);
+    k.rect = (uint2*)(bb + LB.drect);
+    k.counters = counters[i];
+    k.ranges = (uint32_t*)((char*)image[i] + LI.ranges);
+  }
+  pprefix[n] = (uint32_t)ptotal;
+  kprefix[n] = (uint32_t)ktotal;
+  launch[0] = (unsigned)ptotal;
+  launch[1] = (unsigned)ktotal;
+  launch[2] = (unsigned)shm;
+  return GS_OK;
+}
+
+extern "C" int gs_multi_preprocess_keys(int n, const void* dev_set, const unsigned* launch, void* stream) {
+  if (n <= 0 || !dev_set || !launch) return GS_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const char* d = (const char*)dev_set;
+  if (launch[0] > 0) {
+    timing::Scope ts(timing::kPreprocess, s);
+    hipLaunchKernelGGL(k_preprocess_multi, dim3(launch[0]), dim3(256), (size_t)launch[2], s, (const uint32_t*)d,
+                       (const PreArgs*)(d + mpk_pre_offset(n)), n);
+  }
+  {
+    timing::Scope ts(timing::kScan, s);
+    hipLaunchKernelGGL(k_scan_top_multi, dim3(n), dim3(1024), 0, s, (const ScanTop*)(d + mpk_top_offset(n)));
+  }
+  if (launch[1] > 0) {
+    timing::Scope ts(timing::kSort, s);
+    hipLaunchKernelGGL(k_df_keys_multi, dim3(launch[1]), dim3(256), 0, s,
+                       (const uint32_t*)(d + mpk_prefix2_offset(n)), (const DfKeysArgs*)(d + mpk_keys_offset(n)), n);
+  }
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
 extern "C" int gs_render_binning_after_depth_sort(const gs_frame_t* f, int P, const void* geom, uint32_t* counters,

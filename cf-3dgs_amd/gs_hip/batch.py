@@ -302,6 +302,38 @@ class DepthSortSet:
                    "gs_multi_depth_sort")
 
 
+class KeysSet:
+    """The preprocess and depth keys of many solves in three launches
+    (gs_multi_preprocess_keys) instead of three per solve."""
+
+    def __init__(self, slots: Sequence[_Slot], device):
+        lib = _lib.load()
+        n = len(slots)
+        self.n = n
+        vp = ctypes.c_void_p
+        sz = ctypes.c_size_t
+
+        def ptrs(attr):
+            return (vp * n)(*[getattr(s, attr).data_ptr() for s in slots])
+
+        frames = (_lib.GsFrame * n)(*[s.frame for s in slots])
+        gauss = (_lib.GsGaussians * n)(*[s.gauss for s in slots])
+        cap = (ctypes.c_longlong * n)(*[s.capacity for s in slots])
+        nb = int(lib.gs_multi_preprocess_keys_bytes(n))
+        host = (ctypes.c_uint8 * nb)()
+        self.launch_dims = (ctypes.c_uint * 3)()
+        _lib.check(lib.gs_multi_preprocess_keys_build(
+            n, frames, gauss, ptrs("radii"), ptrs("geom"), (sz * n)(*[s.geom.numel() for s in slots]),
+            ptrs("counters"), cap, ptrs("binning"), (sz * n)(*[s.binning.numel() for s in slots]), ptrs("image"),
+            (sz * n)(*[s.image.numel() for s in slots]), host, nb, self.launch_dims),
+            "gs_multi_preprocess_keys_build")
+        self.dev = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+
+    def launch(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_preprocess_keys(self.n, ctypes.c_void_p(self.dev.data_ptr()),
+                                                        self.launch_dims, stream), "gs_multi_preprocess_keys")
+
+
 def launch_slot_loss(s: _Slot, out3_ptr: int, one_ptr: int, lam: float, stream: ctypes.c_void_p) -> None:
     """Fused loss forward (stats to ``out3_ptr``) and backward (d loss scaled
     by the float at ``one_ptr``) of one solve's rendered image."""
@@ -506,9 +538,9 @@ class BatchedPoseSolver:
             raise ValueError("the shared depth sort needs the depth-first binning")
         self.stream_members = {q: [i for i in range(self.n) if self.stream_of[i] == q]
                                for q in sorted(set(self.stream_of))}
-        self.dsets = self.lsets = self.losssets = None
+        self.dsets = self.lsets = self.ksets = self.losssets = None
         if self.shared_sort:
-            self.dsets, self.lsets = self._sort_sets()
+            self.dsets, self.lsets, self.ksets = self._sort_sets()
             self.losssets = self._loss_sets() if self.multi else None
 
     def _frame_sets(self) -> List[FrameSets]:
@@ -537,7 +569,8 @@ class BatchedPoseSolver:
             lsets = {k: ListSortSet([self.slots[i] for i in mem], self.device) for k, mem in zip(keys, parts)}
         except _lib.GsError:
             lsets = None
-        return dsets, lsets
+        ksets = {k: KeysSet([self.slots[i] for i in mem], self.device) for k, mem in zip(keys, parts)}
+        return dsets, lsets, ksets
 
     def _make_slot(self, inst: PoseInstance, capacity: int, i: int) -> _Slot:
         return make_slot(inst, capacity, self.binning, _vp(self.adam.pose, 28 * i), _vp(self.adam.grad, 28 * i))
@@ -612,7 +645,7 @@ class BatchedPoseSolver:
             lead = cur if self.ngroups == 1 else self.streams[sidx[0]]
             lp = ctypes.c_void_p(lead.cuda_stream)
             if self.shared_sort:
-                self._per_slot(lambda i, s, st: launch_slot_keys(s, st), members, lead)
+                self.ksets[g].launch(lp)
                 self.dsets[g].launch(lp)
                 if self.lsets is not None:
                     self._per_slot(lambda i, s, st: launch_slot_dup(s, st), members, lead)
@@ -655,8 +688,7 @@ class BatchedPoseSolver:
             st = self.streams[q]
             sp = ctypes.c_void_p(st.cuda_stream)
             with torch.cuda.stream(st):
-                for i in mem:
-                    launch_slot_keys(self.slots[i], sp)
+                self.ksets[q].launch(sp)
                 self.dsets[q].launch(sp)
                 if self.lsets is not None:
                     for i in mem:
@@ -729,7 +761,7 @@ class BatchedPoseSolver:
             if self.multi:
                 self.sets = self._frame_sets()
             if self.shared_sort:
-                self.dsets, self.lsets = self._sort_sets()
+                self.dsets, self.lsets, self.ksets = self._sort_sets()
                 self.losssets = self._loss_sets() if self.multi else None
         raise RuntimeError(f"binning capacity still exceeded after {max_regrow} regrowths")
 

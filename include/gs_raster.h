@@ -290,6 +290,23 @@ int gs_multi_depth_sort_build(int n, const gs_frame_t* frames, const int* P, con
                               void* const* binning, const size_t* binning_bytes, uint32_t* const* counters,
                               void* host_set, size_t set_bytes, unsigned* blocks);
 int gs_multi_depth_sort(int n, const void* dev_set, unsigned blocks, void* stream);
+/* gs_preprocess followed by gs_render_depth_keys for many frames, in three
+ * launches (every frame's preprocess; every frame's tile-count scan, one
+ * workgroup each; every frame's depth keys), with the results of the
+ * per-frame calls, bit for bit.  The build takes the per-frame arguments of
+ * the two calls as arrays, checks them as those do (frames with P > 0 on
+ * GS_BINNING_DEPTH_FIRST and a positive capacity, as gs_render_depth_keys
+ * requires) and returns launch[3]: the two per-Gaussian launches' workgroup
+ * counts and the preprocess launch's LDS bytes, which gs_multi_preprocess_keys
+ * takes back.  Replaces, for many frames at once, the per-frame preprocess of
+ * forward phase 1 and the key step of phase 2 above. */
+size_t gs_multi_preprocess_keys_bytes(int n);
+int gs_multi_preprocess_keys_build(int n, const gs_frame_t* frames, const gs_gaussians_t* g, int* const* radii,
+                                   void* const* geom, const size_t* geom_bytes, uint32_t* const* counters,
+                                   const long long* capacity, void* const* binning, const size_t* binning_bytes,
+                                   void* const* image, const size_t* image_bytes, void* host_set, size_t set_bytes,
+                                   unsigned* launch);
+int gs_multi_preprocess_keys(int n, const void* dev_set, const unsigned* launch, void* stream);
 int gs_render_binning_after_depth_sort(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters,
                                        long long capacity, void* binning, size_t binning_bytes, void* image,
                                        size_t image_bytes, void* stream);

@@ -451,3 +451,71 @@ def test_multi_pose_bwd_set_on_the_host():
         assert fn(0, dev, blocks, None) == 1
         assert fn(1, None, blocks, None) == 1
         assert fn(1, dev, None, None) == 1
+
+
+def test_multi_preprocess_keys_set_on_the_host():
+    """gs_multi_preprocess_keys_build lays out each frame's preprocess
+    workgroups (one per 256 Gaussians) and depth-key workgroups (enough for
+    the Gaussians and for the tile ranges it clears), and refuses what
+    gs_preprocess / gs_render_depth_keys refuse.  Built on the host only: the
+    device addresses below are never read."""
+    import ctypes
+    from gs_hip import _lib
+    lib = _lib.load()
+    n = 2
+    dims = [(160, 90, 1000), (960, 540, 300)]  # 1000 Gaussians; 300 Gaussians over 60 x 34 tiles
+    frames = (_lib.GsFrame * n)()
+    gauss = (_lib.GsGaussians * n)()
+    for i, (W, H, P) in enumerate(dims):
+        f = frames[i]
+        f.width, f.height, f.sh_degree = W, H, 0
+        f.viewmatrix, f.projmatrix, f.bg = 0x1000, 0x2000, 0x3000
+        f.binning = _lib.BINNING_DEPTH_FIRST
+        g = gauss[i]
+        g.P, g.M = P, 1
+        g.means3D, g.colors_precomp, g.opacities = 0x5000, 0x6000, 0x7000
+        g.scales, g.rotations = 0x8000, 0x9000
+    cap = 50000
+    vp = lambda: (ctypes.c_void_p * n)(*([0x10000000] * n))  # noqa: E731
+    caps = (ctypes.c_longlong * n)(*([cap] * n))
+    sz = ctypes.c_size_t
+    gb = (sz * n)(*[_lib.geom_layout(P).total for _, _, P in dims])
+    bb = (sz * n)(*[_lib.binning_layout(cap, P, W, H, _lib.BINNING_DEPTH_FIRST).total for W, H, P in dims])
+    ib = (sz * n)(*[_lib.image_layout(W, H).total for W, H, _ in dims])
+    nb = int(lib.gs_multi_preprocess_keys_bytes(n))
+    assert lib.gs_multi_preprocess_keys_bytes(0) == 0 and nb > 0
+    host = (ctypes.c_uint8 * nb)()
+    launch = (ctypes.c_uint * 3)()
+
+    def build(nbytes=nb):
+        return lib.gs_multi_preprocess_keys_build(n, frames, gauss, vp(), vp(), gb, vp(), caps, vp(), bb, vp(), ib,
+                                                  host, nbytes, launch)
+
+    assert build() == 0
+    # preprocess: 4 + 2 workgroups; keys: 4, and 16 for the 2 x 2040 range words
+    # of the larger frame; colours precomputed: no SH rows in LDS
+    assert list(launch) == [6, 4 + 16, 0]
+    pprefix = list((ctypes.c_uint32 * (n + 1)).from_buffer(host))
+    kprefix = list((ctypes.c_uint32 * (n + 1)).from_buffer(host, 256))
+    assert pprefix == [0, 4, 6] and kprefix == [0, 4, 20]
+    assert build(nb - 1) == 2
+    # SH coefficients: LDS rows of (3 * 16) | 1 floats for degree 3
+    gauss[0].colors_precomp, gauss[0].shs, gauss[0].M = None, 0xA000, 16
+    frames[0].sh_degree, frames[0].campos = 3, 0xB000
+    assert build() == 0 and launch[2] == 256 * 49 * 4
+    frames[0].sh_degree = 4  # beyond degree 3
+    assert build() == 1
+    frames[0].sh_degree = 3
+    # the 64-bit-key schedule has no depth keys
+    frames[1].binning = _lib.BINNING_SORT64
+    assert build() == 1
+    frames[1].binning = _lib.BINNING_DEPTH_FIRST
+    # a binning workspace smaller than its layout
+    bb[1] -= 1
+    assert build() == 2
+    bb[1] += 1
+    assert build() == 0
+    dev = ctypes.c_void_p(0x10000000)
+    assert lib.gs_multi_preprocess_keys(0, dev, launch, None) == 1
+    assert lib.gs_multi_preprocess_keys(1, None, launch, None) == 1
+    assert lib.gs_multi_preprocess_keys(1, dev, None, None) == 1

@@ -16,8 +16,8 @@ import torch
 
 from gs_hip import _lib
 from gs_hip import raster as _raster
-from gs_hip.batch import (DepthSortSet, ListSortSet, launch_slot_after_sort, launch_slot_dup, launch_slot_keys,
-                          launch_slot_tail, make_slot, start_capacities)
+from gs_hip.batch import (DepthSortSet, KeysSet, ListSortSet, launch_slot_after_sort, launch_slot_dup,
+                          launch_slot_keys, launch_slot_tail, make_slot, start_capacities)
 from gs_hip.pose import build_instance
 
 pytestmark = pytest.mark.gpu
@@ -32,11 +32,13 @@ def _f32(t: torch.Tensor, off: int, count: int) -> np.ndarray:
     return t[off:off + 4 * count].cpu().numpy().view(np.float32)
 
 
+@pytest.mark.parametrize("keys_set", [False, True])
 @pytest.mark.parametrize("shared_list", [False, True])
 @pytest.mark.parametrize("sizes", [[(160, 90, 12000), (320, 180, 40000), (128, 72, 5000)],
                                    [(960, 540, 200000), (480, 270, 60000), (320, 180, 20000)]])
-def test_shared_sorts_equal_per_frame_binning(sizes, shared_list):
-    print("shared depth sort, shared list sort", shared_list, "frames", sizes, flush=True)
+def test_shared_sorts_equal_per_frame_binning(sizes, shared_list, keys_set):
+    print("shared depth sort, shared list sort", shared_list, "one-set preprocess and keys", keys_set, "frames", sizes,
+          flush=True)
     insts = [build_instance(300 + k, W=W, H=H, P=P) for k, (W, H, P) in enumerate(sizes)]
     pose = torch.tensor([0.0, 0, 0, 0, 0, 0, 1], dtype=torch.float32, device=DEV).repeat(len(insts), 1)
     caps = start_capacities(insts, pose, _lib.BINNING_DEPTH_FIRST, 1.3)
@@ -57,9 +59,13 @@ def test_shared_sorts_equal_per_frame_binning(sizes, shared_list):
                                  s.capacity, c(s.binning.data_ptr()), s.binning.numel(), c(s.image.data_ptr()),
                                  s.image.numel(), c(s.color.data_ptr()), c(s.depth.data_ptr()), None, stream),
                    "gs_render")
-    # the split: depth keys per frame, one segmented depth sort, the rest per frame
-    for s in b:
-        launch_slot_keys(s, stream)
+    # the split: preprocess and depth keys per frame (or as one set), one
+    # segmented depth sort, the rest per frame
+    if keys_set:
+        KeysSet(b, DEV).launch(stream)
+    else:
+        for s in b:
+            launch_slot_keys(s, stream)
     DepthSortSet(b, DEV).launch(stream)
     if shared_list:
         for s in b:
@@ -77,6 +83,14 @@ def test_shared_sorts_equal_per_frame_binning(sizes, shared_list):
         print(k, "num_rendered", ca[0], "visible", ca[2], "status", ca[3], flush=True)
         assert (ca == cb).all()
         assert ca[3] == 0 and 0 < ca[0] <= sa.capacity
+        LG = _raster._geom_layout(P)
+        assert torch.equal(sa.radii, sb.radii)
+        tiles = _u32(sa.geom, LG.tiles, P)
+        assert np.array_equal(tiles, _u32(sb.geom, LG.tiles, P))
+        vis = tiles != 0  # (the splat records of culled Gaussians are not written)
+        assert np.array_equal(_u32(sa.geom, LG.splat, 12 * P).reshape(P, 12)[vis],
+                              _u32(sb.geom, LG.splat, 12 * P).reshape(P, 12)[vis])
+        assert np.array_equal(_u32(sa.geom, LG.offsets, P), _u32(sb.geom, LG.offsets, P))
         LB = _raster._binning_layout(sa.capacity, P, W, H, _lib.BINNING_DEPTH_FIRST)
         LI = _raster._image_layout(W, H)
         nt = ((W + 15) // 16) * ((H + 15) // 16)
