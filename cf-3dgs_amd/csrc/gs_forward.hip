@@ -412,19 +412,24 @@ __global__ __launch_bounds__(256) void k_df_keys_multi(const uint32_t* prefix, c
 }
 
 // Workgroup-local inclusive prefix of the tile counts in depth order.
-__global__ __launch_bounds__(256) void k_df_scan(int P, const uint32_t* dvals, const uint32_t* tiles, uint32_t* dscan,
-                                                 uint32_t* dblocks) {
+__device__ __forceinline__ void df_scan_block(int P, const uint32_t* dvals, const uint32_t* tiles, uint32_t* dscan,
+                                              uint32_t* dblocks, int bid) {
   __shared__ uint32_t s_scan[256];
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int k = bid * 256 + threadIdx.x;
   const uint32_t v = k < P ? tiles[dvals[k]] : 0u;
   uint32_t total;
   const uint32_t incl = block_incl_scan_256(v, s_scan, total);
   if (k < P) dscan[k] = incl;
-  if (threadIdx.x == 0) dblocks[blockIdx.x] = total;
+  if (threadIdx.x == 0) dblocks[bid] = total;
+}
+
+__global__ __launch_bounds__(256) void k_df_scan(int P, const uint32_t* dvals, const uint32_t* tiles, uint32_t* dscan,
+                                                 uint32_t* dblocks) {
+  df_scan_block(P, dvals, tiles, dscan, dblocks, blockIdx.x);
 }
 
 // One workgroup: in-place exclusive prefix of n values.
-__global__ __launch_bounds__(1024) void k_excl_scan_1wg(uint32_t* data, int n) {
+__device__ __forceinline__ void excl_scan_block(uint32_t* data, int n) {
   __shared__ uint32_t s[1024];
   __shared__ uint32_t s_carry;
   const int t = threadIdx.x;
@@ -442,6 +447,8 @@ __global__ __launch_bounds__(1024) void k_excl_scan_1wg(uint32_t* data, int n) {
     __syncthreads();
   }
 }
+
+__global__ __launch_bounds__(1024) void k_excl_scan_1wg(uint32_t* data, int n) { excl_scan_block(data, n); }
 
 struct DfDupArgs {
   int P, gx, gy;
@@ -465,9 +472,9 @@ struct DfDupArgs {
 // a run (a run leaves as one contiguous segment, not as one word from each of
 // 64 places per store).  Entry order and values are those of a per-Gaussian
 // loop over the rect, row by row.
-__global__ __launch_bounds__(256) void k_df_duplicate(DfDupArgs a) {
+__device__ __forceinline__ void df_duplicate_block(const DfDupArgs& a, int bid) {
   __shared__ uint32_t s_excl[4][64], s_off[4][64], s_g[4][64], s_xy[4][64], s_w[4][64];
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int k = bid * 256 + threadIdx.x;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t n = a.counters[0];
   if ((long long)n > a.capacity) {
@@ -522,6 +529,40 @@ __global__ __launch_bounds__(256) void k_df_duplicate(DfDupArgs a) {
       a.vals[dst] = s_g[w][j];
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_df_duplicate(DfDupArgs a) { df_duplicate_block(a, blockIdx.x); }
+
+// The depth-first binning's steps between its two sorts for one frame of a
+// multi-frame launch.
+struct DfMid {
+  DfDupArgs dup;
+  uint32_t* dscan;
+  uint32_t* dblocks;
+  int nblocks;
+};
+
+// Many frames' workgroup-local scans of the tile counts in depth order: frame
+// f owns workgroups [prefix[f], prefix[f + 1]).
+__global__ __launch_bounds__(256) void k_df_scan_multi(const uint32_t* prefix, const DfMid* mid, int n) {
+  const uint32_t b = blockIdx.x;
+  const int f = frame_of_block(prefix, n, b);
+  const DfMid m = mid[f];
+  df_scan_block(m.dup.P, m.dup.dvals, m.dup.tiles, m.dscan, m.dblocks, (int)(b - prefix[f]));
+}
+
+// Workgroup f: the exclusive prefix of frame f's block totals.
+__global__ __launch_bounds__(1024) void k_excl_scan_multi(const DfMid* mid) {
+  const DfMid m = mid[blockIdx.x];
+  excl_scan_block(m.dblocks, m.nblocks);
+}
+
+// Many frames' duplicates, on the same workgroup ranges as k_df_scan_multi.
+__global__ __launch_bounds__(256) void k_df_duplicate_multi(const uint32_t* prefix, const DfMid* mid, int n) {
+  const uint32_t b = blockIdx.x;
+  const int f = frame_of_block(prefix, n, b);
+  const DfDupArgs a = mid[f].dup;
+  df_duplicate_block(a, (int)(b - prefix[f]));
 }
 
 struct BlendArgs {
@@ -729,8 +770,8 @@ __global__ __launch_bounds__(256) void k_blend_multi(const uint32_t* prefix, con
 // give, which no result depends on).  A launch lasts as long as its last
 // tiles: the longest lists start first, and consecutive workgroups go to
 // different XCDs, so the long lists of one image region spread over the GPU.
-__global__ __launch_bounds__(1024) void k_tile_ckpt_scan(uint32_t* ranges, int nt, uint32_t* ckpt_base,
-                                                         uint32_t* item_log, uint32_t* tile_order) {
+__device__ __forceinline__ void tile_ckpt_scan_block(uint32_t* ranges, int nt, uint32_t* ckpt_base,
+                                                     uint32_t* item_log, uint32_t* tile_order) {
   __shared__ uint32_t s[1024];
   __shared__ uint32_t s_carry;
   __shared__ uint32_t s_cls[256];
@@ -778,6 +819,25 @@ __global__ __launch_bounds__(1024) void k_tile_ckpt_scan(uint32_t* ranges, int n
   }
   __syncthreads();
   for (int b = t; b < nt; b += 1024) tile_order[atomicAdd(&s_cls[cls_of(b)], 1u)] = (uint32_t)b;
+}
+
+__global__ __launch_bounds__(1024) void k_tile_ckpt_scan(uint32_t* ranges, int nt, uint32_t* ckpt_base,
+                                                         uint32_t* item_log, uint32_t* tile_order) {
+  tile_ckpt_scan_block(ranges, nt, ckpt_base, item_log, tile_order);
+}
+
+struct CkptScan {
+  uint32_t* ranges;
+  uint32_t* ckpt_base;
+  uint32_t* item_log;
+  uint32_t* tile_order;
+  int nt;
+};
+
+// Workgroup f: the checkpoint scan of frame f.
+__global__ __launch_bounds__(1024) void k_tile_ckpt_scan_multi(const CkptScan* cs) {
+  const CkptScan c = cs[blockIdx.x];
+  tile_ckpt_scan_block(c.ranges, c.nt, c.ckpt_base, c.item_log, c.tile_order);
 }
 
 __global__ __launch_bounds__(256) void k_mark_visible(const float* means, int P, const float* view,
@@ -1321,6 +1381,103 @@ extern "C" int gs_multi_preprocess_keys(int n, const void* dev_set, const unsign
     hipLaunchKernelGGL(k_df_keys_multi, dim3(launch[1]), dim3(256), 0, s,
                        (const uint32_t*)(d + mpk_prefix2_offset(n)), (const DfKeysArgs*)(d + mpk_keys_offset(n)), n);
   }
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+// ---- many frames' binning steps around the list sort ----------------------
+// Set layout: uint32 prefix[n + 1] of per-Gaussian workgroup bounds per frame,
+// then DfMid[n] and CkptScan[n], each array at a 256-byte boundary.
+static size_t mbm_mid_offset(int n) { return align_up(sizeof(uint32_t) * (size_t)(n + 1), 256); }
+static size_t mbm_ckpt_offset(int n) { return mbm_mid_offset(n) + align_up(sizeof(DfMid) * (size_t)n, 256); }
+
+extern "C" size_t gs_multi_binning_steps_bytes(int n) {
+  return n <= 0 ? 0 : mbm_ckpt_offset(n) + sizeof(CkptScan) * (size_t)n;
+}
+
+extern "C" int gs_multi_binning_steps_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
+                                            uint32_t* const* counters, const long long* capacity,
+                                            void* const* binning, const size_t* binning_bytes, void* const* image,
+                                            const size_t* image_bytes, void* host_set, size_t set_bytes,
+                                            unsigned* total_blocks) {
+  if (n <= 0 || !frames || !P || !geom || !counters || !capacity || !binning || !binning_bytes || !image ||
+      !image_bytes || !host_set || !total_blocks)
+    return GS_E_ARG;
+  if (set_bytes < gs_multi_binning_steps_bytes(n)) return GS_E_WORKSPACE;
+  uint32_t* prefix = (uint32_t*)host_set;
+  DfMid* mid = (DfMid*)((char*)host_set + mbm_mid_offset(n));
+  CkptScan* ck = (CkptScan*)((char*)host_set + mbm_ckpt_offset(n));
+  unsigned long long total = 0;
+  for (int i = 0; i < n; ++i) {
+    const gs_frame_t* f = &frames[i];
+    // the depth-first schedule's split phases, as gs_render_binning_duplicate
+    // and gs_render_binning_after_list_sort take them
+    if (check_frame(f) != GS_OK || P[i] <= 0 || !geom[i] || !counters[i] || !binning[i] || !image[i] ||
+        capacity[i] <= 0 || capacity[i] > 0x7fffffffLL || f->binning != GS_BINNING_DEPTH_FIRST)
+      return GS_E_ARG;
+    gs_geom_layout_t LG;
+    gs_binning_layout_t LB;
+    gs_image_layout_t LI;
+    gs_geom_layout(P[i], &LG);
+    if (gs_binning_layout(capacity[i], P[i], f->width, f->height, f->binning, &LB) != GS_OK) return GS_E_ARG;
+    gs_image_layout(f->width, f->height, &LI);
+    if (binning_bytes[i] < LB.total || image_bytes[i] < LI.total) return GS_E_WORKSPACE;
+    int gx, gy;
+    const int nt = grid_tiles(f->width, f->height, gx, gy);
+    if (gx > 0xffff || gy > 0xffff) return GS_E_ARG;  // tile rects are packed in 16-bit fields
+    const int nbp = nblocks256(P[i]);
+    prefix[i] = (uint32_t)total;
+    total += (unsigned long long)nbp;
+    if (total > 0x7fffffffULL) return GS_E_ARG;
+    const char* gb = (const char*)geom[i];
+    char* bb = (char*)binning[i];
+    char* ib = (char*)image[i];
+    // where the depth sort (per frame or segmented) leaves its result
+    const int in_b = sort_passes(32) & 1;
+    DfDupArgs& d = mid[i].dup;
+    d.P = P[i];
+    d.gx = gx;
+    d.gy = gy;
+    d.capacity = capacity[i];
+    d.rect = (const uint2*)(bb + LB.drect);
+    d.tiles = (const uint32_t*)(gb + LG.tiles);
+    d.dvals = (const uint32_t*)(bb + (in_b ? LB.dvals_b : LB.dvals_a));
+    d.dscan = (const uint32_t*)(bb + LB.dscan);
+    d.dblocks = (const uint32_t*)(bb + LB.dblocks);
+    d.counters = counters[i];
+    d.keys = (uint32_t*)(bb + LB.keys_dup);
+    d.vals = (uint32_t*)(bb + LB.vals_dup);
+    d.id_bits = LB.id_bits;
+    mid[i].dscan = (uint32_t*)(bb + LB.dscan);
+    mid[i].dblocks = (uint32_t*)(bb + LB.dblocks);
+    mid[i].nblocks = nbp;
+    ck[i].ranges = (uint32_t*)(ib + LI.ranges);
+    ck[i].ckpt_base = (uint32_t*)(ib + LI.ckpt_base);
+    ck[i].item_log = (uint32_t*)(bb + LB.item_log);
+    ck[i].tile_order = (uint32_t*)(ib + LI.tile_order);
+    ck[i].nt = nt;
+  }
+  prefix[n] = (uint32_t)total;
+  *total_blocks = (unsigned)total;
+  return GS_OK;
+}
+
+extern "C" int gs_multi_binning_duplicate(int n, const void* dev_set, unsigned total_blocks, void* stream) {
+  if (n <= 0 || !dev_set) return GS_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t* prefix = (const uint32_t*)dev_set;
+  const DfMid* mid = (const DfMid*)((const char*)dev_set + mbm_mid_offset(n));
+  timing::Scope ts(timing::kDuplicate, s);
+  if (total_blocks > 0) hipLaunchKernelGGL(k_df_scan_multi, dim3(total_blocks), dim3(256), 0, s, prefix, mid, n);
+  hipLaunchKernelGGL(k_excl_scan_multi, dim3(n), dim3(1024), 0, s, mid);
+  if (total_blocks > 0)
+    hipLaunchKernelGGL(k_df_duplicate_multi, dim3(total_blocks), dim3(256), 0, s, prefix, mid, n);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
+}
+
+extern "C" int gs_multi_binning_after_list_sort(int n, const void* dev_set, void* stream) {
+  if (n <= 0 || !dev_set) return GS_E_ARG;
+  hipLaunchKernelGGL(k_tile_ckpt_scan_multi, dim3(n), dim3(1024), 0, (hipStream_t)stream,
+                     (const CkptScan*)((const char*)dev_set + mbm_ckpt_offset(n)));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 

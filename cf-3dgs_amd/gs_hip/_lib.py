@@ -170,6 +170,15 @@ EXPORTS = {
                                                ctypes.POINTER(c_size_t), c_void_p, c_size_t,
                                                ctypes.POINTER(ctypes.c_uint)]),
     "gs_multi_preprocess_keys": (c_int, [c_int, c_void_p, ctypes.POINTER(ctypes.c_uint), c_void_p]),
+    "gs_multi_binning_steps_bytes": (c_size_t, [c_int]),
+    "gs_multi_binning_steps_build": (c_int, [c_int, ctypes.POINTER(GsFrame), ctypes.POINTER(c_int),
+                                             ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                             ctypes.POINTER(c_longlong), ctypes.POINTER(c_void_p),
+                                             ctypes.POINTER(c_size_t), ctypes.POINTER(c_void_p),
+                                             ctypes.POINTER(c_size_t), c_void_p, c_size_t,
+                                             ctypes.POINTER(ctypes.c_uint)]),
+    "gs_multi_binning_duplicate": (c_int, [c_int, c_void_p, ctypes.c_uint, c_void_p]),
+    "gs_multi_binning_after_list_sort": (c_int, [c_int, c_void_p, c_void_p]),
     "gs_multi_pose_bwd_bytes": (c_size_t, [c_int]),
     "gs_multi_pose_bwd_build": (c_int, [c_int, ctypes.POINTER(GsFrame), ctypes.POINTER(GsGaussians),
                                         ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),

@@ -334,6 +334,43 @@ class KeysSet:
                                                         self.launch_dims, stream), "gs_multi_preprocess_keys")
 
 
+class BinningStepsSet:
+    """The binning steps on either side of the shared list sort of many
+    solves: the duplicates (gs_multi_binning_duplicate, three launches) and
+    the checkpoint scans (gs_multi_binning_after_list_sort, one launch)."""
+
+    def __init__(self, slots: Sequence[_Slot], device):
+        lib = _lib.load()
+        n = len(slots)
+        self.n = n
+        vp = ctypes.c_void_p
+        sz = ctypes.c_size_t
+
+        def ptrs(attr):
+            return (vp * n)(*[getattr(s, attr).data_ptr() for s in slots])
+
+        frames = (_lib.GsFrame * n)(*[s.frame for s in slots])
+        P = (ctypes.c_int * n)(*[s.inst.P for s in slots])
+        cap = (ctypes.c_longlong * n)(*[s.capacity for s in slots])
+        nb = int(lib.gs_multi_binning_steps_bytes(n))
+        host = (ctypes.c_uint8 * nb)()
+        blocks = ctypes.c_uint(0)
+        _lib.check(lib.gs_multi_binning_steps_build(
+            n, frames, P, ptrs("geom"), ptrs("counters"), cap, ptrs("binning"),
+            (sz * n)(*[s.binning.numel() for s in slots]), ptrs("image"), (sz * n)(*[s.image.numel() for s in slots]),
+            host, nb, ctypes.byref(blocks)), "gs_multi_binning_steps_build")
+        self.dev = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+        self.blocks = blocks.value
+
+    def duplicate(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_binning_duplicate(self.n, ctypes.c_void_p(self.dev.data_ptr()), self.blocks,
+                                                          stream), "gs_multi_binning_duplicate")
+
+    def after_list_sort(self, stream: ctypes.c_void_p) -> None:
+        _lib.check(_lib.load().gs_multi_binning_after_list_sort(self.n, ctypes.c_void_p(self.dev.data_ptr()), stream),
+                   "gs_multi_binning_after_list_sort")
+
+
 def launch_slot_loss(s: _Slot, out3_ptr: int, one_ptr: int, lam: float, stream: ctypes.c_void_p) -> None:
     """Fused loss forward (stats to ``out3_ptr``) and backward (d loss scaled
     by the float at ``one_ptr``) of one solve's rendered image."""
@@ -538,9 +575,9 @@ class BatchedPoseSolver:
             raise ValueError("the shared depth sort needs the depth-first binning")
         self.stream_members = {q: [i for i in range(self.n) if self.stream_of[i] == q]
                                for q in sorted(set(self.stream_of))}
-        self.dsets = self.lsets = self.ksets = self.losssets = None
+        self.dsets = self.lsets = self.ksets = self.msets = self.losssets = None
         if self.shared_sort:
-            self.dsets, self.lsets, self.ksets = self._sort_sets()
+            self.dsets, self.lsets, self.ksets, self.msets = self._sort_sets()
             self.losssets = self._loss_sets() if self.multi else None
 
     def _frame_sets(self) -> List[FrameSets]:
@@ -558,7 +595,10 @@ class BatchedPoseSolver:
     def _sort_sets(self):
         """Shared depth and list sorts, one per group (one-launch blend
         schedules) or per stream (per-frame schedules); the list sorts stay
-        per frame (None) when a solve's list cannot join a shared one."""
+        per frame (None) when a solve's list cannot join a shared one.  With
+        them, the sets of the steps before and between the sorts (preprocess
+        and depth keys; the duplicates and checkpoint scans, only beside
+        shared list sorts)."""
         parts = self.members if self.multi else list(self.stream_members.values())
         keys = range(len(parts)) if self.multi else list(self.stream_members)
         try:
@@ -570,7 +610,9 @@ class BatchedPoseSolver:
         except _lib.GsError:
             lsets = None
         ksets = {k: KeysSet([self.slots[i] for i in mem], self.device) for k, mem in zip(keys, parts)}
-        return dsets, lsets, ksets
+        msets = None if lsets is None else {k: BinningStepsSet([self.slots[i] for i in mem], self.device)
+                                            for k, mem in zip(keys, parts)}
+        return dsets, lsets, ksets, msets
 
     def _make_slot(self, inst: PoseInstance, capacity: int, i: int) -> _Slot:
         return make_slot(inst, capacity, self.binning, _vp(self.adam.pose, 28 * i), _vp(self.adam.grad, 28 * i))
@@ -648,9 +690,9 @@ class BatchedPoseSolver:
                 self.ksets[g].launch(lp)
                 self.dsets[g].launch(lp)
                 if self.lsets is not None:
-                    self._per_slot(lambda i, s, st: launch_slot_dup(s, st), members, lead)
+                    self.msets[g].duplicate(lp)
                     self.lsets[g].launch(lp)
-                    self._per_slot(lambda i, s, st: launch_slot_tail(s, st, blend=False), members, lead)
+                    self.msets[g].after_list_sort(lp)
                 else:
                     self._per_slot(lambda i, s, st: launch_slot_after_sort(s, st, blend=False), members, lead)
             else:
@@ -691,8 +733,7 @@ class BatchedPoseSolver:
                 self.ksets[q].launch(sp)
                 self.dsets[q].launch(sp)
                 if self.lsets is not None:
-                    for i in mem:
-                        launch_slot_dup(self.slots[i], sp)
+                    self.msets[q].duplicate(sp)
                     self.lsets[q].launch(sp)
                 for i in mem:
                     s = self.slots[i]
@@ -761,7 +802,7 @@ class BatchedPoseSolver:
             if self.multi:
                 self.sets = self._frame_sets()
             if self.shared_sort:
-                self.dsets, self.lsets, self.ksets = self._sort_sets()
+                self.dsets, self.lsets, self.ksets, self.msets = self._sort_sets()
                 self.losssets = self._loss_sets() if self.multi else None
         raise RuntimeError(f"binning capacity still exceeded after {max_regrow} regrowths")
 

@@ -330,6 +330,24 @@ int gs_multi_list_sort_build(int n, const gs_frame_t* frames, const int* P, cons
                              void* const* binning, const size_t* binning_bytes, uint32_t* const* counters,
                              void* const* image, void* host_set, size_t set_bytes, unsigned* blocks);
 int gs_multi_list_sort(int n, const void* dev_set, const unsigned* blocks, void* stream);
+/* The depth-first binning's steps on either side of the list sort for many
+ * frames: gs_multi_binning_duplicate is gs_render_binning_duplicate of every
+ * frame (three launches: the workgroup-local scans of the tile counts in
+ * depth order, one workgroup per frame for the block prefix, the
+ * duplicates); gs_multi_binning_after_list_sort is
+ * gs_render_binning_after_list_sort of every frame (one launch, a workgroup
+ * per frame).  The results are those of the per-frame calls, bit for bit.
+ * The build takes those calls' per-frame arguments as arrays and checks them
+ * as they do; total_blocks is returned by the build.  Replaces, for many
+ * frames at once, the duplicate and the tile-range step of forward phase 2
+ * above. */
+size_t gs_multi_binning_steps_bytes(int n);
+int gs_multi_binning_steps_build(int n, const gs_frame_t* frames, const int* P, const void* const* geom,
+                                 uint32_t* const* counters, const long long* capacity, void* const* binning,
+                                 const size_t* binning_bytes, void* const* image, const size_t* image_bytes,
+                                 void* host_set, size_t set_bytes, unsigned* total_blocks);
+int gs_multi_binning_duplicate(int n, const void* dev_set, unsigned total_blocks, void* stream);
+int gs_multi_binning_after_list_sort(int n, const void* dev_set, void* stream);
 int gs_render_binning_after_list_sort(const gs_frame_t* frame, int P, const void* geom, uint32_t* counters,
                                       long long capacity, void* binning, size_t binning_bytes, void* image,
                                       size_t image_bytes, void* stream);

@@ -519,3 +519,55 @@ def test_multi_preprocess_keys_set_on_the_host():
     assert lib.gs_multi_preprocess_keys(0, dev, launch, None) == 1
     assert lib.gs_multi_preprocess_keys(1, None, launch, None) == 1
     assert lib.gs_multi_preprocess_keys(1, dev, None, None) == 1
+
+
+def test_multi_binning_steps_set_on_the_host():
+    """gs_multi_binning_steps_build lays out one workgroup per 256 Gaussians
+    of each frame and refuses what gs_render_binning_duplicate refuses.  Built
+    on the host only: the device addresses below are never read."""
+    import ctypes
+    from gs_hip import _lib
+    lib = _lib.load()
+    n = 3
+    dims = [(160, 90, 1000), (960, 540, 300), (320, 180, 513)]
+    frames = (_lib.GsFrame * n)()
+    for i, (W, H, _) in enumerate(dims):
+        f = frames[i]
+        f.width, f.height, f.sh_degree = W, H, 0
+        f.viewmatrix, f.projmatrix, f.bg = 0x1000, 0x2000, 0x3000
+        f.binning = _lib.BINNING_DEPTH_FIRST
+    cap = 50000
+    P = (ctypes.c_int * n)(*[p for _, _, p in dims])
+    vp = lambda: (ctypes.c_void_p * n)(*([0x10000000] * n))  # noqa: E731
+    caps = (ctypes.c_longlong * n)(*([cap] * n))
+    sz = ctypes.c_size_t
+    bb = (sz * n)(*[_lib.binning_layout(cap, p, W, H, _lib.BINNING_DEPTH_FIRST).total for W, H, p in dims])
+    ib = (sz * n)(*[_lib.image_layout(W, H).total for W, H, _ in dims])
+    nb = int(lib.gs_multi_binning_steps_bytes(n))
+    assert lib.gs_multi_binning_steps_bytes(0) == 0 and nb > 0
+    host = (ctypes.c_uint8 * nb)()
+    blocks = ctypes.c_uint(0)
+
+    def build(nbytes=nb):
+        return lib.gs_multi_binning_steps_build(n, frames, P, vp(), vp(), caps, vp(), bb, vp(), ib, host, nbytes,
+                                                ctypes.byref(blocks))
+
+    assert build() == 0
+    assert blocks.value == 4 + 2 + 3
+    assert list((ctypes.c_uint32 * (n + 1)).from_buffer(host)) == [0, 4, 6, 9]
+    assert build(nb - 1) == 2
+    frames[2].binning = _lib.BINNING_SORT64  # no depth sort to split around
+    assert build() == 1
+    frames[2].binning = _lib.BINNING_DEPTH_FIRST
+    P[1] = 0  # an empty frame has no depth-first steps
+    assert build() == 1
+    P[1] = 300
+    ib[0] -= 1
+    assert build() == 2
+    ib[0] += 1
+    assert build() == 0
+    dev = ctypes.c_void_p(0x10000000)
+    assert lib.gs_multi_binning_duplicate(0, dev, 1, None) == 1
+    assert lib.gs_multi_binning_duplicate(1, None, 1, None) == 1
+    assert lib.gs_multi_binning_after_list_sort(0, dev, None) == 1
+    assert lib.gs_multi_binning_after_list_sort(1, None, None) == 1

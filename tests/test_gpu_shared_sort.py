@@ -16,8 +16,8 @@ import torch
 
 from gs_hip import _lib
 from gs_hip import raster as _raster
-from gs_hip.batch import (DepthSortSet, KeysSet, ListSortSet, launch_slot_after_sort, launch_slot_dup,
-                          launch_slot_keys, launch_slot_tail, make_slot, start_capacities)
+from gs_hip.batch import (BinningStepsSet, DepthSortSet, FrameSets, KeysSet, ListSortSet, launch_slot_after_sort,
+                          launch_slot_dup, launch_slot_keys, launch_slot_tail, make_slot, start_capacities)
 from gs_hip.pose import build_instance
 
 pytestmark = pytest.mark.gpu
@@ -67,7 +67,14 @@ def test_shared_sorts_equal_per_frame_binning(sizes, shared_list, keys_set):
         for s in b:
             launch_slot_keys(s, stream)
     DepthSortSet(b, DEV).launch(stream)
-    if shared_list:
+    if shared_list and keys_set:
+        # with the one-set duplicates and checkpoint scans, and one blend launch
+        steps = BinningStepsSet(b, DEV)
+        steps.duplicate(stream)
+        ListSortSet(b, DEV).launch(stream)
+        steps.after_list_sort(stream)
+        FrameSets(b, DEV).blend_launch(stream)
+    elif shared_list:
         for s in b:
             launch_slot_dup(s, stream)
         ListSortSet(b, DEV).launch(stream)
