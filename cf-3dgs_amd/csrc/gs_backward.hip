@@ -113,30 +113,33 @@ __device__ __forceinline__ void bwd_items(const uint32_t* item_log, uint32_t* it
 // The backward's set-up in one launch: workgroup 0 builds the work-item table
 // (bwd_items; none when items is null), the others clear the zn accumulator
 // floats at zp.
-// Workgroup bid of a set-up over nblocks workgroups.
-__device__ __forceinline__ void bwd_prepare_block(float* zp, long long zn, const uint32_t* item_log, uint32_t* items,
-                                                  int bid, int nblocks) {
-  const int first = items ? 1 : 0;
-  if (first && bid == 0) {
-    bwd_items(item_log, items);
-    return;
-  }
-  const long long nb = (long long)nblocks - first;
-  for (long long k = ((long long)bid - first) * 1024 + threadIdx.x; k < zn; k += nb * 1024) zp[k] = 0.0f;
-}
-
-__global__ __launch_bounds__(1024) void k_bwd_prepare(float* zp, long long zn, const uint32_t* item_log,
-                                                      uint32_t* items) {
-  bwd_prepare_block(zp, zn, item_log, items, blockIdx.x, gridDim.x);
-}
-
-// The set-up of one frame of a multi-frame launch.
+// The set-up of one frame: zn floats at zp to clear, or, when terms > 0
+// (the deterministic mode's pair partials), only the first num_rendered *
+// terms of them (counters[0] = num_rendered; no pair slot at or past it is
+// written or read).
 struct BwdPrep {
   float* zp;
   long long zn;
   const uint32_t* item_log;
   uint32_t* items;
+  const uint32_t* counters;
+  int terms;
 };
+
+// Workgroup bid of a set-up over nblocks workgroups.
+__device__ __forceinline__ void bwd_prepare_block(const BwdPrep& p, int bid, int nblocks) {
+  const int first = p.items ? 1 : 0;
+  if (first && bid == 0) {
+    bwd_items(p.item_log, p.items);
+    return;
+  }
+  long long zn = p.zn;
+  if (p.terms > 0) zn = min(zn, (long long)p.counters[0] * p.terms);
+  const long long nb = (long long)nblocks - first;
+  for (long long k = ((long long)bid - first) * 1024 + threadIdx.x; k < zn; k += nb * 1024) p.zp[k] = 0.0f;
+}
+
+__global__ __launch_bounds__(1024) void k_bwd_prepare(BwdPrep p) { bwd_prepare_block(p, blockIdx.x, gridDim.x); }
 
 // Many frames in one launch: frame f owns workgroups [prefix[f], prefix[f +
 // 1]), as many as its own launch would have.
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(1024) void k_bwd_prepare_multi(const uint32_t* pref
     else hi = mid - 1;
   }
   const BwdPrep p = prep[lo];
-  bwd_prepare_block(p.zp, p.zn, p.item_log, p.items, (int)(b - prefix[lo]), (int)(prefix[lo + 1] - prefix[lo]));
+  bwd_prepare_block(p, (int)(b - prefix[lo]), (int)(prefix[lo + 1] - prefix[lo]));
 }
 
 // Sum over the 16 lanes of each DPP row; every lane of the row gets the sum.
@@ -1215,13 +1218,15 @@ static int bwd_check(const gs_frame_t* f, const gs_gaussians_t* g, const int* ra
 
 // Phase 1: clear the accumulators and build the work-item table.
 // One frame's set-up and the workgroups it takes (0: nothing to do).
-static BwdPrep bwd_prep_of(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, bool po,
-                           unsigned* grid) {
+static BwdPrep bwd_prep_of(const gs_frame_t* f, int P, long long capacity, const uint32_t* counters,
+                           const RBArgs& r, bool po, unsigned* grid) {
   BwdPrep p;
   // deterministic mode writes every record in k_det_gather; pairs past a
   // tile's last contributor are never visited, so their partials start at 0
   p.zp = f->deterministic ? r.part : r.g_render;
   p.zn = f->deterministic ? capacity * (po ? 5 : 10) : (long long)P * kRG;
+  p.counters = counters;
+  p.terms = f->deterministic ? (po ? 5 : 10) : 0;
   // the work-item table, from the forward's item log (both live in the
   // binning workspace)
   p.item_log = r.item_log;
@@ -1231,10 +1236,11 @@ static BwdPrep bwd_prep_of(const gs_frame_t* f, int P, long long capacity, const
   return p;
 }
 
-static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const RBArgs& r, hipStream_t s, bool po) {
+static void bwd_prepare(const gs_frame_t* f, int P, long long capacity, const uint32_t* counters, const RBArgs& r,
+                        hipStream_t s, bool po) {
   unsigned grid = 0;
-  const BwdPrep p = bwd_prep_of(f, P, capacity, r, po, &grid);
-  if (grid > 0) hipLaunchKernelGGL(k_bwd_prepare, dim3(grid), dim3(1024), 0, s, p.zp, p.zn, p.item_log, p.items);
+  const BwdPrep p = bwd_prep_of(f, P, capacity, counters, r, po, &grid);
+  if (grid > 0) hipLaunchKernelGGL(k_bwd_prepare, dim3(grid), dim3(1024), 0, s, p);
 }
 
 // Phase 2: the render backward of one frame.
@@ -1372,7 +1378,7 @@ extern "C" int gs_backward(GS_BWD_ARGS) {
   hipStream_t s = (hipStream_t)stream;
   const RBArgs r = make_rb_args(f, g->P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
   const bool po = pose_only(f, g, out, dL_ddepth || dL_dalpha);
-  bwd_prepare(f, g->P, capacity, r, s, po);
+  bwd_prepare(f, g->P, capacity, counters, r, s, po);
   bwd_render(f, g->P, capacity, r, s, po);
   bwd_finish(f, g, radii, geom, capacity, counters, out, r, s, po);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
@@ -1409,7 +1415,7 @@ extern "C" int gs_backward_adam(const gs_frame_t* f, const gs_gaussians_t* g, co
   ad.inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
   hipStream_t s = (hipStream_t)stream;
   const RBArgs r = make_rb_args(f, P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
-  bwd_prepare(f, P, capacity, r, s, false);
+  bwd_prepare(f, P, capacity, counters, r, s, false);
   bwd_render(f, P, capacity, r, s, false);
   bwd_finish(f, g, radii, geom, capacity, counters, out, r, s, false, &ad);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
@@ -1420,7 +1426,7 @@ extern "C" int gs_backward_prepare(GS_BWD_ARGS) {
                            scratch_bytes);
   if (rc != GS_OK) return rc;
   const RBArgs r = make_rb_args(f, g->P, geom, capacity, binning, image, dL_dcolor, dL_ddepth, dL_dalpha, scratch);
-  bwd_prepare(f, g->P, capacity, r, (hipStream_t)stream, pose_only(f, g, out, dL_ddepth || dL_dalpha));
+  bwd_prepare(f, g->P, capacity, counters, r, (hipStream_t)stream, pose_only(f, g, out, dL_ddepth || dL_dalpha));
   return hipGetLastError() == hipSuccess ? GS_OK : GS_E_LAUNCH;
 }
 
@@ -1537,7 +1543,7 @@ extern "C" int gs_multi_pose_bwd_build(int n, const gs_frame_t* frames, const gs
     const RBArgs r = make_rb_args(f, P, geom[i], capacity[i], binning[i], image[i], dL_dcolor[i], nullptr, nullptr,
                                   scratch[i]);
     unsigned pgrid = 0;
-    prep[i] = bwd_prep_of(f, P, capacity[i], r, true, &pgrid);
+    prep[i] = bwd_prep_of(f, P, capacity[i], counters[i], r, true, &pgrid);
     pprefix[i] = (uint32_t)ptotal;
     ptotal += pgrid;
     prefix[i] = (uint32_t)total;
